@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X transport step (advection sweeps + TRBDF2, FP64).
+
+Default workload: the production-size closed dipole grid lx1=512 x lx2=256 x lx3=192
+(25.2M cells, 7 species), CFL-limited dt, synthetic inputs generated in HBM.  One
+"step" is the reference's hot path for one time step (src/libgemini.cpp:155-208
+order): state conversions, global ghost fill, x3/x1/x2 MC advection sweeps of the 19
+advected field-species, rhov12v1, rhoe2T and the TRBDF2 energy diffusion of all 7
+species.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--workload dipole512|cart256|cart128]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+METRIC = "cell·species updates/s (adv+TRBDF2, fp64, 25M pts) vs CPU; adjoint/forward ratio"
+UNIT = "cell·species updates/s"
+
+WORKLOADS = {
+    "dipole512": dict(kind="dipole", shape=(512, 256, 192), steps=50,
+                      desc="closed dipole grid 512x256x192 (25.2M cells), 7 species, fp64",
+                      sample=(512, 32, 16)),
+    "cart256": dict(kind="cart", shape=(256, 192, 128), steps=100,
+                    desc="3-D Cartesian grid 256x192x128 (6.3M cells), 7 species, fp64",
+                    sample=(256, 32, 16)),
+    "cart128": dict(kind="cart", shape=(128, 64, 1), steps=1,
+                    desc="2-D Cartesian grid 128x64x1, 7 species, fp64", sample=(128, 64, 1)),
+}
+
+# Algorithmic HBM bytes per cell and launch: each array the kernel must read or write,
+# once.  Sweeps: 19 advected field-species read+written, 7 velocity components and 6
+# metric arrays read; TRBDF2: Ts, ns (7 species each), h1..h3, nn(1:3), J1 read, Ts
+# written; conversions over the padded grid.
+BYTES_PER_CELL = {"sweep1": 408, "sweep2": 408, "sweep3": 408, "trbdf2": 224,
+                  "pre_convert": 280, "rhov12v1": 160, "rhoe2T": 168}
+PADDED_KERNELS = {"pre_convert", "rhov12v1", "rhoe2T"}
+KERNELS = ["pre_convert", "ghost_fill", "halo_periodic", "save_v1b", "sweep3", "sweep1",
+           "sweep2", "rhov12v1", "rhoe2T", "trbdf2", "floor_ghosts"]
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_grid(kind, shape):
+    import gemini_amd as ga
+    return ga.dipole_grid(*shape) if kind == "dipole" else ga.cartesian_grid(*shape)
+
+
+def cpu_baseline(w, dt, seconds=15.0):
+    """Oracle (C restatement, OpenMP over lines/columns) on a bounded sample grid."""
+    import numpy as np
+    import gemini_amd as ga
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    oracle.set_num_threads(threads)
+    g = make_grid(w["kind"], w["sample"])
+    st = ga.synthetic_state(g, seed=1, perturb=(w["kind"] == "cart" and g.lx3 > 1))
+    t0 = time.perf_counter()
+    oracle.transport_step(g, st, dt)
+    t1 = time.perf_counter() - t0
+    n = max(1, min(200, int(seconds / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        oracle.transport_step(g, st, dt)
+    el = time.perf_counter() - t0
+    if not np.isfinite(st.fluidvars).all():
+        log("warning: non-finite values in the CPU sample")
+    value = g.ncells * 7 * n / el
+    return {"value": value, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": f"oracle (plain-C restatement of the reference path) on a "
+                      f"{g.lx1}x{g.lx2}x{g.lx3} {w['kind']} grid of the same field-line "
+                      f"length and profiles, {n} steps in {el:.1f} s after 1 warm-up step"}
+
+
+def load_traffic(workload, kernel):
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d[workload][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="dipole512", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the transport "
+              f"path shards over x2/x3 across GPUs by design, but this build runs on one.",
+              file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on a single rank", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.workload]
+    steps = args.steps if args.steps is not None else w["steps"]
+
+    import torch
+    from gemini_amd.synth_gpu import synthetic_state_device, cfl_dt_device
+    from gemini_amd.transport import Transport
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log(f"workload {args.workload}: {w['desc']}")
+    t0 = time.perf_counter()
+    g = make_grid(w["kind"], w["shape"])
+    st = synthetic_state_device(g, seed=args.seed, device="cuda:0")
+    dt = cfl_dt_device(g, st)
+    torch.cuda.synchronize()
+    log(f"inputs ready in {time.perf_counter() - t0:.1f} s, dt = {dt:.4g} s")
+    tr = Transport(g, st["nn"], st["Tn"], device=0, geometry_on_device=True)
+    tr.set_residency(1)
+    tr.push_blocks(st["fluidvars"], st["aux"], st["electrovars"])
+    del st
+    torch.cuda.empty_cache()
+    log("state in HBM; warm-up")
+    for _ in range(args.warmup):
+        tr.transport_step(dt)
+    tr.synchronize()
+    log(f"timing {steps} steps")
+    tr.kernel_timing_reset()
+    tr.kernel_timing(True)
+    tr.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        tr.transport_step(dt)
+    tr.synchronize()
+    wall = time.perf_counter() - t0
+    tr.kernel_timing(False)
+    ms_step = 1e3 * wall / steps
+    cells = g.ncells
+    padded = g.padded[0] * g.padded[1] * g.padded[2]
+    value = cells * 7 / (ms_step / 1e3)
+    kern = {}
+    for name in KERNELS:
+        tot, n = tr.kernel_time(name)
+        if n == 0:
+            continue
+        avg = tot / n
+        entry = {"ms": round(avg, 4), "launches_per_step": n / steps}
+        if name in BYTES_PER_CELL:
+            nb = BYTES_PER_CELL[name] * (padded if name in PADDED_KERNELS else cells)
+            entry["GB/s"] = round(nb / (avg / 1e3) / 1e9, 1)
+        kern[name] = entry
+    dom = max((k for k in kern if k in BYTES_PER_CELL), key=lambda k: kern[k]["ms"] *
+              kern[k]["launches_per_step"])
+    dom_bytes = BYTES_PER_CELL[dom] * (padded if dom in PADDED_KERNELS else cells)
+    achieved = dom_bytes / (kern[dom]["ms"] / 1e3) / 1e9
+    traffic = load_traffic(args.workload, dom)
+    log(f"{ms_step:.3f} ms/step, {value:.3e} cell-species/s; dominant kernel {dom} "
+        f"{kern[dom]['ms']:.3f} ms ({achieved:.0f} GB/s)")
+    # the TRBDF2 launch is one kernel for all species; report its step share as well
+    step_bytes = 1448 * cells
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle on host cores)")
+        cpu = cpu_baseline(w, dt, args.cpu_seconds)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded profiles generated in HBM)",
+        "config": {"workload": w["desc"], "grid": f"{w['kind']} {g.lx1}x{g.lx2}x{g.lx3}",
+                   "species": 7, "dt_s": dt, "diffusion": "TRBDF2",
+                   "step": "v12rhov1,T2rhoe,ghost fill,sweep3,sweep1,sweep2,rhov12v1,rhoe2T,"
+                           "TRBDF2"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": dom_bytes},
+        "step_hbm": {"algorithmic_bytes": step_bytes,
+                     "GB/s": round(step_bytes / (ms_step / 1e3) / 1e9, 1),
+                     "frac": round(step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels": kern,
+        "cpu_baseline": cpu,
+        "adjoint_forward_ratio": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

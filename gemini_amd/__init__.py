@@ -1,0 +1,3 @@
+"""MI355X-native GEMINI multifluid transport hot path (advection sweeps + TRBDF2)."""
+from .grid import Grid, cartesian_grid, dipole_grid  # noqa: F401
+from .state import State, synthetic_state, empty_state, cfl_dt  # noqa: F401

@@ -1,0 +1,785 @@
+// gemhip_api.hip -- C ABI of the transport library (declared in include/gemini3d_hip.h).
+//
+// Holds the single library context (the reference keeps sizes and the mesh in module
+// globals and Fortran objects; here they are registered once), moves state between the
+// caller's host blocks and HBM according to the residency mode, and implements the
+// reference-named entry points on top of the kernel launchers.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gemini3d_hip.h"
+#include "gemhip_internal.h"
+
+namespace gemhip {
+namespace {
+
+struct TimedLaunch {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct Ctx {
+  bool inited = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int lx1 = 0, lx2 = 0, lx3 = 0;
+  bool sized = false, grid_ok = false, neutrals_ok = false;
+  int xtype = 0;
+  int diffsolvetype = 2;
+  double Teinf = 1500.0;
+  DevGrid g{};
+  DevState s{};
+  double* d_geom = nullptr;
+  double* d_fluid = nullptr;
+  double* d_aux = nullptr;
+  double* d_J1 = nullptr;
+  double* d_vsi = nullptr;
+  double* d_v1b = nullptr;
+  double* d_neut = nullptr;
+  double* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  bool vsi_current = true;      // full intvars%vs*i materialised
+  int residency = 0;
+  const double* pushed_fluid = nullptr;
+  int error_mode = 0;
+  std::string last_error;
+  bool timing = false;
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long>> acc;
+  std::map<std::string, hipEvent_t> open;
+};
+
+Ctx& ctx() {
+  static Ctx c;
+  return c;
+}
+
+void fail(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  Ctx& c = ctx();
+  if (c.error_mode == 0) {
+    fprintf(stderr, "gemini3d_hip: %s\n", buf);
+    fflush(stderr);
+    abort();
+  }
+  if (c.last_error.empty()) c.last_error = buf;
+}
+
+bool hipok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  fail("%s: %s", what, hipGetErrorString(e));
+  return false;
+}
+
+hipEvent_t take_event() {
+  Ctx& c = ctx();
+  if (!c.pool.empty()) {
+    hipEvent_t e = c.pool.back();
+    c.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+void collect_timings() {
+  Ctx& c = ctx();
+  for (auto& t : c.pending) {
+    hipEventSynchronize(t.b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t.a, t.b);
+    auto& a = c.acc[t.name];
+    a.first += ms;
+    a.second += 1;
+    c.pool.push_back(t.a);
+    c.pool.push_back(t.b);
+  }
+  c.pending.clear();
+}
+
+void free_all() {
+  Ctx& c = ctx();
+  for (double** p : {&c.d_geom, &c.d_fluid, &c.d_aux, &c.d_J1, &c.d_vsi, &c.d_v1b, &c.d_neut,
+                     &c.d_scratch}) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+  }
+  c.scratch_bytes = 0;
+  c.sized = c.grid_ok = c.neutrals_ok = false;
+}
+
+bool dev_alloc(double** p, size_t n, const char* what) {
+  if (!hipok(hipMalloc((void**)p, n * sizeof(double)), what)) return false;
+  return hipok(hipMemset(*p, 0, n * sizeof(double)), what);
+}
+
+long nv1i() { Ctx& c = ctx(); return (long)(c.lx1 + 1) * c.lx2 * c.lx3; }
+long nv2i() { Ctx& c = ctx(); return (long)c.lx1 * (c.lx2 + 1) * c.lx3; }
+long nv3i() { Ctx& c = ctx(); return (long)c.lx1 * c.lx2 * (c.lx3 + 1); }
+
+bool ready(bool need_neutrals = false) {
+  Ctx& c = ctx();
+  if (!c.inited) { fail("library not initialised (gemhip_init)"); return false; }
+  if (!c.sized) { fail("grid size not set (gemhip_set_grid_size)"); return false; }
+  if (!c.grid_ok) { fail("grid not registered (gemhip_register_grid)"); return false; }
+  if (need_neutrals && !c.neutrals_ok) {
+    fail("neutral atmosphere not registered (gemhip_register_neutrals)");
+    return false;
+  }
+  return true;
+}
+
+// libgemini_c.f90:63-82: only Cartesian (1) and dipole (2) meshes exist
+bool check_xtype(const int* xtype) {
+  Ctx& c = ctx();
+  if (!xtype || (*xtype != 1 && *xtype != 2)) {
+    fail("Unable to identify object type during conversion from C to Fortran class pointer:  %d",
+         xtype ? *xtype : -1);
+    return false;
+  }
+  if (*xtype != c.xtype) {
+    fail("mesh type %d differs from the registered mesh type %d", *xtype, c.xtype);
+    return false;
+  }
+  return true;
+}
+
+// ---- host <-> HBM movement of whole variables (7 species fields each) ----
+enum Var { NS = 0, VS1 = 1, VS2 = 2, VS3 = 3, TS = 4, RHOVS1 = 5, RHOES = 6, J1 = 7 };
+
+double* dev_var(Var v) {
+  Ctx& c = ctx();
+  const long n7 = 7L * c.g.N;
+  switch (v) {
+    case NS: return c.d_fluid;
+    case VS1: return c.d_fluid + n7;
+    case VS2: return c.d_fluid + 2 * n7;
+    case VS3: return c.d_fluid + 3 * n7;
+    case TS: return c.d_fluid + 4 * n7;
+    case RHOVS1: return c.d_aux;
+    case RHOES: return c.d_aux + n7;
+    default: return c.d_J1;
+  }
+}
+
+const double* host_var(Var v, double* const* fluidvars, double* const* aux, double* const* ev) {
+  Ctx& c = ctx();
+  const long n7 = 7L * c.g.N;
+  switch (v) {
+    case NS: case VS1: case VS2: case VS3: case TS: return *fluidvars + (long)v * n7;
+    case RHOVS1: return *aux;
+    case RHOES: return *aux + n7;
+    default: return *ev + 3L * c.g.N;
+  }
+}
+
+bool check_host(double* const* p, const char* name) {
+  if (!p || !*p) { fail("%s is a null pointer", name); return false; }
+  return true;
+}
+
+struct Io {
+  double* const* fv;
+  double* const* av;
+  double* const* ev;
+};
+
+bool upload(const Io& io, std::initializer_list<Var> vars) {
+  Ctx& c = ctx();
+  if (c.residency == 1) {
+    if (io.fv && *io.fv && c.pushed_fluid && *io.fv != c.pushed_fluid) {
+      fail("fluidvars differs from the block pushed with gemhip_state_push");
+      return false;
+    }
+    return true;
+  }
+  for (Var v : vars) {
+    const size_t n = (v == J1 ? 1L : 7L) * c.g.N;
+    if (!hipok(hipMemcpyAsync(dev_var(v), host_var(v, io.fv, io.av, io.ev), n * sizeof(double),
+                              hipMemcpyDefault, c.stream), "upload"))
+      return false;
+  }
+  return true;
+}
+
+bool download(const Io& io, std::initializer_list<Var> vars) {
+  Ctx& c = ctx();
+  if (c.residency == 1) return true;
+  for (Var v : vars) {
+    const size_t n = 7L * c.g.N;
+    double* h = const_cast<double*>(host_var(v, io.fv, io.av, io.ev));
+    if (!hipok(hipMemcpyAsync(h, dev_var(v), n * sizeof(double), hipMemcpyDefault, c.stream),
+               "download"))
+      return false;
+  }
+  return hipok(hipStreamSynchronize(c.stream), "synchronise");
+}
+
+void sweep_entry(double** fv, double** av, int* xtype, double* dt, int dir, int fields) {
+  if (!ready() || !check_xtype(xtype)) return;
+  if (!dt) { fail("dt is a null pointer"); return; }
+  Ctx& c = ctx();
+  if (!c.vsi_current) {
+    fail("interface velocities are not materialised; call interface_vels_allspec_C first");
+    return;
+  }
+  Io io{fv, av, nullptr};
+  std::vector<Var> vars;
+  if (fields & F_MASS) vars.push_back(NS);
+  if (fields & F_MOMENTUM) vars.push_back(RHOVS1);
+  if (fields & F_ENERGY) vars.push_back(RHOES);
+  if ((fields & F_MASS) && !check_host(fv, "fluidvars")) return;
+  if ((fields & (F_MOMENTUM | F_ENERGY)) && !check_host(av, "fluidauxvars")) return;
+  for (Var v : vars) if (!upload(io, {v})) return;
+  launch_sweep(c.g, c.s, dir, fields, *dt, false, c.stream);
+  if (!hipok(hipGetLastError(), "sweep launch")) return;
+  for (Var v : vars) if (!download(io, {v})) return;
+}
+
+bool ensure_scratch() {
+  Ctx& c = ctx();
+  const size_t need = diffusion_scratch_bytes(c.g);
+  if (c.d_scratch && c.scratch_bytes >= need) return true;
+  if (c.d_scratch) hipFree(c.d_scratch);
+  c.d_scratch = nullptr;
+  if (!hipok(hipMalloc((void**)&c.d_scratch, need), "scratch allocation")) return false;
+  c.scratch_bytes = need;
+  return true;
+}
+
+bool run_energy_diffusion(double dt) {
+  Ctx& c = ctx();
+  if (c.diffsolvetype != 1 && c.diffsolvetype != 2) {
+    fail("Unsupported diffusion solver type/mode:  %d.  Should be either 1 or 2.",
+         c.diffsolvetype);
+    return false;
+  }
+  if (c.lx1 < 4) { fail("energy diffusion needs lx1 >= 4 (got %d)", c.lx1); return false; }
+  if (!ensure_scratch()) return false;
+  launch_energy_diffusion(c.g, c.s, dt, c.diffsolvetype, c.Teinf, c.d_scratch, c.stream);
+  return hipok(hipGetLastError(), "energy diffusion launch");
+}
+
+void halo_fields(std::initializer_list<Var> vars) {
+  Ctx& c = ctx();
+  double* f[8];
+  int n = 0;
+  for (Var v : vars) f[n++] = dev_var(v);
+  launch_halo_periodic(c.g, f, n, 2, LSP, c.stream);
+}
+
+// device evaluation of the scalar routines the kernels use, for the math checks
+__global__ void k_eval_math(int op, const double* x, double* y, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = x[i];
+  y[i] = op == 0 ? pow25(v) : op == 1 ? sqrt(v) : op == 2 ? 1.0 / v : pow(v, 2.5);
+}
+
+}  // namespace
+
+void timing_begin(const char* name, hipStream_t st) {
+  Ctx& c = ctx();
+  if (!c.timing) return;
+  hipEvent_t e = take_event();
+  hipEventRecord(e, st);
+  c.open[name] = e;
+}
+
+void timing_end(const char* name, hipStream_t st) {
+  Ctx& c = ctx();
+  if (!c.timing) return;
+  auto it = c.open.find(name);
+  if (it == c.open.end()) return;
+  hipEvent_t e = take_event();
+  hipEventRecord(e, st);
+  c.pending.push_back({name, it->second, e});
+  c.open.erase(it);
+  if (c.pending.size() > 4096) collect_timings();
+}
+
+}  // namespace gemhip
+
+using namespace gemhip;
+
+extern "C" {
+
+int gemhip_init(int device) {
+  Ctx& c = ctx();
+  if (c.inited) {
+    if (device == c.device) return 0;
+    fail("already initialised on device %d", c.device);
+    return 1;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) { fail("no HIP device available"); return 2; }
+  if (device < 0 || device >= n) { fail("device %d out of range (%d devices)", device, n); return 3; }
+  if (!hipok(hipSetDevice(device), "hipSetDevice")) return 4;
+  if (!hipok(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) return 5;
+  c.device = device;
+  c.inited = true;
+  return 0;
+}
+
+void gemhip_finalize(void) {
+  Ctx& c = ctx();
+  if (!c.inited) return;
+  hipStreamSynchronize(c.stream);
+  collect_timings();
+  for (auto e : c.pool) hipEventDestroy(e);
+  c.pool.clear();
+  free_all();
+  hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+  c.inited = false;
+}
+
+void gemhip_set_error_mode(int mode) { ctx().error_mode = mode ? 1 : 0; }
+const char* gemhip_last_error(void) { return ctx().last_error.c_str(); }
+void gemhip_clear_error(void) { ctx().last_error.clear(); }
+
+int gemhip_set_grid_size(int lx1, int lx2, int lx3) {
+  Ctx& c = ctx();
+  if (!c.inited) { fail("library not initialised (gemhip_init)"); return 1; }
+  // lx1 <= 906 keeps one x1 line's sweep working set (9 arrays of lx1+4 doubles) in the
+  // 64 KiB of LDS a workgroup may address
+  if (lx1 < 4 || lx1 > 906 || lx2 < 1 || lx3 < 1) {
+    fail("grid size %d x %d x %d not supported (4 <= lx1 <= 906, lx2, lx3 >= 1)", lx1, lx2,
+         lx3);
+    return 2;
+  }
+  hipStreamSynchronize(c.stream);
+  free_all();
+  c.lx1 = lx1; c.lx2 = lx2; c.lx3 = lx3;
+  DevGrid& g = c.g;
+  g = DevGrid{};
+  g.lx1 = lx1; g.lx2 = lx2; g.lx3 = lx3;
+  g.L1 = lx1 + 4; g.L2 = lx2 + 4; g.L3 = lx3 + 4;
+  g.N = (long)g.L1 * g.L2 * g.L3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  bool ok = dev_alloc(&c.d_fluid, 35L * g.N, "fluidvars") &&
+            dev_alloc(&c.d_aux, 14L * g.N, "fluidauxvars") &&
+            dev_alloc(&c.d_J1, (size_t)g.N, "J1") &&
+            dev_alloc(&c.d_vsi, 7L * (nv1i() + nv2i() + nv3i()), "interface velocities") &&
+            dev_alloc(&c.d_v1b, 14L * lx2 * lx3, "x1 boundary velocities") &&
+            dev_alloc(&c.d_neut, 4L * ni, "neutrals");
+  if (!ok) { free_all(); return 3; }
+  DevState& s = c.s;
+  s = DevState{};
+  s.ns = dev_var(NS); s.vs1 = dev_var(VS1); s.vs2 = dev_var(VS2); s.vs3 = dev_var(VS3);
+  s.Ts = dev_var(TS); s.rhovs1 = dev_var(RHOVS1); s.rhoes = dev_var(RHOES); s.J1 = c.d_J1;
+  s.vs1i = c.d_vsi; s.vs2i = s.vs1i + 7L * nv1i(); s.vs3i = s.vs2i + 7L * nv2i();
+  s.v1b = c.d_v1b;
+  s.nn = c.d_neut; s.Tn = c.d_neut + 3 * ni;
+  c.sized = true;
+  c.vsi_current = true;
+  c.pushed_fluid = nullptr;
+  return 0;
+}
+
+int gemhip_register_grid(int xtype, int gridflag, int flagper, const int* isglobal6,
+                         const double* h1, const double* h2, const double* h3,
+                         const double* h1x1i, const double* h2x1i, const double* h3x1i,
+                         const double* h1x2i, const double* h2x2i, const double* h3x2i,
+                         const double* h1x3i, const double* h2x3i, const double* h3x3i,
+                         const double* dx1, const double* dx1i, const double* dx2,
+                         const double* dx2i, const double* dx3, const double* dx3i) {
+  Ctx& c = ctx();
+  if (!c.sized) { fail("grid size not set (gemhip_set_grid_size)"); return 1; }
+  if (xtype != 1 && xtype != 2) {
+    fail("Unable to identify object type during conversion from C to Fortran class pointer:  %d",
+         xtype);
+    return 2;
+  }
+  if (gridflag < 0 || gridflag > 2) { fail("gridflag %d not in 0..2", gridflag); return 3; }
+  const double* src[] = {h1, h2, h3, h1x1i, h2x1i, h3x1i, h1x2i, h2x2i, h3x2i,
+                         h1x3i, h2x3i, h3x3i, dx1, dx1i, dx2, dx2i, dx3, dx3i};
+  for (const double* p : src)
+    if (!p) { fail("null geometry pointer"); return 4; }
+  if (!isglobal6) { fail("null isglobal pointer"); return 4; }
+  DevGrid& g = c.g;
+  const long n[] = {g.N, g.N, g.N, nv1i(), nv1i(), nv1i(), nv2i(), nv2i(), nv2i(),
+                    nv3i(), nv3i(), nv3i(), c.lx1 + 3, c.lx1, c.lx2 + 3, c.lx2, c.lx3 + 3, c.lx3};
+  long total = 0;
+  for (long k : n) total += k;
+  if (c.d_geom) hipFree(c.d_geom);
+  c.d_geom = nullptr;
+  if (!hipok(hipMalloc((void**)&c.d_geom, total * sizeof(double)), "geometry")) return 5;
+  const double** dst[] = {&g.h1, &g.h2, &g.h3, &g.h1x1i, &g.h2x1i, &g.h3x1i, &g.h1x2i, &g.h2x2i,
+                          &g.h3x2i, &g.h1x3i, &g.h2x3i, &g.h3x3i, &g.dx1, &g.dx1i, &g.dx2, &g.dx2i,
+                          &g.dx3, &g.dx3i};
+  long off = 0;
+  for (int k = 0; k < 18; ++k) {
+    *dst[k] = c.d_geom + off;
+    if (!hipok(hipMemcpy(c.d_geom + off, src[k], n[k] * sizeof(double), hipMemcpyDefault),
+               "geometry upload"))
+      return 6;
+    off += n[k];
+  }
+  g.gridflag = gridflag;
+  g.flagper = flagper ? 1 : 0;
+  for (int k = 0; k < 6; ++k) g.isglobal[k] = isglobal6[k] ? 1 : 0;
+  c.xtype = xtype;
+  c.grid_ok = true;
+  return 0;
+}
+
+int gemhip_register_neutrals(const double* nn, const double* Tn) {
+  Ctx& c = ctx();
+  if (!c.sized) { fail("grid size not set (gemhip_set_grid_size)"); return 1; }
+  if (!nn || !Tn) { fail("null neutral pointer"); return 2; }
+  const long ni = (long)c.lx1 * c.lx2 * c.lx3;
+  if (!hipok(hipMemcpy(c.d_neut, nn, 3 * ni * sizeof(double), hipMemcpyDefault), "nn upload") ||
+      !hipok(hipMemcpy(c.d_neut + 3 * ni, Tn, ni * sizeof(double), hipMemcpyDefault),
+             "Tn upload"))
+    return 3;
+  c.neutrals_ok = true;
+  return 0;
+}
+
+int gemhip_register_config(int diffsolvetype, double Teinf) {
+  Ctx& c = ctx();
+  c.diffsolvetype = diffsolvetype;
+  c.Teinf = Teinf;
+  return 0;
+}
+
+int gemhip_set_residency(int mode) {
+  Ctx& c = ctx();
+  if (mode != 0 && mode != 1) { fail("residency mode %d not 0 or 1", mode); return 1; }
+  if (c.inited) hipStreamSynchronize(c.stream);
+  c.residency = mode;
+  return 0;
+}
+
+int gemhip_state_push(const double* fluidvars, const double* fluidauxvars,
+                      const double* electrovars) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  if (!fluidvars) { fail("fluidvars is a null pointer"); return 2; }
+  const size_t N = (size_t)c.g.N;
+  bool ok = hipok(hipMemcpyAsync(c.d_fluid, fluidvars, 35 * N * sizeof(double),
+                                 hipMemcpyDefault, c.stream), "push fluidvars");
+  if (ok && fluidauxvars)
+    ok = hipok(hipMemcpyAsync(c.d_aux, fluidauxvars, 14 * N * sizeof(double),
+                              hipMemcpyDefault, c.stream), "push fluidauxvars");
+  if (ok && electrovars)
+    ok = hipok(hipMemcpyAsync(c.d_J1, electrovars + 3 * N, N * sizeof(double),
+                              hipMemcpyDefault, c.stream), "push J1");
+  if (ok) ok = hipok(hipStreamSynchronize(c.stream), "push");
+  c.pushed_fluid = fluidvars;
+  return ok ? 0 : 3;
+}
+
+int gemhip_state_pull(double* fluidvars, double* fluidauxvars) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  const size_t N = (size_t)c.g.N;
+  bool ok = true;
+  if (fluidvars)
+    ok = hipok(hipMemcpyAsync(fluidvars, c.d_fluid, 35 * N * sizeof(double),
+                              hipMemcpyDefault, c.stream), "pull fluidvars");
+  if (ok && fluidauxvars)
+    ok = hipok(hipMemcpyAsync(fluidauxvars, c.d_aux, 14 * N * sizeof(double),
+                              hipMemcpyDefault, c.stream), "pull fluidauxvars");
+  if (ok) ok = hipok(hipStreamSynchronize(c.stream), "pull");
+  return ok ? 0 : 2;
+}
+
+int gemhip_get_interface_vels(double* vs1i, double* vs2i, double* vs3i) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  if (!c.vsi_current) {
+    fail("interface velocities were formed inside gemhip_transport_step and not materialised");
+    return 2;
+  }
+  bool ok = true;
+  if (vs1i) ok = ok && hipok(hipMemcpy(vs1i, c.s.vs1i, 7 * nv1i() * sizeof(double),
+                                       hipMemcpyDefault), "vs1i");
+  if (vs2i) ok = ok && hipok(hipMemcpy(vs2i, c.s.vs2i, 7 * nv2i() * sizeof(double),
+                                       hipMemcpyDefault), "vs2i");
+  if (vs3i) ok = ok && hipok(hipMemcpy(vs3i, c.s.vs3i, 7 * nv3i() * sizeof(double),
+                                       hipMemcpyDefault), "vs3i");
+  return ok ? 0 : 3;
+}
+
+int gemhip_set_interface_vels(const double* vs1i, const double* vs2i, const double* vs3i) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  if (!vs1i || !vs2i || !vs3i) { fail("null interface velocity pointer"); return 2; }
+  hipStreamSynchronize(c.stream);
+  bool ok = hipok(hipMemcpy(c.s.vs1i, vs1i, 7 * nv1i() * sizeof(double), hipMemcpyDefault),
+                  "vs1i") &&
+            hipok(hipMemcpy(c.s.vs2i, vs2i, 7 * nv2i() * sizeof(double), hipMemcpyDefault),
+                  "vs2i") &&
+            hipok(hipMemcpy(c.s.vs3i, vs3i, 7 * nv3i() * sizeof(double), hipMemcpyDefault),
+                  "vs3i");
+  if (!ok) return 3;
+  // x1 boundary planes v1i(1,:,:,:), v1i(lx1+1,:,:,:) for the ghost fill
+  const long lines = (long)c.lx2 * c.lx3;
+  std::vector<double> b(14 * lines);
+  for (int isp = 0; isp < 7; ++isp)
+    for (long l = 0; l < lines; ++l) {
+      const double* v = vs1i + isp * nv1i() + l * (c.lx1 + 1);
+      b[2 * (l + lines * isp)] = v[0];
+      b[2 * (l + lines * isp) + 1] = v[c.lx1];
+    }
+  if (!hipok(hipMemcpy(c.s.v1b, b.data(), b.size() * sizeof(double), hipMemcpyDefault),
+             "v1b"))
+    return 4;
+  c.vsi_current = true;
+  return 0;
+}
+
+int gemhip_transport_step(double dt) {
+  if (!ready(true)) return 1;
+  Ctx& c = ctx();
+  if (c.residency != 1) { fail("gemhip_transport_step needs residency mode 1"); return 2; }
+  const DevGrid& g = c.g;
+  const DevState& s = c.s;
+  hipStream_t st = c.stream;
+  launch_v12rhov1_T2rhoe(g, s, true, true, st);          // libgemini.cpp:155-156
+  launch_set_global_boundaries(g, s, LSP, st);           // :168
+  halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :169
+  launch_save_v1_boundaries(g, s, st);                   // :170 (x1 planes kept for next step)
+  launch_sweep(g, s, 3, F_ALL, dt, true, st);            // :174-176
+  launch_sweep(g, s, 1, F_ALL, dt, true, st);            // :178-180
+  halo_fields({NS, RHOVS1, RHOES});                      // :181
+  launch_sweep(g, s, 2, F_ALL, dt, true, st);            // :183-185
+  launch_rhov12v1(g, s, st);                             // :186
+  launch_rhoe2T(g, s, st);                               // multifluid.f90:179-189
+  if (!hipok(hipGetLastError(), "advection launch")) return 3;
+  c.vsi_current = false;
+  if (!run_energy_diffusion(dt)) return 4;               // :208
+  return 0;
+}
+
+int gemhip_device_state(double** fluidvars, double** rhovs1_rhoes, double** J1p) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  if (fluidvars) *fluidvars = c.d_fluid;
+  if (rhovs1_rhoes) *rhovs1_rhoes = c.d_aux;
+  if (J1p) *J1p = c.d_J1;
+  return 0;
+}
+
+void* gemhip_stream(void) { return (void*)ctx().stream; }
+
+int gemhip_synchronize(void) {
+  Ctx& c = ctx();
+  if (!c.inited) return 1;
+  return hipok(hipStreamSynchronize(c.stream), "synchronise") ? 0 : 2;
+}
+
+int gemhip_kernel_timing(int enable) {
+  Ctx& c = ctx();
+  if (!enable && c.inited) { hipStreamSynchronize(c.stream); collect_timings(); }
+  c.timing = enable != 0;
+  return 0;
+}
+
+int gemhip_kernel_time(const char* name, double* total_ms, long* launches) {
+  Ctx& c = ctx();
+  if (c.inited) collect_timings();
+  auto it = c.acc.find(name ? name : "");
+  if (it == c.acc.end()) {
+    if (total_ms) *total_ms = 0.0;
+    if (launches) *launches = 0;
+    return 1;
+  }
+  if (total_ms) *total_ms = it->second.first;
+  if (launches) *launches = it->second.second;
+  return 0;
+}
+
+int gemhip_kernel_timing_reset(void) {
+  Ctx& c = ctx();
+  if (c.inited) { hipStreamSynchronize(c.stream); collect_timings(); }
+  c.acc.clear();
+  return 0;
+}
+
+// ------------------------------------------------------------ reference-named entries
+
+void v12rhov1_C(double** fluidvars, double** fluidauxvars) {
+  if (!ready() || !check_host(fluidvars, "fluidvars") || !check_host(fluidauxvars, "fluidauxvars"))
+    return;
+  Ctx& c = ctx();
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {NS, VS1})) return;
+  launch_v12rhov1_T2rhoe(c.g, c.s, true, false, c.stream);
+  download(io, {RHOVS1});
+}
+
+void T2rhoe_C(double** fluidvars, double** fluidauxvars) {
+  if (!ready() || !check_host(fluidvars, "fluidvars") || !check_host(fluidauxvars, "fluidauxvars"))
+    return;
+  Ctx& c = ctx();
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {NS, TS})) return;
+  launch_v12rhov1_T2rhoe(c.g, c.s, false, true, c.stream);
+  download(io, {RHOES});
+}
+
+void rhoe2T_C(double** fluidvars, double** fluidauxvars) {
+  if (!ready() || !check_host(fluidvars, "fluidvars") || !check_host(fluidauxvars, "fluidauxvars"))
+    return;
+  Ctx& c = ctx();
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {NS, RHOES})) return;
+  launch_rhoe2T(c.g, c.s, c.stream);
+  download(io, {TS});
+}
+
+void rhov12v1_C(double** fluidvars, double** fluidauxvars) {
+  if (!ready() || !check_host(fluidvars, "fluidvars") || !check_host(fluidauxvars, "fluidauxvars"))
+    return;
+  Ctx& c = ctx();
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {NS, RHOVS1})) return;
+  launch_rhov12v1(c.g, c.s, c.stream);
+  download(io, {NS, VS1});
+}
+
+void interface_vels_allspec_C(int* xtype, void** xC, double** fluidvars, void** intvars, int* lsp) {
+  (void)xC; (void)intvars;
+  if (!ready() || !check_xtype(xtype) || !check_host(fluidvars, "fluidvars")) return;
+  if (!lsp || *lsp > 7 || *lsp < 0) {
+    fail("number of interface vels must be less than or equal to total species number");
+    return;
+  }
+  Ctx& c = ctx();
+  Io io{fluidvars, nullptr, nullptr};
+  if (!upload(io, {VS1, VS2, VS3})) return;
+  if (*lsp > 0) launch_interface_vels(c.g, c.s, *lsp, c.stream);
+  c.vsi_current = true;
+  if (c.residency == 0) hipok(hipStreamSynchronize(c.stream), "synchronise");
+}
+
+void set_global_boundaries_allspec_C(int* xtype, void** xC, double** fluidvars,
+                                     double** fluidauxvars, void** intvars, int* lsp) {
+  (void)xC; (void)intvars;
+  if (!ready() || !check_xtype(xtype) || !check_host(fluidvars, "fluidvars") ||
+      !check_host(fluidauxvars, "fluidauxvars"))
+    return;
+  if (!lsp || *lsp > 7 || *lsp < 0) {
+    fail("number of global boundaries must be less than or equal to total species number");
+    return;
+  }
+  Ctx& c = ctx();
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {NS, VS1, VS2, VS3, RHOVS1, RHOES})) return;
+  if (*lsp > 0) launch_set_global_boundaries(c.g, c.s, *lsp, c.stream);
+  download(io, {NS, VS2, VS3, RHOVS1, RHOES});
+}
+
+#define SWEEP_ENTRY(NAME, DIR, FIELDS)                                                        \
+  void NAME(double** fv, double** av, void** intvars, int* xtype, void** xC, double* dt) {   \
+    (void)intvars; (void)xC;                                                                  \
+    sweep_entry(fv, av, xtype, dt, DIR, FIELDS);                                              \
+  }
+SWEEP_ENTRY(sweep3_allparams_C, 3, F_ALL)
+SWEEP_ENTRY(sweep3_allspec_mass_C, 3, F_MASS)
+SWEEP_ENTRY(sweep3_allspec_momentum_C, 3, F_MOMENTUM)
+SWEEP_ENTRY(sweep3_allspec_energy_C, 3, F_ENERGY)
+SWEEP_ENTRY(sweep1_allparams_C, 1, F_ALL)
+SWEEP_ENTRY(sweep1_allspec_mass_C, 1, F_MASS)
+SWEEP_ENTRY(sweep1_allspec_momentum_C, 1, F_MOMENTUM)
+SWEEP_ENTRY(sweep1_allspec_energy_C, 1, F_ENERGY)
+SWEEP_ENTRY(sweep2_allparams_C, 2, F_ALL)
+SWEEP_ENTRY(sweep2_allspec_mass_C, 2, F_MASS)
+SWEEP_ENTRY(sweep2_allspec_momentum_C, 2, F_MOMENTUM)
+SWEEP_ENTRY(sweep2_allspec_energy_C, 2, F_ENERGY)
+#undef SWEEP_ENTRY
+
+void energy_diffusion_C(void** cfgC, int* xtype, void** xC, double** fluidvars,
+                        double** electrovars, void** intvars, double* dt) {
+  (void)cfgC; (void)xC; (void)intvars;
+  if (!ready(true) || !check_xtype(xtype) || !check_host(fluidvars, "fluidvars") ||
+      !check_host(electrovars, "electrovars"))
+    return;
+  if (!dt) { fail("dt is a null pointer"); return; }
+  Ctx& c = ctx();
+  Io io{fluidvars, nullptr, electrovars};
+  if (!upload(io, {NS, TS, J1})) return;
+  if (!run_energy_diffusion(*dt)) return;
+  download(io, {TS});
+}
+
+void halo_allparams_C(int* xtype, void** xC, double** fluidvars, double** fluidauxvars) {
+  (void)xC;
+  if (!ready() || !check_xtype(xtype)) return;
+  Ctx& c = ctx();
+  if (!c.g.flagper) return;                    // one process, aperiodic: nothing to exchange
+  if (!check_host(fluidvars, "fluidvars") || !check_host(fluidauxvars, "fluidauxvars")) return;
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {NS, RHOVS1, RHOES})) return;
+  halo_fields({NS, RHOVS1, RHOES});
+  download(io, {NS, RHOVS1, RHOES});
+}
+
+void halo_fluidvars_C(int* xtype, void** xC, double** fluidvars, double** fluidauxvars) {
+  (void)xC;
+  if (!ready() || !check_xtype(xtype)) return;
+  Ctx& c = ctx();
+  if (!c.g.flagper) return;
+  if (!check_host(fluidvars, "fluidvars") || !check_host(fluidauxvars, "fluidauxvars")) return;
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {NS, RHOVS1, RHOES, VS2, VS3})) return;
+  halo_fields({NS, RHOVS1, RHOES, VS2, VS3});
+  download(io, {NS, RHOVS1, RHOES, VS2, VS3});
+}
+
+// advec_mpi.f90:14-44: vs2 and vs3 with one ghost layer, species 1..lsp
+void halo_interface_vels_allspec_C(int* xtype, void** xC, double** fluidvars, int* lsp) {
+  (void)xC;
+  if (!ready() || !check_xtype(xtype)) return;
+  if (!lsp || *lsp > 7 || *lsp < 0) {
+    fail("number of haloed species must be less than or equal to total species number");
+    return;
+  }
+  Ctx& c = ctx();
+  if (!c.g.flagper) return;
+  if (!check_host(fluidvars, "fluidvars")) return;
+  Io io{fluidvars, nullptr, nullptr};
+  if (!upload(io, {VS2, VS3})) return;
+  double* f[2] = {dev_var(VS2), dev_var(VS3)};
+  if (*lsp > 0) launch_halo_periodic(c.g, f, 2, 1, *lsp, c.stream);
+  download(io, {VS2, VS3});
+}
+
+int gemhip_eval_math(int op, const double* x, double* y, long n) {
+  Ctx& c = ctx();
+  if (!c.inited) { fail("library not initialised (gemhip_init)"); return 1; }
+  if (op < 0 || op > 3 || n < 0 || (n > 0 && (!x || !y))) { fail("bad gemhip_eval_math call"); return 2; }
+  if (n == 0) return 0;
+  double *dx = nullptr, *dy = nullptr;
+  bool ok = hipok(hipMalloc((void**)&dx, n * sizeof(double)), "eval alloc") &&
+            hipok(hipMalloc((void**)&dy, n * sizeof(double)), "eval alloc") &&
+            hipok(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyDefault), "eval upload");
+  if (ok) {
+    hipLaunchKernelGGL(k_eval_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, op,
+                       dx, dy, n);
+    ok = hipok(hipGetLastError(), "eval launch") &&
+         hipok(hipStreamSynchronize(c.stream), "eval sync") &&
+         hipok(hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDefault), "eval download");
+  }
+  if (dx) hipFree(dx);
+  if (dy) hipFree(dy);
+  return ok ? 0 : 3;
+}
+
+}  // extern "C"

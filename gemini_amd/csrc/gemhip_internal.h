@@ -1,0 +1,68 @@
+// gemhip_internal.h -- device-side views of the transport state and the kernel
+// launchers shared by the kernel files and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gemhip_math.h"
+
+namespace gemhip {
+
+// Geometry in HBM, same layouts as the reference's curvmesh arrays (Fortran order,
+// x1 fastest; ghosted arrays start at index -1).
+struct DevGrid {
+  int lx1, lx2, lx3;
+  int L1, L2, L3;          // lxN + 4
+  long N;                  // L1*L2*L3, one ghosted field
+  int gridflag, flagper;
+  int isglobal[6];         // x1min x1max x2min x2max x3min x3max
+  const double *h1, *h2, *h3;
+  const double *h1x1i, *h2x1i, *h3x1i;
+  const double *h1x2i, *h2x2i, *h3x2i;
+  const double *h1x3i, *h2x3i, *h3x3i;
+  const double *dx1, *dx1i, *dx2, *dx2i, *dx3, *dx3i;
+};
+
+// State pointers in HBM.  ns..Ts point into one fluidvars block of 35 fields,
+// rhovs1/rhoes into a 14-field auxiliary block; each species field is N long.
+struct DevState {
+  double *ns, *vs1, *vs2, *vs3, *Ts;
+  double *rhovs1, *rhoes;
+  const double *J1;
+  double *vs1i, *vs2i, *vs3i;   // materialised interface velocities (intvars%vs*i)
+  double *v1b;                  // v1i(1,:,:,:) and v1i(lx1+1,:,:,:): (2, lx2, lx3, 7)
+  const double *nn, *Tn;        // nn(lx1,lx2,lx3,1:3), Tn(lx1,lx2,lx3)
+};
+
+__host__ __device__ inline long gix(const DevGrid& g, int i1, int i2, int i3) {
+  return (long)(i1 + 1) + (long)g.L1 * ((long)(i2 + 1) + (long)g.L2 * (long)(i3 + 1));
+}
+
+// field-selection bits for the sweeps (multifluid.f90:49-126)
+enum : int { F_MASS = 1, F_MOMENTUM = 2, F_ENERGY = 4, F_ALL = 7 };
+
+// ---- launchers (kernels_advec.hip) ----
+void launch_v12rhov1_T2rhoe(const DevGrid& g, const DevState& s, bool mom, bool energy,
+                            hipStream_t st);
+void launch_rhov12v1(const DevGrid& g, const DevState& s, hipStream_t st);
+void launch_rhoe2T(const DevGrid& g, const DevState& s, hipStream_t st);
+void launch_set_global_boundaries(const DevGrid& g, const DevState& s, int lsp, hipStream_t st);
+void launch_halo_periodic(const DevGrid& g, double* const* fields, int nfields, int lhalo,
+                          int nspec, hipStream_t st);
+void launch_interface_vels(const DevGrid& g, const DevState& s, int lsp, hipStream_t st);
+void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t st);
+// dir 1,2,3; fields = F_* bits; onthefly: interface velocities from the cell velocities
+// (bit-identical to the materialised ones) instead of intvars%vs*i.
+void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, double dt,
+                  bool onthefly, hipStream_t st);
+
+// ---- launchers (kernels_diffusion.hip) ----
+size_t diffusion_scratch_bytes(const DevGrid& g);
+void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
+                             double Teinf, double* scratch, hipStream_t st);
+
+// kernel-timing hooks implemented by the API layer
+void timing_begin(const char* name, hipStream_t st);
+void timing_end(const char* name, hipStream_t st);
+
+}  // namespace gemhip

@@ -1,0 +1,530 @@
+// kernels_advec.hip -- advection substep of the transport path on gfx950.
+//
+// Reference: src/numerical/advection/advec.f90 (ghost fill :21-149, interface
+// velocities :155-229, sweeps :233-381, MC limiter :384-447),
+// src/multifluid/multifluid.f90:49-126 (species/field selection) and :179-233
+// (state conversions), src/mpimod/mpihalo.f90:147-152 (periodic x3 wrap).
+//
+// Every kernel keeps the reference's arithmetic order; the file is built with
+// -ffp-contract=off so results are bit-identical to the Fortran/C restatement.
+// Arrays are the reference's Fortran layouts (x1 contiguous), so:
+//  * x1 sweeps give each (x2,x3) line to one workgroup and stage it in LDS,
+//  * x2 and x3 sweeps map threads onto the contiguous x1 index and march each
+//    thread along its line, keeping the MC stencil in registers -- every load and
+//    store is a coalesced 64-lane access and no LDS or barrier is needed.
+#include "gemhip_internal.h"
+
+namespace gemhip {
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline int blocks_for(long n, int tpb = TPB) {
+  long b = (n + tpb - 1) / tpb;
+  return (int)(b > 65535L * 16 ? 65535L * 16 : b);
+}
+
+// ---------------------------------------------------------------- conversions
+// multifluid.f90:193-214 (T2rhoe, v12rhov1) over the whole padded grid
+__global__ void k_pre_convert(DevGrid g, DevState s, int mom, int energy) {
+  const int isp = blockIdx.y + 1;
+  const long o = (long)(isp - 1) * g.N;
+  const double msp = ms(isp), gm1 = gammas(isp) - 1.0;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < g.N;
+       k += (long)gridDim.x * blockDim.x) {
+    const double n = s.ns[o + k];
+    if (mom) s.rhovs1[o + k] = (n * msp) * s.vs1[o + k];
+    if (energy) s.rhoes[o + k] = ((n * kB) * s.Ts[o + k]) / gm1;
+  }
+}
+
+// multifluid.f90:218-233 (rhov12v1)
+__global__ void k_rhov12v1(DevGrid g, DevState s) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < g.N;
+       k += (long)gridDim.x * blockDim.x) {
+    double chrgflux = 0.0;
+#pragma unroll
+    for (int isp = 1; isp <= LSP - 1; ++isp) {
+      const long o = (long)(isp - 1) * g.N + k;
+      const double n = s.ns[o];
+      const double v = s.rhovs1[o] / (ms(isp) * fmax_f(n, mindensdiv));
+      s.vs1[o] = v;
+      chrgflux = chrgflux + (n * qs(isp)) * v;
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int isp = 1; isp <= LSP - 1; ++isp) sum = sum + s.ns[(long)(isp - 1) * g.N + k];
+    const long o7 = (long)(LSP - 1) * g.N + k;
+    s.ns[o7] = sum;
+    s.vs1[o7] = -(((1.0 / fmax_f(sum, mindensdiv)) / qs(LSP)) * chrgflux);
+  }
+}
+
+// multifluid.f90:179-189 (rhoe2T)
+__global__ void k_rhoe2T(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1;
+  const long o = (long)(isp - 1) * g.N;
+  const double c = (gammas(isp) - 1.0) / kB;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < g.N;
+       k += (long)gridDim.x * blockDim.x) {
+    const double t = (c * s.rhoes[o + k]) / fmax_f(s.ns[o + k], mindensdiv);
+    s.Ts[o + k] = fmax_f(t, 100.0);
+  }
+}
+
+// ---------------------------------------------------------------- global ghost fill
+// advec.f90:42-79: x1 ends, one thread per padded (x2,x3) column and species
+__global__ void k_bc_x1(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.L2 * g.L3) return;
+  const int i2 = (int)(t % g.L2) - 1, i3 = (int)(t / g.L2) - 1;
+  const bool inner = i2 >= 1 && i2 <= g.lx2 && i3 >= 1 && i3 <= g.lx3;
+  const long o = (long)(isp - 1) * g.N + gix(g, 0, i2, i3) - 1;  // index of x1 = -1
+  double* n = s.ns + o + 1;        // n[i1] for i1 = -1..lx1+2 -> n[-1..]
+  double* r = s.rhovs1 + o + 1;
+  double* e = s.rhoes + o + 1;
+  const double* u = s.vs1 + o + 1;
+  const double msp = ms(isp);
+  const int lx1 = g.lx1;
+  const long vb = 2L * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) + 2L * g.lx2 * g.lx3 * (isp - 1);
+  if (g.isglobal[0]) {
+    if (inner) {
+      const double coeff = n[2] / n[3];
+      n[0] = fmin_f(coeff * n[1], n[1]);
+      n[-1] = fmin_f(coeff * n[0], n[0]);
+      r[0] = 2.0 * s.v1b[vb] - u[1];
+    }
+    r[-1] = (r[0] + r[0]) - u[1];
+    r[-1] = (r[-1] * n[-1]) * msp;
+    r[0] = (r[0] * n[0]) * msp;
+    const double Tb = e[1] / n[1];
+    e[0] = Tb * n[0];
+    e[-1] = Tb * n[-1];
+  }
+  if (g.isglobal[1]) {
+    if (inner) {
+      const double coeff = n[lx1 - 1] / n[lx1 - 2];
+      n[lx1 + 1] = fmin_f(coeff * n[lx1], n[lx1]);
+      n[lx1 + 2] = fmin_f(coeff * n[lx1 + 1], n[lx1 + 1]);
+      r[lx1 + 1] = 2.0 * s.v1b[vb + 1] - u[lx1];
+    }
+    r[lx1 + 2] = (r[lx1 + 1] + r[lx1 + 1]) - u[lx1];
+    r[lx1 + 1] = (r[lx1 + 1] * n[lx1 + 1]) * msp;
+    r[lx1 + 2] = (r[lx1 + 2] * n[lx1 + 2]) * msp;
+    const double Tb = e[lx1] / n[lx1];
+    e[lx1 + 1] = Tb * n[lx1 + 1];
+    e[lx1 + 2] = Tb * n[lx1 + 2];
+  }
+}
+
+// advec.f90:83-104: x2 ends, every x1 and x3 position, fields vs2, ns, rhovs1, rhoes
+__global__ void k_bc_x2(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1, q = blockIdx.z;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.L1 * g.L3) return;
+  const int i1 = (int)(t % g.L1) - 1, i3 = (int)(t / g.L1) - 1;
+  double* base = q == 0 ? s.vs2 : q == 1 ? s.ns : q == 2 ? s.rhovs1 : s.rhoes;
+  double* f = base + (long)(isp - 1) * g.N;
+  if (g.isglobal[2]) {
+    const double v = f[gix(g, i1, 1, i3)];
+    f[gix(g, i1, 0, i3)] = v;
+    f[gix(g, i1, -1, i3)] = v;
+  }
+  if (g.isglobal[3]) {
+    const double v = f[gix(g, i1, g.lx2, i3)];
+    f[gix(g, i1, g.lx2 + 1, i3)] = v;
+    f[gix(g, i1, g.lx2 + 2, i3)] = v;
+  }
+}
+
+// advec.f90:107-132: x3 ends unless periodic, every x1 and x2 position
+__global__ void k_bc_x3(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1, q = blockIdx.z;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.L1 * g.L2) return;
+  const int i1 = (int)(t % g.L1) - 1, i2 = (int)(t / g.L1) - 1;
+  double* base = q == 0 ? s.vs3 : q == 1 ? s.ns : q == 2 ? s.rhovs1 : s.rhoes;
+  double* f = base + (long)(isp - 1) * g.N;
+  if (g.isglobal[4]) {
+    const double v = f[gix(g, i1, i2, 1)];
+    f[gix(g, i1, i2, 0)] = v;
+    f[gix(g, i1, i2, -1)] = v;
+  }
+  if (g.isglobal[5]) {
+    const double v = f[gix(g, i1, i2, g.lx3)];
+    f[gix(g, i1, i2, g.lx3 + 1)] = v;
+    f[gix(g, i1, i2, g.lx3 + 2)] = v;
+  }
+}
+
+// ---------------------------------------------------------------- periodic x3 wrap
+struct FieldList { double* p[8]; };
+
+// mpihalo.f90:147-152 with one process along x3: every x1 (ghosts included), interior
+// x2.  Sources are read before the destinations are written, as Fortran's array
+// assignment does, which matters only when lx3 < lhalo.
+__global__ void k_halo_periodic(DevGrid g, FieldList fl, int lhalo) {
+  const int isp = blockIdx.y + 1;
+  double* f = fl.p[blockIdx.z] + (long)(isp - 1) * g.N;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.L1 * g.lx2) return;
+  const int i1 = (int)(t % g.L1) - 1, i2 = (int)(t / g.L1) + 1;
+  double tmp[2];
+  for (int h = 0; h < lhalo; ++h) tmp[h] = f[gix(g, i1, i2, g.lx3 - (lhalo - 1) + h)];
+  for (int h = 0; h < lhalo; ++h) f[gix(g, i1, i2, 1 - lhalo + h)] = tmp[h];
+  for (int h = 0; h < lhalo; ++h) tmp[h] = f[gix(g, i1, i2, 1 + h)];
+  for (int h = 0; h < lhalo; ++h) f[gix(g, i1, i2, g.lx3 + 1 + h)] = tmp[h];
+}
+
+// ---------------------------------------------------------------- interface velocities
+// advec.f90:171-196: x1 interface velocity k (1..lx1+1) of line (i2,i3) from the cell
+// velocities u(-1..lx1+2) of that line.
+__device__ __forceinline__ double v1i_at(const DevGrid& g, const double* u, int k) {
+  const int lx1 = g.lx1;
+  if (k >= 2 && k <= lx1) return 0.5 * (u[k - 1] + u[k]);
+  if (k == 1) {
+    if (g.isglobal[0]) {
+      if (g.gridflag == 1) return fmin_f(0.5 * (u[1] + u[2]), 0.0);
+      return u[1];
+    }
+    return 0.5 * (u[0] + u[1]);
+  }
+  if (g.isglobal[1]) {
+    if (g.gridflag == 2) return fmax_f(0.5 * (u[lx1 - 1] + u[lx1]), 0.0);
+    return u[lx1];
+  }
+  return 0.5 * (u[lx1] + u[lx1 + 1]);
+}
+
+__global__ void k_vs1i(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1;
+  const long n = (long)(g.lx1 + 1) * g.lx2 * g.lx3;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int k = (int)(t % (g.lx1 + 1)) + 1;
+  const long line = t / (g.lx1 + 1);
+  const int i2 = (int)(line % g.lx2) + 1, i3 = (int)(line / g.lx2) + 1;
+  const double* u = s.vs1 + (long)(isp - 1) * g.N + gix(g, 0, i2, i3);  // u[i1] = vs1(i1)
+  const double v = v1i_at(g, u, k);
+  s.vs1i[(long)(isp - 1) * n + t] = v;
+  const long vb = 2L * (line + (long)g.lx2 * g.lx3 * (isp - 1));
+  if (k == 1) s.v1b[vb] = v;
+  if (k == g.lx1 + 1) s.v1b[vb + 1] = v;
+}
+
+__global__ void k_vs2i(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1;
+  const long n = (long)g.lx1 * (g.lx2 + 1) * g.lx3;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i1 = (int)(t % g.lx1) + 1;
+  const long r = t / g.lx1;
+  const int k = (int)(r % (g.lx2 + 1)) + 1, i3 = (int)(r / (g.lx2 + 1)) + 1;
+  const double* w = s.vs2 + (long)(isp - 1) * g.N;
+  s.vs2i[(long)(isp - 1) * n + t] = 0.5 * (w[gix(g, i1, k - 1, i3)] + w[gix(g, i1, k, i3)]);
+}
+
+__global__ void k_vs3i(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1;
+  const long n = (long)g.lx1 * g.lx2 * (g.lx3 + 1);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i1 = (int)(t % g.lx1) + 1;
+  const long r = t / g.lx1;
+  const int i2 = (int)(r % g.lx2) + 1, k = (int)(r / g.lx2) + 1;
+  const double* w = s.vs3 + (long)(isp - 1) * g.N;
+  s.vs3i[(long)(isp - 1) * n + t] = 0.5 * (w[gix(g, i1, i2, k - 1)] + w[gix(g, i1, i2, k)]);
+}
+
+// only the x1-boundary interface velocities, which the next step's ghost fill reads
+__global__ void k_save_v1b(DevGrid g, DevState s) {
+  const int isp = blockIdx.y + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.lx2 * g.lx3) return;
+  const int i2 = (int)(t % g.lx2) + 1, i3 = (int)(t / g.lx2) + 1;
+  const double* u = s.vs1 + (long)(isp - 1) * g.N + gix(g, 0, i2, i3);
+  const long vb = 2L * (t + (long)g.lx2 * g.lx3 * (isp - 1));
+  s.v1b[vb] = v1i_at(g, u, 1);
+  s.v1b[vb + 1] = v1i_at(g, u, g.lx1 + 1);
+}
+
+// ---------------------------------------------------------------- x1 sweep
+// advec.f90:283-309 + :384-435 for one (x2,x3) line per workgroup, all species.
+// LDS holds the line (f), the one-sided slopes, the limited slopes, the fluxes and the
+// per-line metric products; each phase is one pass of the reference's loops.
+__global__ __launch_bounds__(256) void k_sweep1(DevGrid g, DevState s, int fields, double dt,
+                                                int onthefly) {
+  extern __shared__ double sh[];
+  const int lx1 = g.lx1, L1 = g.L1;
+  const int i2 = blockIdx.x + 1, i3 = blockIdx.y + 1;
+  double* sf = sh;               // f(-1..lx1+2)        -> sf[k+1]
+  double* srs = sf + L1;         // rslope(-1..lx1+1)    -> srs[k+1]
+  double* ssl = srs + L1;        // slope(0..lx1+1)      -> ssl[k]
+  double* sph = ssl + L1;        // phi(1..lx1+1)        -> sph[k]
+  double* sv = sph + L1;         // v1i(1..lx1+1)        -> sv[k]
+  double* svq = sv + L1;         // v1i/h1i              -> svq[k]
+  double* sha1 = svq + L1;       // (h1*h2)*h3 (1..lx1)  -> sha1[k]
+  double* sha2 = sha1 + L1;      // h2x1i*h3x1i          -> sha2[k]
+  double* sh1i = sha2 + L1;      // h1x1i                -> sh1i[k]
+  const long c0 = gix(g, 0, i2, i3);            // cell (0,i2,i3)
+  const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;  // iface k -> if0+k
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = tid + 1; k <= lx1 + 1; k += nt) {
+    if (k <= lx1) sha1[k] = (g.h1[c0 + k] * g.h2[c0 + k]) * g.h3[c0 + k];
+    sha2[k] = g.h2x1i[if0 + k] * g.h3x1i[if0 + k];
+    sh1i[k] = g.h1x1i[if0 + k];
+  }
+  const long nif = (long)(lx1 + 1) * g.lx2 * g.lx3;
+  for (int isp = 1; isp <= LSP; ++isp) {
+    const long so = (long)(isp - 1) * g.N;
+    __syncthreads();
+    for (int k = tid + 1; k <= lx1 + 1; k += nt) {
+      double v;
+      if (onthefly) v = v1i_at(g, s.vs1 + so + c0, k);
+      else v = s.vs1i[(long)(isp - 1) * nif + if0 + k];
+      sv[k] = v;
+      svq[k] = v / sh1i[k];
+    }
+    for (int q = 0; q < 3; ++q) {
+      const int bit = 1 << q;
+      if (!(fields & bit)) continue;
+      if (q < 2 && isp == LSP) continue;            // mass, momentum: species 1..6
+      double* f = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;  // f[k] = f(k)
+      __syncthreads();
+      for (int k = tid - 1; k <= lx1 + 2; k += nt) sf[k + 1] = f[k];
+      __syncthreads();
+      for (int k = tid - 1; k <= lx1 + 1; k += nt)
+        srs[k + 1] = (sf[k + 2] - sf[k + 1]) / g.dx1[k + 1];
+      __syncthreads();
+      for (int k = tid; k <= lx1 + 1; k += nt) {
+        const double cs = (sf[k + 2] - sf[k]) / (g.dx1[k] + g.dx1[k + 1]);
+        ssl[k] = minmod(cs, minmod(2.0 * srs[k], 2.0 * srs[k + 1]));
+      }
+      __syncthreads();
+      for (int k = tid + 1; k <= lx1 + 1; k += nt) {
+        const double v = sv[k];
+        if (v < 0.0)
+          sph[k] = sf[k + 1] * v - ((0.5 * v) * (g.dx1[k] + svq[k] * dt)) * ssl[k];
+        else
+          sph[k] = sf[k] * v + ((0.5 * v) * (g.dx1[k] - svq[k] * dt)) * ssl[k - 1];
+      }
+      __syncthreads();
+      for (int k = tid + 1; k <= lx1; k += nt)
+        f[k] = sf[k + 1] - ((dt * (sha2[k + 1] * sph[k + 1] - sha2[k] * sph[k])) / g.dx1i[k - 1])
+                               / sha1[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- x2 / x3 sweeps
+// advec.f90:313-381 + :384-435.  Thread (i1, line, species): marches along its line
+// keeping f(k-1), f(k), f(k+1), the last one-sided slope, limited slope and flux in
+// registers for up to three fields (ns, rhovs1, rhoes) that share the velocity and
+// metric factors.  frank 1 (momentum) uses h1^2 in both metric products
+// (advec.f90:336-338, 372-374).
+template <int DIR>
+__global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fields, double dt,
+                                                 int onthefly) {
+  const int lx = DIR == 2 ? g.lx2 : g.lx3;
+  const int i1 = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int isp = threadIdx.y + 1;
+  if (i1 > g.lx1) return;
+  const int other = blockIdx.y + 1;                  // i3 for DIR 2, i2 for DIR 3
+  const long cs_ = DIR == 2 ? (long)g.L1 : (long)g.L1 * g.L2;   // cell stride along the line
+  const long c0 = DIR == 2 ? gix(g, i1, 0, other) : gix(g, i1, other, 0);  // cell k -> c0+k*cs_
+  const long istr = DIR == 2 ? (long)g.lx1 : (long)g.lx1 * g.lx2;          // iface stride
+  const long if0 = DIR == 2 ? (long)(i1 - 1) + (long)g.lx1 * ((long)(g.lx2 + 1) * (other - 1)) - istr
+                            : (long)(i1 - 1) + (long)g.lx1 * (other - 1) - istr;  // iface k -> if0+k*istr
+  const double* hA = DIR == 2 ? g.h1x2i : g.h1x3i;   // metric factor that frank squares
+  const double* hB = DIR == 2 ? g.h3x2i : g.h2x3i;
+  const double* hI = DIR == 2 ? g.h2x2i : g.h3x3i;   // h1i of advec1D_MC_curv
+  const double* dx = DIR == 2 ? g.dx2 : g.dx3;
+  const double* dxi = DIR == 2 ? g.dx2i : g.dx3i;
+  const long so = (long)(isp - 1) * g.N;
+  const double* w = (DIR == 2 ? s.vs2 : s.vs3) + so + c0;
+  const double* vi = (DIR == 2 ? s.vs2i : s.vs3i) +
+                     (long)(isp - 1) * (DIR == 2 ? (long)g.lx1 * (g.lx2 + 1) * g.lx3
+                                                 : (long)g.lx1 * g.lx2 * (g.lx3 + 1));
+  double* fp[3];
+  bool act[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    fp[q] = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;
+    act[q] = (fields & (1 << q)) && (q == 2 || isp < LSP);
+  }
+  double fa[3], fb[3], fc[3], rsp[3], slp[3], php[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    fa[q] = fb[q] = fc[q] = rsp[q] = slp[q] = php[q] = 0.0;
+    if (!act[q]) continue;
+    fa[q] = fp[q][-cs_];       // f(-1)
+    fb[q] = fp[q][0];          // f(0)
+    fc[q] = fp[q][cs_];        // f(1)
+    rsp[q] = (fb[q] - fa[q]) / dx[0];
+  }
+  double wprev = w[0];         // vs(0) for the on-the-fly interface velocity
+  double h2p0 = 0.0, h2p1 = 0.0;   // ha2i(k-1) for frank 0 / frank 1
+  for (int k = 0; k <= lx + 1; ++k) {
+    const double dxk = dx[k], dxk1 = dx[k + 1];
+    double v = 0.0, vq = 0.0, h2f0 = 0.0, h2f1 = 0.0, ha1f0 = 1.0, ha1f1 = 1.0, dxim = 1.0;
+    if (k >= 1) {
+      const long ik = if0 + (long)k * istr;
+      if (onthefly) {
+        const double wk = w[(long)k * cs_];
+        v = 0.5 * (wprev + wk);
+        wprev = wk;
+      } else {
+        v = vi[ik];
+      }
+      vq = v / hI[ik];
+      const double a = hA[ik], b = hB[ik];
+      h2f0 = a * b;
+      h2f1 = (a * a) * b;
+      if (k >= 2) {
+        const long cm = c0 + (long)(k - 1) * cs_;
+        const double h1 = g.h1[cm], h2 = g.h2[cm], h3 = g.h3[cm];
+        ha1f0 = (h1 * h2) * h3;
+        ha1f1 = ((h1 * h1) * h2) * h3;
+        dxim = dxi[k - 2];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (!act[q]) continue;
+      const double rs = (fc[q] - fb[q]) / dxk1;
+      const double csl = (fc[q] - fa[q]) / (dxk + dxk1);
+      const double sl = minmod(csl, minmod(2.0 * rsp[q], 2.0 * rs));
+      if (k >= 1) {
+        double ph;
+        if (v < 0.0) ph = fb[q] * v - ((0.5 * v) * (dxk + vq * dt)) * sl;
+        else ph = fa[q] * v + ((0.5 * v) * (dxk - vq * dt)) * slp[q];
+        const bool fr1 = (q == 1);
+        const double h2k = fr1 ? h2f1 : h2f0;
+        if (k >= 2) {
+          const double h2m = fr1 ? h2p1 : h2p0;
+          const double ha1 = fr1 ? ha1f1 : ha1f0;
+          fp[q][(long)(k - 1) * cs_] = fa[q] - ((dt * (h2k * ph - h2m * php[q])) / dxim) / ha1;
+        }
+        php[q] = ph;
+      }
+      slp[q] = sl;
+      rsp[q] = rs;
+      fa[q] = fb[q];
+      fb[q] = fc[q];
+      if (k + 2 <= lx + 2) fc[q] = fp[q][(long)(k + 2) * cs_];
+    }
+    if (k >= 1) { h2p0 = h2f0; h2p1 = h2f1; }
+  }
+}
+
+// Ts ghost cells: floor at 100 K (the reference leaves them undefined, see the
+// diffusion kernel); interior columns only have x1 ghosts.
+__global__ void k_floor_ghosts(DevGrid g, double* Ts) {
+  const int isp = blockIdx.y + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.L2 * g.L3) return;
+  const int i2 = (int)(t % g.L2) - 1, i3 = (int)(t / g.L2) - 1;
+  double* T = Ts + (long)(isp - 1) * g.N + gix(g, -1, i2, i3);
+  const bool inner = i2 >= 1 && i2 <= g.lx2 && i3 >= 1 && i3 <= g.lx3;
+  for (int k = 0; k < g.L1; ++k) {
+    if (inner && k >= 2 && k < g.lx1 + 2) continue;
+    T[k] = fmax_f(T[k], 100.0);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+void launch_v12rhov1_T2rhoe(const DevGrid& g, const DevState& s, bool mom, bool energy,
+                            hipStream_t st) {
+  timing_begin("pre_convert", st);
+  hipLaunchKernelGGL(k_pre_convert, dim3(blocks_for(g.N), LSP), dim3(TPB), 0, st, g, s,
+                     (int)mom, (int)energy);
+  timing_end("pre_convert", st);
+}
+
+void launch_rhov12v1(const DevGrid& g, const DevState& s, hipStream_t st) {
+  timing_begin("rhov12v1", st);
+  hipLaunchKernelGGL(k_rhov12v1, dim3(blocks_for(g.N)), dim3(TPB), 0, st, g, s);
+  timing_end("rhov12v1", st);
+}
+
+void launch_rhoe2T(const DevGrid& g, const DevState& s, hipStream_t st) {
+  timing_begin("rhoe2T", st);
+  hipLaunchKernelGGL(k_rhoe2T, dim3(blocks_for(g.N), LSP), dim3(TPB), 0, st, g, s);
+  timing_end("rhoe2T", st);
+}
+
+void launch_set_global_boundaries(const DevGrid& g, const DevState& s, int lsp,
+                                  hipStream_t st) {
+  timing_begin("ghost_fill", st);
+  hipLaunchKernelGGL(k_bc_x1, dim3(blocks_for((long)g.L2 * g.L3), lsp), dim3(TPB), 0, st, g, s);
+  hipLaunchKernelGGL(k_bc_x2, dim3(blocks_for((long)g.L1 * g.L3), lsp, 4), dim3(TPB), 0, st, g, s);
+  if (!g.flagper)
+    hipLaunchKernelGGL(k_bc_x3, dim3(blocks_for((long)g.L1 * g.L2), lsp, 4), dim3(TPB), 0, st, g,
+                       s);
+  timing_end("ghost_fill", st);
+}
+
+void launch_halo_periodic(const DevGrid& g, double* const* fields, int nfields, int lhalo,
+                          int nspec, hipStream_t st) {
+  if (!g.flagper || nfields <= 0) return;
+  FieldList fl;
+  for (int i = 0; i < 8; ++i) fl.p[i] = i < nfields ? fields[i] : nullptr;
+  timing_begin("halo_periodic", st);
+  hipLaunchKernelGGL(k_halo_periodic, dim3(blocks_for((long)g.L1 * g.lx2), nspec, nfields),
+                     dim3(TPB), 0, st, g, fl, lhalo);
+  timing_end("halo_periodic", st);
+}
+
+void launch_interface_vels(const DevGrid& g, const DevState& s, int lsp, hipStream_t st) {
+  timing_begin("interface_vels", st);
+  hipLaunchKernelGGL(k_vs1i, dim3(blocks_for((long)(g.lx1 + 1) * g.lx2 * g.lx3), lsp), dim3(TPB),
+                     0, st, g, s);
+  hipLaunchKernelGGL(k_vs2i, dim3(blocks_for((long)g.lx1 * (g.lx2 + 1) * g.lx3), lsp), dim3(TPB),
+                     0, st, g, s);
+  hipLaunchKernelGGL(k_vs3i, dim3(blocks_for((long)g.lx1 * g.lx2 * (g.lx3 + 1)), lsp), dim3(TPB),
+                     0, st, g, s);
+  timing_end("interface_vels", st);
+}
+
+void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t st) {
+  timing_begin("save_v1b", st);
+  hipLaunchKernelGGL(k_save_v1b, dim3(blocks_for((long)g.lx2 * g.lx3), LSP), dim3(TPB), 0, st, g,
+                     s);
+  timing_end("save_v1b", st);
+}
+
+void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, double dt,
+                  bool onthefly, hipStream_t st) {
+  if (dir == 1) {
+    if (g.lx1 <= 1) return;
+    const size_t lds = sizeof(double) * 9 * (size_t)g.L1;
+    if (lds > 65536) return;                 // rejected by gemhip_set_grid_size
+    timing_begin("sweep1", st);
+    hipLaunchKernelGGL(k_sweep1, dim3(g.lx2, g.lx3), dim3(256), lds, st, g, s, fields, dt,
+                       (int)onthefly);
+    timing_end("sweep1", st);
+  } else if (dir == 2) {
+    if (g.lx2 <= 1) return;
+    timing_begin("sweep2", st);
+    hipLaunchKernelGGL(k_sweep23<2>, dim3((g.lx1 + 63) / 64, g.lx3), dim3(64, LSP), 0, st, g, s,
+                       fields, dt, (int)onthefly);
+    timing_end("sweep2", st);
+  } else {
+    if (g.lx3 <= 1) return;
+    timing_begin("sweep3", st);
+    hipLaunchKernelGGL(k_sweep23<3>, dim3((g.lx1 + 63) / 64, g.lx2), dim3(64, LSP), 0, st, g, s,
+                       fields, dt, (int)onthefly);
+    timing_end("sweep3", st);
+  }
+}
+
+void launch_floor_ghosts(const DevGrid& g, double* Ts, hipStream_t st) {
+  hipLaunchKernelGGL(k_floor_ghosts, dim3(blocks_for((long)g.L2 * g.L3), LSP), dim3(TPB), 0, st, g,
+                     Ts);
+}
+
+}  // namespace gemhip

@@ -1,0 +1,238 @@
+"""State blocks of the transport path and the seeded synthetic-state generator.
+
+Memory layout follows the reference exactly (src/libgemini.f90:401-417, 462-519;
+C allocation src/libgemini.cpp:62-64), Fortran order, two ghost cells per side:
+
+* ``fluidvars(-1:lx1+2,-1:lx2+2,-1:lx3+2,5*lsp)``: ns 1-7, vs1 8-14, vs2 15-21,
+  vs3 22-28, Ts 29-35,
+* ``fluidauxvars(...,2*lsp+9)``: rhovs1 1-7, rhoes 8-14, then nine MHD fields
+  this path never touches,
+* ``electrovars(...,7)``: E1 E2 E3 J1 J2 J3 Phi (only J1 is read here),
+* interface velocities ``vs1i(1:lx1+1,1:lx2,1:lx3,lsp)``, ``vs2i``, ``vs3i``
+  (libgemini.f90:284-291), zero before the first step,
+* neutral inputs ``nn(1:lx1,1:lx2,1:lx3,1:3)`` (O, N2, O2 densities, the three
+  the electron conductivity reads) and ``Tn(1:lx1,1:lx2,1:lx3)``.
+
+The generator produces the profiles the benchmark configurations describe:
+Chapman-layer ion densities peaking near 300 km with 5 % lognormal noise,
+smooth 300-3000 K temperatures, smooth random ion drifts (|vs1| <= 200 m/s,
+species-independent E x B drifts |vs2|,|vs3| <= 500 m/s, electron vs1 from zero
+net parallel current), exponential neutral densities, a 200-1000 K neutral
+temperature and a ~1e-6 A/m^2 parallel current (tapered at low density).  Everything is drawn from
+``numpy.random.default_rng(seed)``, so a (grid, seed) pair always yields the
+same bytes.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .grid import Grid
+
+LSP = 7
+MINDENS = 1.0e-100  # phys_consts.in.f90:82
+
+
+@dataclass
+class State:
+    fluidvars: np.ndarray
+    fluidauxvars: np.ndarray
+    electrovars: np.ndarray
+    vs1i: np.ndarray
+    vs2i: np.ndarray
+    vs3i: np.ndarray
+    nn: np.ndarray
+    Tn: np.ndarray
+
+    @property
+    def ns(self):
+        return self.fluidvars[..., 0:7]
+
+    @property
+    def vs1(self):
+        return self.fluidvars[..., 7:14]
+
+    @property
+    def vs2(self):
+        return self.fluidvars[..., 14:21]
+
+    @property
+    def vs3(self):
+        return self.fluidvars[..., 21:28]
+
+    @property
+    def Ts(self):
+        return self.fluidvars[..., 28:35]
+
+    @property
+    def rhovs1(self):
+        return self.fluidauxvars[..., 0:7]
+
+    @property
+    def rhoes(self):
+        return self.fluidauxvars[..., 7:14]
+
+    @property
+    def J1(self):
+        return self.electrovars[..., 3]
+
+    def copy(self):
+        return State(*(np.array(a, order="F", copy=True) for a in
+                       (self.fluidvars, self.fluidauxvars, self.electrovars, self.vs1i,
+                        self.vs2i, self.vs3i, self.nn, self.Tn)))
+
+
+def empty_state(g: Grid) -> State:
+    L = g.padded
+    z = lambda *s: np.zeros(s, dtype=np.float64, order="F")
+    return State(z(*L, 5 * LSP), z(*L, 2 * LSP + 9), z(*L, 7),
+                 z(g.lx1 + 1, g.lx2, g.lx3, LSP), z(g.lx1, g.lx2 + 1, g.lx3, LSP),
+                 z(g.lx1, g.lx2, g.lx3 + 1, LSP), z(g.lx1, g.lx2, g.lx3, 3),
+                 z(g.lx1, g.lx2, g.lx3))
+
+
+def _chapman(z, zpk, H):
+    zeta = (z - zpk) / H
+    return np.exp(0.5 * (1.0 - zeta - np.exp(-zeta)))
+
+
+def _sig(z, z0, w):
+    return 0.5 * (1.0 + np.tanh((z - z0) / w))
+
+
+def synthetic_state(g: Grid, seed: int = 0, perturb: bool = False) -> State:
+    """Seeded synthetic plasma state on grid ``g`` (see module docstring).
+
+    ``perturb`` adds 3-D Gaussian density perturbations (amplitude 20 %, widths
+    20-50 km), as the 3-D Cartesian configuration asks.
+
+    Work is done on the transposed views: a Fortran (L1, L2, L3) field is a C-order
+    (L3, L2, L1) array, so every 3-D array below is built and written contiguously.
+    """
+    rng = np.random.default_rng(seed)
+    st = empty_state(g)
+    L1, L2, L3 = g.padded
+    sh = (L3, L2, L1)
+    FV = st.fluidvars.T                 # (35, L3, L2, L1)
+    # profiles depend on (x1, x2) only: build them on the (L2, L1) plane
+    z = g.alt_plane()                                           # (L2, L1)
+    nmax = 10.0 ** rng.uniform(11.0, 12.0, size=(L2, 1))       # per-line F-region peak
+    nF = nmax * _chapman(z, 300e3, 60e3)
+    nE = 0.05 * nmax * _chapman(z, 110e3, 12e3)
+    fO = _sig(z, 180e3, 25e3)
+    fH = _sig(z, 1000e3, 150e3)
+    plasmasphere = 3e8 * np.exp(-np.maximum(z - 600e3, 0.0) / 6000e3) * _sig(z, 500e3, 100e3)
+    mol = nF * (1.0 - fO) + nE
+    prof = [
+        nF * fO * (1.0 - fH),                      # O+
+        0.55 * mol,                                # NO+
+        0.03 * mol,                                # N2+
+        0.40 * mol,                                # O2+
+        0.02 * nF * fO,                            # N+
+        nF * fO * fH + plasmasphere,               # H+
+    ]
+    for isp in range(6):
+        noise = rng.standard_normal(sh)
+        noise *= 0.05
+        np.exp(noise, out=noise)
+        noise *= prof[isp][None, :, :]
+        np.maximum(noise, MINDENS, out=FV[isp])
+    if perturb:
+        _gaussian_perturbations(g, rng, FV)
+    np.sum(FV[0:6], axis=0, out=FV[6])
+
+    Tn2 = 1000.0 - 800.0 * np.exp(-np.maximum(z - 80e3, 0.0) / 40e3)   # (L2, L1)
+    Ti = Tn2 + (1500.0 - Tn2) * _sig(z, 300e3, 100e3)
+    Te = Tn2 + (3000.0 - Tn2) * _sig(z, 200e3, 80e3)
+    for isp in range(6):
+        FV[28 + isp][...] = Ti[None, :, :]
+    FV[34][...] = Te[None, :, :]
+    # ion parallel drifts: smooth random fields within +-200 m/s
+    for isp in range(LSP - 1):
+        _smooth_field(rng, sh, out=FV[7 + isp], scale=200.0)
+    # electron parallel drift from zero net parallel current, as rhov12v1 sets it
+    # (multifluid.f90:229): vs1(7) = sum_i ns_i vs1_i / ns_7
+    flux = np.zeros(sh)
+    for isp in range(6):
+        flux += FV[isp] * FV[7 + isp]
+    np.divide(flux, np.maximum(FV[6], 1e-5), out=FV[13])
+    # perpendicular (E x B) drifts are species independent: +-500 m/s
+    _smooth_field(rng, sh, out=FV[14], scale=500.0)
+    _smooth_field(rng, sh, out=FV[21], scale=500.0)
+    for isp in range(1, LSP):
+        FV[14 + isp][...] = FV[14]
+        FV[21 + isp][...] = FV[21]
+    # parallel current ~1e-6 A/m^2, tapered where n_e < 1e10 m^-3 so the implied
+    # electron drift J1/(n_e e) stays below ~600 m/s (a potential solve gives such currents)
+    J = st.electrovars.T[3]
+    _smooth_field(rng, sh, out=J, scale=1e-6)
+    J *= np.minimum(FV[6] / 1e10, 1.0)
+
+    zi = z[2:g.lx2 + 2, 2:g.lx1 + 2]                            # (lx2, lx1)
+    Hs = lambda H0, z0: H0 + 0.06 * np.maximum(zi - z0, 0.0)
+    nO = 3e17 * np.exp(-(zi - 100e3) / Hs(8e3, 100e3))
+    nN2 = 1e19 * np.exp(-(zi - 80e3) / Hs(7e3, 80e3))
+    nO2 = 2.5e18 * np.exp(-(zi - 80e3) / Hs(6.5e3, 80e3))
+    NN = st.nn.T                                                 # (3, lx3, lx2, lx1)
+    for k, a in enumerate((nO, nN2, nO2)):
+        NN[k][...] = a[None, :, :]
+    st.Tn.T[...] = Tn2[None, 2:g.lx2 + 2, 2:g.lx1 + 2]
+    return st
+
+
+def _smooth_field(rng, shape, out, scale=1.0, nmodes: int = 3, noise: float = 0.05):
+    """scale * (random field in [-1, 1]): a few separable low-wavenumber modes plus 5 %
+    white noise, written into ``out`` (C-order array of ``shape``)."""
+    u = [np.linspace(0.0, 1.0, n) for n in shape]
+    out[...] = 0.0
+    for _ in range(nmodes):
+        parts = [np.sin(2 * np.pi * rng.uniform(0.3, 2.0) * ui + rng.uniform(0, 2 * np.pi))
+                 for ui in u]
+        plane = rng.uniform(0.5, 1.0) * parts[1][:, None] * parts[2][None, :]
+        out += parts[0][:, None, None] * plane[None, :, :]
+    out *= (1.0 - noise) * scale / max(np.max(np.abs(out)), 1e-30)
+    out += (noise * scale) * rng.uniform(-1.0, 1.0, size=shape)
+    return out
+
+
+def _gaussian_perturbations(g: Grid, rng, FV, nblobs: int = 6):
+    x1 = g.x1[None, None, :]
+    x2 = g.x2[None, :, None]
+    x3 = g.x3[:, None, None]
+    pert = np.zeros(g.padded[::-1])
+    for _ in range(nblobs):
+        c1 = rng.uniform(g.x1[2], g.x1[g.lx1 + 1])
+        c2 = rng.uniform(g.x2[2], g.x2[g.lx2 + 1])
+        c3 = rng.uniform(g.x3[2], g.x3[g.lx3 + 1])
+        w = rng.uniform(20e3, 50e3)
+        pert += np.exp(-((x1 - c1) ** 2 + (x2 - c2) ** 2 + (x3 - c3) ** 2) / (2 * w * w))
+    fac = 1.0 + 0.2 * np.minimum(pert, 1.0)
+    for isp in range(6):
+        FV[isp] *= fac
+
+
+def cfl_dt(g: Grid, st: State, tcfl: float = 0.9, dtmax: float = 0.5) -> float:
+    """Time step from the reference's CFL rule (temporal.f90:12-49), capped at dtmax.
+
+    dl_i = h_i * dx_i (interior), sound speed from ion and electron temperatures.
+    """
+    kB = 1.38064852e-23
+    amu = 1.660539040e-27
+    ms = np.array([16, 30, 28, 32, 14, 1, 5.485799090e-4]) * amu
+    sl = (slice(2, g.lx3 + 2), slice(2, g.lx2 + 2), slice(2, g.lx1 + 2))   # (x3, x2, x1)
+    FV = st.fluidvars.T
+    dl1 = g.dx1i[None, None, :] * g.h1.T[sl]
+    dl2 = g.dx2i[None, :, None] * g.h2.T[sl]
+    dl3 = g.dx3i[:, None, None] * g.h3.T[sl]
+    Te = FV[34][sl]
+    rate = 0.0
+    for isp in range(LSP):
+        if isp < LSP - 1:
+            vsnd = np.sqrt(kB * FV[28 + isp][sl] / ms[isp] + 5.0 / 3.0 * kB * Te / ms[isp])
+        else:
+            vsnd = 0.0
+        rate = max(rate, float(np.max((vsnd + np.abs(FV[7 + isp][sl])) / dl1)),
+                   float(np.max(np.abs(FV[14 + isp][sl]) / dl2)),
+                   float(np.max(np.abs(FV[21 + isp][sl]) / dl3)))
+    return min(dtmax, tcfl / rate) if rate > 0 else dtmax

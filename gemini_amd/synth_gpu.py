@@ -1,0 +1,133 @@
+"""Synthetic transport state generated directly in HBM with torch (benchmark inputs).
+
+Same profiles as :func:`gemini_amd.state.synthetic_state` (Chapman-layer ions,
+smooth temperatures, smooth drifts, tapered parallel current, exponential
+neutrals), drawn from a seeded ``torch.Generator`` on the device so the
+25M-cell configurations need no host memory.  Tensors are C-order with the
+species/field index first, e.g. ``fluidvars`` is (35, L3, L2, L1): the same bytes
+as the reference's Fortran ``fluidvars(-1:lx1+2,-1:lx2+2,-1:lx3+2,35)``.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .grid import Grid
+
+LSP = 7
+MINDENS = 1.0e-100
+
+
+def _chapman(z, zpk, H):
+    zeta = (z - zpk) / H
+    return torch.exp(0.5 * (1.0 - zeta - torch.exp(-zeta)))
+
+
+def _sig(z, z0, w):
+    return 0.5 * (1.0 + torch.tanh((z - z0) / w))
+
+
+def _uniform(gen, lo, hi, shape, dev):
+    return torch.rand(shape, generator=gen, device=dev, dtype=torch.float64) * (hi - lo) + lo
+
+
+def _smooth_field(gen, shape, out, scale, dev, nmodes=3, noise=0.05):
+    u = [torch.linspace(0.0, 1.0, n, device=dev, dtype=torch.float64) for n in shape]
+    out.zero_()
+    for _ in range(nmodes):
+        k = _uniform(gen, 0.3, 2.0, (3,), dev)
+        ph = _uniform(gen, 0.0, 2 * math.pi, (3,), dev)
+        parts = [torch.sin(2 * math.pi * k[i] * u[i] + ph[i]) for i in range(3)]
+        amp = _uniform(gen, 0.5, 1.0, (1,), dev)
+        plane = amp * parts[1][:, None] * parts[2][None, :]
+        out += parts[0][:, None, None] * plane[None, :, :]
+    out *= (1.0 - noise) * scale / out.abs().max().clamp_min(1e-30)
+    out += (noise * scale) * _uniform(gen, -1.0, 1.0, shape, dev)
+    return out
+
+
+def synthetic_state_device(g: Grid, seed: int = 0, device="cuda"):
+    """Dict of device tensors: fluidvars (35,L3,L2,L1), aux (14,L3,L2,L1) (rhovs1,
+    rhoes left zero), electrovars (7,L3,L2,L1) with J1 in [3], nn (3,lx3,lx2,lx1),
+    Tn (lx3,lx2,lx1)."""
+    dev = torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    L1, L2, L3 = g.padded
+    sh = (L3, L2, L1)
+    f64 = torch.float64
+    FV = torch.empty((35,) + sh, device=dev, dtype=f64)
+    z = torch.as_tensor(g.alt_plane(), device=dev)                  # (L2, L1)
+    nmax = 10.0 ** _uniform(gen, 11.0, 12.0, (L2, 1), dev)
+    nF = nmax * _chapman(z, 300e3, 60e3)
+    nE = 0.05 * nmax * _chapman(z, 110e3, 12e3)
+    fO = _sig(z, 180e3, 25e3)
+    fH = _sig(z, 1000e3, 150e3)
+    plasmasphere = 3e8 * torch.exp(-(z - 600e3).clamp_min(0.0) / 6000e3) * _sig(z, 500e3, 100e3)
+    mol = nF * (1.0 - fO) + nE
+    prof = [nF * fO * (1.0 - fH), 0.55 * mol, 0.03 * mol, 0.40 * mol, 0.02 * nF * fO,
+            nF * fO * fH + plasmasphere]
+    for isp in range(6):
+        noise = torch.randn(sh, generator=gen, device=dev, dtype=f64)
+        noise.mul_(0.05).exp_().mul_(prof[isp][None])
+        torch.clamp_min(noise, MINDENS, out=FV[isp])
+        del noise
+    torch.sum(FV[0:6], dim=0, out=FV[6])
+    Tn2 = 1000.0 - 800.0 * torch.exp(-(z - 80e3).clamp_min(0.0) / 40e3)
+    Ti = Tn2 + (1500.0 - Tn2) * _sig(z, 300e3, 100e3)
+    Te = Tn2 + (3000.0 - Tn2) * _sig(z, 200e3, 80e3)
+    for isp in range(6):
+        FV[28 + isp].copy_(Ti[None].expand(sh))
+    FV[34].copy_(Te[None].expand(sh))
+    for isp in range(LSP - 1):
+        _smooth_field(gen, sh, FV[7 + isp], 200.0, dev)
+    flux = torch.zeros(sh, device=dev, dtype=f64)
+    for isp in range(6):
+        flux.addcmul_(FV[isp], FV[7 + isp])
+    torch.div(flux, FV[6].clamp_min(1e-5), out=FV[13])
+    del flux
+    _smooth_field(gen, sh, FV[14], 500.0, dev)
+    _smooth_field(gen, sh, FV[21], 500.0, dev)
+    for isp in range(1, LSP):
+        FV[14 + isp].copy_(FV[14])
+        FV[21 + isp].copy_(FV[21])
+    electrovars = torch.zeros((7,) + sh, device=dev, dtype=f64)
+    J1 = electrovars[3]
+    _smooth_field(gen, sh, J1, 1e-6, dev)
+    J1.mul_((FV[6] / 1e10).clamp_max(1.0))
+    zi = z[2:g.lx2 + 2, 2:g.lx1 + 2]
+    Hs = lambda H0, z0: H0 + 0.06 * (zi - z0).clamp_min(0.0)
+    nn = torch.empty((3, g.lx3, g.lx2, g.lx1), device=dev, dtype=f64)
+    for k, a in enumerate((3e17 * torch.exp(-(zi - 100e3) / Hs(8e3, 100e3)),
+                           1e19 * torch.exp(-(zi - 80e3) / Hs(7e3, 80e3)),
+                           2.5e18 * torch.exp(-(zi - 80e3) / Hs(6.5e3, 80e3)))):
+        nn[k].copy_(a[None].expand(g.lx3, g.lx2, g.lx1))
+    Tn = Tn2[2:g.lx2 + 2, 2:g.lx1 + 2][None].expand(g.lx3, g.lx2, g.lx1).contiguous()
+    aux = torch.zeros((14,) + sh, device=dev, dtype=f64)
+    return dict(fluidvars=FV, aux=aux, electrovars=electrovars, nn=nn, Tn=Tn)
+
+
+def cfl_dt_device(g: Grid, st, tcfl: float = 0.9, dtmax: float = 0.5) -> float:
+    """temporal.f90:12-49 evaluated on the device tensors, capped at dtmax."""
+    kB = 1.38064852e-23
+    amu = 1.660539040e-27
+    ms = np.array([16, 30, 28, 32, 14, 1, 5.485799090e-4]) * amu
+    dev = st["fluidvars"].device
+    sl = (slice(2, g.lx3 + 2), slice(2, g.lx2 + 2), slice(2, g.lx1 + 2))
+    FV = st["fluidvars"]
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    h = lambda n: t(g.planes[n][0])[2:g.lx2 + 2, 2:g.lx1 + 2][None]
+    dl1 = t(g.dx1i)[None, None, :] * h("h1")
+    dl2 = t(g.dx2i)[None, :, None] * h("h2")
+    dl3 = t(g.dx3i)[:, None, None] * h("h3")
+    Te = FV[34][sl]
+    rate = 0.0
+    for isp in range(LSP):
+        vsnd = torch.sqrt(kB * FV[28 + isp][sl] / ms[isp] + 5.0 / 3.0 * kB * Te / ms[isp]) \
+            if isp < LSP - 1 else 0.0
+        rate = max(rate, float(((vsnd + FV[7 + isp][sl].abs()) / dl1).max()),
+                   float((FV[14 + isp][sl].abs() / dl2).max()),
+                   float((FV[21 + isp][sl].abs() / dl3).max()))
+    return min(dtmax, tcfl / rate) if rate > 0 else dtmax

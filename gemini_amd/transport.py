@@ -1,0 +1,278 @@
+"""Python front end of the HIP transport library (ctypes over include/gemini3d_hip.h).
+
+``Transport`` mirrors the hot-path interface the reference exposes to its drivers
+(src/libgemini.f90:731-1075 ``*_in`` wrappers and the C shims of
+src/libgemini_c.f90:445-848): the same operation names, acting in place on the
+reference's state blocks (a :class:`gemini_amd.state.State`).  Every call goes
+through the C ABI into libgemhip.so; there is no other implementation behind it,
+and a missing or unloadable library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .grid import Grid
+from .state import State
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgemhip.so")
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+class GemhipError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libgemhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GemhipError(f"{path} is missing: run `python -m gemini_amd.build` first")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    for name in ("gemhip_init", "gemhip_set_grid_size", "gemhip_register_grid",
+                 "gemhip_register_neutrals", "gemhip_register_config", "gemhip_set_residency",
+                 "gemhip_state_push", "gemhip_state_pull", "gemhip_get_interface_vels",
+                 "gemhip_set_interface_vels", "gemhip_transport_step", "gemhip_device_state",
+                 "gemhip_synchronize", "gemhip_kernel_timing", "gemhip_kernel_time",
+                 "gemhip_kernel_timing_reset", "gemhip_eval_math"):
+        getattr(lib, name).restype = C.c_int
+    lib.gemhip_last_error.restype = C.c_char_p
+    lib.gemhip_stream.restype = C.c_void_p
+    lib.gemhip_transport_step.argtypes = [C.c_double]
+    lib.gemhip_register_config.argtypes = [C.c_int, C.c_double]
+    lib.gemhip_kernel_time.argtypes = [C.c_char_p, _dp, C.POINTER(C.c_long)]
+    lib.gemhip_eval_math.argtypes = [C.c_int, _dp, _dp, C.c_long]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    """double* of a float64 numpy array in Fortran order, or of a contiguous float64
+    torch tensor (host or device; the library copies with hipMemcpyDefault)."""
+    if isinstance(a, np.ndarray):
+        if a.dtype != np.float64 or not a.flags.f_contiguous:
+            raise GemhipError("arrays must be float64 in Fortran order")
+        return a.ctypes.data_as(_dp)
+    if a.dtype.__repr__() != "torch.float64" or not a.is_contiguous():
+        raise GemhipError("tensors must be contiguous float64")
+    return C.cast(C.c_void_p(a.data_ptr()), _dp)
+
+
+class _Ref:
+    """A C pointer variable whose address is passed as double** (by reference)."""
+
+    def __init__(self, a: np.ndarray | None):
+        self.a = a
+        self.v = C.c_void_p(a.ctypes.data if a is not None else None)
+
+    @property
+    def ref(self):
+        return C.byref(self.v)
+
+
+class Transport:
+    """Drop-in hot path: holds the registered mesh, neutrals and config in the library."""
+
+    XTYPE = {"cartesian": 1, "dipole": 2}
+
+    GEOM = ("h1", "h2", "h3", "h1x1i", "h2x1i", "h3x1i", "h1x2i", "h2x2i", "h3x2i",
+            "h1x3i", "h2x3i", "h3x3i")
+
+    def __init__(self, grid: Grid, nn, Tn, device: int = 0, diffsolvetype: int = 2,
+                 Teinf: float = 1500.0, geometry_on_device: bool = False):
+        """nn (lx1,lx2,lx3,>=3) and Tn (lx1,lx2,lx3): Fortran numpy arrays, or torch
+        tensors holding the same bytes.  With geometry_on_device the x3-invariant metric
+        planes are expanded to 3-D arrays in HBM by torch instead of on the host."""
+        self.lib = load_library()
+        self.g = grid
+        self.lib.gemhip_set_error_mode(1)
+        self._call("gemhip_init", C.c_int(device))
+        self._call("gemhip_set_grid_size", grid.lx1, grid.lx2, grid.lx3)
+        if geometry_on_device:
+            geo = [grid.device_array(n, f"cuda:{device}") for n in self.GEOM]
+        else:
+            geo = [np.asfortranarray(getattr(grid, n), dtype=np.float64) for n in self.GEOM]
+        self._keep = geo + [np.ascontiguousarray(getattr(grid, n), dtype=np.float64) for n in
+                            ("dx1", "dx1i", "dx2", "dx2i", "dx3", "dx3i")]
+        isg = (C.c_int * 6)(*[int(bool(v)) for v in grid.isglobal])
+        self.xtype = self.XTYPE[grid.kind]
+        self._call("gemhip_register_grid", self.xtype, int(grid.gridflag), int(bool(grid.flagper)),
+                   isg, *[_p(a) for a in self._keep])
+        self.register_neutrals(nn, Tn)
+        self.configure(diffsolvetype, Teinf)
+        self._xt = C.c_int(self.xtype)
+        self._null = C.c_void_p(None)
+        self.residency = 0
+        self._call("gemhip_set_residency", 0)
+
+    # ------------------------------------------------------------------ plumbing
+    def _check(self, rc=0):
+        msg = self.lib.gemhip_last_error().decode()
+        if msg or rc:
+            self.lib.gemhip_clear_error()
+            raise GemhipError(msg or f"error code {rc}")
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(*args)
+        self._check(rc)
+        return rc
+
+    def register_neutrals(self, nn, Tn):
+        if isinstance(nn, np.ndarray):
+            nn = np.asfortranarray(nn[..., :3], dtype=np.float64)
+            Tn = np.asfortranarray(Tn, dtype=np.float64)
+        self._call("gemhip_register_neutrals", _p(nn), _p(Tn))
+        self._keep_neut = (nn, Tn)
+
+    def configure(self, diffsolvetype=2, Teinf=1500.0):
+        self.diffsolvetype, self.Teinf = diffsolvetype, Teinf
+        self._call("gemhip_register_config", int(diffsolvetype), float(Teinf))
+
+    def set_residency(self, mode: int):
+        self._call("gemhip_set_residency", int(mode))
+        self.residency = mode
+
+    def push(self, st: State):
+        """Upload the state blocks and its interface velocities (the reference keeps the
+        latter in intvars, which persists between steps)."""
+        self._call("gemhip_state_push", _p(st.fluidvars), _p(st.fluidauxvars), _p(st.electrovars))
+        self.set_interface_vels(st)
+        self._pushed = st
+
+    def push_blocks(self, fluidvars, fluidauxvars, electrovars):
+        """Push raw blocks (numpy or torch) laid out like the reference's."""
+        self._call("gemhip_state_push", _p(fluidvars), _p(fluidauxvars), _p(electrovars))
+
+    def pull(self, st: State):
+        self._call("gemhip_state_pull", _p(st.fluidvars), _p(st.fluidauxvars))
+
+    def get_interface_vels(self, st: State):
+        self._call("gemhip_get_interface_vels", _p(st.vs1i), _p(st.vs2i), _p(st.vs3i))
+
+    def set_interface_vels(self, st: State):
+        self._call("gemhip_set_interface_vels", _p(st.vs1i), _p(st.vs2i), _p(st.vs3i))
+
+    def synchronize(self):
+        self._call("gemhip_synchronize")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.gemhip_stream()
+
+    def device_state(self):
+        f, a, j = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+        self._call("gemhip_device_state", C.byref(f), C.byref(a), C.byref(j))
+        return C.cast(f, C.c_void_p).value, C.cast(a, C.c_void_p).value, C.cast(j, C.c_void_p).value
+
+    def eval_math(self, op: int, x: np.ndarray) -> np.ndarray:
+        """Device evaluation of a scalar routine (0 x**2.5, 1 sqrt, 2 1/x, 3 device pow)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._call("gemhip_eval_math", int(op), x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                   x.size)
+        return y
+
+    def kernel_timing(self, enable: bool):
+        self._call("gemhip_kernel_timing", int(enable))
+
+    def kernel_time(self, name: str):
+        ms, n = C.c_double(), C.c_long()
+        self.lib.gemhip_kernel_time(name.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def kernel_timing_reset(self):
+        self._call("gemhip_kernel_timing_reset")
+
+    # ------------------------------------------------------------------ reference operations
+    def _fa(self, st: State):
+        return _Ref(st.fluidvars), _Ref(st.fluidauxvars)
+
+    def _void(self, name, *args):
+        getattr(self.lib, name)(*args)
+        self._check()
+
+    def v12rhov1(self, st):
+        f, a = self._fa(st)
+        self._void("v12rhov1_C", f.ref, a.ref)
+
+    def T2rhoe(self, st):
+        f, a = self._fa(st)
+        self._void("T2rhoe_C", f.ref, a.ref)
+
+    def rhoe2T(self, st):
+        f, a = self._fa(st)
+        self._void("rhoe2T_C", f.ref, a.ref)
+
+    def rhov12v1(self, st):
+        f, a = self._fa(st)
+        self._void("rhov12v1_C", f.ref, a.ref)
+
+    def set_global_boundaries_allspec(self, st, lsp=7):
+        f, a = self._fa(st)
+        n = C.c_int(lsp)
+        self._void("set_global_boundaries_allspec_C", C.byref(self._xt), C.byref(self._null),
+                   f.ref, a.ref, C.byref(self._null), C.byref(n))
+
+    def interface_vels_allspec(self, st, lsp=7):
+        f = _Ref(st.fluidvars)
+        n = C.c_int(lsp)
+        self._void("interface_vels_allspec_C", C.byref(self._xt), C.byref(self._null), f.ref,
+                   C.byref(self._null), C.byref(n))
+
+    def halo_fluidvars(self, st):
+        f, a = self._fa(st)
+        self._void("halo_fluidvars_C", C.byref(self._xt), C.byref(self._null), f.ref, a.ref)
+
+    def halo_allparams(self, st):
+        f, a = self._fa(st)
+        self._void("halo_allparams_C", C.byref(self._xt), C.byref(self._null), f.ref, a.ref)
+
+    def halo_interface_vels_allspec(self, st, lsp=7):
+        f = _Ref(st.fluidvars)
+        n = C.c_int(lsp)
+        self._void("halo_interface_vels_allspec_C", C.byref(self._xt), C.byref(self._null), f.ref,
+                   C.byref(n))
+
+    def sweep(self, st, direction: int, which: str, dt: float):
+        """sweep{direction}_allspec_{which}_C; which in mass, momentum, energy, allparams."""
+        f, a = self._fa(st)
+        d = C.c_double(dt)
+        name = f"sweep{direction}_allparams_C" if which == "allparams" else \
+            f"sweep{direction}_allspec_{which}_C"
+        self._void(name, f.ref, a.ref, C.byref(self._null), C.byref(self._xt),
+                   C.byref(self._null), C.byref(d))
+
+    def energy_diffusion(self, st, dt: float):
+        f, e = _Ref(st.fluidvars), _Ref(st.electrovars)
+        d = C.c_double(dt)
+        self._void("energy_diffusion_C", C.byref(self._null), C.byref(self._xt),
+                   C.byref(self._null), f.ref, e.ref, C.byref(self._null), C.byref(d))
+
+    def fluid_adv_hotpath(self, st, dt: float):
+        """The hot-path calls of src/libgemini.cpp:155-208 one by one, then rhoe2T before
+        the energy diffusion, as the oracle's transport step orders them."""
+        self.v12rhov1(st)
+        self.T2rhoe(st)
+        self.set_global_boundaries_allspec(st, 7)
+        self.halo_fluidvars(st)
+        self.interface_vels_allspec(st, 7)
+        for d in (3, 1):
+            for w in ("mass", "momentum", "energy"):
+                self.sweep(st, d, w, dt)
+        self.halo_allparams(st)
+        for w in ("mass", "momentum", "energy"):
+            self.sweep(st, 2, w, dt)
+        self.rhov12v1(st)
+        self.rhoe2T(st)
+        self.energy_diffusion(st, dt)
+
+    def transport_step(self, dt: float):
+        """Fused step on the HBM-resident state (residency mode 1)."""
+        self._call("gemhip_transport_step", C.c_double(dt))

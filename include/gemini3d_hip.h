@@ -1,0 +1,159 @@
+/*
+ * gemini3d_hip.h -- C ABI of the MI355X (gfx950) multifluid-transport library.
+ *
+ * Part 1 re-exports, with identical names and signatures, the hot-path entry
+ * points of the reference's public C API (include/gemini3d.h), so the
+ * reference's C++ driver fluid_adv (src/libgemini.cpp:143-234) or the Fortran
+ * main (app/main.f90:258-383, through gemini_amd/fortran/gemhip_bind.f90)
+ * compiles against this library unchanged.  As in the reference every
+ * argument is passed by reference and sizes come from library state set once
+ * (the reference keeps them in module globals, grid.f90:12; here
+ * gemhip_set_grid_size).  The opaque void** arguments (cfgC, xC, intvars) are
+ * Fortran objects C cannot read; this library takes their contents once
+ * through the gemhip_register_* calls of part 2 and ignores the pointers,
+ * except that *xtype must equal the registered mesh type.
+ *
+ * Errors: where the reference executes `error stop` (advec.f90:145,225,241,
+ * 258,275; multifluid.f90:334-336; libgemini_c.f90:75-78) this library prints
+ * the message and calls abort() by default; after gemhip_set_error_mode(1) it
+ * records the message (gemhip_last_error) and returns instead.
+ *
+ * Data residency: in mode 0 (default) each part-1 call copies the host blocks
+ * it reads to HBM, runs, copies back what it writes and synchronises, so the
+ * caller's host arrays are current after every call, as in the reference.  In
+ * mode 1 the HBM copy is authoritative between gemhip_state_push and
+ * gemhip_state_pull and the calls only enqueue kernels on the library stream.
+ */
+#ifndef GEMINI3D_HIP_H
+#define GEMINI3D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- part 1: reference gemini3d.h hot-path entry points ---------------- */
+
+/* gemini3d.h:64, libgemini_c.f90:445-455 -> multifluid.f90:206-214 */
+void v12rhov1_C(double** fluidvars, double** fluidauxvars);
+/* gemini3d.h:65, libgemini_c.f90:459-469 -> multifluid.f90:193-202 */
+void T2rhoe_C(double** fluidvars, double** fluidauxvars);
+/* gemini3d.h:83, libgemini_c.f90:797-807 -> multifluid.f90:179-189 */
+void rhoe2T_C(double** fluidvars, double** fluidauxvars);
+/* gemini3d.h:79, libgemini_c.f90:748-758 -> multifluid.f90:218-233 */
+void rhov12v1_C(double** fluidvars, double** fluidauxvars);
+/* gemini3d.h:66, libgemini_c.f90:473-488 -> advec.f90:155-229 */
+void interface_vels_allspec_C(int* xtype, void** xC, double** fluidvars, void** intvars, int* lsp);
+/* gemini3d.h:101, libgemini_c.f90:491-509 -> advec.f90:21-149 */
+void set_global_boundaries_allspec_C(int* xtype, void** xC, double** fluidvars,
+                                     double** fluidauxvars, void** intvars, int* lsp);
+/* gemini3d.h:67-78, libgemini_c.f90:513-744 -> multifluid.f90:49-126 -> advec.f90:233-381 */
+void sweep3_allparams_C(double** fluidvars, double** fluidauxvars, void** intvars, int* xtype,
+                        void** xC, double* dt);
+void sweep3_allspec_mass_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                           int* xtype, void** xC, double* dt);
+void sweep3_allspec_momentum_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                               int* xtype, void** xC, double* dt);
+void sweep3_allspec_energy_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                             int* xtype, void** xC, double* dt);
+void sweep1_allparams_C(double** fluidvars, double** fluidauxvars, void** intvars, int* xtype,
+                        void** xC, double* dt);
+void sweep1_allspec_mass_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                           int* xtype, void** xC, double* dt);
+void sweep1_allspec_momentum_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                               int* xtype, void** xC, double* dt);
+void sweep1_allspec_energy_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                             int* xtype, void** xC, double* dt);
+void sweep2_allparams_C(double** fluidvars, double** fluidauxvars, void** intvars, int* xtype,
+                        void** xC, double* dt);
+void sweep2_allspec_mass_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                           int* xtype, void** xC, double* dt);
+void sweep2_allspec_momentum_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                               int* xtype, void** xC, double* dt);
+void sweep2_allspec_energy_C(double** fluidvars, double** fluidauxvars, void** intvars,
+                             int* xtype, void** xC, double* dt);
+/* gemini3d.h:84, libgemini_c.f90:827-848 -> multifluid.f90:309-342 */
+void energy_diffusion_C(void** cfgC, int* xtype, void** xC, double** fluidvars,
+                        double** electrovars, void** intvars, double* dt);
+/* gemini3d.h:135-137, libgemini_mpi_c.f90:414-458 -> multifluid_mpi.f90:12-41 and
+   advec_mpi.f90:33-44.
+   One process owns the whole grid: the exchange reduces to the periodic x3 wrap of
+   mpihalo.f90:147-152 when the grid is periodic, and to nothing otherwise. */
+void halo_interface_vels_allspec_C(int* xtype, void** xC, double** fluidvars, int* lsp);
+void halo_allparams_C(int* xtype, void** xC, double** fluidvars, double** fluidauxvars);
+void halo_fluidvars_C(int* xtype, void** xC, double** fluidvars, double** fluidauxvars);
+
+/* ---------------- part 2: library set-up (no counterpart in the reference ABI) ---------------- */
+/* Array arguments of part 2 may be host or HBM addresses: copies use hipMemcpyDefault. */
+
+/* Select the HIP device and create the library stream.  Returns 0 or an error code. */
+int gemhip_init(int device);
+/* Release every HBM buffer and the stream. */
+void gemhip_finalize(void);
+/* 0: abort() on error like the reference's `error stop`; 1: record and return. */
+void gemhip_set_error_mode(int mode);
+/* Message of the last error, or "" (reset by gemhip_clear_error). */
+const char* gemhip_last_error(void);
+void gemhip_clear_error(void);
+
+/* Subgrid sizes, the counterpart of the grid-module globals lx1,lx2,lx3 (grid.f90:12). */
+int gemhip_set_grid_size(int lx1, int lx2, int lx3);
+/* Mesh contents the reference keeps in its curvmesh object (meshobj.f90:29-208):
+   xtype (1 Cartesian, 2 dipole, libgemini_c.f90:69-74), gridflag, flagper, the six
+   isglobal flags (grid.f90:448-507) and host pointers to h1,h2,h3 (lx1+4,lx2+4,lx3+4),
+   hNx1i (lx1+1,lx2,lx3), hNx2i (lx1,lx2+1,lx3), hNx3i (lx1,lx2,lx3+1), dxN (lxN+3),
+   dxNi (lxN).  The arrays are copied to HBM. */
+int gemhip_register_grid(int xtype, int gridflag, int flagper, const int* isglobal6,
+                         const double* h1, const double* h2, const double* h3,
+                         const double* h1x1i, const double* h2x1i, const double* h3x1i,
+                         const double* h1x2i, const double* h2x2i, const double* h3x2i,
+                         const double* h1x3i, const double* h2x3i, const double* h3x3i,
+                         const double* dx1, const double* dx1i, const double* dx2,
+                         const double* dx2i, const double* dx3, const double* dx3i);
+/* intvars%atmos%nn(lx1,lx2,lx3,1:3) (slices lx1*lx2*lx3 apart) and Tn(lx1,lx2,lx3). */
+int gemhip_register_neutrals(const double* nn, const double* Tn);
+/* cfg%diffsolvetype (1 backward Euler, 2 TRBDF2) and cfg%Teinf (config.f90:88). */
+int gemhip_register_config(int diffsolvetype, double Teinf);
+
+/* Residency mode (see above). */
+int gemhip_set_residency(int mode);
+/* Copy host blocks to HBM: fluidvars (...,35), the first 14 fields of fluidauxvars
+   (rhovs1, rhoes) and field 4 of electrovars (J1).  Either auxiliary pointer may be
+   NULL to leave that part untouched. */
+int gemhip_state_push(const double* fluidvars, const double* fluidauxvars,
+                      const double* electrovars);
+/* Copy HBM state back: fluidvars (...,35) and the first 14 fields of fluidauxvars. */
+int gemhip_state_pull(double* fluidvars, double* fluidauxvars);
+/* Interface velocities held by the library (the reference's intvars%vs1i/vs2i/vs3i).
+   Arrays are (lx1+1,lx2,lx3,7), (lx1,lx2+1,lx3,7), (lx1,lx2,lx3+1,7). */
+int gemhip_get_interface_vels(double* vs1i, double* vs2i, double* vs3i);
+int gemhip_set_interface_vels(const double* vs1i, const double* vs2i, const double* vs3i);
+
+/* One transport step on the HBM state, in the order of src/libgemini.cpp:155-186 and
+   :208: v12rhov1, T2rhoe, set_global_boundaries_allspec, halo_fluidvars,
+   interface_vels_allspec, sweep3/sweep1/halo_allparams/sweep2 of mass, momentum and
+   energy, rhov12v1, rhoe2T, energy_diffusion.  Kernels are fused where the result is
+   bit-identical (interface velocities are formed inside the sweeps).  Requires
+   residency mode 1.  Returns 0 or an error code. */
+int gemhip_transport_step(double dt);
+
+/* HBM pointers of the state blocks (mode 1), for zero-copy callers. */
+int gemhip_device_state(double** fluidvars, double** rhovs1_rhoes, double** J1);
+/* The library's HIP stream (a hipStream_t) for event timing and ordering. */
+void* gemhip_stream(void);
+int gemhip_synchronize(void);
+
+/* Per-kernel timing with HIP events on the library stream: enable, then read the
+   accumulated milliseconds and launch counts by kernel name. */
+int gemhip_kernel_timing(int enable);
+int gemhip_kernel_time(const char* name, double* total_ms, long* launches);
+int gemhip_kernel_timing_reset(void);
+
+/* Evaluate on the device the scalar routines the kernels use, for checks against the
+   host libm: op 0 x**2.5 as the conductivity uses it, 1 sqrt, 2 reciprocal 1/x,
+   3 the device library pow(x, 2.5).  x and y are host arrays of n doubles. */
+int gemhip_eval_math(int op, const double* x, double* y, long n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

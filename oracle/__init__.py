@@ -1,0 +1,264 @@
+"""ctypes front end of the oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads ``oracle/liboracle.so`` (built from gemini_oracle.c by ``oracle/Makefile``)
+and exposes the CPU restatement of the reference's transport path on the
+numpy arrays of :mod:`gemini_amd.grid` / :mod:`gemini_amd.state`.  Only the
+tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use it, and
+only as the checker; the product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+class _OrcGrid(C.Structure):
+    _fields_ = [("lx1", C.c_int), ("lx2", C.c_int), ("lx3", C.c_int),
+                ("gridflag", C.c_int), ("flagper", C.c_int), ("isglobal", C.c_int * 6)] + \
+               [(n, _dp) for n in ("h1", "h2", "h3", "h1x1i", "h2x1i", "h3x1i",
+                                   "h1x2i", "h2x2i", "h3x2i", "h1x3i", "h2x3i", "h3x3i",
+                                   "dx1", "dx1i", "dx2", "dx2i", "dx3", "dx3i")]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_minmod.restype = C.c_double
+        _lib.orc_minmod.argtypes = [C.c_double, C.c_double]
+        for name in ("orc_gbsv", "orc_energy_diffusion", "orc_transport_step", "orc_num_threads"):
+            getattr(_lib, name).restype = C.c_int
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous), a.flags
+    return a.ctypes.data_as(_dp)
+
+
+class OracleGrid:
+    """Keeps the numpy arrays alive while the C struct points into them."""
+
+    def __init__(self, g):
+        self.g = g
+        s = _OrcGrid()
+        s.lx1, s.lx2, s.lx3 = g.lx1, g.lx2, g.lx3
+        s.gridflag = int(g.gridflag)
+        s.flagper = int(bool(g.flagper))
+        for k in range(6):
+            s.isglobal[k] = int(bool(g.isglobal[k]))
+        self._keep = []
+        for n in ("h1", "h2", "h3", "h1x1i", "h2x1i", "h3x1i", "h1x2i", "h2x2i", "h3x2i",
+                  "h1x3i", "h2x3i", "h3x3i", "dx1", "dx1i", "dx2", "dx2i", "dx3", "dx3i"):
+            a = np.asfortranarray(getattr(g, n), dtype=np.float64)
+            self._keep.append(a)
+            setattr(s, n, ptr(a))
+        self.s = s
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _f(a):
+    assert a.dtype == np.float64 and a.flags.f_contiguous
+    return ptr(a)
+
+
+def transport_step(g, st, dt, diffsolvetype=2, Teinf=1500.0):
+    """Advance ``st`` (in place) by one oracle transport step; returns st."""
+    og = OracleGrid(g)
+    rc = lib().orc_transport_step(og.ref, C.c_double(dt), _f(st.fluidvars), _f(st.fluidauxvars),
+                                  _f(st.electrovars), _f(st.vs1i), _f(st.vs2i), _f(st.vs3i),
+                                  _f(st.nn), _f(st.Tn), C.c_int(diffsolvetype),
+                                  C.c_double(Teinf))
+    if rc != 0:
+        raise RuntimeError(f"oracle transport step failed ({rc})")
+    return st
+
+
+def set_pow_mode(mode: int):
+    """0: x**2.5 via libm pow as the reference; 1: correctly rounded x**2.5."""
+    lib().orc_set_pow_mode(C.c_int(mode))
+
+
+def eval_pow25(x, mode):
+    """x**2.5 elementwise: mode 0 libm pow, mode 1 correctly rounded."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().orc_eval_pow25(ptr(x), ptr(y), C.c_long(x.size), C.c_int(mode))
+    return y
+
+
+class pow_mode:
+    """Context manager selecting the oracle's x**2.5 evaluation."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        set_pow_mode(self.mode)
+
+    def __exit__(self, *exc):
+        set_pow_mode(0)
+
+
+def set_num_threads(n: int):
+    os.environ["OMP_NUM_THREADS"] = str(n)
+    try:
+        C.CDLL("libgomp.so.1").omp_set_num_threads(C.c_int(n))
+    except OSError:
+        pass
+
+
+# ---- finer-grained entry points used by the unit tests ----
+
+def constants():
+    ms, qs, gm, tc = (np.zeros(7), np.zeros(7), np.zeros(7), np.zeros(6))
+    kB, e = C.c_double(), C.c_double()
+    lib().orc_constants(ptr(ms), ptr(qs), ptr(gm), C.byref(kB), C.byref(e), ptr(tc))
+    return dict(ms=ms, qs=qs, gammas=gm, kB=kB.value, elchrg=e.value, thermal_coeff=tc)
+
+
+def minmod(a, b):
+    return lib().orc_minmod(C.c_double(a), C.c_double(b))
+
+
+def advec1D(f, v, dt, dx, dxi, ha1, ha2i, h1i):
+    lx = len(f) - 4
+    out = np.zeros(lx + 4)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (f, v, dx, dxi, ha1, ha2i, h1i)]
+    lib().orc_advec1D_MC_curv(C.c_int(lx), ptr(arrs[0]), ptr(arrs[1]), C.c_double(dt),
+                              ptr(arrs[2]), ptr(arrs[3]), ptr(arrs[4]), ptr(arrs[5]),
+                              ptr(arrs[6]), ptr(out))
+    return out
+
+
+def gbsv(ab, b, kl=2):
+    """LAPACK-band solve; ab (3*kl+1, n) Fortran band array, returns (x, lu, ipiv, info)."""
+    ab = np.array(ab, dtype=np.float64, order="F", copy=True)
+    x = np.array(b, dtype=np.float64, copy=True)
+    n = ab.shape[1]
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = lib().orc_gbsv(C.c_int(n), C.c_int(kl), ptr(ab), ptr(x),
+                          ipiv.ctypes.data_as(C.POINTER(C.c_int)))
+    return x, ab, ipiv, info
+
+
+def _col(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def trbdf2_1d(Ts, A, B, Cc, D, E, Tmin, Tmax, dt, dx1, dx1i):
+    lx1 = len(Ts)
+    out = np.zeros(lx1)
+    a = [_col(v) for v in (Ts, A, B, Cc, D, E, dx1, dx1i)]
+    lib().orc_TRBDF21D(C.c_int(lx1), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]),
+                       ptr(a[5]), C.c_double(Tmin), C.c_double(Tmax), C.c_double(dt),
+                       ptr(a[6]), ptr(a[7]), ptr(out))
+    return out
+
+
+def backeuler_1d(Ts, A, B, Cc, D, E, Tmin, Tmax, dt, dx1, dx1i):
+    lx1 = len(Ts)
+    out = np.zeros(lx1)
+    coeffs = np.zeros((lx1, 3), order="F")
+    rhs = np.zeros(lx1)
+    a = [_col(v) for v in (Ts, A, B, Cc, D, E, dx1, dx1i)]
+    lib().orc_backEuler1D(C.c_int(lx1), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]),
+                          ptr(a[5]), C.c_double(Tmin), C.c_double(Tmax), C.c_double(dt),
+                          ptr(a[6]), ptr(a[7]), ptr(out), ptr(coeffs), ptr(rhs))
+    return out, coeffs, rhs
+
+
+def interface_vels(g, st, lsp=7):
+    og = OracleGrid(g)
+    lib().orc_interface_vels_allspec(og.ref, _f(st.fluidvars[..., 7:14]),
+                                     _f(st.fluidvars[..., 14:21]), _f(st.fluidvars[..., 21:28]),
+                                     _f(st.vs1i), _f(st.vs2i), _f(st.vs3i), C.c_int(lsp))
+    return st
+
+
+def sweep(g, direction, fs, vsi, dt, frank, lsp):
+    """fs: (L1,L2,L3,7) Fortran array, modified in place."""
+    og = OracleGrid(g)
+    lib().orc_sweep_allspec(og.ref, C.c_int(direction), _f(fs), _f(vsi), C.c_double(dt),
+                            C.c_int(frank), C.c_int(lsp))
+    return fs
+
+
+def _views(st):
+    fv, fa = st.fluidvars, st.fluidauxvars
+    return dict(ns=fv[..., 0:7], vs1=fv[..., 7:14], vs2=fv[..., 14:21], vs3=fv[..., 21:28],
+                Ts=fv[..., 28:35], rhovs1=fa[..., 0:7], rhoes=fa[..., 7:14],
+                J1=st.electrovars[..., 3])
+
+
+def op(g, st, name, dt=None, lsp=7, direction=None, which=None, diffsolvetype=2,
+       Teinf=1500.0):
+    """Apply one reference operation of the transport path to ``st`` in place.
+
+    Names follow the reference's C entry points: v12rhov1, T2rhoe, rhoe2T, rhov12v1,
+    set_global_boundaries_allspec, halo_fluidvars, halo_allparams,
+    halo_interface_vels_allspec, interface_vels_allspec, sweep (direction 1-3, which
+    mass/momentum/energy/allparams) and energy_diffusion.
+    """
+    og = OracleGrid(g)
+    L = lib()
+    v = {k: _f(a) for k, a in _views(st).items()}
+    if name == "v12rhov1":
+        L.orc_v12rhov1(og.ref, v["ns"], v["vs1"], v["rhovs1"])
+    elif name == "T2rhoe":
+        L.orc_T2rhoe(og.ref, v["ns"], v["Ts"], v["rhoes"])
+    elif name == "rhoe2T":
+        L.orc_rhoe2T(og.ref, v["ns"], v["rhoes"], v["Ts"])
+    elif name == "rhov12v1":
+        L.orc_rhov12v1(og.ref, v["ns"], v["rhovs1"], v["vs1"])
+    elif name == "set_global_boundaries_allspec":
+        L.orc_set_global_boundaries_allspec(og.ref, v["ns"], v["rhovs1"], v["vs1"], v["vs2"],
+                                            v["vs3"], v["rhoes"], _f(st.vs1i), C.c_int(lsp))
+    elif name in ("halo_fluidvars", "halo_allparams"):
+        names = ["ns", "rhovs1", "rhoes"] + (["vs2", "vs3"] if name == "halo_fluidvars" else [])
+        for k in names:
+            L.orc_halo_periodic(og.ref, v[k], C.c_int(7), C.c_int(2))
+    elif name == "halo_interface_vels_allspec":
+        for k in ("vs2", "vs3"):
+            L.orc_halo_periodic(og.ref, v[k], C.c_int(lsp), C.c_int(1))
+    elif name == "interface_vels_allspec":
+        L.orc_interface_vels_allspec(og.ref, v["vs1"], v["vs2"], v["vs3"], _f(st.vs1i),
+                                     _f(st.vs2i), _f(st.vs3i), C.c_int(lsp))
+    elif name == "sweep":
+        vsi = {1: st.vs1i, 2: st.vs2i, 3: st.vs3i}[direction]
+        parts = {"mass": [("ns", 0, 6)], "momentum": [("rhovs1", 1, 6)],
+                 "energy": [("rhoes", 0, 7)]}
+        todo = parts["mass"] + parts["momentum"] + parts["energy"] if which == "allparams" \
+            else parts[which]
+        for field, frank, nsp in todo:
+            L.orc_sweep_allspec(og.ref, C.c_int(direction), v[field], _f(vsi), C.c_double(dt),
+                                C.c_int(frank), C.c_int(nsp))
+    elif name == "energy_diffusion":
+        rc = L.orc_energy_diffusion(og.ref, C.c_double(dt), v["ns"], v["Ts"], v["J1"],
+                                    _f(st.nn), _f(st.Tn), C.c_int(diffsolvetype),
+                                    C.c_double(Teinf))
+        if rc != 0:
+            raise RuntimeError(f"oracle energy diffusion failed ({rc})")
+    else:
+        raise KeyError(name)
+    return st

@@ -1,0 +1,783 @@
+/*
+ * gemini_oracle.c -- TEST INFRASTRUCTURE ONLY (parity checker and CPU baseline).
+ *
+ * A plain-C restatement of the reference's multifluid transport hot path,
+ * written to reproduce its floating-point results: every expression keeps the
+ * reference's evaluation order (Fortran left-to-right, parentheses kept), no
+ * reciprocal is substituted for a divide, and the file is compiled with
+ * -ffp-contract=off so no multiply-add is fused (the reference is built for
+ * the x86-64 baseline, which has no FMA).  x**2.5 is libm pow(), as gfortran
+ * emits for Ts**(5.0_wp/2).
+ *
+ * Reference files followed (paths relative to the reference root):
+ *   src/numerical/advection/advec.f90        (advection, ghost fill, interface velocities)
+ *   src/numerical/diffusion/PDEparabolic.f90 (TRBDF21D, backEuler1D)
+ *   src/numerical/diffusion/diffusion.f90    (diffusion_prep, TRBDF23D/backEuler3D)
+ *   src/collisions/collisions.f90:51-53,162-199 (thermal_conduct)
+ *   src/multifluid/multifluid.f90:179-233,309-342 (state conversions, energy_diffusion)
+ *   src/mpimod/mpihalo.f90:147-152           (single-process periodic x3 wrap)
+ *   src/vendor/lapack95/gbsv.in.f90:58-97    (-> LAPACK dgbsv, Netlib dgbtrf/dgbtf2/
+ *                                             dgbtrs/dtbsv/dger/dscal/idamax semantics)
+ *   src/numerical/constants/phys_consts.in.f90 (constants)
+ */
+#include "gemini_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define LSP 7
+
+/* ---- constants: phys_consts.in.f90:24-53,82-84; collisions.f90:51-53 ---- */
+static const double kB = 1.38064852e-23;
+static const double elchrg = 1.60217662e-19;
+static const double amu = 1.660539040e-27;
+static const double mindensdiv = 1.0e-5;
+
+static double ms_(int isp) {              /* [16,30,28,32,14,1,5.485799090e-4]*amu */
+  static const double m[LSP] = {16.0, 30.0, 28.0, 32.0, 14.0, 1.0, 5.485799090e-4};
+  return m[isp - 1] * amu;
+}
+static double qs_(int isp) { return (isp < LSP ? 1.0 : -1.0) * elchrg; }
+static double gammas_(int isp) {
+  return (isp == 2 || isp == 3 || isp == 4) ? 7.0 / 5.0 : 5.0 / 3.0;
+}
+/* [real(wp) :: 0.1019e-12, ...]: default-real (single precision) literals
+   converted to real64, so each coefficient is the float nearest the decimal. */
+static double thermal_coeff_(int isp) {
+  static const float c[LSP - 1] = {0.1019e-12f, 0.0747e-12f, 0.0754e-12f,
+                                   0.0701e-12f, 0.1068e-12f, 0.3986e-12f};
+  return (double)c[isp - 1];
+}
+
+void orc_constants(double *ms7, double *qs7, double *gammas7, double *kB_, double *elchrg_,
+                   double *thermal_coeff6) {
+  for (int i = 1; i <= LSP; ++i) {
+    ms7[i - 1] = ms_(i);
+    qs7[i - 1] = qs_(i);
+    gammas7[i - 1] = gammas_(i);
+  }
+  for (int i = 1; i < LSP; ++i) thermal_coeff6[i - 1] = thermal_coeff_(i);
+  *kB_ = kB;
+  *elchrg_ = elchrg;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* x**2.5 as the reference evaluates Ts**(5.0_wp/2): libm pow (mode 0, default).  Mode 1
+   instead rounds x^2.5 correctly (x^2 exactly via an FMA split times sqrt(x) with one
+   Newton correction, in double-double, then one rounding), the method the GPU library
+   uses; it lets the tests check the GPU bit for bit apart from libm's pow, which
+   returns the other neighbour of the exact value in about 0.2 % of arguments. */
+static int pow_mode = 0;
+void orc_set_pow_mode(int mode) { pow_mode = mode; }
+
+static double pow25(double x) {
+  if (pow_mode == 0 || !(x > 0x1p-400 && x < 0x1p+400)) return pow(x, 2.5);
+  const double s = sqrt(x);
+  const double e = fma(-s, s, x);
+  const double sl = e / (2.0 * s);
+  const double qh = x * x;
+  const double ql = fma(x, x, -qh);
+  const double ph = qh * s;
+  const double pl = fma(qh, s, -ph) + (qh * sl + ql * s);
+  return ph + pl;
+}
+
+/* y = x**2.5 elementwise with the given mode (for the math checks) */
+void orc_eval_pow25(const double *x, double *y, long n, int mode) {
+  const int saved = pow_mode;
+  pow_mode = mode;
+  for (long i = 0; i < n; ++i) y[i] = pow25(x[i]);
+  pow_mode = saved;
+}
+
+static double fmin_f(double a, double b) { return (b < a) ? b : a; }  /* Fortran MIN */
+static double fmax_f(double a, double b) { return (b > a) ? b : a; }  /* Fortran MAX */
+
+/* ---- index helpers (Fortran layout, lower bound -1 for ghosted arrays) ---- */
+typedef struct { long L1, L2, L3, N; } dims_t;
+static dims_t dims_of(const orc_grid *g) {
+  dims_t d;
+  d.L1 = g->lx1 + 4; d.L2 = g->lx2 + 4; d.L3 = g->lx3 + 4;
+  d.N = d.L1 * d.L2 * d.L3;
+  return d;
+}
+#define GIX(d, i1, i2, i3) ((long)((i1) + 1) + (d).L1 * ((long)((i2) + 1) + (d).L2 * (long)((i3) + 1)))
+
+/* ---- advec.f90:438-447 ---- */
+double orc_minmod(double a, double b) {
+  if (a * b <= 0.0) return 0.0;
+  if (fabs(a) < fabs(b)) return a;
+  return b;
+}
+
+/* ---- advec.f90:384-435 ----
+   f(-1:lx+2), v1i(1:lx+1), dx1(0:lx+2), dx1i(1:lx), ha1(-1:lx+2), ha2i(1:lx+1),
+   h1i(1:lx+1); out(-1:lx+2).  C arrays are 0-based: f[k] = f(k-1), v1i[k] = v1i(k+1),
+   dx1[k] = dx1(k), dx1i[k] = dx1i(k+1). */
+void orc_advec1D_MC_curv(int lx, const double *f, const double *v1i, double dt,
+                         const double *dx1, const double *dx1i, const double *ha1,
+                         const double *ha2i, const double *h1i, double *out) {
+#define F(i) f[(i) + 1]
+#define HA1(i) ha1[(i) + 1]
+#define V(i) v1i[(i) - 1]
+#define HA2I(i) ha2i[(i) - 1]
+#define H1I(i) h1i[(i) - 1]
+#define DX(i) dx1[(i)]
+#define DXI(i) dx1i[(i) - 1]
+  if (lx > 1) {
+    double *slope = (double *)malloc(sizeof(double) * (size_t)(lx + 2));  /* 0:lx+1 */
+    double *phi = (double *)malloc(sizeof(double) * (size_t)(lx + 2));    /* 1:lx+1 */
+    double lslope = (F(0) - F(-1)) / DX(0);
+    for (int i = 0; i <= lx + 1; ++i) {
+      double rslope = (F(i + 1) - F(i)) / DX(i + 1);
+      double cslope = (F(i + 1) - F(i - 1)) / (DX(i) + DX(i + 1));
+      slope[i] = orc_minmod(cslope, orc_minmod(2.0 * lslope, 2.0 * rslope));
+      lslope = rslope;
+    }
+    for (int i = 1; i <= lx + 1; ++i) {
+      double v = V(i);
+      if (v < 0.0)
+        phi[i] = F(i) * v - ((0.5 * v) * (DX(i) + (v / H1I(i)) * dt)) * slope[i];
+      else
+        phi[i] = F(i - 1) * v + ((0.5 * v) * (DX(i) - (v / H1I(i)) * dt)) * slope[i - 1];
+    }
+    for (int i = 1; i <= lx; ++i)
+      out[i + 1] = F(i) - ((dt * (HA2I(i + 1) * phi[i + 1] - HA2I(i) * phi[i])) / DXI(i)) / HA1(i);
+    out[0] = F(-1); out[1] = F(0);
+    out[lx + 2] = F(lx + 1); out[lx + 3] = F(lx + 2);
+    free(slope);
+    free(phi);
+  } else {
+    for (int i = 0; i < lx + 4; ++i) out[i] = f[i];
+  }
+#undef F
+#undef HA1
+#undef V
+#undef HA2I
+#undef H1I
+#undef DX
+#undef DXI
+}
+
+/* ---- multifluid.f90:179-233: pointwise state conversions over the padded grid ---- */
+void orc_v12rhov1(const orc_grid *g, const double *ns, const double *vs1, double *rhovs1) {
+  dims_t d = dims_of(g);
+  for (int isp = 1; isp <= LSP; ++isp) {
+    long o = (long)(isp - 1) * d.N;
+    for (long k = 0; k < d.N; ++k) rhovs1[o + k] = (ns[o + k] * ms_(isp)) * vs1[o + k];
+  }
+}
+
+void orc_T2rhoe(const orc_grid *g, const double *ns, const double *Ts, double *rhoes) {
+  dims_t d = dims_of(g);
+  for (int isp = 1; isp <= LSP; ++isp) {
+    long o = (long)(isp - 1) * d.N;
+    double gm1 = gammas_(isp) - 1.0;
+    for (long k = 0; k < d.N; ++k) rhoes[o + k] = ((ns[o + k] * kB) * Ts[o + k]) / gm1;
+  }
+}
+
+void orc_rhov12v1(const orc_grid *g, double *ns, const double *rhovs1, double *vs1) {
+  dims_t d = dims_of(g);
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < d.N; ++k) {
+    double chrgflux = 0.0;
+    for (int isp = 1; isp <= LSP - 1; ++isp) {
+      long o = (long)(isp - 1) * d.N + k;
+      vs1[o] = rhovs1[o] / (ms_(isp) * fmax_f(ns[o], mindensdiv));
+      chrgflux = chrgflux + (ns[o] * qs_(isp)) * vs1[o];
+    }
+    double s = 0.0;                                   /* sum(ns(:,:,:,1:lsp-1),4) */
+    for (int isp = 1; isp <= LSP - 1; ++isp) s = s + ns[(long)(isp - 1) * d.N + k];
+    long o7 = (long)(LSP - 1) * d.N + k;
+    ns[o7] = s;
+    vs1[o7] = -(((1.0 / fmax_f(ns[o7], mindensdiv)) / qs_(LSP)) * chrgflux);
+  }
+}
+
+void orc_rhoe2T(const orc_grid *g, const double *ns, const double *rhoes, double *Ts) {
+  dims_t d = dims_of(g);
+  for (int isp = 1; isp <= LSP; ++isp) {
+    long o = (long)(isp - 1) * d.N;
+    double c = (gammas_(isp) - 1.0) / kB;
+    for (long k = 0; k < d.N; ++k) {
+      double t = (c * rhoes[o + k]) / fmax_f(ns[o + k], mindensdiv);
+      Ts[o + k] = fmax_f(t, 100.0);
+    }
+  }
+}
+
+/* ---- advec.f90:21-149: global-boundary ghost fill (one species at a time) ---- */
+static void set_global_boundaries_1(const orc_grid *g, int isp, double *ns, double *rhovs1,
+                                    const double *vs1, double *vs2, double *vs3,
+                                    double *rhoes, const double *v1i) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long o = (long)(isp - 1) * d.N;
+  double *n = ns + o, *r = rhovs1 + o, *e = rhoes + o, *w2 = vs2 + o, *w3 = vs3 + o;
+  const double *u = vs1 + o;
+  const double msp = ms_(isp);
+  /* v1i(1:lx1+1,1:lx2,1:lx3) */
+#define V1I(i1, i2, i3) v1i[((i1) - 1) + (long)(lx1 + 1) * (((i2) - 1) + (long)lx2 * ((i3) - 1))]
+#define A3(p, i1, i2, i3) (p)[GIX(d, i1, i2, i3)]
+  if (g->isglobal[0]) {                                   /* :42-60 */
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2) {
+        double coeff = A3(n, 2, i2, i3) / A3(n, 3, i2, i3);
+        A3(n, 0, i2, i3) = fmin_f(coeff * A3(n, 1, i2, i3), A3(n, 1, i2, i3));
+        A3(n, -1, i2, i3) = fmin_f(coeff * A3(n, 0, i2, i3), A3(n, 0, i2, i3));
+      }
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        A3(r, 0, i2, i3) = 2.0 * V1I(1, i2, i3) - A3(u, 1, i2, i3);
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+        A3(r, -1, i2, i3) = (A3(r, 0, i2, i3) + A3(r, 0, i2, i3)) - A3(u, 1, i2, i3);
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+        for (int i1 = -1; i1 <= 0; ++i1)
+          A3(r, i1, i2, i3) = (A3(r, i1, i2, i3) * A3(n, i1, i2, i3)) * msp;
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2) {
+        double Tb = A3(e, 1, i2, i3) / A3(n, 1, i2, i3);
+        A3(e, 0, i2, i3) = Tb * A3(n, 0, i2, i3);
+        A3(e, -1, i2, i3) = Tb * A3(n, -1, i2, i3);
+      }
+  }
+  if (g->isglobal[1]) {                                   /* :61-79 */
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2) {
+        double coeff = A3(n, lx1 - 1, i2, i3) / A3(n, lx1 - 2, i2, i3);
+        A3(n, lx1 + 1, i2, i3) = fmin_f(coeff * A3(n, lx1, i2, i3), A3(n, lx1, i2, i3));
+        A3(n, lx1 + 2, i2, i3) = fmin_f(coeff * A3(n, lx1 + 1, i2, i3), A3(n, lx1 + 1, i2, i3));
+      }
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        A3(r, lx1 + 1, i2, i3) = 2.0 * V1I(lx1 + 1, i2, i3) - A3(u, lx1, i2, i3);
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+        A3(r, lx1 + 2, i2, i3) = (A3(r, lx1 + 1, i2, i3) + A3(r, lx1 + 1, i2, i3)) - A3(u, lx1, i2, i3);
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+        for (int i1 = lx1 + 1; i1 <= lx1 + 2; ++i1)
+          A3(r, i1, i2, i3) = (A3(r, i1, i2, i3) * A3(n, i1, i2, i3)) * msp;
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2) {
+        double Tb = A3(e, lx1, i2, i3) / A3(n, lx1, i2, i3);
+        A3(e, lx1 + 1, i2, i3) = Tb * A3(n, lx1 + 1, i2, i3);
+        A3(e, lx1 + 2, i2, i3) = Tb * A3(n, lx1 + 2, i2, i3);
+      }
+  }
+  /* :83-104, x2 edges: copy the edge cell over both ghosts, all x1 and x3 */
+  double *x2f[4] = {w2, n, r, e};
+  if (g->isglobal[2])
+    for (int q = 0; q < 4; ++q)
+      for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1) {
+          A3(x2f[q], i1, 0, i3) = A3(x2f[q], i1, 1, i3);
+          A3(x2f[q], i1, -1, i3) = A3(x2f[q], i1, 1, i3);
+        }
+  if (g->isglobal[3])
+    for (int q = 0; q < 4; ++q)
+      for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1) {
+          A3(x2f[q], i1, lx2 + 1, i3) = A3(x2f[q], i1, lx2, i3);
+          A3(x2f[q], i1, lx2 + 2, i3) = A3(x2f[q], i1, lx2, i3);
+        }
+  /* :107-132, x3 edges unless periodic: all x1 and x2 */
+  double *x3f[4] = {w3, n, r, e};
+  if (!g->flagper) {
+    if (g->isglobal[4])
+      for (int q = 0; q < 4; ++q)
+        for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+          for (int i1 = -1; i1 <= lx1 + 2; ++i1) {
+            A3(x3f[q], i1, i2, 0) = A3(x3f[q], i1, i2, 1);
+            A3(x3f[q], i1, i2, -1) = A3(x3f[q], i1, i2, 1);
+          }
+    if (g->isglobal[5])
+      for (int q = 0; q < 4; ++q)
+        for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+          for (int i1 = -1; i1 <= lx1 + 2; ++i1) {
+            A3(x3f[q], i1, i2, lx3 + 1) = A3(x3f[q], i1, i2, lx3);
+            A3(x3f[q], i1, i2, lx3 + 2) = A3(x3f[q], i1, i2, lx3);
+          }
+  }
+#undef V1I
+#undef A3
+}
+
+void orc_set_global_boundaries_allspec(const orc_grid *g, double *ns, double *rhovs1,
+                                       const double *vs1, double *vs2, double *vs3,
+                                       double *rhoes, const double *vs1i, int lsp) {
+  long nv1i = (long)(g->lx1 + 1) * g->lx2 * g->lx3;
+  for (int isp = 1; isp <= lsp; ++isp)
+    set_global_boundaries_1(g, isp, ns, rhovs1, vs1, vs2, vs3, rhoes,
+                            vs1i + (long)(isp - 1) * nv1i);
+}
+
+/* ---- mpihalo.f90:147-152: single-process periodic x3 wrap (lid3 == 1) ---- */
+void orc_halo_periodic(const orc_grid *g, double *param, int nspec, int lhalo) {
+  if (!g->flagper) return;
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  double *tmp = (double *)malloc(sizeof(double) * (size_t)(lx1 + 4) * (size_t)lhalo);
+  for (int s = 0; s < nspec; ++s) {
+    double *p = param + (long)s * d.N;
+    for (int i2 = 1; i2 <= lx2; ++i2) {
+      /* param(-1:lx1+2,1:lx2,1-lhalo:0) = param(-1:lx1+2,1:lx2,lx3-(lhalo-1):lx3) */
+      for (int h = 0; h < lhalo; ++h)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1)
+          tmp[(i1 + 1) + (long)(lx1 + 4) * h] = p[GIX(d, i1, i2, lx3 - (lhalo - 1) + h)];
+      for (int h = 0; h < lhalo; ++h)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1)
+          p[GIX(d, i1, i2, 1 - lhalo + h)] = tmp[(i1 + 1) + (long)(lx1 + 4) * h];
+      /* param(-1:lx1+2,1:lx2,lx3+1:lx3+lhalo) = param(-1:lx1+2,1:lx2,1:lhalo) */
+      for (int h = 0; h < lhalo; ++h)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1)
+          tmp[(i1 + 1) + (long)(lx1 + 4) * h] = p[GIX(d, i1, i2, 1 + h)];
+      for (int h = 0; h < lhalo; ++h)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1)
+          p[GIX(d, i1, i2, lx3 + 1 + h)] = tmp[(i1 + 1) + (long)(lx1 + 4) * h];
+    }
+  }
+  free(tmp);
+}
+
+/* ---- advec.f90:155-229: interface velocities ---- */
+void orc_interface_vels_allspec(const orc_grid *g, const double *vs1, const double *vs2,
+                                const double *vs3, double *vs1i, double *vs2i, double *vs3i,
+                                int lsp) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long n1 = (long)(lx1 + 1) * lx2 * lx3, n2 = (long)lx1 * (lx2 + 1) * lx3,
+             n3 = (long)lx1 * lx2 * (lx3 + 1);
+  for (int isp = 1; isp <= lsp; ++isp) {
+    const double *u1 = vs1 + (long)(isp - 1) * d.N, *u2 = vs2 + (long)(isp - 1) * d.N,
+                 *u3 = vs3 + (long)(isp - 1) * d.N;
+    double *v1 = vs1i + (long)(isp - 1) * n1, *v2 = vs2i + (long)(isp - 1) * n2,
+           *v3 = vs3i + (long)(isp - 1) * n3;
+#define V1(i1, i2, i3) v1[((i1) - 1) + (long)(lx1 + 1) * (((i2) - 1) + (long)lx2 * ((i3) - 1))]
+#define V2(i1, i2, i3) v2[((i1) - 1) + (long)lx1 * (((i2) - 1) + (long)(lx2 + 1) * ((i3) - 1))]
+#define V3(i1, i2, i3) v3[((i1) - 1) + (long)lx1 * (((i2) - 1) + (long)lx2 * ((i3) - 1))]
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2) {
+        for (int i1 = 2; i1 <= lx1; ++i1)
+          V1(i1, i2, i3) = 0.5 * (u1[GIX(d, i1 - 1, i2, i3)] + u1[GIX(d, i1, i2, i3)]);
+        if (g->isglobal[0]) {
+          if (g->gridflag == 0) V1(1, i2, i3) = u1[GIX(d, 1, i2, i3)];
+          else if (g->gridflag == 1) V1(1, i2, i3) = fmin_f(V1(2, i2, i3), 0.0);
+          else V1(1, i2, i3) = u1[GIX(d, 1, i2, i3)];
+        } else {
+          V1(1, i2, i3) = 0.5 * (u1[GIX(d, 0, i2, i3)] + u1[GIX(d, 1, i2, i3)]);
+        }
+        if (g->isglobal[1]) {
+          if (g->gridflag == 0) V1(lx1 + 1, i2, i3) = u1[GIX(d, lx1, i2, i3)];
+          else if (g->gridflag == 1) V1(lx1 + 1, i2, i3) = u1[GIX(d, lx1, i2, i3)];
+          else V1(lx1 + 1, i2, i3) = fmax_f(V1(lx1, i2, i3), 0.0);
+        } else {
+          V1(lx1 + 1, i2, i3) = 0.5 * (u1[GIX(d, lx1, i2, i3)] + u1[GIX(d, lx1 + 1, i2, i3)]);
+        }
+      }
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2 + 1; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1)
+          V2(i1, i2, i3) = 0.5 * (u2[GIX(d, i1, i2 - 1, i3)] + u2[GIX(d, i1, i2, i3)]);
+    for (int i3 = 1; i3 <= lx3 + 1; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1)
+          V3(i1, i2, i3) = 0.5 * (u3[GIX(d, i1, i2, i3 - 1)] + u3[GIX(d, i1, i2, i3)]);
+#undef V1
+#undef V2
+#undef V3
+  }
+}
+
+/* ---- advec.f90:233-381: dimensionally split sweeps ---- */
+void orc_sweep_allspec(const orc_grid *g, int dir, double *fs, const double *vsi, double dt,
+                       int frank, int lsp) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const int lx = dir == 1 ? lx1 : dir == 2 ? lx2 : lx3;
+  const long nvi = dir == 1 ? (long)(lx1 + 1) * lx2 * lx3
+                 : dir == 2 ? (long)lx1 * (lx2 + 1) * lx3 : (long)lx1 * lx2 * (lx3 + 1);
+  const double *dx = dir == 1 ? g->dx1 : dir == 2 ? g->dx2 : g->dx3;
+  const double *dxi = dir == 1 ? g->dx1i : dir == 2 ? g->dx2i : g->dx3i;
+  const int nlineA = dir == 1 ? lx2 : lx1, nlineB = dir == 3 ? lx2 : lx3;
+  for (int isp = 1; isp <= lsp; ++isp) {
+    double *f = fs + (long)(isp - 1) * d.N;
+    const double *vi = vsi + (long)(isp - 1) * nvi;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int b = 1; b <= nlineB; ++b)
+      for (int a = 1; a <= nlineA; ++a) {
+        double fl[lx + 4], out[lx + 4], ha1[lx + 4], vl[lx + 1], ha2i[lx + 1], h1i[lx + 1];
+        for (int k = -1; k <= lx + 2; ++k) {
+          long c = dir == 1 ? GIX(d, k, a, b) : dir == 2 ? GIX(d, a, k, b) : GIX(d, a, b, k);
+          fl[k + 1] = f[c];
+          double h1 = g->h1[c], h2 = g->h2[c], h3 = g->h3[c];
+          if (dir == 1 || frank == 0) ha1[k + 1] = (h1 * h2) * h3;   /* :302, :334, :370 */
+          else ha1[k + 1] = ((h1 * h1) * h2) * h3;                     /* :337, :373 */
+        }
+        for (int k = 1; k <= lx + 1; ++k) {
+          long ci;
+          if (dir == 1) {
+            ci = (k - 1) + (long)(lx1 + 1) * ((a - 1) + (long)lx2 * (b - 1));
+            ha2i[k - 1] = g->h2x1i[ci] * g->h3x1i[ci];                  /* :303 */
+            h1i[k - 1] = g->h1x1i[ci];
+          } else if (dir == 2) {
+            ci = (a - 1) + (long)lx1 * ((k - 1) + (long)(lx2 + 1) * (b - 1));
+            double p = g->h1x2i[ci];
+            ha2i[k - 1] = frank == 0 ? p * g->h3x2i[ci] : (p * p) * g->h3x2i[ci];
+            h1i[k - 1] = g->h2x2i[ci];
+          } else {
+            ci = (a - 1) + (long)lx1 * ((b - 1) + (long)lx2 * (k - 1));
+            double p = g->h1x3i[ci];
+            ha2i[k - 1] = frank == 0 ? p * g->h2x3i[ci] : (p * p) * g->h2x3i[ci];
+            h1i[k - 1] = g->h3x3i[ci];
+          }
+          vl[k - 1] = vi[ci];
+        }
+        orc_advec1D_MC_curv(lx, fl, vl, dt, dx, dxi, ha1, ha2i, h1i, out);
+        for (int k = -1; k <= lx + 2; ++k) {
+          long c = dir == 1 ? GIX(d, k, a, b) : dir == 2 ? GIX(d, a, k, b) : GIX(d, a, b, k);
+          f[c] = out[k + 1];
+        }
+      }
+  }
+}
+
+/* ---- collisions.f90:162-199 ---- */
+void orc_thermal_conduct(const orc_grid *g, int isp, const double *Ts, const double *ns,
+                         const double *nn, const double *J1, double *lambda, double *beta) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  for (int i3 = 1; i3 <= lx3; ++i3)
+    for (int i2 = 1; i2 <= lx2; ++i2)
+      for (int i1 = 1; i1 <= lx1; ++i1) {
+        long c = GIX(d, i1, i2, i3);
+        long k = (i1 - 1) + (long)lx1 * ((i2 - 1) + (long)lx2 * (i3 - 1));
+        double T = Ts[c];
+        if (isp < LSP) {
+          lambda[k] = thermal_coeff_(isp) * pow25(T);
+          beta[k] = 0.0;
+        } else {
+          double n1 = nn[k], n2 = nn[ni + k], n3 = nn[2 * ni + k];
+          double sq = sqrt(T);
+          double br = ((n1 * 1.1e-16) * (1.0 + 5.7e-4 * T) + ((n2 * 2.82e-17) * sq) * (1.0 - 1.21e-4 * T))
+                      + (n3 * 2.2e-16) * (1.0 + 3.6e-2 * sq);
+          double den = 1.0 + ((3.22e4 * (T * T)) / ns[c]) * br;
+          lambda[k] = (((elchrg * 100.0) * 7.7e5) * pow25(T)) / den;
+          beta[k] = (((5.0 / 2.0) * kB) / elchrg) * J1[c];
+        }
+      }
+}
+
+/* ---- LAPACK dgbsv restated (Netlib reference semantics) ----
+   ab is the LAPACK band array AB(ldab=2*kl+ku+1, n), column-major, 1-based in the
+   macros below; b is the single right-hand side.  Returns INFO. */
+#define AB(i, j) ab[((i) - 1) + (long)ldab * ((j) - 1)]
+static int dgbtf2_(int n, int kl, int ku, double *ab, int ldab, int *ipiv) {
+  int info = 0;
+  const int kv = ku + kl;
+  for (int j = ku + 2; j <= (kv < n ? kv : n); ++j)       /* zero fill-in columns ku+2..kv */
+    for (int i = kv - j + 2; i <= kl; ++i) AB(i, j) = 0.0;
+  int ju = 1;
+  for (int j = 1; j <= n; ++j) {
+    if (j + kv <= n)
+      for (int i = 1; i <= kl; ++i) AB(i, j + kv) = 0.0;
+    const int km = kl < n - j ? kl : n - j;
+    int jp = 1;                                             /* idamax: first maximal |x| */
+    double dmax = fabs(AB(kv + 1, j));
+    for (int i = 2; i <= km + 1; ++i) {
+      double v = fabs(AB(kv + i, j));
+      if (v > dmax) { jp = i; dmax = v; }
+    }
+    ipiv[j - 1] = jp + j - 1;
+    if (AB(kv + jp, j) != 0.0) {
+      int jt = j + ku + jp - 1 < n ? j + ku + jp - 1 : n;
+      if (jt > ju) ju = jt;
+      if (jp != 1)                                          /* dswap along the row */
+        for (int k = 0; k <= ju - j; ++k) {
+          double t = AB(kv + jp - k, j + k);
+          AB(kv + jp - k, j + k) = AB(kv + 1 - k, j + k);
+          AB(kv + 1 - k, j + k) = t;
+        }
+      if (km > 0) {
+        const double rp = 1.0 / AB(kv + 1, j);              /* dscal(km, ONE/pivot) */
+        for (int i = 1; i <= km; ++i) AB(kv + 1 + i, j) = rp * AB(kv + 1 + i, j);
+        if (ju > j)                                         /* dger(km, ju-j, -ONE, ...) */
+          for (int k = 1; k <= ju - j; ++k) {
+            const double y = AB(kv + 1 - k, j + k);
+            if (y != 0.0) {
+              const double temp = -1.0 * y;
+              for (int i = 1; i <= km; ++i)
+                AB(kv + 1 + i - k, j + k) = AB(kv + 1 + i - k, j + k) + AB(kv + 1 + i, j) * temp;
+            }
+          }
+      }
+    } else if (info == 0) {
+      info = j;
+    }
+  }
+  return info;
+}
+
+static void dgbtrs_n_(int n, int kl, int ku, const double *ab, int ldab, const int *ipiv,
+                      double *b) {
+  const int kd = ku + kl + 1;
+  if (kl > 0)
+    for (int j = 1; j <= n - 1; ++j) {
+      const int lm = kl < n - j ? kl : n - j;
+      const int l = ipiv[j - 1];
+      if (l != j) { double t = b[l - 1]; b[l - 1] = b[j - 1]; b[j - 1] = t; }
+      const double y = b[j - 1];                            /* dger(lm, 1, -ONE, ...) */
+      if (y != 0.0) {
+        const double temp = -1.0 * y;
+        for (int i = 1; i <= lm; ++i) b[j + i - 1] = b[j + i - 1] + AB(kd + i, j) * temp;
+      }
+    }
+  const int k = kl + ku, kplus1 = k + 1;                   /* dtbsv('U','N','N') */
+  for (int j = n; j >= 1; --j) {
+    if (b[j - 1] != 0.0) {
+      const int l = kplus1 - j;
+      b[j - 1] = b[j - 1] / AB(kplus1, j);
+      const double temp = b[j - 1];
+      for (int i = j - 1; i >= (1 > j - k ? 1 : j - k); --i)
+        b[i - 1] = b[i - 1] - temp * AB(l + i, j);
+    }
+  }
+}
+#undef AB
+
+int orc_gbsv(int n, int kl, double *ab, double *b, int *ipiv) {
+  const int ldab = 3 * kl + 1;                  /* gbsv.in.f90:72,88: KU = LDA-2*KL-1 */
+  const int ku = ldab - 2 * kl - 1;
+  int info = dgbtf2_(n, kl, ku, ab, ldab, ipiv);  /* dgbtrf picks dgbtf2 for KU <= 64 */
+  if (info == 0) dgbtrs_n_(n, kl, ku, ab, ldab, ipiv, b);
+  return info;
+}
+
+/* ---- PDEparabolic.f90:19-218 ----
+   Ts,A..E(1:lx1) -> arrays [0..lx1-1]; dx1(0:lx1+2) -> [0..lx1+2]; dx1i(1:lx1) -> [0..]. */
+void orc_TRBDF21D(int lx1, const double *Ts, const double *A, const double *B,
+                  const double *C, const double *D, const double *E, double Tsminx1,
+                  double Tsmaxx1, double dt, const double *dx1, const double *dx1i,
+                  double *out) {
+  const int ll = 2;
+  double *M = (double *)calloc((size_t)7 * (size_t)lx1, sizeof(double));
+  double *Dh = (double *)malloc(sizeof(double) * (size_t)lx1);
+  double *TR = (double *)malloc(sizeof(double) * (size_t)lx1);
+  int *ipiv = (int *)malloc(sizeof(int) * (size_t)lx1);
+#define MM(r, c) M[((r) - 1) + 7L * ((c) - 1)]
+#define T_(i) Ts[(i) - 1]
+#define A_(i) A[(i) - 1]
+#define B_(i) B[(i) - 1]
+#define C_(i) C[(i) - 1]
+#define E_(i) E[(i) - 1]
+#define DH(i) Dh[(i) - 1]
+#define DX(i) dx1[(i)]
+#define DXI(i) dx1i[(i) - 1]
+  DH(1) = 0.0;
+  for (int i = 2; i <= lx1; ++i) DH(i) = 0.5 * (D[i - 2] + D[i - 1]);
+
+  /* TR half step */
+  int i = 1;
+  MM(ll + 3, i) = 1.0; MM(ll + 2, i + 1) = 0.0; MM(ll + 1, i + 2) = 0.0;
+  TR[i - 1] = Tsminx1;
+  for (i = 2; i <= lx1 - 1; ++i) {
+    const double t1 = (((C_(i) * DH(i)) / DXI(i)) / DX(i)) / 2.0;
+    const double t3 = (((C_(i) * DH(i + 1)) / DXI(i)) / DX(i + 1)) / 2.0;
+    const double t2 = (B_(i) / (DX(i + 1) + DX(i))) / 2.0;
+    if (i >= 3) MM(ll + 5, i - 2) = 0.0;
+    MM(ll + 4, i - 1) = -t1 + t2;
+    MM(ll + 3, i) = ((1.0 / (dt / 2.0) - A_(i) / 2.0) + t3) + t1;
+    MM(ll + 2, i + 1) = -t3 - t2;
+    if (i <= lx1 - 2) MM(ll + 1, i + 2) = 0.0;
+    double r = T_(i) / (dt / 2.0) + E_(i);
+    if (i >= 3) r = r - MM(ll + 5, i - 2) * T_(i - 2);
+    r = r - MM(ll + 4, i - 1) * T_(i - 1);
+    r = r - ((-(A_(i) / 2.0) + t3) + t1) * T_(i);
+    r = r - MM(ll + 2, i + 1) * T_(i + 1);
+    if (i <= lx1 - 2) r = r - MM(ll + 1, i + 2) * T_(i + 2);
+    TR[i - 1] = r;
+  }
+  i = lx1;
+  MM(ll + 5, i - 2) = 0.0; MM(ll + 4, i - 1) = 0.0; MM(ll + 3, i) = 1.0;
+  TR[i - 1] = Tsmaxx1;
+  orc_gbsv(lx1, 2, M, TR, ipiv);
+
+  /* BDF2 step */
+  i = 1;
+  MM(ll + 3, i) = 1.0; MM(ll + 2, i + 1) = 0.0; MM(ll + 1, i + 2) = 0.0;
+  out[i - 1] = Tsminx1;
+  for (i = 2; i <= lx1 - 1; ++i) {
+    const double s1 = ((C_(i) * DH(i)) / DXI(i)) / DX(i);
+    const double s3 = ((C_(i) * DH(i + 1)) / DXI(i)) / DX(i + 1);
+    const double s2 = B_(i) / (DX(i + 1) + DX(i));
+    if (i >= 3) MM(ll + 5, i - 2) = 0.0;
+    MM(ll + 4, i - 1) = -s1 + s2;
+    MM(ll + 3, i) = ((1.0 / (dt / 3.0) - A_(i)) + s3) + s1;
+    MM(ll + 2, i + 1) = -s3 - s2;
+    if (i <= lx1 - 2) MM(ll + 1, i + 2) = 0.0;
+    out[i - 1] = (E_(i) - ((1.0 / 3.0) * T_(i)) / (dt / 3.0)) + ((4.0 / 3.0) * TR[i - 1]) / (dt / 3.0);
+  }
+  i = lx1;
+  MM(ll + 5, i - 2) = 0.0; MM(ll + 4, i - 1) = 0.0; MM(ll + 3, i) = 1.0;
+  out[i - 1] = Tsmaxx1;
+  orc_gbsv(lx1, 2, M, out, ipiv);
+  free(M); free(Dh); free(TR); free(ipiv);
+}
+
+/* ---- PDEparabolic.f90:221-346 (coeffs(lx1,3) and rhs(lx1) optional outputs) ---- */
+void orc_backEuler1D(int lx1, const double *Ts, const double *A, const double *B,
+                     const double *C, const double *D, const double *E, double Tsminx1,
+                     double Tsmaxx1, double dt, const double *dx1, const double *dx1i,
+                     double *out, double *coeffs, double *rhs) {
+  const int ll = 2;
+  double *M = (double *)calloc((size_t)7 * (size_t)lx1, sizeof(double));
+  double *Dh = (double *)malloc(sizeof(double) * (size_t)lx1);
+  int *ipiv = (int *)malloc(sizeof(int) * (size_t)lx1);
+  DH(1) = 0.0;
+  for (int i = 2; i <= lx1; ++i) DH(i) = 0.5 * (D[i - 2] + D[i - 1]);
+  for (int i = 1; i <= lx1; ++i) out[i - 1] = T_(i) / dt + E_(i);
+  int i = 1;
+  MM(ll + 3, i) = 1.0; MM(ll + 2, i + 1) = 0.0; MM(ll + 1, i + 2) = 0.0;
+  out[i - 1] = Tsminx1;
+  for (i = 2; i <= lx1 - 1; ++i) {
+    const double s1 = ((C_(i) * DH(i)) / DXI(i)) / DX(i);
+    const double s3 = ((C_(i) * DH(i + 1)) / DXI(i)) / DX(i + 1);
+    const double s2 = B_(i) / (DX(i + 1) + DX(i));
+    if (i >= 3) MM(ll + 5, i - 2) = 0.0;
+    MM(ll + 4, i - 1) = -s1 + s2;
+    MM(ll + 3, i) = ((1.0 / dt - A_(i)) + s3) + s1;
+    MM(ll + 2, i + 1) = -s3 - s2;
+    if (i <= lx1 - 2) MM(ll + 1, i + 2) = 0.0;
+  }
+  i = lx1;
+  MM(ll + 5, i - 2) = 0.0; MM(ll + 4, i - 1) = 0.0; MM(ll + 3, i) = 1.0;
+  out[i - 1] = Tsmaxx1;
+  if (rhs) for (int k = 0; k < lx1; ++k) rhs[k] = out[k];
+  orc_gbsv(lx1, 2, M, out, ipiv);
+  if (coeffs)
+    for (int k = 1; k <= lx1; ++k) {
+      coeffs[(k - 1)] = MM(ll + 2, k);
+      coeffs[(k - 1) + lx1] = MM(ll + 3, k);
+      coeffs[(k - 1) + 2L * lx1] = MM(ll + 4, k);
+    }
+  free(M); free(Dh); free(ipiv);
+#undef MM
+#undef T_
+#undef A_
+#undef B_
+#undef C_
+#undef E_
+#undef DH
+#undef DX
+#undef DXI
+}
+
+/* ---- multifluid.f90:309-342 with diffusion.f90:24-152 ----
+   nn(1:lx1,1:lx2,1:lx3,>=3) and Tn(1:lx1,1:lx2,1:lx3) interior arrays; J1 ghosted.
+   Returns 0, or -1 for an unsupported diffsolvetype (the reference's error stop).
+   Ghost cells of Ts, which the reference leaves undefined (the TRBDF23D result is only
+   assigned on 1:lx1 x 1:lx2 x 1:lx3, diffusion.f90:136,149), keep their input values
+   and are floored at 100 like every other cell. */
+int orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double *Ts,
+                         const double *J1, const double *nn, const double *Tn,
+                         int diffsolvetype, double Teinf) {
+  if (diffsolvetype != 1 && diffsolvetype != 2) return -1;
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  double *lambda = (double *)malloc(sizeof(double) * (size_t)ni);
+  double *beta = (double *)malloc(sizeof(double) * (size_t)ni);
+  for (int isp = 1; isp <= LSP; ++isp) {
+    double *T = Ts + (long)(isp - 1) * d.N;
+    const double *n = ns + (long)(isp - 1) * d.N;
+    orc_thermal_conduct(g, isp, T, n, nn, J1, lambda, beta);
+    const double gm1kB = (gammas_(isp) - 1.0) / kB;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2) {
+        double Tc[lx1], A[lx1], B[lx1], C[lx1], D[lx1], E[lx1], res[lx1];
+        for (int i1 = 1; i1 <= lx1; ++i1) {
+          long c = GIX(d, i1, i2, i3);
+          long k = (i1 - 1) + (long)lx1 * ((i2 - 1) + (long)lx2 * (i3 - 1));
+          double h1 = g->h1[c], h2 = g->h2[c], h3 = g->h3[c];
+          A[i1 - 1] = 0.0;
+          C[i1 - 1] = (gm1kB / fmax_f(n[c], mindensdiv)) / ((h1 * h2) * h3);
+          B[i1 - 1] = (C[i1 - 1] * beta[k]) / h1;
+          D[i1 - 1] = ((lambda[k] * h2) * h3) / h1;
+          E[i1 - 1] = 0.0;
+          Tc[i1 - 1] = T[c];
+        }
+        long kb = (long)lx1 * ((i2 - 1) + (long)lx2 * (i3 - 1));
+        double Tmin, Tmax;
+        if (g->gridflag == 0) { Tmin = Tn[kb]; Tmax = Tn[kb + lx1 - 1]; }
+        else if (g->gridflag == 1) { Tmax = Tn[kb + lx1 - 1]; Tmin = Teinf; }
+        else { Tmin = Tn[kb]; Tmax = Teinf; }
+        if (diffsolvetype == 2)
+          orc_TRBDF21D(lx1, Tc, A, B, C, D, E, Tmin, Tmax, dt, g->dx1, g->dx1i, res);
+        else
+          orc_backEuler1D(lx1, Tc, A, B, C, D, E, Tmin, Tmax, dt, g->dx1, g->dx1i, res, 0, 0);
+        for (int i1 = 1; i1 <= lx1; ++i1) T[GIX(d, i1, i2, i3)] = res[i1 - 1];
+      }
+    for (long k = 0; k < d.N; ++k) T[k] = fmax_f(T[k], 100.0);
+  }
+  free(lambda);
+  free(beta);
+  return 0;
+}
+
+/* ---- one transport step in the order of src/libgemini.cpp:155-186 and :208
+   (app/main.f90:277-355): v12rhov1, T2rhoe, set_global_boundaries_allspec,
+   halo_fluidvars, interface_vels_allspec, sweep3 (mass, momentum, energy), sweep1,
+   halo_allparams, sweep2, rhov12v1, then rhoe2T (multifluid.f90:179-189) to bring the
+   advected energy back to temperature, then energy_diffusion.  The compression,
+   artificial-viscosity and clean_param substeps between them are outside this path.
+   Block layout: fluidvars(...,35) = ns 1-7, vs1 8-14, vs2 15-21, vs3 22-28, Ts 29-35;
+   fluidauxvars(...,>=14) = rhovs1 1-7, rhoes 8-14; electrovars(...,4) = J1. */
+int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *fluidauxvars,
+                       const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
+                       const double *nn, const double *Tn, int diffsolvetype, double Teinf) {
+  dims_t d = dims_of(g);
+  double *ns = fluidvars, *vs1 = fluidvars + 7 * d.N, *vs2 = fluidvars + 14 * d.N,
+         *vs3 = fluidvars + 21 * d.N, *Ts = fluidvars + 28 * d.N;
+  double *rhovs1 = fluidauxvars, *rhoes = fluidauxvars + 7 * d.N;
+  const double *J1 = electrovars + 3 * d.N;
+  orc_v12rhov1(g, ns, vs1, rhovs1);
+  orc_T2rhoe(g, ns, Ts, rhoes);
+  orc_set_global_boundaries_allspec(g, ns, rhovs1, vs1, vs2, vs3, rhoes, vs1i, LSP);
+  orc_halo_periodic(g, ns, LSP, 2);
+  orc_halo_periodic(g, rhovs1, LSP, 2);
+  orc_halo_periodic(g, rhoes, LSP, 2);
+  orc_halo_periodic(g, vs2, LSP, 2);
+  orc_halo_periodic(g, vs3, LSP, 2);
+  orc_interface_vels_allspec(g, vs1, vs2, vs3, vs1i, vs2i, vs3i, LSP);
+  orc_sweep_allspec(g, 3, ns, vs3i, dt, 0, 6);
+  orc_sweep_allspec(g, 3, rhovs1, vs3i, dt, 1, 6);
+  orc_sweep_allspec(g, 3, rhoes, vs3i, dt, 0, 7);
+  orc_sweep_allspec(g, 1, ns, vs1i, dt, 0, 6);
+  orc_sweep_allspec(g, 1, rhovs1, vs1i, dt, 0, 6);
+  orc_sweep_allspec(g, 1, rhoes, vs1i, dt, 0, 7);
+  orc_halo_periodic(g, ns, LSP, 2);
+  orc_halo_periodic(g, rhovs1, LSP, 2);
+  orc_halo_periodic(g, rhoes, LSP, 2);
+  orc_sweep_allspec(g, 2, ns, vs2i, dt, 0, 6);
+  orc_sweep_allspec(g, 2, rhovs1, vs2i, dt, 1, 6);
+  orc_sweep_allspec(g, 2, rhoes, vs2i, dt, 0, 7);
+  orc_rhov12v1(g, ns, rhovs1, vs1);
+  orc_rhoe2T(g, ns, rhoes, Ts);
+  return orc_energy_diffusion(g, dt, ns, Ts, J1, nn, Tn, diffsolvetype, Teinf);
+}

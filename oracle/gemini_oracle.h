@@ -1,0 +1,95 @@
+/*
+ * gemini_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C, IEEE double, no FMA contraction) of the GEMINI
+ * multifluid-transport hot path: MC-limited finite-volume advection sweeps
+ * and TRBDF2 / backward-Euler field-aligned energy diffusion with LAPACK
+ * dgbsv semantics.  It is the parity checker for the HIP library in
+ * gemini_amd/ and the CPU baseline timed by bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product path never calls it.
+ *
+ * Array conventions follow the reference (Fortran column-major, x1 fastest):
+ *   ghosted cell arrays  A(-1:lx1+2,-1:lx2+2,-1:lx3+2[,isp])
+ *   x1-interface arrays  A(1:lx1+1,1:lx2,1:lx3[,isp])
+ *   x2-interface arrays  A(1:lx1,1:lx2+1,1:lx3[,isp])
+ *   x3-interface arrays  A(1:lx1,1:lx2,1:lx3+1[,isp])
+ *   interior arrays      A(1:lx1,1:lx2,1:lx3[,k])
+ *   dxN(0:lxN+2) and dxNi(1:lxN) one-dimensional
+ * All species counts are the reference's lsp = 7.
+ */
+#ifndef GEMINI_ORACLE_H
+#define GEMINI_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int lx1, lx2, lx3;
+  int gridflag;          /* meshobj.f90:563-569: 0 closed dipole, 1 inverted, 2 standard */
+  int flagper;           /* x3 periodic (meshobj.f90 flagper) */
+  int isglobal[6];       /* grid.f90:448-507: x1min,x1max,x2min,x2max,x3min,x3max */
+  const double *h1, *h2, *h3;                 /* ghosted cell-centred metric factors */
+  const double *h1x1i, *h2x1i, *h3x1i;        /* x1 interfaces */
+  const double *h1x2i, *h2x2i, *h3x2i;        /* x2 interfaces */
+  const double *h1x3i, *h2x3i, *h3x3i;        /* x3 interfaces */
+  const double *dx1, *dx1i, *dx2, *dx2i, *dx3, *dx3i;
+} orc_grid;
+
+/* constants (phys_consts.in.f90) exported so tests can read what the oracle uses */
+void orc_constants(double *ms7, double *qs7, double *gammas7, double *kB, double *elchrg,
+                   double *thermal_coeff6);
+
+double orc_minmod(double a, double b);
+void orc_advec1D_MC_curv(int lx, const double *f, const double *v1i, double dt,
+                         const double *dx1, const double *dx1i, const double *ha1,
+                         const double *ha2i, const double *h1i, double *out);
+
+void orc_v12rhov1(const orc_grid *g, const double *ns, const double *vs1, double *rhovs1);
+void orc_T2rhoe(const orc_grid *g, const double *ns, const double *Ts, double *rhoes);
+void orc_rhov12v1(const orc_grid *g, double *ns, const double *rhovs1, double *vs1);
+void orc_rhoe2T(const orc_grid *g, const double *ns, const double *rhoes, double *Ts);
+
+void orc_set_global_boundaries_allspec(const orc_grid *g, double *ns, double *rhovs1,
+                                       const double *vs1, double *vs2, double *vs3,
+                                       double *rhoes, const double *vs1i, int lsp);
+void orc_halo_periodic(const orc_grid *g, double *param, int nspec, int lhalo);
+void orc_interface_vels_allspec(const orc_grid *g, const double *vs1, const double *vs2,
+                                const double *vs3, double *vs1i, double *vs2i, double *vs3i,
+                                int lsp);
+/* dir = 1,2,3; frank = 0 scalar, 1 first component of a rank-1 tensor */
+void orc_sweep_allspec(const orc_grid *g, int dir, double *fs, const double *vsi, double dt,
+                       int frank, int lsp);
+
+void orc_thermal_conduct(const orc_grid *g, int isp, const double *Ts, const double *ns,
+                         const double *nn, const double *J1, double *lambda, double *beta);
+int  orc_gbsv(int n, int kl, double *ab, double *b, int *ipiv);
+void orc_TRBDF21D(int lx1, const double *Ts, const double *A, const double *B,
+                  const double *C, const double *D, const double *E, double Tsminx1,
+                  double Tsmaxx1, double dt, const double *dx1, const double *dx1i,
+                  double *out);
+void orc_backEuler1D(int lx1, const double *Ts, const double *A, const double *B,
+                     const double *C, const double *D, const double *E, double Tsminx1,
+                     double Tsmaxx1, double dt, const double *dx1, const double *dx1i,
+                     double *out, double *coeffs, double *rhs);
+int  orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double *Ts,
+                          const double *J1, const double *nn, const double *Tn,
+                          int diffsolvetype, double Teinf);
+
+/* One transport step in the reference's fluid_adv order (see the .c file). */
+int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *fluidauxvars,
+                       const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
+                       const double *nn, const double *Tn, int diffsolvetype, double Teinf);
+
+/* 0: x**2.5 by libm pow, as the reference (default); 1: correctly rounded x**2.5 */
+void orc_set_pow_mode(int mode);
+void orc_eval_pow25(const double *x, double *y, long n, int mode);
+
+/* threads used by the OpenMP loops (1 when built without OpenMP) */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
