@@ -486,6 +486,13 @@ void orc_thermal_conduct(const orc_grid *g, int isp, const double *Ts, const dou
 /* ---- LAPACK dgbsv restated (Netlib reference semantics) ----
    ab is the LAPACK band array AB(ldab=2*kl+ku+1, n), column-major, 1-based in the
    macros below; b is the single right-hand side.  Returns INFO. */
+/* BLAS arithmetic of the band LU and solve: 0 (default) the Netlib reference BLAS, every
+   product and sum rounded separately; 1 fused multiply-add in the dger (factorisation and forward solve) and dtbsv updates, as
+   optimised BLAS libraries (OpenBLAS on FMA-capable x86) evaluate them.  Mode 1 exists
+   only so the tests can reproduce such a library bit for bit. */
+static int blas_fma = 0;
+void orc_set_blas_fma(int mode) { blas_fma = mode; }
+
 #define AB(i, j) ab[((i) - 1) + (long)ldab * ((j) - 1)]
 static int dgbtf2_(int n, int kl, int ku, double *ab, int ldab, int *ipiv) {
   int info = 0;
@@ -522,7 +529,9 @@ static int dgbtf2_(int n, int kl, int ku, double *ab, int ldab, int *ipiv) {
             if (y != 0.0) {
               const double temp = -1.0 * y;
               for (int i = 1; i <= km; ++i)
-                AB(kv + 1 + i - k, j + k) = AB(kv + 1 + i - k, j + k) + AB(kv + 1 + i, j) * temp;
+                AB(kv + 1 + i - k, j + k) =
+                    blas_fma ? fma(AB(kv + 1 + i, j), temp, AB(kv + 1 + i - k, j + k))
+                             : AB(kv + 1 + i - k, j + k) + AB(kv + 1 + i, j) * temp;
             }
           }
       }
@@ -544,7 +553,9 @@ static void dgbtrs_n_(int n, int kl, int ku, const double *ab, int ldab, const i
       const double y = b[j - 1];                            /* dger(lm, 1, -ONE, ...) */
       if (y != 0.0) {
         const double temp = -1.0 * y;
-        for (int i = 1; i <= lm; ++i) b[j + i - 1] = b[j + i - 1] + AB(kd + i, j) * temp;
+        for (int i = 1; i <= lm; ++i)
+          b[j + i - 1] = blas_fma ? fma(AB(kd + i, j), temp, b[j + i - 1])
+                                  : b[j + i - 1] + AB(kd + i, j) * temp;
       }
     }
   const int k = kl + ku, kplus1 = k + 1;                   /* dtbsv('U','N','N') */
@@ -554,7 +565,7 @@ static void dgbtrs_n_(int n, int kl, int ku, const double *ab, int ldab, const i
       b[j - 1] = b[j - 1] / AB(kplus1, j);
       const double temp = b[j - 1];
       for (int i = j - 1; i >= (1 > j - k ? 1 : j - k); --i)
-        b[i - 1] = b[i - 1] - temp * AB(l + i, j);
+        b[i - 1] = blas_fma ? fma(-temp, AB(l + i, j), b[i - 1]) : b[i - 1] - temp * AB(l + i, j);
     }
   }
 }

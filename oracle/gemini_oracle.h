@@ -85,6 +85,9 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
 /* 0: x**2.5 by libm pow, as the reference (default); 1: correctly rounded x**2.5 */
 void orc_set_pow_mode(int mode);
 void orc_eval_pow25(const double *x, double *y, long n, int mode);
+/* 0: Netlib reference BLAS arithmetic in the band solve (default); 1: fused multiply-add
+   updates as optimised BLAS libraries evaluate them */
+void orc_set_blas_fma(int mode);
 
 /* threads used by the OpenMP loops (1 when built without OpenMP) */
 int orc_num_threads(void);
