@@ -101,7 +101,20 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (3 launches) to calibrate the "
+                         "FETCH_SIZE/WRITE_SIZE counters, then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 27                                   # 2^27 doubles = 1 GiB
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the transport "
               f"path shards over x2/x3 across GPUs by design, but this build runs on one.",

@@ -31,7 +31,11 @@ import numpy as np
 from .grid import Grid
 
 LSP = 7
-MINDENS = 1.0e-100  # phys_consts.in.f90:82
+MINDENS = 1.0e-100
+# every ion species carries a 10 m^-3 floor (negligible against 1e9-1e12 m^-3 plasma), so
+# no species is so rare that ratios of advected quantities (vs1 = rhovs1/(ms ns)) run away
+# in a transport-only run without the reference's source and clean-up substeps
+BACKGROUND = 10.0  # phys_consts.in.f90:82
 
 
 @dataclass
@@ -136,7 +140,7 @@ def synthetic_state(g: Grid, seed: int = 0, perturb: bool = False) -> State:
         noise = rng.standard_normal(sh)
         noise *= 0.05
         np.exp(noise, out=noise)
-        noise *= prof[isp][None, :, :]
+        noise *= (prof[isp] + BACKGROUND)[None, :, :]
         np.maximum(noise, MINDENS, out=FV[isp])
     if perturb:
         _gaussian_perturbations(g, rng, FV)

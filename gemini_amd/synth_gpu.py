@@ -18,6 +18,10 @@ from .grid import Grid
 
 LSP = 7
 MINDENS = 1.0e-100
+# every ion species carries a 10 m^-3 floor (negligible against 1e9-1e12 m^-3 plasma), so
+# no species is so rare that ratios of advected quantities (vs1 = rhovs1/(ms ns)) run away
+# in a transport-only run without the reference's source and clean-up substeps
+BACKGROUND = 10.0
 
 
 def _chapman(z, zpk, H):
@@ -71,7 +75,7 @@ def synthetic_state_device(g: Grid, seed: int = 0, device="cuda"):
             nF * fO * fH + plasmasphere]
     for isp in range(6):
         noise = torch.randn(sh, generator=gen, device=dev, dtype=f64)
-        noise.mul_(0.05).exp_().mul_(prof[isp][None])
+        noise.mul_(0.05).exp_().mul_((prof[isp] + BACKGROUND)[None])
         torch.clamp_min(noise, MINDENS, out=FV[isp])
         del noise
     torch.sum(FV[0:6], dim=0, out=FV[6])
