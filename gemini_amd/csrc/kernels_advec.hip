@@ -251,69 +251,91 @@ __global__ void k_save_v1b(DevGrid g, DevState s) {
 }
 
 // ---------------------------------------------------------------- x1 sweep
-// advec.f90:283-309 + :384-435 for one (x2,x3) line per workgroup, all species.
-// LDS holds the line (f), the one-sided slopes, the limited slopes, the fluxes and the
-// per-line metric products; each phase is one pass of the reference's loops.
-__global__ __launch_bounds__(256) void k_sweep1(DevGrid g, DevState s, int fields, double dt,
+// advec.f90:283-309 + :384-435.  Workgroup = one (x2,x3) line, wave w = species w+1.
+// The wave walks the line in chunks of 64 consecutive cells, lane l at position
+// p = 64c + l - 1, so every load and store is a coalesced 64-lane access.  From f(p-1),
+// f(p), f(p+1) a lane forms the one-sided slope rs(p), the limited slope sl(p) and the
+// wall flux phi(p); rs(p-1), sl(p-1) and phi(p-1) come from the neighbouring lane by a
+// wave shuffle (lane 0 takes them from lane 63 of the previous chunk), and the lane
+// writes the updated cell p-1.  Reads of a chunk all precede its writes, and a chunk only
+// writes cells the next chunk no longer reads, so the in-place update sees original data.
+__device__ __forceinline__ double shfl_up1(double v, double carry, int lane) {
+  const double u = __shfl_up(v, 1, 64);
+  return lane == 0 ? carry : u;
+}
+
+__global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, int fields, double dt,
                                                 int onthefly) {
-  extern __shared__ double sh[];
-  const int lx1 = g.lx1, L1 = g.L1;
+  const int lx1 = g.lx1;
   const int i2 = blockIdx.x + 1, i3 = blockIdx.y + 1;
-  double* sf = sh;               // f(-1..lx1+2)        -> sf[k+1]
-  double* srs = sf + L1;         // rslope(-1..lx1+1)    -> srs[k+1]
-  double* ssl = srs + L1;        // slope(0..lx1+1)      -> ssl[k]
-  double* sph = ssl + L1;        // phi(1..lx1+1)        -> sph[k]
-  double* sv = sph + L1;         // v1i(1..lx1+1)        -> sv[k]
-  double* svq = sv + L1;         // v1i/h1i              -> svq[k]
-  double* sha1 = svq + L1;       // (h1*h2)*h3 (1..lx1)  -> sha1[k]
-  double* sha2 = sha1 + L1;      // h2x1i*h3x1i          -> sha2[k]
-  double* sh1i = sha2 + L1;      // h1x1i                -> sh1i[k]
-  const long c0 = gix(g, 0, i2, i3);            // cell (0,i2,i3)
-  const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;  // iface k -> if0+k
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int k = tid + 1; k <= lx1 + 1; k += nt) {
-    if (k <= lx1) sha1[k] = (g.h1[c0 + k] * g.h2[c0 + k]) * g.h3[c0 + k];
-    sha2[k] = g.h2x1i[if0 + k] * g.h3x1i[if0 + k];
-    sh1i[k] = g.h1x1i[if0 + k];
-  }
+  const int lane = threadIdx.x, isp = threadIdx.y + 1;
+  const long c0 = gix(g, 0, i2, i3);                                       // cell p -> c0+p
+  const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;  // iface p -> if0+p
+  const long so = (long)(isp - 1) * g.N;
   const long nif = (long)(lx1 + 1) * g.lx2 * g.lx3;
-  for (int isp = 1; isp <= LSP; ++isp) {
-    const long so = (long)(isp - 1) * g.N;
-    __syncthreads();
-    for (int k = tid + 1; k <= lx1 + 1; k += nt) {
-      double v;
-      if (onthefly) v = v1i_at(g, s.vs1 + so + c0, k);
-      else v = s.vs1i[(long)(isp - 1) * nif + if0 + k];
-      sv[k] = v;
-      svq[k] = v / sh1i[k];
+  const double* u = s.vs1 + so + c0;                                        // u[p] = vs1(p)
+  double* fp[3];
+  bool act[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    fp[q] = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;
+    act[q] = (fields & (1 << q)) && (q == 2 || isp < LSP);
+  }
+  double crs[3] = {0, 0, 0}, csl[3] = {0, 0, 0}, cph[3] = {0, 0, 0};  // carries from lane 63
+  double ch2 = 0.0;
+  const int nchunk = (lx1 + 3 + 63) / 64;                                   // p = -1 .. lx1+1
+  for (int c = 0; c < nchunk; ++c) {
+    const int p = 64 * c + lane - 1;
+    const bool inl = p <= lx1 + 1;
+    // interface quantities at wall p (1..lx1+1) and metric products
+    double v = 0.0, vq = 0.0, h2p = 0.0, ha1m = 1.0, dxp = 0.0, dxim = 1.0;
+    const bool wall = p >= 1 && p <= lx1 + 1;
+    if (wall) {
+      v = onthefly ? v1i_at(g, u, p) : s.vs1i[(long)(isp - 1) * nif + if0 + p];
+      vq = v / g.h1x1i[if0 + p];
+      h2p = g.h2x1i[if0 + p] * g.h3x1i[if0 + p];
+      dxp = g.dx1[p];
     }
+    if (p >= 2 && p <= lx1 + 1) {
+      const long cm = c0 + p - 1;
+      ha1m = (g.h1[cm] * g.h2[cm]) * g.h3[cm];
+      dxim = g.dx1i[p - 2];
+    }
+    const double h2m = shfl_up1(h2p, ch2, lane);
+    ch2 = __shfl(h2p, 63, 64);
+    const double dx_p = inl ? g.dx1[p + 1] : 1.0;               // dx(p+1)
+    const double dx_0 = (inl && p >= 0) ? g.dx1[p] : 1.0;       // dx(p)
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const int bit = 1 << q;
-      if (!(fields & bit)) continue;
-      if (q < 2 && isp == LSP) continue;            // mass, momentum: species 1..6
-      double* f = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;  // f[k] = f(k)
-      __syncthreads();
-      for (int k = tid - 1; k <= lx1 + 2; k += nt) sf[k + 1] = f[k];
-      __syncthreads();
-      for (int k = tid - 1; k <= lx1 + 1; k += nt)
-        srs[k + 1] = (sf[k + 2] - sf[k + 1]) / g.dx1[k + 1];
-      __syncthreads();
-      for (int k = tid; k <= lx1 + 1; k += nt) {
-        const double cs = (sf[k + 2] - sf[k]) / (g.dx1[k] + g.dx1[k + 1]);
-        ssl[k] = minmod(cs, minmod(2.0 * srs[k], 2.0 * srs[k + 1]));
+      if (!act[q]) continue;
+      const double* f = fp[q];
+      double fm1 = 0.0, f0 = 0.0, f1 = 0.0;
+      if (inl) {
+        f0 = f[p];
+        f1 = f[p + 1];
+        if (p >= 0) fm1 = f[p - 1];
       }
-      __syncthreads();
-      for (int k = tid + 1; k <= lx1 + 1; k += nt) {
-        const double v = sv[k];
-        if (v < 0.0)
-          sph[k] = sf[k + 1] * v - ((0.5 * v) * (g.dx1[k] + svq[k] * dt)) * ssl[k];
-        else
-          sph[k] = sf[k] * v + ((0.5 * v) * (g.dx1[k] - svq[k] * dt)) * ssl[k - 1];
+      const double rs = inl ? (f1 - f0) / dx_p : 0.0;            // rs(p)
+      const double rsm = shfl_up1(rs, crs[q], lane);             // rs(p-1)
+      double sl = 0.0;
+      if (inl && p >= 0) {
+        const double cs = (f1 - fm1) / (dx_0 + dx_p);
+        sl = minmod(cs, minmod(2.0 * rsm, 2.0 * rs));
       }
-      __syncthreads();
-      for (int k = tid + 1; k <= lx1; k += nt)
-        f[k] = sf[k + 1] - ((dt * (sha2[k + 1] * sph[k + 1] - sha2[k] * sph[k])) / g.dx1i[k - 1])
-                               / sha1[k];
+      const double slm = shfl_up1(sl, csl[q], lane);             // sl(p-1)
+      double ph = 0.0;
+      if (wall) {
+        if (v < 0.0) ph = f0 * v - ((0.5 * v) * (dxp + vq * dt)) * sl;
+        else ph = fm1 * v + ((0.5 * v) * (dxp - vq * dt)) * slm;
+      }
+      const double phm = shfl_up1(ph, cph[q], lane);             // phi(p-1)
+      if (p >= 2 && p <= lx1 + 1) {
+        const double out = fm1 - ((dt * (h2p * ph - h2m * phm)) / dxim) / ha1m;
+        fp[q][p - 1] = out;
+      }
+      crs[q] = __shfl(rs, 63, 64);
+      csl[q] = __shfl(sl, 63, 64);
+      cph[q] = __shfl(ph, 63, 64);
     }
   }
 }
@@ -328,9 +350,11 @@ template <int DIR>
 __global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fields, double dt,
                                                  int onthefly) {
   const int lx = DIR == 2 ? g.lx2 : g.lx3;
-  const int i1 = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  // lanes past lx1 stay in the loop (it holds block barriers): they address column lx1,
+  // which keeps their loads in bounds, and write nothing
+  const bool live = (int)(blockIdx.x * blockDim.x + threadIdx.x) < g.lx1;
+  const int i1 = live ? (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 : g.lx1;
   const int isp = threadIdx.y + 1;
-  if (i1 > g.lx1) return;
   const int other = blockIdx.y + 1;                  // i3 for DIR 2, i2 for DIR 3
   const long cs_ = DIR == 2 ? (long)g.L1 : (long)g.L1 * g.L2;   // cell stride along the line
   const long c0 = DIR == 2 ? gix(g, i1, 0, other) : gix(g, i1, other, 0);  // cell k -> c0+k*cs_
@@ -352,7 +376,7 @@ __global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fiel
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     fp[q] = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;
-    act[q] = (fields & (1 << q)) && (q == 2 || isp < LSP);
+    act[q] = live && (fields & (1 << q)) && (q == 2 || isp < LSP);
   }
   double fa[3], fb[3], fc[3], rsp[3], slp[3], php[3];
 #pragma unroll
@@ -416,6 +440,9 @@ __global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fiel
       if (k + 2 <= lx + 2) fc[q] = fp[q][(long)(k + 2) * cs_];
     }
     if (k >= 1) { h2p0 = h2f0; h2p1 = h2f1; }
+    // keep the seven species waves of the block in step so the metric factors they all
+    // read are still cached when the slower waves reach them
+    if ((k & 7) == 7) __syncthreads();
   }
 }
 
@@ -501,10 +528,8 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
                   bool onthefly, hipStream_t st) {
   if (dir == 1) {
     if (g.lx1 <= 1) return;
-    const size_t lds = sizeof(double) * 9 * (size_t)g.L1;
-    if (lds > 65536) return;                 // rejected by gemhip_set_grid_size
     timing_begin("sweep1", st);
-    hipLaunchKernelGGL(k_sweep1, dim3(g.lx2, g.lx3), dim3(256), lds, st, g, s, fields, dt,
+    hipLaunchKernelGGL(k_sweep1, dim3(g.lx2, g.lx3), dim3(64, LSP), 0, st, g, s, fields, dt,
                        (int)onthefly);
     timing_end("sweep1", st);
   } else if (dir == 2) {
