@@ -41,10 +41,10 @@ WORKLOADS = {
 # metric arrays read; TRBDF2: Ts, ns (7 species each), h1..h3, nn(1:3), J1 read, Ts
 # written; conversions over the padded grid.
 BYTES_PER_CELL = {"sweep1": 408, "sweep2": 408, "sweep3": 408, "trbdf2": 224,
-                  "pre_convert": 280, "rhov12v1": 160, "rhoe2T": 168}
-PADDED_KERNELS = {"pre_convert", "rhov12v1", "rhoe2T"}
+                  "pre_convert": 280, "post_convert": 272, "rhov12v1": 160, "rhoe2T": 168}
+PADDED_KERNELS = {"pre_convert", "post_convert", "rhov12v1", "rhoe2T"}
 KERNELS = ["pre_convert", "ghost_fill", "halo_periodic", "save_v1b", "sweep3", "sweep1",
-           "sweep2", "rhov12v1", "rhoe2T", "trbdf2", "floor_ghosts"]
+           "sweep2", "post_convert", "rhov12v1", "rhoe2T", "trbdf2", "floor_ghosts"]
 
 
 def log(*a):

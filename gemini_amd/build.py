@@ -26,11 +26,15 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source into the shared library (always recompiles)."""
+def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
+    """Compile every source into the shared library (always recompiles).
+
+    ``defines`` are extra ``NAME=VALUE`` preprocessor settings (tile sizes, pipeline
+    depths) for tuning builds written to ``out``."""
     cmd = [hipcc(), "-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17",
            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-result", "-o", LIB + ".tmp"]
+           "-Wall", "-Wno-unused-result", "-o", out + ".tmp"]
+    cmd += [f"-D{d}" for d in defines]
     cmd += [os.path.join(CSRC, s) for s in SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -38,8 +42,8 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libgemhip.so")
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -354,10 +354,8 @@ void gemhip_clear_error(void) { ctx().last_error.clear(); }
 int gemhip_set_grid_size(int lx1, int lx2, int lx3) {
   Ctx& c = ctx();
   if (!c.inited) { fail("library not initialised (gemhip_init)"); return 1; }
-  // lx1 <= 906 keeps one x1 line's sweep working set (9 arrays of lx1+4 doubles) in the
-  // 64 KiB of LDS a workgroup may address
-  if (lx1 < 4 || lx1 > 906 || lx2 < 1 || lx3 < 1) {
-    fail("grid size %d x %d x %d not supported (4 <= lx1 <= 906, lx2, lx3 >= 1)", lx1, lx2,
+  if (lx1 < 4 || lx2 < 1 || lx3 < 1) {
+    fail("grid size %d x %d x %d not supported (lx1 >= 4, lx2, lx3 >= 1)", lx1, lx2,
          lx3);
     return 2;
   }
@@ -559,8 +557,7 @@ int gemhip_transport_step(double dt) {
   launch_sweep(g, s, 1, F_ALL, dt, true, st);            // :178-180
   halo_fields({NS, RHOVS1, RHOES});                      // :181
   launch_sweep(g, s, 2, F_ALL, dt, true, st);            // :183-185
-  launch_rhov12v1(g, s, st);                             // :186
-  launch_rhoe2T(g, s, st);                               // multifluid.f90:179-189
+  launch_post_convert(g, s, st);                         // :186, then multifluid.f90:179-189
   if (!hipok(hipGetLastError(), "advection launch")) return 3;
   c.vsi_current = false;
   if (!run_energy_diffusion(dt)) return 4;               // :208

@@ -46,6 +46,7 @@ void launch_v12rhov1_T2rhoe(const DevGrid& g, const DevState& s, bool mom, bool 
                             hipStream_t st);
 void launch_rhov12v1(const DevGrid& g, const DevState& s, hipStream_t st);
 void launch_rhoe2T(const DevGrid& g, const DevState& s, hipStream_t st);
+void launch_post_convert(const DevGrid& g, const DevState& s, hipStream_t st);  // both above
 void launch_set_global_boundaries(const DevGrid& g, const DevState& s, int lsp, hipStream_t st);
 void launch_halo_periodic(const DevGrid& g, double* const* fields, int nfields, int lhalo,
                           int nspec, hipStream_t st);

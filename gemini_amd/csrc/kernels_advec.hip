@@ -73,6 +73,36 @@ __global__ void k_rhoe2T(DevGrid g, DevState s) {
   }
 }
 
+// rhov12v1 then rhoe2T of every padded cell in one pass: both are pointwise, so doing
+// them cell by cell gives what the two whole-array passes give, with ns read once.
+__global__ void k_post_convert(DevGrid g, DevState s) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < g.N;
+       k += (long)gridDim.x * blockDim.x) {
+    double chrgflux = 0.0, sum = 0.0;
+    double n[LSP];
+#pragma unroll
+    for (int isp = 1; isp <= LSP - 1; ++isp) {
+      const long o = (long)(isp - 1) * g.N + k;
+      n[isp - 1] = s.ns[o];
+      const double v = s.rhovs1[o] / (ms(isp) * fmax_f(n[isp - 1], mindensdiv));
+      s.vs1[o] = v;
+      chrgflux = chrgflux + (n[isp - 1] * qs(isp)) * v;
+    }
+#pragma unroll
+    for (int isp = 1; isp <= LSP - 1; ++isp) sum = sum + n[isp - 1];
+    n[LSP - 1] = sum;
+    const long o7 = (long)(LSP - 1) * g.N + k;
+    s.ns[o7] = sum;
+    s.vs1[o7] = -(((1.0 / fmax_f(sum, mindensdiv)) / qs(LSP)) * chrgflux);
+#pragma unroll
+    for (int isp = 1; isp <= LSP; ++isp) {
+      const long o = (long)(isp - 1) * g.N + k;
+      const double t = (((gammas(isp) - 1.0) / kB) * s.rhoes[o]) / fmax_f(n[isp - 1], mindensdiv);
+      s.Ts[o] = fmax_f(t, 100.0);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- global ghost fill
 // advec.f90:42-79: x1 ends, one thread per padded (x2,x3) column and species
 __global__ void k_bc_x1(DevGrid g, DevState s) {
@@ -340,13 +370,22 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, int field
   }
 }
 
+#ifndef SWEEP_PF
+#define SWEEP_PF 4      // x2/x3 march pipeline depth (steps of loads in flight)
+#endif
+
 // ---------------------------------------------------------------- x2 / x3 sweeps
 // advec.f90:313-381 + :384-435.  Thread (i1, line, species): marches along its line
 // keeping f(k-1), f(k), f(k+1), the last one-sided slope, limited slope and flux in
 // registers for up to three fields (ns, rhovs1, rhoes) that share the velocity and
 // metric factors.  frank 1 (momentum) uses h1^2 in both metric products
 // (advec.f90:336-338, 372-374).
-template <int DIR>
+//
+// The march is software-pipelined PF steps deep: step k consumes the field values,
+// velocity and metric factors loaded PF steps earlier and issues the loads of step
+// k+PF.  Field loads run ahead of the in-place stores (step k stores cell k-1 and loads
+// cell k+2+PF, which no earlier step has written), so every value read is the original.
+template <int DIR, int PF>
 __global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fields, double dt,
                                                  int onthefly) {
   const int lx = DIR == 2 ? g.lx2 : g.lx3;
@@ -378,6 +417,22 @@ __global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fiel
     fp[q] = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;
     act[q] = live && (fields & (1 << q)) && (q == 2 || isp < LSP);
   }
+  // pipeline registers of the steps in flight
+  double pf[3][PF], pv[PF], pI[PF], pA[PF], pB[PF], p1[PF], p2[PF], p3[PF];
+  auto issue = [&](int k, int u) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (act[q] && k + 2 <= lx + 2) pf[q][u] = fp[q][(long)(k + 2) * cs_];
+    if (k >= 1 && k <= lx + 1) {
+      const long ik = if0 + (long)k * istr;
+      pv[u] = onthefly ? w[(long)k * cs_] : vi[ik];
+      pI[u] = hI[ik]; pA[u] = hA[ik]; pB[u] = hB[ik];
+      if (k >= 2) {
+        const long cm = c0 + (long)(k - 1) * cs_;
+        p1[u] = g.h1[cm]; p2[u] = g.h2[cm]; p3[u] = g.h3[cm];
+      }
+    }
+  };
   double fa[3], fb[3], fc[3], rsp[3], slp[3], php[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -389,76 +444,103 @@ __global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fiel
     rsp[q] = (fb[q] - fa[q]) / dx[0];
   }
   double wprev = w[0];         // vs(0) for the on-the-fly interface velocity
-  double h2p0 = 0.0, h2p1 = 0.0;   // ha2i(k-1) for frank 0 / frank 1
-  for (int k = 0; k <= lx + 1; ++k) {
-    const double dxk = dx[k], dxk1 = dx[k + 1];
-    double v = 0.0, vq = 0.0, h2f0 = 0.0, h2f1 = 0.0, ha1f0 = 1.0, ha1f1 = 1.0, dxim = 1.0;
-    if (k >= 1) {
-      const long ik = if0 + (long)k * istr;
-      if (onthefly) {
-        const double wk = w[(long)k * cs_];
-        v = 0.5 * (wprev + wk);
-        wprev = wk;
-      } else {
-        v = vi[ik];
-      }
-      vq = v / hI[ik];
-      const double a = hA[ik], b = hB[ik];
-      h2f0 = a * b;
-      h2f1 = (a * a) * b;
-      if (k >= 2) {
-        const long cm = c0 + (long)(k - 1) * cs_;
-        const double h1 = g.h1[cm], h2 = g.h2[cm], h3 = g.h3[cm];
-        ha1f0 = (h1 * h2) * h3;
-        ha1f1 = ((h1 * h1) * h2) * h3;
-        dxim = dxi[k - 2];
-      }
-    }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (!act[q]) continue;
-      const double rs = (fc[q] - fb[q]) / dxk1;
-      const double csl = (fc[q] - fa[q]) / (dxk + dxk1);
-      const double sl = minmod(csl, minmod(2.0 * rsp[q], 2.0 * rs));
+  for (int u = 0; u < PF; ++u) issue(u, u);
+  double h2p0 = 0.0, h2p1 = 0.0;   // ha2i(k-1) for frank 0 / frank 1
+  for (int kb = 0; kb <= lx + 1; kb += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int k = kb + u;
+      if (k > lx + 1) break;
+      const double dxk = dx[k], dxk1 = dx[k + 1];
+      double v = 0.0, vq = 0.0, h2f0 = 0.0, h2f1 = 0.0, ha1f0 = 1.0, ha1f1 = 1.0, dxim = 1.0;
       if (k >= 1) {
-        double ph;
-        if (v < 0.0) ph = fb[q] * v - ((0.5 * v) * (dxk + vq * dt)) * sl;
-        else ph = fa[q] * v + ((0.5 * v) * (dxk - vq * dt)) * slp[q];
-        const bool fr1 = (q == 1);
-        const double h2k = fr1 ? h2f1 : h2f0;
-        if (k >= 2) {
-          const double h2m = fr1 ? h2p1 : h2p0;
-          const double ha1 = fr1 ? ha1f1 : ha1f0;
-          fp[q][(long)(k - 1) * cs_] = fa[q] - ((dt * (h2k * ph - h2m * php[q])) / dxim) / ha1;
+        if (onthefly) {
+          v = 0.5 * (wprev + pv[u]);
+          wprev = pv[u];
+        } else {
+          v = pv[u];
         }
-        php[q] = ph;
+        vq = v / pI[u];
+        const double a = pA[u], b = pB[u];
+        h2f0 = a * b;
+        h2f1 = (a * a) * b;
+        if (k >= 2) {
+          const double h1 = p1[u], h2 = p2[u], h3 = p3[u];
+          ha1f0 = (h1 * h2) * h3;
+          ha1f1 = ((h1 * h1) * h2) * h3;
+          dxim = dxi[k - 2];
+        }
       }
-      slp[q] = sl;
-      rsp[q] = rs;
-      fa[q] = fb[q];
-      fb[q] = fc[q];
-      if (k + 2 <= lx + 2) fc[q] = fp[q][(long)(k + 2) * cs_];
+      double nxt[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) nxt[q] = pf[q][u];
+      issue(k + PF, u);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        if (!act[q]) continue;
+        const double rs = (fc[q] - fb[q]) / dxk1;
+        const double csl = (fc[q] - fa[q]) / (dxk + dxk1);
+        const double sl = minmod(csl, minmod(2.0 * rsp[q], 2.0 * rs));
+        if (k >= 1) {
+          double ph;
+          if (v < 0.0) ph = fb[q] * v - ((0.5 * v) * (dxk + vq * dt)) * sl;
+          else ph = fa[q] * v + ((0.5 * v) * (dxk - vq * dt)) * slp[q];
+          const bool fr1 = (q == 1);
+          const double h2k = fr1 ? h2f1 : h2f0;
+          if (k >= 2) {
+            const double h2m = fr1 ? h2p1 : h2p0;
+            const double ha1 = fr1 ? ha1f1 : ha1f0;
+            fp[q][(long)(k - 1) * cs_] = fa[q] - ((dt * (h2k * ph - h2m * php[q])) / dxim) / ha1;
+          }
+          php[q] = ph;
+        }
+        slp[q] = sl;
+        rsp[q] = rs;
+        fa[q] = fb[q];
+        fb[q] = fc[q];
+        fc[q] = nxt[q];
+      }
+      if (k >= 1) { h2p0 = h2f0; h2p1 = h2f1; }
     }
-    if (k >= 1) { h2p0 = h2f0; h2p1 = h2f1; }
     // keep the seven species waves of the block in step so the metric factors they all
     // read are still cached when the slower waves reach them
-    if ((k & 7) == 7) __syncthreads();
+    if (((kb / PF) & 1) == 1) __syncthreads();
   }
 }
 
 // Ts ghost cells: floor at 100 K (the reference leaves them undefined, see the
-// diffusion kernel); interior columns only have x1 ghosts.
+// diffusion kernel).  One thread per ghost cell: first the four x1 ghosts of every
+// interior column, then every cell of the columns with an x2 or x3 ghost index.
+__device__ __forceinline__ int ghost_index(int j, int lx) { return j < 2 ? j - 1 : lx + j - 1; }
+
 __global__ void k_floor_ghosts(DevGrid g, double* Ts) {
-  const int isp = blockIdx.y + 1;
+  double* T = Ts + (long)blockIdx.y * g.N;
+  const long n1 = 4L * g.lx2 * g.lx3;
+  const long ncol = 4L * g.L2 + 4L * g.lx3;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.L2 * g.L3) return;
-  const int i2 = (int)(t % g.L2) - 1, i3 = (int)(t / g.L2) - 1;
-  double* T = Ts + (long)(isp - 1) * g.N + gix(g, -1, i2, i3);
-  const bool inner = i2 >= 1 && i2 <= g.lx2 && i3 >= 1 && i3 <= g.lx3;
-  for (int k = 0; k < g.L1; ++k) {
-    if (inner && k >= 2 && k < g.lx1 + 2) continue;
-    T[k] = fmax_f(T[k], 100.0);
+  long c;
+  if (t < n1) {
+    const long col = t / 4;
+    c = gix(g, ghost_index((int)(t % 4), g.lx1), (int)(col % g.lx2) + 1, (int)(col / g.lx2) + 1);
+  } else if (t < n1 + ncol * g.L1) {
+    const long r = t - n1;
+    const int i1 = (int)(r % g.L1) - 1;
+    const long col = r / g.L1;
+    int i2, i3;
+    if (col < 4L * g.L2) {
+      i2 = (int)(col % g.L2) - 1;
+      i3 = ghost_index((int)(col / g.L2), g.lx3);
+    } else {
+      const long cc = col - 4L * g.L2;
+      i3 = (int)(cc % g.lx3) + 1;
+      i2 = ghost_index((int)(cc / g.lx3), g.lx2);
+    }
+    c = gix(g, i1, i2, i3);
+  } else {
+    return;
   }
+  T[c] = fmax_f(T[c], 100.0);
 }
 
 }  // namespace
@@ -476,6 +558,12 @@ void launch_rhov12v1(const DevGrid& g, const DevState& s, hipStream_t st) {
   timing_begin("rhov12v1", st);
   hipLaunchKernelGGL(k_rhov12v1, dim3(blocks_for(g.N)), dim3(TPB), 0, st, g, s);
   timing_end("rhov12v1", st);
+}
+
+void launch_post_convert(const DevGrid& g, const DevState& s, hipStream_t st) {
+  timing_begin("post_convert", st);
+  hipLaunchKernelGGL(k_post_convert, dim3(blocks_for(g.N)), dim3(TPB), 0, st, g, s);
+  timing_end("post_convert", st);
 }
 
 void launch_rhoe2T(const DevGrid& g, const DevState& s, hipStream_t st) {
@@ -535,21 +623,21 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
   } else if (dir == 2) {
     if (g.lx2 <= 1) return;
     timing_begin("sweep2", st);
-    hipLaunchKernelGGL(k_sweep23<2>, dim3((g.lx1 + 63) / 64, g.lx3), dim3(64, LSP), 0, st, g, s,
+    hipLaunchKernelGGL((k_sweep23<2, SWEEP_PF>), dim3((g.lx1 + 63) / 64, g.lx3), dim3(64, LSP), 0, st, g, s,
                        fields, dt, (int)onthefly);
     timing_end("sweep2", st);
   } else {
     if (g.lx3 <= 1) return;
     timing_begin("sweep3", st);
-    hipLaunchKernelGGL(k_sweep23<3>, dim3((g.lx1 + 63) / 64, g.lx2), dim3(64, LSP), 0, st, g, s,
+    hipLaunchKernelGGL((k_sweep23<3, SWEEP_PF>), dim3((g.lx1 + 63) / 64, g.lx2), dim3(64, LSP), 0, st, g, s,
                        fields, dt, (int)onthefly);
     timing_end("sweep3", st);
   }
 }
 
 void launch_floor_ghosts(const DevGrid& g, double* Ts, hipStream_t st) {
-  hipLaunchKernelGGL(k_floor_ghosts, dim3(blocks_for((long)g.L2 * g.L3), LSP), dim3(TPB), 0, st, g,
-                     Ts);
+  const long n = 4L * g.lx2 * g.lx3 + (4L * g.L2 + 4L * g.lx3) * g.L1;
+  hipLaunchKernelGGL(k_floor_ghosts, dim3(blocks_for(n), LSP), dim3(TPB), 0, st, g, Ts);
 }
 
 }  // namespace gemhip

@@ -37,29 +37,36 @@ struct Elim {
 };
 
 // ---------------------------------------------------------------- column-group kernel
-// One wavefront owns 64 columns of one species.  The input rows arrive in chunks of R
-// rows: the wave loads the chunk of all 64 columns cooperatively (lane -> (row, column),
-// so every load instruction reads whole 128-byte segments), computes the conductivity
-// and the coefficients C, B, D of each (row, column) in parallel, and leaves T, C, B, D
-// in LDS.  Each lane then runs the sequential pivoted elimination of its own column over
-// the chunk.  Outputs are staged through LDS the same way, so global stores are
-// coalesced too.  The LU rows, the per-row coefficients needed again by the BDF2 stage
-// and the TR solution live in the lane-interleaved scratch area.
+// One wavefront owns 64 columns of one species.  Each pass over the rows is one of:
+//   forward (per stage): the input rows arrive in chunks of R rows; the wave loads a
+//     chunk of all 64 columns cooperatively (lane -> (row, column), so every load
+//     instruction reads whole 128-byte segments), computes the conductivity and the
+//     coefficients C, B, D of each (row, column) in parallel into LDS, and each lane
+//     then runs the sequential pivoted elimination of its own column over the chunk;
+//   back substitution (per stage): each lane walks its column from row N to 1 with the
+//     factors of the next DIFF_PF rows already loaded.
+// The BDF2 stage recomputes its coefficients from the inputs (one more read of Ts, ns
+// and the metric factors) instead of keeping them from the TR stage.  Scratch per (row,
+// lane): the pivot row U0, U1, its right-hand side y, the TR solution and U2, the
+// pivoting fill-in, which a column only writes from its first row swap on.  Outputs
+// are staged through LDS so global stores are coalesced too.  Blocks are numbered so
+// that the seven species of a column group run on one XCD and share its L2.
 constexpr int LP = 65;         // padded LDS row (conflict-free transposed access)
 #ifndef DIFF_CHUNK
-#define DIFF_CHUNK 8       // rows per chunk: 4 x 8 x 65 doubles of LDS per wave
+#define DIFF_CHUNK 8           // rows per chunk: 4 x 8 x 65 doubles of LDS per wave
 #endif
-constexpr int NS2 = 9;         // scratch slots: U0 U1 U2 y t1f t2f t3f T TR
+#ifndef DIFF_PF
+#define DIFF_PF 8              // rows of factors in flight in the back substitutions
+#endif
+enum : int { SU0 = 0, SU1 = 1, SY = 2, STR = 3, SU2 = 4, NSLOT = 5 };
 
 struct Group {
-  int isp, lane, col, valid, N;
-  long cbase;                  // gix(0,i2,i3) of this lane's column
-  long kb;                     // interior index of (1,i2,i3)
-  double* scr;
+  int N;
+  double* scr;                 // this lane's scratch: slot k of row i at scr[((i-1)*NSLOT+k)*64]
 };
 
 __device__ __forceinline__ double& S2(const Group& q, int i, int k) {
-  return q.scr[((long)(i - 1) * NS2 + k) * 64];
+  return q.scr[((long)(i - 1) * NSLOT + k) * 64];
 }
 
 __device__ __forceinline__ void column_of(const DevGrid& g, int gcol, int& i2, int& i3) {
@@ -93,9 +100,10 @@ __device__ __forceinline__ void cell_coeffs(const DevGrid& g, const DevState& s,
   Dr = ((lambda * h2) * h3) / h1;
 }
 
-// pivoted elimination step j (dgbtf2 + forward dgbtrs) bringing in row j+1
-__device__ __forceinline__ void elim2(const Group& q, Elim& e, int j, double a, double b,
-                                      double cc, double d) {
+// pivoted elimination step j (dgbtf2 + forward dgbtrs) bringing in row j+1; fsw is the
+// first step of this column that swapped rows (U2 is stored from there on)
+__device__ __forceinline__ void elim2(const Group& q, Elim& e, int& fsw, int j, double a,
+                                      double b, double cc, double d) {
   const bool swap = fabs(a) > fabs(e.r0);
   double p0, p1, p2, pd, o0, o1, o2, od;
   if (swap) { p0 = a; p1 = b; p2 = cc; pd = d; o0 = e.r0; o1 = e.r1; o2 = 0.0; od = e.rb; }
@@ -109,29 +117,44 @@ __device__ __forceinline__ void elim2(const Group& q, Elim& e, int j, double a, 
   } else {
     e.singular = true;
   }
-  S2(q, j, 0) = p0; S2(q, j, 1) = p1; S2(q, j, 2) = p2; S2(q, j, 3) = pd;
+  if (swap && j < fsw) fsw = j;
+  S2(q, j, SU0) = p0; S2(q, j, SU1) = p1; S2(q, j, SY) = pd;
+  if (j >= fsw) S2(q, j, SU2) = p2;
   e.r0 = o1; e.r1 = o2; e.rb = od;
 }
 
-// dtbsv over the stored factors; out(i, x) for i = N..1; rhs(i) when singular
-template <typename OUT, typename RHS>
-__device__ __forceinline__ void backsub2(const Group& q, bool singular, OUT out, RHS rhs) {
+// dtbsv over the stored factors (column-oriented: row i takes x(i+2)*U2 off first, then
+// x(i+1)*U1, and divides by U0; a zero x(j) contributes nothing).  U2 is zero above the
+// first swap and is not read there.  A singular factorisation leaves the right-hand side,
+// which the caller has stored in the y slot, as the result (dgbsv skips dgbtrs).
+template <int PF, typename OUT>
+__device__ __forceinline__ void backsub2(const Group& q, bool singular, int fsw, OUT out) {
   const int N = q.N;
+  double pu0[PF], pu1[PF], py[PF];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    const int i = N - u;
+    if (i >= 1) { pu0[u] = S2(q, i, SU0); pu1[u] = S2(q, i, SU1); py[u] = S2(q, i, SY); }
+  }
   double x1 = 0.0, x2 = 0.0;
-#pragma unroll 4
-  for (int i = N; i >= 1; --i) {
-    double acc;
-    if (singular) {
-      acc = rhs(i);
-    } else {
-      acc = S2(q, i, 3);
-      if (i + 2 <= N && x2 != 0.0) acc = acc - x2 * S2(q, i, 2);
-      if (i + 1 <= N && x1 != 0.0) acc = acc - x1 * S2(q, i, 1);
-      if (acc != 0.0) acc = acc / S2(q, i, 0);
+  for (int ib = N; ib >= 1; ib -= PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int i = ib - u;
+      if (i < 1) break;
+      const double u0 = pu0[u], u1 = pu1[u], y = py[u];
+      const int ip = i - PF;
+      if (ip >= 1) { pu0[u] = S2(q, ip, SU0); pu1[u] = S2(q, ip, SU1); py[u] = S2(q, ip, SY); }
+      double acc = y;
+      if (!singular) {
+        if (i + 2 <= N && x2 != 0.0 && i >= fsw) acc = acc - x2 * S2(q, i, SU2);
+        if (i + 1 <= N && x1 != 0.0) acc = acc - x1 * u1;
+        if (acc != 0.0) acc = acc / u0;
+      }
+      out(i, acc);
+      x2 = x1;
+      x1 = acc;
     }
-    out(i, acc);
-    x2 = x1;
-    x1 = acc;
   }
 }
 
@@ -141,101 +164,115 @@ __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double*
                                                   double Teinf) {
   __shared__ double sT[R][LP], sC[R][LP], sB[R][LP], sD[R][LP];
   const long ncol = (long)g.lx2 * g.lx3;
-  const long gps = (ncol + 63) / 64;                 // groups per species
-  const long grp = blockIdx.x;
+  const long gps = (ncol + 63) / 64;                 // column groups per species
+  const long slot = blockIdx.x / 8;
+  const long cg = (slot / LSP) * 8 + blockIdx.x % 8;  // column group (same XCD for all species)
+  if (cg >= gps) return;
+  const int isp = (int)(slot % LSP) + 1;
   const int lane = threadIdx.x;
-  Group q;
-  q.isp = (int)(grp / gps) + 1;
-  q.lane = lane;
-  q.col = (int)((grp % gps) * 64 + lane);
-  q.valid = q.col < ncol;
-  q.N = g.lx1;
+  const int col = (int)(cg * 64 + lane);
+  const bool valid = col < ncol;
   const int N = g.lx1;
-  const long so = (long)(q.isp - 1) * g.N;
+  Group q;
+  q.N = N;
+  q.scr = scratch + ((long)(isp - 1) * gps + cg) * (long)N * NSLOT * 64 + lane;
+  const long so = (long)(isp - 1) * g.N;
   const long ni = (long)g.lx1 * g.lx2 * g.lx3;
-  const long colbase0 = (grp % gps) * 64;            // first column of the group
-  int i2, i3;
-  column_of(g, q.valid ? q.col : 0, i2, i3);
-  q.cbase = gix(g, 0, i2, i3);
-  q.kb = (long)g.lx1 * (q.valid ? q.col : 0);
-  q.scr = scratch + grp * (long)N * NS2 * 64 + lane;
-  const double gm1kB = (gammas(q.isp) - 1.0) / kB;
+  const long colbase0 = cg * 64;                     // first column of the group
+  const long kb = (long)g.lx1 * (valid ? col : 0);
+  const double gm1kB = (gammas(isp) - 1.0) / kB;
   const double* dx1 = g.dx1;
   const double* dx1i = g.dx1i - 1;
   double Tmin, Tmax;
   {
-    const double Tn1 = s.Tn[q.kb], TnN = s.Tn[q.kb + N - 1];
+    const double Tn1 = s.Tn[kb], TnN = s.Tn[kb + N - 1];
     if (g.gridflag == 0) { Tmin = Tn1; Tmax = TnN; }
     else if (g.gridflag == 1) { Tmax = TnN; Tmin = Teinf; }
     else { Tmin = Tn1; Tmax = Teinf; }
   }
-  const Coef& s1 = backeuler ? be : tr;
   // cooperative mapping of a chunk: lane -> (row lr, column sub-index cs)
   const int lr = lane % R, cs = lane / R;
   constexpr int CPI = 64 / R;                        // columns per load instruction
 
-  // ---------------- stage 1 forward: arrivals of rows 1..N ----------------
-  Elim e{1.0, 0.0, Tmin, false};
-  double Tm = 0.0, Tc = 0.0, Cc = 0.0, Bc = 0.0, Dm = 0.0, Dc = 0.0;
-  for (int r0 = 1; r0 <= N; r0 += R) {
-    __syncthreads();
-    for (int k = 0; k < 64 / CPI; ++k) {
-      const int cc = k * CPI + cs;
-      const long gcol = colbase0 + cc;
-      const int r = r0 + lr;
-      if (gcol < ncol && r <= N) {
-        int a2, a3;
-        column_of(g, (int)gcol, a2, a3);
-        double T, C_, B_, D_;
-        cell_coeffs(g, s, q.isp, so, gix(g, 0, a2, a3), (long)g.lx1 * gcol, ni, r, gm1kB, T, C_,
-                    B_, D_);
-        sT[lr][cc] = T; sC[lr][cc] = C_; sB[lr][cc] = B_; sD[lr][cc] = D_;
-      }
-    }
-    __syncthreads();
-    const int rend = min(R, N - r0 + 1);
-    for (int k = 0; k < rend; ++k) {
-      const int r = r0 + k;
-      const double Tp = sT[k][lane], Cp = sC[k][lane], Bp = sB[k][lane], Dp = sD[k][lane];
-      if (r >= 3) {
-        const int ro = r - 1;                       // row being finalised (2..N-1)
-        const double Dhr = 0.5 * (Dm + Dc), Dhr1 = 0.5 * (Dc + Dp);
-        const double t1f = ((Cc * Dhr) / dx1i[ro]) / dx1[ro];
-        const double t3f = ((Cc * Dhr1) / dx1i[ro]) / dx1[ro + 1];
-        const double t2f = Bc / (dx1[ro + 1] + dx1[ro]);
-        S2(q, ro, 4) = t1f; S2(q, ro, 5) = t2f; S2(q, ro, 6) = t3f; S2(q, ro, 7) = Tc;
-        double a, b, c, d;
-        if (backeuler) {
-          a = -t1f + t2f; b = (s1.K + t3f) + t1f; c = -t3f - t2f; d = Tc / s1.dt;
-        } else {
-          const double t1 = t1f / 2.0, t2 = t2f / 2.0, t3 = t3f / 2.0;
-          a = -t1 + t2; b = (s1.K + t3) + t1; c = -t3 - t2;
-          d = ((Tc / s1.dt2 - a * Tm) - (t3 + t1) * Tc) - c * Tp;
+  // Forward pass of stage 1 (TR, or backward Euler) or stage 2 (BDF2).  mode 0
+  // eliminates and stores the factors, mode 1 stores the stage's right-hand side in the
+  // y slot (for a singular factorisation), mode 2 (other lanes of that pass) does nothing.
+  auto forward = [&](int stage, int mode, Elim& e, int& fsw) {
+    const Coef& k1 = stage == 2 ? bdf : (backeuler ? be : tr);
+    e = Elim{1.0, 0.0, Tmin, false};
+    fsw = N + 1;
+    if (mode == 1) S2(q, 1, SY) = Tmin;
+    double Tm = 0.0, Tc = 0.0, Cc = 0.0, Bc = 0.0, Dm = 0.0, Dc = 0.0;
+    for (int r0 = 1; r0 <= N; r0 += R) {
+      double trv[R];                                 // TR of the rows finalised below
+      if (stage == 2) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int ro = r0 + k - 1;
+          trv[k] = (ro >= 2 && ro <= N - 1) ? S2(q, ro, STR) : 0.0;
         }
-        elim2(q, e, ro - 1, a, b, c, d);
       }
-      Tm = Tc; Tc = Tp;
-      Cc = Cp; Bc = Bp;
-      Dm = Dc; Dc = Dp;
+      __syncthreads();
+      for (int k = 0; k < 64 / CPI; ++k) {
+        const int cc = k * CPI + cs;
+        const long gcol = colbase0 + cc;
+        const int r = r0 + lr;
+        if (gcol < ncol && r <= N) {
+          int a2, a3;
+          column_of(g, (int)gcol, a2, a3);
+          double T, C_, B_, D_;
+          cell_coeffs(g, s, isp, so, gix(g, 0, a2, a3), (long)g.lx1 * gcol, ni, r, gm1kB, T, C_,
+                      B_, D_);
+          sT[lr][cc] = T; sC[lr][cc] = C_; sB[lr][cc] = B_; sD[lr][cc] = D_;
+        }
+      }
+      __syncthreads();
+      const int rend = min(R, N - r0 + 1);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        if (k >= rend) break;
+        const int r = r0 + k;
+        const double Tp = sT[k][lane], Cp = sC[k][lane], Bp = sB[k][lane], Dp = sD[k][lane];
+        if (r >= 3) {
+          const int ro = r - 1;                     // row being finalised (2..N-1)
+          const double Dhr = 0.5 * (Dm + Dc), Dhr1 = 0.5 * (Dc + Dp);
+          const double t1f = ((Cc * Dhr) / dx1i[ro]) / dx1[ro];
+          const double t3f = ((Cc * Dhr1) / dx1i[ro]) / dx1[ro + 1];
+          const double t2f = Bc / (dx1[ro + 1] + dx1[ro]);
+          double a, b, c, d;
+          if (stage == 2) {                         // PDEparabolic.f90:146-216
+            a = -t1f + t2f; b = (k1.K + t3f) + t1f; c = -t3f - t2f;
+            d = -((k1.c13 * Tc) / k1.dt3) + (k1.c43 * trv[k]) / k1.dt3;
+          } else if (backeuler) {                   // PDEparabolic.f90:221-346
+            a = -t1f + t2f; b = (k1.K + t3f) + t1f; c = -t3f - t2f; d = Tc / k1.dt;
+          } else {                                  // PDEparabolic.f90:19-145
+            const double t1 = t1f / 2.0, t2 = t2f / 2.0, t3 = t3f / 2.0;
+            a = -t1 + t2; b = (k1.K + t3) + t1; c = -t3 - t2;
+            d = ((Tc / k1.dt2 - a * Tm) - (t3 + t1) * Tc) - c * Tp;
+          }
+          if (mode == 0) elim2(q, e, fsw, ro - 1, a, b, c, d);
+          else if (mode == 1) S2(q, ro, SY) = d;
+        }
+        Tm = Tc; Tc = Tp;
+        Cc = Cp; Bc = Bp;
+        Dm = Dc; Dc = Dp;
+      }
     }
-  }
-  S2(q, 1, 7) = 0.0;                                 // rows 1 and N keep no coefficients
-  elim2(q, e, N - 1, 0.0, 1.0, 0.0, Tmax);           // Dirichlet row N
-  if (e.r0 == 0.0) e.singular = true;
-  S2(q, N, 0) = e.r0; S2(q, N, 1) = 0.0; S2(q, N, 2) = 0.0; S2(q, N, 3) = e.rb;
+    if (mode == 1) S2(q, N, SY) = Tmax;
+    if (mode != 0) return;
+    elim2(q, e, fsw, N - 1, 0.0, 1.0, 0.0, Tmax);     // Dirichlet row N
+    if (e.r0 == 0.0) e.singular = true;
+    S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SY) = e.rb;
+  };
 
-  // rhs of the first stage, recomputed only when the factorisation was singular
-  auto rhs1 = [&](int i) -> double {
-    if (i == 1) return Tmin;
-    if (i == N) return Tmax;
-    const double Ti = S2(q, i, 7);
-    const double t1f = S2(q, i, 4), t2f = S2(q, i, 5), t3f = S2(q, i, 6);
-    if (backeuler) return Ti / s1.dt;
-    const double t1 = t1f / 2.0, t2 = t2f / 2.0, t3 = t3f / 2.0;
-    const double a = -t1 + t2, c = -t3 - t2;
-    const double Tl = i - 1 >= 2 ? S2(q, i - 1, 7) : s.Ts[so + q.cbase + 1];
-    const double Tr = i + 1 <= N - 1 ? S2(q, i + 1, 7) : s.Ts[so + q.cbase + N];
-    return ((Ti / s1.dt2 - a * Tl) - (t3 + t1) * Ti) - c * Tr;
+  // a forward pass, then the right-hand side again for lanes whose factorisation is singular
+  auto factor = [&](int stage, Elim& e, int& fsw) {
+    forward(stage, 0, e, fsw);
+    if (__any(e.singular)) {
+      Elim e2;
+      int f2;
+      forward(stage, e.singular ? 1 : 2, e2, f2);
+    }
   };
 
   // final temperatures go out through LDS in chunks of R rows
@@ -258,38 +295,18 @@ __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double*
     }
   };
 
+  Elim e;
+  int fsw;
+  factor(1, e, fsw);
+  __syncthreads();
   if (backeuler) {
-    __syncthreads();
-    backsub2(q, e.singular, store_out, rhs1);
+    backsub2<DIFF_PF>(q, e.singular, fsw, store_out);
     return;
   }
-  backsub2(q, e.singular, [&](int i, double x) { S2(q, i, 8) = x; }, rhs1);
-
-  // ---------------- BDF2 stage (PDEparabolic.f90:146-216) ----------------
-  Elim f{1.0, 0.0, Tmin, false};
-#pragma unroll 4
-  for (int j = 1; j <= N - 1; ++j) {
-    const int r = j + 1;
-    double a, b, c, d;
-    if (r < N) {
-      const double t1f = S2(q, r, 4), t2f = S2(q, r, 5), t3f = S2(q, r, 6);
-      a = -t1f + t2f;
-      b = (bdf.K + t3f) + t1f;
-      c = -t3f - t2f;
-      d = -((bdf.c13 * S2(q, r, 7)) / bdf.dt3) + (bdf.c43 * S2(q, r, 8)) / bdf.dt3;
-    } else {
-      a = 0.0; b = 1.0; c = 0.0; d = Tmax;
-    }
-    elim2(q, f, j, a, b, c, d);
-  }
-  if (f.r0 == 0.0) f.singular = true;
-  S2(q, N, 0) = f.r0; S2(q, N, 1) = 0.0; S2(q, N, 2) = 0.0; S2(q, N, 3) = f.rb;
+  backsub2<DIFF_PF>(q, e.singular, fsw, [&](int i, double x) { S2(q, i, STR) = x; });
+  factor(2, e, fsw);
   __syncthreads();
-  backsub2(q, f.singular, store_out, [&](int i) -> double {
-    if (i == 1) return Tmin;
-    if (i == N) return Tmax;
-    return -((bdf.c13 * S2(q, i, 7)) / bdf.dt3) + (bdf.c43 * S2(q, i, 8)) / bdf.dt3;
-  });
+  backsub2<DIFF_PF>(q, e.singular, fsw, store_out);
 }
 
 }  // namespace
@@ -299,7 +316,7 @@ void launch_floor_ghosts(const DevGrid& g, double* Ts, hipStream_t st);
 size_t diffusion_scratch_bytes(const DevGrid& g) {
   const long ncol = (long)g.lx2 * g.lx3;
   const long groups = (ncol + 63) / 64 * LSP;
-  return (size_t)groups * (size_t)g.lx1 * NS2 * 64 * sizeof(double);
+  return (size_t)groups * (size_t)g.lx1 * NSLOT * 64 * sizeof(double);
 }
 
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
@@ -310,9 +327,9 @@ void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int
   bdf.c13 = 1.0 / 3.0; bdf.c43 = 4.0 / 3.0;
   be.dt = dt; be.K = 1.0 / dt;
   const long ncol = (long)g.lx2 * g.lx3;
-  const long groups = (ncol + 63) / 64 * LSP;
+  const long blocks = ((ncol + 63) / 64 + 7) / 8 * 8 * LSP;   // XCD-major numbering
   timing_begin("trbdf2", st);
-  hipLaunchKernelGGL(k_diffusion<DIFF_CHUNK>, dim3((unsigned)groups), dim3(64), 0, st, g, s,
+  hipLaunchKernelGGL(k_diffusion<DIFF_CHUNK>, dim3((unsigned)blocks), dim3(64), 0, st, g, s,
                      scratch, tr, bdf, be, diffsolvetype == 1 ? 1 : 0, Teinf);
   timing_end("trbdf2", st);
   timing_begin("floor_ghosts", st);

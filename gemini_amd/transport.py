@@ -18,7 +18,9 @@ from .grid import Grid
 from .state import State
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgemhip.so")
+# GEMHIP_LIBRARY selects another build of the same library (e.g. one compiled with
+# different tile or pipeline-depth settings)
+LIB_PATH = os.environ.get("GEMHIP_LIBRARY") or os.path.join(HERE, "libgemhip.so")
 
 _dp = C.POINTER(C.c_double)
 _lib = None
