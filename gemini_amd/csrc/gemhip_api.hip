@@ -354,9 +354,12 @@ void gemhip_clear_error(void) { ctx().last_error.clear(); }
 int gemhip_set_grid_size(int lx1, int lx2, int lx3) {
   Ctx& c = ctx();
   if (!c.inited) { fail("library not initialised (gemhip_init)"); return 1; }
-  if (lx1 < 4 || lx2 < 1 || lx3 < 1) {
-    fail("grid size %d x %d x %d not supported (lx1 >= 4, lx2, lx3 >= 1)", lx1, lx2,
-         lx3);
+  // a field's cell index fits in 32 bits (2^31 ghosted cells are 16 GiB per field, far
+  // more than the 35 fields of a state can hold in one GPU's memory)
+  if (lx1 < 4 || lx2 < 1 || lx3 < 1 ||
+      (long)(lx1 + 4) * (lx2 + 4) * (lx3 + 4) >= (1L << 31)) {
+    fail("grid size %d x %d x %d not supported (lx1 >= 4, lx2, lx3 >= 1, fewer than 2^31 "
+         "ghosted cells)", lx1, lx2, lx3);
     return 2;
   }
   hipStreamSynchronize(c.stream);

@@ -280,120 +280,192 @@ __global__ void k_save_v1b(DevGrid g, DevState s) {
   s.v1b[vb + 1] = v1i_at(g, u, g.lx1 + 1);
 }
 
+// ---------------------------------------------------------------- sweeps
+// The fields one sweep launch advances for a range of species: up to three arrays
+// (ns, rhovs1, rhoes) sharing the species' velocity and the metric factors, each with
+// the tensor rank of its first dimension (advec.f90:313-381: rank 1 squares h1 in the
+// x2/x3 metric products).  Species 1..6 carry all three, species 7 (electrons) only
+// energy (multifluid.f90:49-126), so the launches for the two groups are separate and
+// every wave of a launch runs the same instruction stream.
+struct SweepFields {
+  double* f[3];
+  int frank[3];
+};
+
+// Every memory operation in the steady state of a sweep is unconditional: loads use
+// clamped indices (a lane past the end reads a valid cell it does not use) and stores
+// either write an updated interior cell or put an unchanged ghost value back.  With no
+// branches around memory operations the compiler's wait counters stay exact, so loads
+// issued for later steps remain in flight while the current step computes.
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
 // ---------------------------------------------------------------- x1 sweep
-// advec.f90:283-309 + :384-435.  Workgroup = one (x2,x3) line, wave w = species w+1.
+// advec.f90:283-309 + :384-435.  Workgroup = one (x2,x3) line, wave w = species w+isp0.
 // The wave walks the line in chunks of 64 consecutive cells, lane l at position
-// p = 64c + l - 1, so every load and store is a coalesced 64-lane access.  From f(p-1),
-// f(p), f(p+1) a lane forms the one-sided slope rs(p), the limited slope sl(p) and the
-// wall flux phi(p); rs(p-1), sl(p-1) and phi(p-1) come from the neighbouring lane by a
-// wave shuffle (lane 0 takes them from lane 63 of the previous chunk), and the lane
-// writes the updated cell p-1.  Reads of a chunk all precede its writes, and a chunk only
-// writes cells the next chunk no longer reads, so the in-place update sees original data.
-__device__ __forceinline__ double shfl_up1(double v, double carry, int lane) {
-  const double u = __shfl_up(v, 1, 64);
-  return lane == 0 ? carry : u;
+// p = 64c + l - 1 holding f(p), f(p+1), the velocity and metric factors at interface /
+// cell p, dx(p+1) and dx1i(p-2).  f(p-1), rs(p-1), sl(p-1), phi(p-1) and the other left
+// neighbours move one lane up by a DPP wave shift (lane 0 takes lane 63's of the previous
+// chunk), and the lane writes the update of cell p-1.  Two register buffers alternate:
+// the loads of chunk c+1 are issued before chunk c is computed, and chunk c writes only
+// cells chunk c+1 no longer reads.  The chunk count is rounded up to even; the extra
+// chunk only puts the last ghost value back.
+__device__ __forceinline__ double lane_up1(double v, double carry) {
+  // lane i receives lane i-1's value, lane 0 the carry (DPP wave_shr:1)
+  const long long vi = __double_as_longlong(v), ci = __double_as_longlong(carry);
+  const int lo = __builtin_amdgcn_update_dpp((int)ci, (int)vi, 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ci >> 32), (int)(vi >> 32), 0x138, 0xf, 0xf,
+                                             false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane63(double v) {
+  const long long vi = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)vi, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(vi >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, int fields, double dt,
-                                                int onthefly) {
+template <int NQ>
+struct S1Chunk {
+  double f[NQ], fn[NQ];     // f(p), f(p+1)
+  double u;                 // vs1(p) (on the fly) or vs1i(p)
+  double i1, i2, i3;        // h1x1i, h2x1i, h3x1i at interface p
+  double c1, c2, c3;        // h1, h2, h3 at cell p
+  double dxp1, dxim;        // dx1(p+1), dx1i(p-2)
+};
+
+template <int NQ, bool OTF>
+__device__ __forceinline__ void s1_load(const DevGrid& g, double* const* fp, const double* u,
+                                        const double* v1i, long c0, long if0, int p,
+                                        S1Chunk<NQ>& d) {
+  const int lx1 = g.lx1;
+  const int pc = clampi(p, -1, lx1 + 2), pn = clampi(p + 1, -1, lx1 + 2);
+  const int pi = clampi(p, 1, lx1 + 1);
+  const int pcell = clampi(p, 1, lx1);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    d.f[q] = fp[q][pc];
+    d.fn[q] = fp[q][pn];
+  }
+  d.u = OTF ? u[pc] : v1i[if0 + pi];
+  d.i1 = g.h1x1i[if0 + pi]; d.i2 = g.h2x1i[if0 + pi]; d.i3 = g.h3x1i[if0 + pi];
+  d.c1 = g.h1[c0 + pcell]; d.c2 = g.h2[c0 + pcell]; d.c3 = g.h3[c0 + pcell];
+  d.dxp1 = g.dx1[clampi(p + 1, 0, lx1 + 2)];
+  d.dxim = g.dx1i[clampi(p - 2, 0, lx1 - 1)];
+}
+
+template <int NQ>
+struct S1Carry {
+  double rs[NQ], sl[NQ], ph[NQ], f[NQ];
+  double u, h2, ha, dx;
+};
+
+template <int NQ, bool OTF>
+__device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, const double* u,
+                                         const double* v1i, long c0, long if0, int c, int lane,
+                                         double dt, double vb1, double vbN,
+                                         const S1Chunk<NQ>& cur, S1Chunk<NQ>& nxt,
+                                         S1Carry<NQ>& k) {
+  const int lx1 = g.lx1;
+  const int p = 64 * c + lane - 1;
+  s1_load<NQ, OTF>(g, fp, u, v1i, c0, if0, p + 64, nxt);                  // next chunk
+  const bool upd = p >= 2 && p <= lx1 + 1;
+  const double um = lane_up1(cur.u, k.u);                                  // vs1(p-1)
+  double v;
+  if (OTF) v = p == 1 ? vb1 : (p == lx1 + 1 ? vbN : 0.5 * (um + cur.u));
+  else v = cur.u;
+  const double vq = v / cur.i1;
+  const double h2p = cur.i2 * cur.i3;
+  const double h2m = lane_up1(h2p, k.h2);
+  const double hap = (cur.c1 * cur.c2) * cur.c3;                           // ha1 of cell p
+  const double ha1m = lane_up1(hap, k.ha);                                 // ha1 of cell p-1
+  const double dx_p = cur.dxp1;                                            // dx(p+1)
+  const double dx_0 = lane_up1(dx_p, k.dx);                                // dx(p)
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double f0 = cur.f[q], f1 = cur.fn[q];
+    const double fm1 = lane_up1(f0, k.f[q]);
+    const double rs = (f1 - f0) / dx_p;                                    // rs(p)
+    const double rsm = lane_up1(rs, k.rs[q]);                              // rs(p-1)
+    const double cs = (f1 - fm1) / (dx_0 + dx_p);
+    const double sl = minmod(cs, minmod(2.0 * rsm, 2.0 * rs));            // sl(p), p >= 0
+    const double slm = lane_up1(sl, k.sl[q]);                              // sl(p-1)
+    const double phn = f0 * v - ((0.5 * v) * (dx_0 + vq * dt)) * sl;
+    const double phv = fm1 * v + ((0.5 * v) * (dx_0 - vq * dt)) * slm;
+    const double ph = v < 0.0 ? phn : phv;                                 // phi(p), p >= 1
+    const double phm = lane_up1(ph, k.ph[q]);                              // phi(p-1)
+    const double out = fm1 - ((dt * (h2p * ph - h2m * phm)) / cur.dxim) / ha1m;
+    // cells 1..lx1 take the update; the other lanes put the ghost value they cover back
+    fp[q][clampi(p - 1, -1, lx1 + 2)] = upd ? out : (p < 0 ? f0 : fm1);
+    k.rs[q] = lane63(rs);
+    k.sl[q] = lane63(sl);
+    k.ph[q] = lane63(ph);
+    k.f[q] = lane63(f0);
+  }
+  k.u = lane63(cur.u);
+  k.h2 = lane63(h2p);
+  k.ha = lane63(hap);
+  k.dx = lane63(dx_p);
+}
+
+template <int NQ, bool OTF>
+__global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFields F, int isp0,
+                                                double dt) {
   const int lx1 = g.lx1;
   const int i2 = blockIdx.x + 1, i3 = blockIdx.y + 1;
-  const int lane = threadIdx.x, isp = threadIdx.y + 1;
+  const int lane = threadIdx.x, isp = isp0 + threadIdx.y;
   const long c0 = gix(g, 0, i2, i3);                                       // cell p -> c0+p
   const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;  // iface p -> if0+p
   const long so = (long)(isp - 1) * g.N;
   const long nif = (long)(lx1 + 1) * g.lx2 * g.lx3;
+  double* fp[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) fp[q] = F.f[q] + so + c0;
   const double* u = s.vs1 + so + c0;                                        // u[p] = vs1(p)
-  double* fp[3];
-  bool act[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    fp[q] = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;
-    act[q] = (fields & (1 << q)) && (q == 2 || isp < LSP);
+  const double* v1i = s.vs1i + (long)(isp - 1) * nif;
+  // boundary interface velocities (advec.f90:171-196), the same for the whole wave
+  double vb1 = 0.0, vbN = 0.0;
+  if (OTF) {
+    vb1 = v1i_at(g, u, 1);
+    vbN = v1i_at(g, u, lx1 + 1);
   }
-  double crs[3] = {0, 0, 0}, csl[3] = {0, 0, 0}, cph[3] = {0, 0, 0};  // carries from lane 63
-  double ch2 = 0.0;
-  const int nchunk = (lx1 + 3 + 63) / 64;                                   // p = -1 .. lx1+1
-  for (int c = 0; c < nchunk; ++c) {
-    const int p = 64 * c + lane - 1;
-    const bool inl = p <= lx1 + 1;
-    // interface quantities at wall p (1..lx1+1) and metric products
-    double v = 0.0, vq = 0.0, h2p = 0.0, ha1m = 1.0, dxp = 0.0, dxim = 1.0;
-    const bool wall = p >= 1 && p <= lx1 + 1;
-    if (wall) {
-      v = onthefly ? v1i_at(g, u, p) : s.vs1i[(long)(isp - 1) * nif + if0 + p];
-      vq = v / g.h1x1i[if0 + p];
-      h2p = g.h2x1i[if0 + p] * g.h3x1i[if0 + p];
-      dxp = g.dx1[p];
-    }
-    if (p >= 2 && p <= lx1 + 1) {
-      const long cm = c0 + p - 1;
-      ha1m = (g.h1[cm] * g.h2[cm]) * g.h3[cm];
-      dxim = g.dx1i[p - 2];
-    }
-    const double h2m = shfl_up1(h2p, ch2, lane);
-    ch2 = __shfl(h2p, 63, 64);
-    const double dx_p = inl ? g.dx1[p + 1] : 1.0;               // dx(p+1)
-    const double dx_0 = (inl && p >= 0) ? g.dx1[p] : 1.0;       // dx(p)
+  S1Carry<NQ> k;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (!act[q]) continue;
-      const double* f = fp[q];
-      double fm1 = 0.0, f0 = 0.0, f1 = 0.0;
-      if (inl) {
-        f0 = f[p];
-        f1 = f[p + 1];
-        if (p >= 0) fm1 = f[p - 1];
-      }
-      const double rs = inl ? (f1 - f0) / dx_p : 0.0;            // rs(p)
-      const double rsm = shfl_up1(rs, crs[q], lane);             // rs(p-1)
-      double sl = 0.0;
-      if (inl && p >= 0) {
-        const double cs = (f1 - fm1) / (dx_0 + dx_p);
-        sl = minmod(cs, minmod(2.0 * rsm, 2.0 * rs));
-      }
-      const double slm = shfl_up1(sl, csl[q], lane);             // sl(p-1)
-      double ph = 0.0;
-      if (wall) {
-        if (v < 0.0) ph = f0 * v - ((0.5 * v) * (dxp + vq * dt)) * sl;
-        else ph = fm1 * v + ((0.5 * v) * (dxp - vq * dt)) * slm;
-      }
-      const double phm = shfl_up1(ph, cph[q], lane);             // phi(p-1)
-      if (p >= 2 && p <= lx1 + 1) {
-        const double out = fm1 - ((dt * (h2p * ph - h2m * phm)) / dxim) / ha1m;
-        fp[q][p - 1] = out;
-      }
-      crs[q] = __shfl(rs, 63, 64);
-      csl[q] = __shfl(sl, 63, 64);
-      cph[q] = __shfl(ph, 63, 64);
-    }
+  for (int q = 0; q < NQ; ++q) k.rs[q] = k.sl[q] = k.ph[q] = k.f[q] = 0.0;
+  k.u = k.h2 = 0.0;
+  k.ha = k.dx = 1.0;
+  const int nchunk = ((lx1 + 3 + 63) / 64 + 1) & ~1;                       // p = -1 .. lx1+1
+  S1Chunk<NQ> A, B;
+  s1_load<NQ, OTF>(g, fp, u, v1i, c0, if0, lane - 1, A);
+  for (int c = 0; c < nchunk; c += 2) {
+    s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k);
+    s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c + 1, lane, dt, vb1, vbN, B, A, k);
   }
 }
-
-#ifndef SWEEP_PF
-#define SWEEP_PF 4      // x2/x3 march pipeline depth (steps of loads in flight)
-#endif
 
 // ---------------------------------------------------------------- x2 / x3 sweeps
 // advec.f90:313-381 + :384-435.  Thread (i1, line, species): marches along its line
 // keeping f(k-1), f(k), f(k+1), the last one-sided slope, limited slope and flux in
-// registers for up to three fields (ns, rhovs1, rhoes) that share the velocity and
-// metric factors.  frank 1 (momentum) uses h1^2 in both metric products
-// (advec.f90:336-338, 372-374).
+// registers for up to three fields that share the velocity and metric factors.
 //
 // The march is software-pipelined PF steps deep: step k consumes the field values,
 // velocity and metric factors loaded PF steps earlier and issues the loads of step
 // k+PF.  Field loads run ahead of the in-place stores (step k stores cell k-1 and loads
 // cell k+2+PF, which no earlier step has written), so every value read is the original.
-template <int DIR, int PF>
-__global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fields, double dt,
-                                                 int onthefly) {
+// Steps 0 and 1 (no store) are peeled off so every step of the main loop stores.
+#ifndef SWEEP_PF
+#define SWEEP_PF 2      // x2/x3 march pipeline depth (steps of loads in flight) with 2-3 fields
+#endif
+#ifndef SWEEP_PF1
+#define SWEEP_PF1 4     // the same for single-field launches (fewer registers per step)
+#endif
+
+template <int DIR, int PF, int NQ, bool OTF>
+__global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, SweepFields F, int isp0,
+                                                 double dt) {
   const int lx = DIR == 2 ? g.lx2 : g.lx3;
-  // lanes past lx1 stay in the loop (it holds block barriers): they address column lx1,
-  // which keeps their loads in bounds, and write nothing
-  const bool live = (int)(blockIdx.x * blockDim.x + threadIdx.x) < g.lx1;
-  const int i1 = live ? (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 : g.lx1;
-  const int isp = threadIdx.y + 1;
+  // lanes past lx1 redo column lx1's work: same inputs, same values written to the same cells
+  const int i1 = min((int)(blockIdx.x * blockDim.x + threadIdx.x), g.lx1 - 1) + 1;
+  const int isp = isp0 + threadIdx.y;
   const int other = blockIdx.y + 1;                  // i3 for DIR 2, i2 for DIR 3
   const long cs_ = DIR == 2 ? (long)g.L1 : (long)g.L1 * g.L2;   // cell stride along the line
   const long c0 = DIR == 2 ? gix(g, i1, 0, other) : gix(g, i1, other, 0);  // cell k -> c0+k*cs_
@@ -410,102 +482,102 @@ __global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, int fiel
   const double* vi = (DIR == 2 ? s.vs2i : s.vs3i) +
                      (long)(isp - 1) * (DIR == 2 ? (long)g.lx1 * (g.lx2 + 1) * g.lx3
                                                  : (long)g.lx1 * g.lx2 * (g.lx3 + 1));
-  double* fp[3];
-  bool act[3];
+  double* fp[NQ];
+  bool fr1[NQ];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    fp[q] = (q == 0 ? s.ns : q == 1 ? s.rhovs1 : s.rhoes) + so + c0;
-    act[q] = live && (fields & (1 << q)) && (q == 2 || isp < LSP);
+  for (int q = 0; q < NQ; ++q) {
+    fp[q] = F.f[q] + so + c0;
+    fr1[q] = F.frank[q] == 1;
   }
   // pipeline registers of the steps in flight
-  double pf[3][PF], pv[PF], pI[PF], pA[PF], pB[PF], p1[PF], p2[PF], p3[PF];
+  double pf[NQ][PF], pv[PF], pI[PF], pA[PF], pB[PF], p1[PF], p2[PF], p3[PF];
   auto issue = [&](int k, int u) {
+    const int kf = min(k + 2, lx + 2);
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-      if (act[q] && k + 2 <= lx + 2) pf[q][u] = fp[q][(long)(k + 2) * cs_];
-    if (k >= 1 && k <= lx + 1) {
-      const long ik = if0 + (long)k * istr;
-      pv[u] = onthefly ? w[(long)k * cs_] : vi[ik];
-      pI[u] = hI[ik]; pA[u] = hA[ik]; pB[u] = hB[ik];
-      if (k >= 2) {
-        const long cm = c0 + (long)(k - 1) * cs_;
-        p1[u] = g.h1[cm]; p2[u] = g.h2[cm]; p3[u] = g.h3[cm];
-      }
-    }
+    for (int q = 0; q < NQ; ++q) pf[q][u] = fp[q][(long)kf * cs_];
+    const int ki = clampi(k, 1, lx + 1);
+    const long ik = if0 + (long)ki * istr;
+    pv[u] = OTF ? w[(long)ki * cs_] : vi[ik];
+    pI[u] = hI[ik]; pA[u] = hA[ik]; pB[u] = hB[ik];
+    const long cm = c0 + (long)clampi(k - 1, 1, lx) * cs_;
+    p1[u] = g.h1[cm]; p2[u] = g.h2[cm]; p3[u] = g.h3[cm];
   };
-  double fa[3], fb[3], fc[3], rsp[3], slp[3], php[3];
+  double fa[NQ], fb[NQ], fc[NQ], rsp[NQ], slp[NQ], php[NQ];
+  double dxcur = dx[0];
+  double wprev = w[0];         // vs(0) for the on-the-fly interface velocity
+  double h2p0 = 0.0, h2p1 = 0.0;   // ha2i(k-1) for frank 0 / frank 1
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    fa[q] = fb[q] = fc[q] = rsp[q] = slp[q] = php[q] = 0.0;
-    if (!act[q]) continue;
+  for (int q = 0; q < NQ; ++q) {
     fa[q] = fp[q][-cs_];       // f(-1)
     fb[q] = fp[q][0];          // f(0)
     fc[q] = fp[q][cs_];        // f(1)
-    rsp[q] = (fb[q] - fa[q]) / dx[0];
+    rsp[q] = (fb[q] - fa[q]) / dxcur;
+    slp[q] = php[q] = 0.0;
   }
-  double wprev = w[0];         // vs(0) for the on-the-fly interface velocity
 #pragma unroll
   for (int u = 0; u < PF; ++u) issue(u, u);
-  double h2p0 = 0.0, h2p1 = 0.0;   // ha2i(k-1) for frank 0 / frank 1
-  for (int kb = 0; kb <= lx + 1; kb += PF) {
+  // mode 0: step 0 (slopes only), 1: step 1 (flux, no store), 2: steps 2..lx+1
+  auto step = [&](int k, int u, int mode) {
+    const double dxk = dxcur, dxk1 = dx[k + 1];
+    dxcur = dxk1;
+    const double wv = pv[u], hIv = pI[u], a = pA[u], b = pB[u], h1 = p1[u], h2 = p2[u], h3 = p3[u];
+    double nxt[NQ];
 #pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      const int k = kb + u;
-      if (k > lx + 1) break;
-      const double dxk = dx[k], dxk1 = dx[k + 1];
-      double v = 0.0, vq = 0.0, h2f0 = 0.0, h2f1 = 0.0, ha1f0 = 1.0, ha1f1 = 1.0, dxim = 1.0;
-      if (k >= 1) {
-        if (onthefly) {
-          v = 0.5 * (wprev + pv[u]);
-          wprev = pv[u];
-        } else {
-          v = pv[u];
-        }
-        vq = v / pI[u];
-        const double a = pA[u], b = pB[u];
-        h2f0 = a * b;
-        h2f1 = (a * a) * b;
-        if (k >= 2) {
-          const double h1 = p1[u], h2 = p2[u], h3 = p3[u];
-          ha1f0 = (h1 * h2) * h3;
-          ha1f1 = ((h1 * h1) * h2) * h3;
-          dxim = dxi[k - 2];
-        }
-      }
-      double nxt[3];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) nxt[q] = pf[q][u];
-      issue(k + PF, u);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        if (!act[q]) continue;
-        const double rs = (fc[q] - fb[q]) / dxk1;
-        const double csl = (fc[q] - fa[q]) / (dxk + dxk1);
-        const double sl = minmod(csl, minmod(2.0 * rsp[q], 2.0 * rs));
-        if (k >= 1) {
-          double ph;
-          if (v < 0.0) ph = fb[q] * v - ((0.5 * v) * (dxk + vq * dt)) * sl;
-          else ph = fa[q] * v + ((0.5 * v) * (dxk - vq * dt)) * slp[q];
-          const bool fr1 = (q == 1);
-          const double h2k = fr1 ? h2f1 : h2f0;
-          if (k >= 2) {
-            const double h2m = fr1 ? h2p1 : h2p0;
-            const double ha1 = fr1 ? ha1f1 : ha1f0;
-            fp[q][(long)(k - 1) * cs_] = fa[q] - ((dt * (h2k * ph - h2m * php[q])) / dxim) / ha1;
-          }
-          php[q] = ph;
-        }
-        slp[q] = sl;
-        rsp[q] = rs;
-        fa[q] = fb[q];
-        fb[q] = fc[q];
-        fc[q] = nxt[q];
-      }
-      if (k >= 1) { h2p0 = h2f0; h2p1 = h2f1; }
+    for (int q = 0; q < NQ; ++q) nxt[q] = pf[q][u];
+    issue(k + PF, u);
+    double v = 0.0, vq = 0.0, h2f0 = 0.0, h2f1 = 0.0, ha1f0 = 1.0, ha1f1 = 1.0, dxim = 1.0;
+    if (mode >= 1) {
+      if (OTF) { v = 0.5 * (wprev + wv); wprev = wv; }
+      else v = wv;
+      vq = v / hIv;
+      h2f0 = a * b;
+      h2f1 = (a * a) * b;
     }
-    // keep the seven species waves of the block in step so the metric factors they all
-    // read are still cached when the slower waves reach them
-    if (((kb / PF) & 1) == 1) __syncthreads();
+    if (mode == 2) {
+      ha1f0 = (h1 * h2) * h3;
+      ha1f1 = ((h1 * h1) * h2) * h3;
+      dxim = dxi[k - 2];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double rs = (fc[q] - fb[q]) / dxk1;
+      const double csl = (fc[q] - fa[q]) / (dxk + dxk1);
+      const double sl = minmod(csl, minmod(2.0 * rsp[q], 2.0 * rs));
+      if (mode >= 1) {
+        const double phn = fb[q] * v - ((0.5 * v) * (dxk + vq * dt)) * sl;
+        const double phv = fa[q] * v + ((0.5 * v) * (dxk - vq * dt)) * slp[q];
+        const double ph = v < 0.0 ? phn : phv;
+        if (mode == 2) {
+          const double h2k = fr1[q] ? h2f1 : h2f0;
+          const double h2m = fr1[q] ? h2p1 : h2p0;
+          const double ha1 = fr1[q] ? ha1f1 : ha1f0;
+          fp[q][(long)(k - 1) * cs_] = fa[q] - ((dt * (h2k * ph - h2m * php[q])) / dxim) / ha1;
+        }
+        php[q] = ph;
+      }
+      slp[q] = sl;
+      rsp[q] = rs;
+      fa[q] = fb[q];
+      fb[q] = fc[q];
+      fc[q] = nxt[q];
+    }
+    if (mode >= 1) { h2p0 = h2f0; h2p1 = h2f1; }
+  };
+  step(0, 0, 0);
+  step(1, 1 % PF, 1);
+  int kb = 2;
+  for (; kb + PF - 1 <= lx + 1; kb += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) step(kb + u, (2 + u) % PF, 2);
+    // keep the species waves of the block in step so the metric factors they all read
+    // are still cached when the slower waves reach them (no memory fence: nothing is
+    // exchanged between the waves)
+    __builtin_amdgcn_s_barrier();
+  }
+#pragma unroll
+  for (int u = 0; u < PF - 1; ++u) {
+    if (kb + u > lx + 1) break;
+    step(kb + u, (2 + u) % PF, 2);
   }
 }
 
@@ -612,27 +684,55 @@ void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t 
   timing_end("save_v1b", st);
 }
 
+namespace {
+template <int NQ>
+void sweep_launch(const DevGrid& g, const DevState& s, int dir, const SweepFields& F, int isp0,
+                  int nsp, double dt, bool otf, hipStream_t st) {
+  const dim3 blk(64, nsp);
+  if (dir == 1) {
+    const dim3 grd(g.lx2, g.lx3);
+    if (otf) hipLaunchKernelGGL((k_sweep1<NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
+    else hipLaunchKernelGGL((k_sweep1<NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
+  } else if (dir == 2) {
+    constexpr int PF = NQ == 1 ? SWEEP_PF1 : SWEEP_PF;
+    const dim3 grd((g.lx1 + 63) / 64, g.lx3);
+    if (otf) hipLaunchKernelGGL((k_sweep23<2, PF, NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
+    else hipLaunchKernelGGL((k_sweep23<2, PF, NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
+  } else {
+    constexpr int PF = NQ == 1 ? SWEEP_PF1 : SWEEP_PF;
+    const dim3 grd((g.lx1 + 63) / 64, g.lx2);
+    if (otf) hipLaunchKernelGGL((k_sweep23<3, PF, NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
+    else hipLaunchKernelGGL((k_sweep23<3, PF, NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
+  }
+}
+}  // namespace
+
 void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, double dt,
                   bool onthefly, hipStream_t st) {
-  if (dir == 1) {
-    if (g.lx1 <= 1) return;
-    timing_begin("sweep1", st);
-    hipLaunchKernelGGL(k_sweep1, dim3(g.lx2, g.lx3), dim3(64, LSP), 0, st, g, s, fields, dt,
-                       (int)onthefly);
-    timing_end("sweep1", st);
-  } else if (dir == 2) {
-    if (g.lx2 <= 1) return;
-    timing_begin("sweep2", st);
-    hipLaunchKernelGGL((k_sweep23<2, SWEEP_PF>), dim3((g.lx1 + 63) / 64, g.lx3), dim3(64, LSP), 0, st, g, s,
-                       fields, dt, (int)onthefly);
-    timing_end("sweep2", st);
+  const int lx = dir == 1 ? g.lx1 : dir == 2 ? g.lx2 : g.lx3;
+  if (lx <= 1) return;                       // advec1D_MC_curv leaves lx = 1 lines alone
+  SweepFields A{};
+  int na = 0;
+  if (fields & F_MASS) { A.f[na] = s.ns; A.frank[na++] = 0; }
+  if (fields & F_MOMENTUM) { A.f[na] = s.rhovs1; A.frank[na++] = dir == 1 ? 0 : 1; }
+  if (fields & F_ENERGY) { A.f[na] = s.rhoes; A.frank[na++] = 0; }
+  if (na == 0) return;
+  const char* name = dir == 1 ? "sweep1" : dir == 2 ? "sweep2" : "sweep3";
+  timing_begin(name, st);
+  if (na == 1 && fields == F_ENERGY) {       // energy alone: all seven species in one launch
+    sweep_launch<1>(g, s, dir, A, 1, LSP, dt, onthefly, st);
   } else {
-    if (g.lx3 <= 1) return;
-    timing_begin("sweep3", st);
-    hipLaunchKernelGGL((k_sweep23<3, SWEEP_PF>), dim3((g.lx1 + 63) / 64, g.lx2), dim3(64, LSP), 0, st, g, s,
-                       fields, dt, (int)onthefly);
-    timing_end("sweep3", st);
+    if (na == 1) sweep_launch<1>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);
+    else if (na == 2) sweep_launch<2>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);
+    else sweep_launch<3>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);
+    if (fields & F_ENERGY) {                 // electrons: energy only
+      SweepFields E{};
+      E.f[0] = s.rhoes;
+      E.frank[0] = 0;
+      sweep_launch<1>(g, s, dir, E, LSP, 1, dt, onthefly, st);
+    }
   }
+  timing_end(name, st);
 }
 
 void launch_floor_ghosts(const DevGrid& g, double* Ts, hipStream_t st) {
