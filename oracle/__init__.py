@@ -120,6 +120,17 @@ class pow_mode:
         set_pow_mode(0)
 
 
+def reset_swap_count():
+    lib().orc_reset_swap_count()
+
+
+def swap_count() -> int:
+    """Row interchanges made by the band solves since reset_swap_count()."""
+    f = lib().orc_swap_count
+    f.restype = C.c_long
+    return int(f())
+
+
 def set_num_threads(n: int):
     os.environ["OMP_NUM_THREADS"] = str(n)
     try:

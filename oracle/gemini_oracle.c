@@ -571,10 +571,22 @@ static void dgbtrs_n_(int n, int kl, int ku, const double *ab, int ldab, const i
 }
 #undef AB
 
+/* rows interchanged by the factorisations since the last reset (tests use it to show a
+   case exercises pivoting) */
+static long swap_count = 0;
+void orc_reset_swap_count(void) { swap_count = 0; }
+long orc_swap_count(void) { return swap_count; }
+
 int orc_gbsv(int n, int kl, double *ab, double *b, int *ipiv) {
   const int ldab = 3 * kl + 1;                  /* gbsv.in.f90:72,88: KU = LDA-2*KL-1 */
   const int ku = ldab - 2 * kl - 1;
   int info = dgbtf2_(n, kl, ku, ab, ldab, ipiv);  /* dgbtrf picks dgbtf2 for KU <= 64 */
+  long sw = 0;
+  for (int j = 0; j < n; ++j) sw += ipiv[j] != j + 1;
+  if (sw) {
+#pragma omp atomic
+    swap_count += sw;
+  }
   if (info == 0) dgbtrs_n_(n, kl, ku, ab, ldab, ipiv, b);
   return info;
 }

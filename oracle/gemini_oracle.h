@@ -65,6 +65,9 @@ void orc_sweep_allspec(const orc_grid *g, int dir, double *fs, const double *vsi
 void orc_thermal_conduct(const orc_grid *g, int isp, const double *Ts, const double *ns,
                          const double *nn, const double *J1, double *lambda, double *beta);
 int  orc_gbsv(int n, int kl, double *ab, double *b, int *ipiv);
+/* number of row interchanges made by orc_gbsv since the last reset */
+void orc_reset_swap_count(void);
+long orc_swap_count(void);
 void orc_TRBDF21D(int lx1, const double *Ts, const double *A, const double *B,
                   const double *C, const double *D, const double *E, double Tsminx1,
                   double Tsmaxx1, double dt, const double *dx1, const double *dx1i,

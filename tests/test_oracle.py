@@ -167,6 +167,21 @@ def test_band_solver_singular_returns_rhs():
     assert info == 4 and np.array_equal(x, r)
 
 
+@pytest.mark.parametrize("kind,shape", [("cart", (24, 10, 7)), ("dipole", (20, 9, 5))])
+def test_energy_diffusion_cases_pivot(kind, shape):
+    """The synthetic states of the parity tests drive LAPACK's partial pivoting: the
+    identity Dirichlet row is tiny next to conduction-dominated interior rows, so most
+    elimination steps swap rows and the fill-in U2 is in use."""
+    g = ga.cartesian_grid(*shape) if kind == "cart" else ga.dipole_grid(*shape)
+    st = ga.synthetic_state(g, seed=1, perturb=True)
+    dt = ga.cfl_dt(g, st)
+    for solver in (2, 1):
+        oracle.reset_swap_count()
+        oracle.op(g, st.copy(), "energy_diffusion", dt=dt, diffsolvetype=solver)
+        rows = g.lx1 * g.lx2 * g.lx3 * 7 * (2 if solver == 2 else 1)
+        assert oracle.swap_count() > rows // 4
+
+
 # ------------------------------------------------------------------ TRBDF2 closed form
 @pytest.mark.parametrize("mode", [1, 3, 7])
 def test_trbdf2_sine_mode_closed_form(mode):
