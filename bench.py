@@ -56,30 +56,47 @@ def make_grid(kind, shape):
     return ga.dipole_grid(*shape) if kind == "dipole" else ga.cartesian_grid(*shape)
 
 
-def cpu_baseline(w, dt, seconds=15.0):
-    """Oracle (C restatement, OpenMP over lines/columns) on a bounded sample grid."""
+def cpu_baseline(w, dt, seconds=15.0, block=20):
+    """Oracle (C restatement, OpenMP over lines/columns) on a bounded sample grid: blocks of
+    `block` steps, each from a fresh copy of the seeded state, with only the steps timed."""
     import numpy as np
     import gemini_amd as ga
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     oracle.set_num_threads(threads)
     g = make_grid(w["kind"], w["sample"])
-    st = ga.synthetic_state(g, seed=1, perturb=(w["kind"] == "cart" and g.lx3 > 1))
-    t0 = time.perf_counter()
-    oracle.transport_step(g, st, dt)
-    t1 = time.perf_counter() - t0
-    n = max(1, min(200, int(seconds / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    for _ in range(n):
-        oracle.transport_step(g, st, dt)
-    el = time.perf_counter() - t0
-    if not np.isfinite(st.fluidvars).all():
+    st0 = ga.synthetic_state(g, seed=1, perturb=(w["kind"] == "cart" and g.lx3 > 1))
+    oracle.transport_step(g, st0.copy(), dt)          # warm-up
+    n, el, finite = 0, 0.0, True
+    while el < seconds and n < 400:
+        st = st0.copy()
+        t0 = time.perf_counter()
+        for _ in range(block):
+            oracle.transport_step(g, st, dt)
+        el += time.perf_counter() - t0
+        n += block
+        finite = finite and bool(np.isfinite(st.fluidvars).all())
+    if not finite:
         log("warning: non-finite values in the CPU sample")
     value = g.ncells * 7 * n / el
     return {"value": value, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": f"oracle (plain-C restatement of the reference path) on a "
-                      f"{g.lx1}x{g.lx2}x{g.lx3} {w['kind']} grid of the same field-line "
-                      f"length and profiles, {n} steps in {el:.1f} s after 1 warm-up step"}
+            "sample": f"oracle (plain-C restatement of the reference path, OpenMP) on a "
+                      f"{g.lx1}x{g.lx2}x{g.lx3} {w['kind']} grid with the same field-line "
+                      f"length and profiles: {n} steps in {el:.1f} s, in blocks of {block} "
+                      f"from the seeded state, after 1 warm-up step"}
+
+
+def state_is_finite(tr, g):
+    """Copy the state blocks back into HBM tensors (after timing) and check them."""
+    import torch
+    n = g.padded[0] * g.padded[1] * g.padded[2]
+    fv = torch.empty(35 * n, dtype=torch.float64, device="cuda:0")
+    aux = torch.empty(14 * n, dtype=torch.float64, device="cuda:0")
+    tr.pull_blocks(fv, aux)
+    ok = bool(torch.isfinite(fv).all().item()) and bool(torch.isfinite(aux).all().item())
+    del fv, aux
+    torch.cuda.empty_cache()
+    return ok
 
 
 def load_traffic(workload, kernel):
@@ -159,6 +176,9 @@ def main():
     tr.synchronize()
     wall = time.perf_counter() - t0
     tr.kernel_timing(False)
+    finite = state_is_finite(tr, g)
+    if not finite:
+        log("warning: non-finite values in the GPU state after the timed steps")
     ms_step = 1e3 * wall / steps
     cells = g.ncells
     padded = g.padded[0] * g.padded[1] * g.padded[2]
@@ -203,7 +223,7 @@ def main():
         "config": {"workload": w["desc"], "grid": f"{w['kind']} {g.lx1}x{g.lx2}x{g.lx3}",
                    "species": 7, "dt_s": dt, "diffusion": "TRBDF2",
                    "step": "v12rhov1,T2rhoe,ghost fill,sweep3,sweep1,sweep2,rhov12v1,rhoe2T,"
-                           "TRBDF2"},
+                           "TRBDF2", "state_finite_after_timing": finite},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

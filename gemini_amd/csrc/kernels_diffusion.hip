@@ -64,6 +64,15 @@ constexpr int LP = 65;         // padded LDS row (conflict-free transposed acces
 #ifndef DIFF_CHUNK
 #define DIFF_CHUNK 8           // rows per chunk: 4 x 8 x 65 doubles of LDS per wave
 #endif
+#ifndef DIFF_WAVES_PER_SIMD
+#define DIFF_WAVES_PER_SIMD 2  // register budget: 512 / this many VGPRs per lane
+#endif
+#ifndef DIFF_BATCH_I
+#define DIFF_BATCH_I 8         // ion cells per lane whose inputs are loaded together
+#endif
+#ifndef DIFF_BATCH_E
+#define DIFF_BATCH_E 4         // the same for electron cells (nine inputs each)
+#endif
 enum : int { SU0 = 0, SU1 = 1, SY = 2, STR = 3, SU2 = 4, NSLOT = 5 };
 
 struct Group {
@@ -155,7 +164,7 @@ __device__ __forceinline__ void elim2(const Group& q, Elim& e, int& fsw, int j, 
 }
 
 template <int R>
-__global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double* scratch,
+__global__ __launch_bounds__(64, DIFF_WAVES_PER_SIMD) void k_diffusion(DevGrid g, DevState s, double* scratch,
                                                   Coef tr, Coef bdf, Coef be, int backeuler,
                                                   double Teinf);
 
@@ -191,8 +200,8 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   // cooperative mapping of a chunk: lane -> (row lr, column sub-index cs); R loads per lane
   const int lr = lane % R, cs = lane / R;
   constexpr int CPI = 64 / R;                        // columns per load instruction
-  constexpr int BATCH = ELEC ? 4 : 8;                // cells whose inputs are in flight together
-  static_assert(R % (ELEC ? 4 : 8) == 0 || R < (ELEC ? 4 : 8), "chunk and batch sizes");
+  constexpr int BATCH = ELEC ? DIFF_BATCH_E : DIFF_BATCH_I;   // cells whose inputs load together
+  static_assert(R % BATCH == 0 || R < BATCH, "chunk and batch sizes");
   constexpr int NB = R < BATCH ? R : BATCH;
   int cbk[R];                                        // gix(0, i2, i3) of the columns this lane loads
   int kbk[R];                                        // and their interior offsets lx1*(column)
@@ -399,7 +408,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 }
 
 template <int R>
-__global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double* scratch,
+__global__ __launch_bounds__(64, DIFF_WAVES_PER_SIMD) void k_diffusion(DevGrid g, DevState s, double* scratch,
                                                   Coef tr, Coef bdf, Coef be, int backeuler,
                                                   double Teinf) {
   __shared__ double sT[R][LP], sC[R][LP], sB[R][LP], sD[R][LP];

@@ -152,6 +152,10 @@ class Transport:
         """Push raw blocks (numpy or torch) laid out like the reference's."""
         self._call("gemhip_state_push", _p(fluidvars), _p(fluidauxvars), _p(electrovars))
 
+    def pull_blocks(self, fluidvars, fluidauxvars):
+        """Copy the resident state into raw blocks (numpy or torch, host or HBM)."""
+        self._call("gemhip_state_pull", _p(fluidvars), _p(fluidauxvars))
+
     def pull(self, st: State):
         self._call("gemhip_state_pull", _p(st.fluidvars), _p(st.fluidauxvars))
 
