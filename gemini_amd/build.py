@@ -46,5 +46,47 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     return out
 
 
+ROOT = os.path.dirname(HERE)
+DRIVERS = os.path.join(ROOT, "tests", "drivers")
+DRIVER_BIN = os.path.join(DRIVERS, "bin")
+
+
+def flang():
+    for c in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang", "flang"):
+        if c and (os.path.sep not in c or os.path.exists(c)):
+            return c
+    return None
+
+
+def build_drivers(verbose: bool = False) -> list:
+    """Compile the Fortran binding module and the two host drivers the test suite runs
+    (tests/drivers/*): a Fortran program using gemini_amd/fortran/gemhip_bind.f90 and a
+    C++ program using include/gemini3d_hip.h, both linked against libgemhip.so."""
+    os.makedirs(DRIVER_BIN, exist_ok=True)
+    rpath = "-Wl,-rpath,$ORIGIN/../../../gemini_amd"
+    link = ["-L" + HERE, "-lgemhip", rpath]
+    built = []
+    cmds = [["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+             os.path.join(DRIVERS, "capi_driver.cpp"), "-o", os.path.join(DRIVER_BIN, "capi_driver")]
+            + link]
+    fc = flang()
+    if fc:
+        mod = os.path.join(DRIVER_BIN, "gemhip_bind.o")
+        cmds.append([fc, "-O2", "-c", os.path.join(HERE, "fortran", "gemhip_bind.f90"), "-o", mod,
+                     "-module-dir", DRIVER_BIN])
+        cmds.append([fc, "-O2", "-I" + DRIVER_BIN, os.path.join(DRIVERS, "fortran_driver.f90"), mod,
+                     "-o", os.path.join(DRIVER_BIN, "fortran_driver")] + link)
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=DRIVER_BIN)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"building {cmd[-len(link) - 1]} failed")
+        built.append(cmd[cmd.index("-o") + 1])
+    return built
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_drivers(verbose=True))

@@ -294,10 +294,6 @@ struct SweepFields {
   // its three field slots all point at rhoes with rank 0, so the three identical updates
   // write the same values to the same cells
   int elec_energy;
-  // one wave per field-species (NQ = 1): the launch is one-dimensional and numbered so
-  // that the 19 field-species of a line run on one XCD one after another (they share the
-  // line's velocity-free metric factors in its L2); nx x ny lines
-  int split, nx, ny;
 };
 
 // Every memory operation in the steady state of a sweep is unconditional: loads use
@@ -415,45 +411,19 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
   k.dx = lane63(dx_p);
 }
 
-// Block coordinates of a launch.  In split mode blockIdx.x encodes (line, z) XCD-major,
-// z = 3 (isp - 1) + q for species 1..6 and z = 18 for the electrons' energy; returns false
-// for the padding blocks past the last line.
-constexpr int NZ = 3 * (LSP - 1) + 1;
-__device__ __forceinline__ bool block_coords(SweepFields& F, int isp0, int& bx, int& by,
-                                             int& isp) {
-  if (!F.split) {
-    bx = blockIdx.x; by = blockIdx.y; isp = isp0 + threadIdx.y;
-    return true;
-  }
-  const int slot = blockIdx.x / 8;
-  const int line = (slot / NZ) * 8 + blockIdx.x % 8;
-  if (line >= F.nx * F.ny) return false;
-  const int z = slot % NZ;
-  bx = line % F.nx; by = line / F.nx;
-  int q;
-  if (z < 3 * (LSP - 1)) { isp = z / 3 + 1; q = z % 3; }
-  else { isp = LSP; q = 2; }
-  F.f[0] = F.f[q];
-  F.frank[0] = F.frank[q];
-  return true;
-}
-
 template <int NQ, bool OTF>
 __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFields F, int isp0,
                                                 double dt) {
   const int lx1 = g.lx1;
-  int bx, by, isp;
-  if (!block_coords(F, isp0, bx, by, isp)) return;
-  const int i2 = bx + 1, i3 = by + 1;
-  const int lane = threadIdx.x;
+  const int i2 = blockIdx.x + 1, i3 = blockIdx.y + 1;
+  const int lane = threadIdx.x, isp = isp0 + threadIdx.y;
   const long c0 = gix(g, 0, i2, i3);                                       // cell p -> c0+p
   const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;  // iface p -> if0+p
   const long so = (long)(isp - 1) * g.N;
   const long nif = (long)(lx1 + 1) * g.lx2 * g.lx3;
   double* fp[NQ];
-  const bool e7 = F.elec_energy && isp == LSP;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) fp[q] = (e7 ? F.f[NQ - 1] : F.f[q]) + so + c0;
+  for (int q = 0; q < NQ; ++q) fp[q] = F.f[q] + so + c0;
   const double* u = s.vs1 + so + c0;                                        // u[p] = vs1(p)
   const double* v1i = s.vs1i + (long)(isp - 1) * nif;
   // boundary interface velocities (advec.f90:171-196), the same for the whole wave
@@ -489,22 +459,24 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFiel
 #ifndef SWEEP_PF
 #define SWEEP_PF 2      // x2/x3 march pipeline depth (steps of loads in flight) with 2-3 fields
 #endif
-#ifndef SWEEP_SPLIT
-#define SWEEP_SPLIT 0   // 1: one wave per field-species instead of one per species
+#ifndef SWEEP_E7_IN_BLOCK
+#define SWEEP_E7_IN_BLOCK 1   // x2/x3: electrons as a seventh wave of the species block
+#endif
+#ifndef SWEEP23_WAVES
+#define SWEEP23_WAVES         // e.g. __attribute__((amdgpu_waves_per_eu(3))) to cap registers
 #endif
 #ifndef SWEEP_PF1
 #define SWEEP_PF1 4     // the same for single-field launches (fewer registers per step)
 #endif
 
 template <int DIR, int PF, int NQ, bool OTF>
-__global__ __launch_bounds__(448) void k_sweep23(DevGrid g, DevState s, SweepFields F, int isp0,
-                                                 double dt) {
+__global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevState s,
+                                                               SweepFields F, int isp0, double dt) {
   const int lx = DIR == 2 ? g.lx2 : g.lx3;
-  int bx, by, isp;
-  if (!block_coords(F, isp0, bx, by, isp)) return;
   // lanes past lx1 redo column lx1's work: same inputs, same values written to the same cells
-  const int i1 = min((int)(bx * blockDim.x + threadIdx.x), g.lx1 - 1) + 1;
-  const int other = by + 1;                          // i3 for DIR 2, i2 for DIR 3
+  const int i1 = min((int)(blockIdx.x * blockDim.x + threadIdx.x), g.lx1 - 1) + 1;
+  const int isp = isp0 + threadIdx.y;
+  const int other = blockIdx.y + 1;                  // i3 for DIR 2, i2 for DIR 3
   const long cs_ = DIR == 2 ? (long)g.L1 : (long)g.L1 * g.L2;   // cell stride along the line
   const long c0 = DIR == 2 ? gix(g, i1, 0, other) : gix(g, i1, other, 0);  // cell k -> c0+k*cs_
   const long istr = DIR == 2 ? (long)g.lx1 : (long)g.lx1 * g.lx2;          // iface stride
@@ -725,14 +697,10 @@ void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t 
 
 namespace {
 template <int NQ>
-void sweep_launch(const DevGrid& g, const DevState& s, int dir, SweepFields F, int isp0,
+void sweep_launch(const DevGrid& g, const DevState& s, int dir, const SweepFields& F, int isp0,
                   int nsp, double dt, bool otf, hipStream_t st) {
   const dim3 blk(64, nsp);
-  const int nx = dir == 1 ? g.lx2 : (g.lx1 + 63) / 64;
-  const int ny = dir == 1 ? g.lx3 : dir == 2 ? g.lx3 : g.lx2;
-  F.nx = nx;
-  F.ny = ny;
-  const dim3 grd = F.split ? dim3((unsigned)(((long)nx * ny + 7) / 8 * 8 * NZ)) : dim3(nx, ny);
+  const dim3 grd(dir == 1 ? g.lx2 : (g.lx1 + 63) / 64, dir == 1 ? g.lx3 : dir == 2 ? g.lx3 : g.lx2);
   if (dir == 1) {
     if (otf) hipLaunchKernelGGL((k_sweep1<NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
     else hipLaunchKernelGGL((k_sweep1<NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
@@ -762,11 +730,9 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
   timing_begin(name, st);
   if (na == 1 && fields == F_ENERGY) {       // energy alone: all seven species in one launch
     sweep_launch<1>(g, s, dir, A, 1, LSP, dt, onthefly, st);
-  } else if (na == 3 && SWEEP_SPLIT) {       // all fields, one wave per field-species
-    A.split = 1;
-    sweep_launch<1>(g, s, dir, A, 1, 1, dt, onthefly, st);
-  } else if (na == 3 && dir != 1) {          // x2/x3, all fields: the electron wave joins the
-    A.elec_energy = 1;                       // block and does energy only
+  } else if (na == 3 && dir != 1 && SWEEP_E7_IN_BLOCK) {
+    // x2/x3, all fields: the electron wave joins the block and does energy only
+    A.elec_energy = 1;
     sweep_launch<3>(g, s, dir, A, 1, LSP, dt, onthefly, st);
   } else {                                   // (x1: six-wave blocks pack the CUs better)
     if (na == 1) sweep_launch<1>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);

@@ -1,0 +1,117 @@
+// A C++ host driving the transport hot path through include/gemini3d_hip.h in the order of
+// the reference's fluid_adv (src/libgemini.cpp:155-208): the reference-named entry points
+// take the address of each state-block pointer, sizes come from library state.  Test
+// program: reads a case written by the test suite (case.txt, in.bin), runs nsteps steps
+// and writes out.bin.  mode 0 calls the entry points one by one on host blocks; mode 1
+// pushes the state once and runs gemhip_transport_step on HBM.
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "gemini3d_hip.h"
+
+namespace {
+
+std::vector<double> take(FILE* f, size_t n) {
+  std::vector<double> v(n);
+  if (fread(v.data(), sizeof(double), n, f) != n) {
+    fprintf(stderr, "capi_driver: short read\n");
+    exit(2);
+  }
+  return v;
+}
+
+void check(int rc, const char* what) {
+  if (rc != 0) {
+    fprintf(stderr, "capi_driver: %s failed (%d): %s\n", what, rc, gemhip_last_error());
+    exit(3);
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc != 2) { fprintf(stderr, "usage: capi_driver <case dir>\n"); return 1; }
+  const std::string dir = argv[1];
+  int lx1, lx2, lx3, xtype, gridflag, flagper, isg[6], solver, nsteps, mode;
+  double dt, Teinf;
+  FILE* c = fopen((dir + "/case.txt").c_str(), "r");
+  if (!c || fscanf(c, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &lx1, &lx2, &lx3,
+                   &xtype, &gridflag, &flagper, &isg[0], &isg[1], &isg[2], &isg[3], &isg[4],
+                   &isg[5], &solver, &nsteps, &mode, &dt, &Teinf) != 17) {
+    fprintf(stderr, "capi_driver: bad case.txt\n");
+    return 1;
+  }
+  fclose(c);
+  const size_t L = (size_t)(lx1 + 4) * (lx2 + 4) * (lx3 + 4);
+  const size_t n1 = (size_t)(lx1 + 1) * lx2 * lx3, n2 = (size_t)lx1 * (lx2 + 1) * lx3,
+               n3 = (size_t)lx1 * lx2 * (lx3 + 1), ni = (size_t)lx1 * lx2 * lx3;
+  FILE* f = fopen((dir + "/in.bin").c_str(), "rb");
+  if (!f) { fprintf(stderr, "capi_driver: no in.bin\n"); return 1; }
+  auto h1 = take(f, L), h2 = take(f, L), h3 = take(f, L);
+  auto h1x1i = take(f, n1), h2x1i = take(f, n1), h3x1i = take(f, n1);
+  auto h1x2i = take(f, n2), h2x2i = take(f, n2), h3x2i = take(f, n2);
+  auto h1x3i = take(f, n3), h2x3i = take(f, n3), h3x3i = take(f, n3);
+  auto dx1 = take(f, lx1 + 3), dx1i = take(f, lx1), dx2 = take(f, lx2 + 3), dx2i = take(f, lx2);
+  auto dx3 = take(f, lx3 + 3), dx3i = take(f, lx3);
+  auto nn = take(f, 3 * ni), Tn = take(f, ni);
+  auto fluid = take(f, 35 * L), aux = take(f, 23 * L), electro = take(f, 7 * L);
+  auto vs1i = take(f, 7 * n1), vs2i = take(f, 7 * n2), vs3i = take(f, 7 * n3);
+  fclose(f);
+
+  check(gemhip_init(0), "gemhip_init");
+  gemhip_set_error_mode(1);
+  check(gemhip_set_grid_size(lx1, lx2, lx3), "gemhip_set_grid_size");
+  check(gemhip_register_grid(xtype, gridflag, flagper, isg, h1.data(), h2.data(), h3.data(),
+                             h1x1i.data(), h2x1i.data(), h3x1i.data(), h1x2i.data(),
+                             h2x2i.data(), h3x2i.data(), h1x3i.data(), h2x3i.data(),
+                             h3x3i.data(), dx1.data(), dx1i.data(), dx2.data(), dx2i.data(),
+                             dx3.data(), dx3i.data()),
+        "gemhip_register_grid");
+  check(gemhip_register_neutrals(nn.data(), Tn.data()), "gemhip_register_neutrals");
+  check(gemhip_register_config(solver, Teinf), "gemhip_register_config");
+  check(gemhip_set_interface_vels(vs1i.data(), vs2i.data(), vs3i.data()), "set_interface_vels");
+
+  double* fluidvars = fluid.data();
+  double* fluidauxvars = aux.data();
+  double* electrovars = electro.data();
+  void* xC = nullptr;       // the reference's curvmesh, cfg and intvars objects: their
+  void* intvars = nullptr;  // contents were registered above
+  void* cfgC = nullptr;
+  int lsp = 7;
+  if (mode == 1) {
+    check(gemhip_set_residency(1), "set_residency");
+    check(gemhip_state_push(fluidvars, fluidauxvars, electrovars), "state_push");
+    for (int k = 0; k < nsteps; ++k) check(gemhip_transport_step(dt), "transport_step");
+    check(gemhip_state_pull(fluidvars, fluidauxvars), "state_pull");
+  } else {
+    for (int k = 0; k < nsteps; ++k) {
+      // src/libgemini.cpp:155-186
+      v12rhov1_C(&fluidvars, &fluidauxvars);
+      T2rhoe_C(&fluidvars, &fluidauxvars);
+      set_global_boundaries_allspec_C(&xtype, &xC, &fluidvars, &fluidauxvars, &intvars, &lsp);
+      halo_fluidvars_C(&xtype, &xC, &fluidvars, &fluidauxvars);
+      interface_vels_allspec_C(&xtype, &xC, &fluidvars, &intvars, &lsp);
+      sweep3_allparams_C(&fluidvars, &fluidauxvars, &intvars, &xtype, &xC, &dt);
+      sweep1_allparams_C(&fluidvars, &fluidauxvars, &intvars, &xtype, &xC, &dt);
+      halo_allparams_C(&xtype, &xC, &fluidvars, &fluidauxvars);
+      sweep2_allparams_C(&fluidvars, &fluidauxvars, &intvars, &xtype, &xC, &dt);
+      rhov12v1_C(&fluidvars, &fluidauxvars);
+      // multifluid.f90:179-189, then src/libgemini.cpp:208
+      rhoe2T_C(&fluidvars, &fluidauxvars);
+      energy_diffusion_C(&cfgC, &xtype, &xC, &fluidvars, &electrovars, &intvars, &dt);
+      if (gemhip_last_error()[0]) check(1, "step");
+    }
+    check(gemhip_get_interface_vels(vs1i.data(), vs2i.data(), vs3i.data()), "get_interface_vels");
+  }
+  FILE* o = fopen((dir + "/out.bin").c_str(), "wb");
+  fwrite(fluid.data(), sizeof(double), fluid.size(), o);
+  fwrite(aux.data(), sizeof(double), 14 * L, o);
+  fwrite(vs1i.data(), sizeof(double), vs1i.size(), o);
+  fwrite(vs2i.data(), sizeof(double), vs2i.size(), o);
+  fwrite(vs3i.data(), sizeof(double), vs3i.size(), o);
+  fclose(o);
+  gemhip_finalize();
+  return 0;
+}

@@ -35,3 +35,42 @@ def compare(got, ref, exact=(), close=(), rtol=1e-12, interior_close=True):
             worst = max(worst, r)
             assert r <= rtol, f"{name} species {s + 1}: normwise rel diff {r:.3e} > {rtol:g}"
     return worst
+
+
+# ------------------------------------------------------------------ host-driver cases
+DRIVER_ARRAYS = ("h1", "h2", "h3", "h1x1i", "h2x1i", "h3x1i", "h1x2i", "h2x2i", "h3x2i",
+                 "h1x3i", "h2x3i", "h3x3i", "dx1", "dx1i", "dx2", "dx2i", "dx3", "dx3i")
+
+
+def write_case(d, g, st, dt, nsteps, mode, solver=2, Teinf=1500.0):
+    """Case files read by tests/drivers/*: case.txt (sizes, flags, step control) and
+    in.bin (every input array, Fortran order, little-endian float64, concatenated)."""
+    import os
+    isg = " ".join(str(int(bool(v))) for v in g.isglobal)
+    xtype = 1 if g.kind == "cartesian" else 2
+    with open(os.path.join(d, "case.txt"), "w") as f:
+        f.write(f"{g.lx1} {g.lx2} {g.lx3} {xtype} {int(g.gridflag)} {int(bool(g.flagper))} {isg} "
+                f"{solver} {nsteps} {mode} {dt!r} {Teinf!r}\n")
+    parts = [np.asarray(getattr(g, n), dtype="<f8") for n in DRIVER_ARRAYS]
+    parts += [st.nn[..., :3], st.Tn, st.fluidvars, st.fluidauxvars, st.electrovars,
+              st.vs1i, st.vs2i, st.vs3i]
+    with open(os.path.join(d, "in.bin"), "wb") as f:
+        for a in parts:
+            f.write(np.asarray(a, dtype="<f8").tobytes(order="F"))
+
+
+def read_case_output(d, st):
+    """Load out.bin (fluidvars, rhovs1/rhoes, vs1i, vs2i, vs3i) into a copy of st."""
+    import os
+    out = st.copy()
+    raw = np.fromfile(os.path.join(d, "out.bin"), dtype="<f8")
+    pos = 0
+    for name, sl in (("fluidvars", None), ("fluidauxvars", 14), ("vs1i", None), ("vs2i", None),
+                     ("vs3i", None)):
+        a = getattr(out, name)
+        view = a[..., :sl] if sl else a
+        n = view.size
+        view[...] = raw[pos:pos + n].reshape(view.shape, order="F")
+        pos += n
+    assert pos == raw.size
+    return out

@@ -70,3 +70,17 @@ def test_front_end_fails_loudly_without_library(tmp_path):
             transport.load_library(str(tmp_path / "libgemhip.so"))
     finally:
         transport._lib = saved
+
+
+def test_fortran_binding_declares_every_entry_point():
+    """gemini_amd/fortran/gemhip_bind.f90 binds every function include/gemini3d_hip.h
+    declares in part 1 (the reference's hot-path C API) by its C name."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "gemini3d_hip.h")).read()
+    part1 = hdr[hdr.index("-- part 1:"):hdr.index("-- part 2:")]
+    names = set(re.findall(r"void\s+(\w+_C)\s*\(", part1))
+    f90 = open(os.path.join(root, "gemini_amd", "fortran", "gemhip_bind.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(\w+)"\)', f90))
+    assert len(names) == 22
+    assert names <= bound, sorted(names - bound)
