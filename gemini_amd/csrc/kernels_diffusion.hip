@@ -39,33 +39,36 @@ struct Elim {
 };
 
 // ---------------------------------------------------------------- column-group kernel
-// One wavefront (the whole workgroup) owns 64 columns of one species.  Each pass over
-// the rows is one of:
-//   forward (per stage): the input rows arrive in chunks of R rows; the wave loads a
-//     chunk of all 64 columns cooperatively (lane -> (row, column), so every load
-//     instruction reads whole 64-byte segments), computes the conductivity and the
-//     coefficients C, B, D of each (row, column) in parallel into LDS, and each lane
-//     then runs the sequential pivoted elimination of its own column over the chunk;
-//   back substitution (per stage): each lane walks its column from row N to 1, R rows
-//     at a time, with the factors of the next R rows already loaded.
-// The BDF2 stage recomputes its coefficients from the inputs (one more read of Ts, ns
-// and the metric factors) instead of keeping them from the TR stage.  Scratch per (row,
-// lane): the pivot row U0, U1, its right-hand side y, the TR solution and U2, the
-// pivoting fill-in, which matters only from a column's first row swap on.  Outputs are
-// staged through LDS so global stores are coalesced too.  Blocks are numbered so that
-// the seven species of a column group run on one XCD and share its L2.
+// One wavefront (the whole workgroup) owns 64 columns of one species, one lane per
+// column, and makes four passes over the rows:
+//   1 (top-down): the input rows arrive in chunks of R rows.  The wave loads a chunk of
+//     all 64 columns cooperatively (lane -> (row, column), so each load instruction reads
+//     whole 64-byte segments), computes conductivity and the coefficients C, B, D of each
+//     (row, column) in parallel into LDS, and each lane then builds its column's rows
+//     and runs the sequential pivoted elimination of the first solve (TR, or backward
+//     Euler) with its right-hand side.  For TRBDF2 the same pass also factors the BDF2
+//     matrix, which does not depend on the TR solution: its pivot rows, multipliers and
+//     row interchanges are stored, with the part of its right-hand side that depends on
+//     T alone;
+//   2 (bottom-up): back substitution of the first solve (into Ts for backward Euler);
+//   3 (top-down, TRBDF2): the BDF2 right-hand side from the TR solution, taken through
+//     the stored interchanges and multipliers (dgbtrs's forward solve);
+//   4 (bottom-up, TRBDF2): back substitution of the BDF2 solve into Ts.
+// Every value is formed with the operations, and in the order, LAPACK's dgbtf2, dgbtrs
+// and dtbsv would use on the reference's band matrix: factorising a matrix before its
+// right-hand side is known changes nothing in either.  The pivot rows, right-hand sides
+// and solutions live in lane-interleaved HBM scratch (one 512-byte segment per row and
+// slot for the wave); outputs are staged through LDS so global stores are coalesced.
+// Blocks are numbered so that the seven species of a column group run on one XCD.
 //
 // The steady state of every pass issues its memory operations unconditionally (clamped
-// indices; stores that must not land are redirected to a slot a later store of the same
-// lane overwrites, or that is never read), so the compiler's wait counters stay exact and
-// the loads issued ahead stay in flight.  Lanes of a group past the last column solve
-// the last column again and write the same values to the same places.
+// indices; stores that must not land go where a later store of the same lane overwrites
+// them), so the compiler's wait counters stay exact and loads issued ahead stay in
+// flight.  Lanes of a group past the last column solve the last column again and write
+// the same values to the same places.
 constexpr int LP = 65;         // padded LDS row (conflict-free transposed access)
 #ifndef DIFF_CHUNK
 #define DIFF_CHUNK 8           // rows per chunk: 4 x 8 x 65 doubles of LDS per wave
-#endif
-#ifndef DIFF_WAVES_PER_SIMD
-#define DIFF_WAVES_PER_SIMD 2  // register budget: 512 / this many VGPRs per lane
 #endif
 #ifndef DIFF_BATCH_I
 #define DIFF_BATCH_I 8         // ion cells per lane whose inputs are loaded together
@@ -73,11 +76,20 @@ constexpr int LP = 65;         // padded LDS row (conflict-free transposed acces
 #ifndef DIFF_BATCH_E
 #define DIFF_BATCH_E 4         // the same for electron cells (nine inputs each)
 #endif
-enum : int { SU0 = 0, SU1 = 1, SY = 2, STR = 3, SU2 = 4, NSLOT = 5 };
+// scratch slots per (row, lane)
+enum : int {
+  SU0 = 0, SU1 = 1, SU2 = 2, SY = 3,   // first solve: pivot row (U0 U1 U2) and its rhs y;
+                                       // pass 3 reuses SY for the BDF2 rhs
+  STR = 4,                             // TR solution
+  SV0 = 5, SV1 = 6, SV2 = 7, SL = 8,   // BDF2 pivot rows and multipliers
+  SAT = 9,                             // -((1/3 T)/(dt/3)): the T part of the BDF2 rhs
+  NSLOT = 10
+};
 
 struct Group {
   int N;
   double* scr;                 // this lane's scratch: slot k of row i at scr[((i-1)*NSLOT+k)*64]
+  int* swb;                    // BDF2 row interchanges, bit (j&31) of word j>>5 at swb[(j>>5)*64]
 };
 
 __device__ __forceinline__ double& S2(const Group& q, int i, int k) {
@@ -132,13 +144,11 @@ __device__ __forceinline__ void cell_coeffs(const CellIn<ELEC>& c, int isp, doub
   Dr = ((lambda * c.h2) * c.h3) / c.h1;
 }
 
-// pivoted elimination step j (dgbtf2 + forward dgbtrs) bringing in row j+1, applied only
-// when `real`; fsw is the first step of this column that swapped rows.  Every store is
-// made: a step that is not real stores into row sj, which the real step sj stores over
-// later, and U2 before the first swap goes to row 1's U2 slot, which is read only when
-// the first swap is at step 1 (and then every U2 store is a real one).
-__device__ __forceinline__ void elim2(const Group& q, Elim& e, int& fsw, int j, int sj, bool real,
-                                      double a, double b, double cc, double d) {
+// Pivoted elimination step j (dgbtf2 with dgbtrs's forward solve) bringing in row j+1 =
+// (a, b, cc | d), applied to the state only when `real`.  The row at pivot position j is
+// stored at row sj (a step that is not real stores where the real step sj stores later).
+__device__ __forceinline__ void elim_rhs(const Group& q, Elim& e, int j, int sj, bool real,
+                                         double a, double b, double cc, double d) {
   const bool swap = fabs(a) > fabs(e.r0);
   double p0, p1, p2, pd, o0, o1, o2, od;
   if (swap) { p0 = a; p1 = b; p2 = cc; pd = d; o0 = e.r0; o1 = e.r1; o2 = 0.0; od = e.rb; }
@@ -153,37 +163,65 @@ __device__ __forceinline__ void elim2(const Group& q, Elim& e, int& fsw, int j, 
   } else {
     sing = true;
   }
-  const int f2 = (real && swap && j < fsw) ? j : fsw;
-  S2(q, sj, SU0) = p0; S2(q, sj, SU1) = p1; S2(q, sj, SY) = pd;
-  S2(q, sj >= f2 ? sj : 1, SU2) = p2;
+  S2(q, sj, SU0) = p0; S2(q, sj, SU1) = p1; S2(q, sj, SU2) = p2; S2(q, sj, SY) = pd;
   if (real) {
-    fsw = f2;
     e.singular = e.singular || sing;
     e.r0 = o1; e.r1 = o2; e.rb = od;
   }
 }
 
-template <int R>
-__global__ __launch_bounds__(64, DIFF_WAVES_PER_SIMD) void k_diffusion(DevGrid g, DevState s, double* scratch,
-                                                  Coef tr, Coef bdf, Coef be, int backeuler,
-                                                  double Teinf);
+// The same step on the matrix alone (the BDF2 factorisation): stores the pivot row, the
+// multiplier l and the interchange bit that the right-hand side will need in pass 3.
+struct Fact {
+  double r0, r1;
+  bool singular;
+  int word, bits;              // interchange bits of steps 32*word .. 32*word+31
+};
 
-// The whole solve of one column group; ELEC selects the electron conductivity.
-template <int R, bool ELEC>
+__device__ __forceinline__ void elim_mat(const Group& q, Fact& f, int j, int sj, bool real,
+                                         double a, double b, double cc) {
+  const bool swap = fabs(a) > fabs(f.r0);
+  double p0, p1, p2, o0, o1, o2;
+  if (swap) { p0 = a; p1 = b; p2 = cc; o0 = f.r0; o1 = f.r1; o2 = 0.0; }
+  else      { p0 = f.r0; p1 = f.r1; p2 = 0.0; o0 = a; o1 = b; o2 = cc; }
+  bool sing = false;
+  double l = 0.0;
+  if (p0 != 0.0) {
+    const double rp = 1.0 / p0;
+    l = rp * o0;
+    if (p1 != 0.0) o1 = o1 + l * (-p1);
+    if (p2 != 0.0) o2 = o2 + l * (-p2);
+  } else {
+    sing = true;
+  }
+  S2(q, sj, SV0) = p0; S2(q, sj, SV1) = p1; S2(q, sj, SV2) = p2; S2(q, sj, SL) = l;
+  if (real) {
+    f.singular = f.singular || sing;
+    f.r0 = o1; f.r1 = o2;
+    const int w = j >> 5;
+    f.bits = (w == f.word ? f.bits : 0) | ((swap ? 1 : 0) << (j & 31));
+    f.word = w;
+  }
+  q.swb[f.word * 64] = f.bits;
+}
+
+template <int R, bool BE, bool ELEC>
 __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState& s,
-                                                double* scratch, const Coef& tr, const Coef& bdf,
-                                                const Coef& be, int backeuler, double Teinf,
-                                                int isp, long cg, long gps,
-                                                double (*sT)[LP], double (*sC)[LP],
-                                                double (*sB)[LP], double (*sD)[LP]) {
+                                                double* scratch, int* swbits, const Coef& k1,
+                                                const Coef& bdf, double Teinf, int isp,
+                                                long cg, long gps, double (*sT)[LP],
+                                                double (*sC)[LP], double (*sB)[LP],
+                                                double (*sD)[LP]) {
   const long ncol = (long)g.lx2 * g.lx3;
   const int lane = threadIdx.x;
   const long colbase0 = cg * 64;                     // first column of the group
   const int col = (int)min(colbase0 + lane, ncol - 1);
   const int N = g.lx1;
+  const long grp = (long)(isp - 1) * gps + cg;
   Group q;
   q.N = N;
-  q.scr = scratch + ((long)(isp - 1) * gps + cg) * (long)N * NSLOT * 64 + lane;
+  q.scr = scratch + grp * (long)N * NSLOT * 64 + lane;
+  q.swb = swbits + grp * (long)((N + 31) / 32) * 64 + lane;
   const long so = (long)(isp - 1) * g.N;
   const long ni = (long)g.lx1 * g.lx2 * g.lx3;
   const long kb = (long)g.lx1 * col;
@@ -200,7 +238,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   // cooperative mapping of a chunk: lane -> (row lr, column sub-index cs); R loads per lane
   const int lr = lane % R, cs = lane / R;
   constexpr int CPI = 64 / R;                        // columns per load instruction
-  constexpr int BATCH = ELEC ? DIFF_BATCH_E : DIFF_BATCH_I;   // cells whose inputs load together
+  constexpr int BATCH = ELEC ? DIFF_BATCH_E : DIFF_BATCH_I;
   static_assert(R % BATCH == 0 || R < BATCH, "chunk and batch sizes");
   constexpr int NB = R < BATCH ? R : BATCH;
   int cbk[R];                                        // gix(0, i2, i3) of the columns this lane loads
@@ -232,20 +270,12 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     }
   };
 
-  // Forward pass of stage 1 (TR, or backward Euler) or stage 2 (BDF2): rows arrive in
-  // chunks; row r's arrival completes the coefficients of row ro = r-1, whose elimination
-  // is step ro-1.  Rows 1 and 2 (and rows past N) are not real steps.
-  auto forward = [&](int stage, Elim& e, int& fsw) {
-    const Coef& k1 = stage == 2 ? bdf : (backeuler ? be : tr);
-    e = Elim{1.0, 0.0, Tmin, false};
-    fsw = N + 1;
+  // walk the rows in chunks; row r's arrival completes the coefficients of row ro = r-1
+  // (2..N-1), handed to `row(ro, real, t1f, t2f, t3f, Tm, Tc, Tp)`; rows 1 and 2 and rows
+  // past N are not real and the handler must leave its state alone for them
+  auto rows = [&](auto row) {
     double Tm = 0.0, Tc = 0.0, Cc = 0.0, Bc = 0.0, Dm = 0.0, Dc = 1.0;
     for (int r0 = 1; r0 <= N; r0 += R) {
-      double trv[R];                                 // TR of the rows finalised below
-      if (stage == 2) {
-#pragma unroll
-        for (int k = 0; k < R; ++k) trv[k] = S2(q, clampi(r0 + k - 1, 1, N), STR);
-      }
       __builtin_amdgcn_wave_barrier();
       load_chunk(r0);
       __builtin_amdgcn_wave_barrier();
@@ -253,105 +283,89 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
       for (int k = 0; k < R; ++k) {
         const int r = r0 + k;
         const double Tp = sT[k][lane], Cp = sC[k][lane], Bp = sB[k][lane], Dp = sD[k][lane];
-        const int ro = r - 1;                       // row being finalised (2..N-1)
-        const bool real = r >= 3 && r <= N;
+        const int ro = r - 1;
         const int rc = clampi(ro, 2, N - 1);
         const double Dhr = 0.5 * (Dm + Dc), Dhr1 = 0.5 * (Dc + Dp);
         const double t1f = ((Cc * Dhr) / dx1i[rc]) / dx1[rc];
         const double t3f = ((Cc * Dhr1) / dx1i[rc]) / dx1[rc + 1];
         const double t2f = Bc / (dx1[rc + 1] + dx1[rc]);
-        double a, b, c, d;
-        if (stage == 2) {                           // PDEparabolic.f90:146-216
-          a = -t1f + t2f; b = (k1.K + t3f) + t1f; c = -t3f - t2f;
-          d = -((k1.c13 * Tc) / k1.dt3) + (k1.c43 * trv[k]) / k1.dt3;
-        } else if (backeuler) {                     // PDEparabolic.f90:221-346
-          a = -t1f + t2f; b = (k1.K + t3f) + t1f; c = -t3f - t2f; d = Tc / k1.dt;
-        } else {                                    // PDEparabolic.f90:19-145
-          const double t1 = t1f / 2.0, t2 = t2f / 2.0, t3 = t3f / 2.0;
-          a = -t1 + t2; b = (k1.K + t3) + t1; c = -t3 - t2;
-          d = ((Tc / k1.dt2 - a * Tm) - (t3 + t1) * Tc) - c * Tp;
-        }
-        elim2(q, e, fsw, ro - 1, clampi(ro - 1, 1, N - 1), real, a, b, c, d);
+        row(ro, r >= 3 && r <= N, t1f, t2f, t3f, Tm, Tc, Tp);
         Tm = Tc; Tc = Tp;
         Cc = Cp; Bc = Bp;
         Dm = Dc; Dc = Dp;
       }
     }
-    elim2(q, e, fsw, N - 1, N - 1, true, 0.0, 1.0, 0.0, Tmax);   // Dirichlet row N
-    if (e.r0 == 0.0) e.singular = true;
-    S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SY) = e.rb;
-    S2(q, N >= fsw ? N : 1, SU2) = 0.0;
   };
 
-  // For lanes whose factorisation is singular, dgbsv returns the right-hand side
-  // unchanged: store it in the y slot (a slow pass that never runs in practice).
-  auto rhs_pass = [&](int stage, bool mine) {
-    const Coef& k1 = stage == 2 ? bdf : (backeuler ? be : tr);
+  // rows of the first solve (PDEparabolic.f90:19-145 TR half step, :221-346 backward Euler)
+  auto first_row = [&](double t1f, double t2f, double t3f, double Tm, double Tc, double Tp,
+                       double& a, double& b, double& c, double& d) {
+    if (BE) {
+      a = -t1f + t2f; b = (k1.K + t3f) + t1f; c = -t3f - t2f; d = Tc / k1.dt;
+    } else {
+      const double t1 = t1f / 2.0, t2 = t2f / 2.0, t3 = t3f / 2.0;
+      a = -t1 + t2; b = (k1.K + t3) + t1; c = -t3 - t2;
+      d = ((Tc / k1.dt2 - a * Tm) - (t3 + t1) * Tc) - c * Tp;
+    }
+  };
+
+  // pass 1
+  Elim e{1.0, 0.0, Tmin, false};
+  Fact f{1.0, 0.0, false, 0, 0};
+  rows([&](int ro, bool real, double t1f, double t2f, double t3f, double Tm, double Tc,
+           double Tp) {
+    const int j = ro - 1, sj = clampi(j, 1, N - 1);
+    double a, b, c, d;
+    first_row(t1f, t2f, t3f, Tm, Tc, Tp, a, b, c, d);
+    elim_rhs(q, e, j, sj, real, a, b, c, d);
+    if (!BE) {                                       // PDEparabolic.f90:146-216 matrix
+      elim_mat(q, f, j, sj, real, -t1f + t2f, (bdf.K + t3f) + t1f, -t3f - t2f);
+      S2(q, clampi(ro, 1, N), SAT) = -((bdf.c13 * Tc) / bdf.dt3);
+    }
+  });
+  elim_rhs(q, e, N - 1, N - 1, true, 0.0, 1.0, 0.0, Tmax);       // Dirichlet row N
+  if (e.r0 == 0.0) e.singular = true;
+  S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SU2) = 0.0; S2(q, N, SY) = e.rb;
+  if (!BE) {
+    elim_mat(q, f, N - 1, N - 1, true, 0.0, 1.0, 0.0);
+    if (f.r0 == 0.0) f.singular = true;
+    S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0; S2(q, N, SV2) = 0.0;
+  }
+  // a singular factorisation leaves dgbsv's right-hand side as the solution: for the
+  // lanes concerned, store the first solve's right-hand side in the y slot (a slow pass
+  // that never runs on physical inputs)
+  if (__any(e.singular)) {
+    const bool mine = e.singular;
     if (mine) S2(q, 1, SY) = Tmin;
-    double Tm = 0.0, Tc = 0.0, Cc = 0.0, Bc = 0.0, Dm = 0.0, Dc = 1.0;
-    for (int r0 = 1; r0 <= N; r0 += R) {
-      __builtin_amdgcn_wave_barrier();
-      load_chunk(r0);
-      __builtin_amdgcn_wave_barrier();
-      for (int k = 0; k < R && r0 + k <= N; ++k) {
-        const int r = r0 + k;
-        const double Tp = sT[k][lane], Cp = sC[k][lane], Bp = sB[k][lane], Dp = sD[k][lane];
-        if (r >= 3 && mine) {
-          const int ro = r - 1;
-          const double Dhr = 0.5 * (Dm + Dc), Dhr1 = 0.5 * (Dc + Dp);
-          const double t1f = ((Cc * Dhr) / dx1i[ro]) / dx1[ro];
-          const double t3f = ((Cc * Dhr1) / dx1i[ro]) / dx1[ro + 1];
-          const double t2f = Bc / (dx1[ro + 1] + dx1[ro]);
-          double d;
-          if (stage == 2) {
-            d = -((k1.c13 * Tc) / k1.dt3) + (k1.c43 * S2(q, ro, STR)) / k1.dt3;
-          } else if (backeuler) {
-            d = Tc / k1.dt;
-          } else {
-            const double t1 = t1f / 2.0, t2 = t2f / 2.0, t3 = t3f / 2.0;
-            const double a = -t1 + t2, c = -t3 - t2;
-            d = ((Tc / k1.dt2 - a * Tm) - (t3 + t1) * Tc) - c * Tp;
-          }
-          S2(q, ro, SY) = d;
-        }
-        Tm = Tc; Tc = Tp;
-        Cc = Cp; Bc = Bp;
-        Dm = Dc; Dc = Dp;
-      }
-    }
+    rows([&](int ro, bool real, double t1f, double t2f, double t3f, double Tm, double Tc,
+             double Tp) {
+      double a, b, c, d;
+      first_row(t1f, t2f, t3f, Tm, Tc, Tp, a, b, c, d);
+      if (mine && real) S2(q, ro, SY) = d;
+    });
     if (mine) S2(q, N, SY) = Tmax;
-  };
+  }
 
-  auto factor = [&](int stage, Elim& e, int& fsw) {
-    forward(stage, e, fsw);
-    if (__any(e.singular)) rhs_pass(stage, e.singular);
-  };
-
-  // dtbsv over the stored factors (column-oriented: row i takes x(i+2)*U2 off first, then
-  // x(i+1)*U1, and divides by U0; a zero x(j) contributes nothing).  U2 is zero above the
-  // first swap; WU2 = false when no lane of the wave swapped.  A singular factorisation
-  // leaves the right-hand side, stored in the y slot, as the result.  Rows are taken in
-  // the aligned chunks [kR+1, kR+R] from the top; each chunk's results go to `put`,
-  // then `flush(k)` runs.
-  auto backsub = [&](auto wu2, bool singular, int fsw, auto put, auto flush) {
-    constexpr bool WU2 = decltype(wu2)::value;
-    double pu0[R], pu1[R], py[R], pu2[R];
+  // dtbsv over stored pivot rows (slots s0 s1 s2, rhs sy), column-oriented: row i takes
+  // x(i+2)*U2 off first, then x(i+1)*U1, and divides by U0; a zero x(j) contributes
+  // nothing.  Rows are taken in the aligned chunks [kR+1, kR+R] from the top; each
+  // chunk's results go to `put`, then `flush(k)` runs.
+  auto backsub = [&](int s0, int s1, int s2, int sy, bool singular, auto put, auto flush) {
+    double pu0[R], pu1[R], pu2[R], py[R];
     const int kt = (N - 1) / R;                      // top chunk
     auto fetch = [&](int i, int u) {
       const int ic = clampi(i, 1, N);
-      pu0[u] = S2(q, ic, SU0); pu1[u] = S2(q, ic, SU1); py[u] = S2(q, ic, SY);
-      if (WU2) pu2[u] = S2(q, ic, SU2);
+      pu0[u] = S2(q, ic, s0); pu1[u] = S2(q, ic, s1); pu2[u] = S2(q, ic, s2); py[u] = S2(q, ic, sy);
     };
 #pragma unroll
     for (int u = 0; u < R; ++u) fetch(kt * R + 1 + u, u);     // row i lives in slot (i-1) % R
     double x1 = 0.0, x2 = 0.0;
     auto row = [&](int i, int u, bool live) {
-      const double u0 = pu0[u], u1 = pu1[u], y = py[u];
-      const double u2 = WU2 ? pu2[u] : 0.0;
+      const double u0 = pu0[u], u1 = pu1[u], u2 = pu2[u], y = py[u];
       fetch(i - R, u);
       double acc = y;
       if (!singular) {
-        if (WU2 && i + 2 <= N && x2 != 0.0 && i >= fsw) acc = acc - x2 * u2;
+        if (i + 2 <= N && x2 != 0.0) acc = acc - x2 * u2;
         if (i + 1 <= N && x1 != 0.0) acc = acc - x1 * u1;
         if (acc != 0.0) acc = acc / u0;
       }
@@ -387,30 +401,60 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     }
     __builtin_amdgcn_wave_barrier();
   };
-  auto put_tr = [&](int i, int u, double x) { S2(q, i, STR) = x; };
-  auto flush_none = [&](int) {};
 
-  Elim e;
-  int fsw;
-  factor(1, e, fsw);
-  const bool any_swap = __any(fsw <= N);
-  if (backeuler) {
-    if (any_swap) backsub(std::true_type{}, e.singular, fsw, put_out, flush_out);
-    else backsub(std::false_type{}, e.singular, fsw, put_out, flush_out);
+  // pass 2
+  __builtin_amdgcn_wave_barrier();
+  if (BE) {
+    backsub(SU0, SU1, SU2, SY, e.singular, put_out, flush_out);
     return;
   }
-  if (any_swap) backsub(std::true_type{}, e.singular, fsw, put_tr, flush_none);
-  else backsub(std::false_type{}, e.singular, fsw, put_tr, flush_none);
-  factor(2, e, fsw);
-  const bool any_swap2 = __any(fsw <= N);
-  if (any_swap2) backsub(std::true_type{}, e.singular, fsw, put_out, flush_out);
-  else backsub(std::false_type{}, e.singular, fsw, put_out, flush_out);
+  backsub(SU0, SU1, SU2, SY, e.singular, [&](int i, int u, double x) { S2(q, i, STR) = x; },
+          [&](int) {});
+
+  // pass 3: d(r) = AT(r) + ((4/3) TR(r))/(dt/3) for rows 2..N-1 (Tmin, Tmax at the ends),
+  // through the interchanges and multipliers of pass 1; singular lanes keep d itself
+  {
+    constexpr int PF = R;
+    double pat[PF], ptr[PF], pl[PF];
+    int pb[PF];
+    auto fetch = [&](int r, int u) {                 // inputs of step j = r-1
+      const int rc = clampi(r, 1, N), jc = clampi(r - 1, 1, N - 1);
+      pat[u] = S2(q, rc, SAT); ptr[u] = S2(q, rc, STR); pl[u] = S2(q, jc, SL);
+      pb[u] = q.swb[(jc >> 5) * 64];
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u) fetch(2 + u, u);
+    const bool fs = f.singular;
+    S2(q, 1, SY) = Tmin;
+    double rb = Tmin;
+    for (int j0 = 1; j0 <= N - 1; j0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int j = j0 + u, r = j + 1;
+        const double at = pat[u], trr = ptr[u], l = pl[u];
+        const int bits = pb[u];
+        fetch(r + PF, u);
+        const bool live = j <= N - 1;
+        const double d = r >= N ? Tmax : at + (bdf.c43 * trr) / bdf.dt3;
+        const bool swap = (bits >> (j & 31)) & 1;
+        const double pd = swap ? d : rb;
+        double od = swap ? rb : d;
+        if (!fs && pd != 0.0) od = od + l * (-pd);
+        // steps past N-1 store on row N, which the store after the loop overwrites
+        S2(q, live ? (fs ? r : j) : N, SY) = fs ? d : pd;
+        rb = live ? od : rb;
+      }
+    }
+    S2(q, N, SY) = fs ? Tmax : rb;
+  }
+
+  // pass 4
+  backsub(SV0, SV1, SV2, SY, f.singular, put_out, flush_out);
 }
 
-template <int R>
-__global__ __launch_bounds__(64, DIFF_WAVES_PER_SIMD) void k_diffusion(DevGrid g, DevState s, double* scratch,
-                                                  Coef tr, Coef bdf, Coef be, int backeuler,
-                                                  double Teinf) {
+template <int R, bool BE>
+__global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double* scratch,
+                                                  int* swbits, Coef k1, Coef bdf, double Teinf) {
   __shared__ double sT[R][LP], sC[R][LP], sB[R][LP], sD[R][LP];
   const long ncol = (long)g.lx2 * g.lx3;
   const long gps = (ncol + 63) / 64;                 // column groups per species
@@ -419,19 +463,25 @@ __global__ __launch_bounds__(64, DIFF_WAVES_PER_SIMD) void k_diffusion(DevGrid g
   if (cg >= gps) return;
   const int isp = (int)(slot % LSP) + 1;
   if (isp == LSP)
-    diffusion_group<R, true>(g, s, scratch, tr, bdf, be, backeuler, Teinf, isp, cg, gps, sT, sC, sB, sD);
+    diffusion_group<R, BE, true>(g, s, scratch, swbits, k1, bdf, Teinf, isp, cg, gps, sT, sC, sB, sD);
   else
-    diffusion_group<R, false>(g, s, scratch, tr, bdf, be, backeuler, Teinf, isp, cg, gps, sT, sC, sB, sD);
+    diffusion_group<R, BE, false>(g, s, scratch, swbits, k1, bdf, Teinf, isp, cg, gps, sT, sC, sB, sD);
 }
 
 }  // namespace
 
 void launch_floor_ghosts(const DevGrid& g, double* Ts, hipStream_t st);
 
+namespace {
+long diffusion_groups(const DevGrid& g) { return ((long)g.lx2 * g.lx3 + 63) / 64 * LSP; }
+size_t factor_bytes(const DevGrid& g) {
+  return (size_t)diffusion_groups(g) * (size_t)g.lx1 * NSLOT * 64 * sizeof(double);
+}
+}  // namespace
+
+// pivot rows, right-hand sides and solutions, then the BDF2 interchange bits
 size_t diffusion_scratch_bytes(const DevGrid& g) {
-  const long ncol = (long)g.lx2 * g.lx3;
-  const long groups = (ncol + 63) / 64 * LSP;
-  return (size_t)groups * (size_t)g.lx1 * NSLOT * 64 * sizeof(double);
+  return factor_bytes(g) + (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
 }
 
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
@@ -443,9 +493,15 @@ void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int
   be.dt = dt; be.K = 1.0 / dt;
   const long ncol = (long)g.lx2 * g.lx3;
   const long blocks = ((ncol + 63) / 64 + 7) / 8 * 8 * LSP;   // XCD-major numbering
+  int* swbits = (int*)((char*)scratch + factor_bytes(g));
+  const dim3 grd((unsigned)blocks), blk(64);
   timing_begin("trbdf2", st);
-  hipLaunchKernelGGL(k_diffusion<DIFF_CHUNK>, dim3((unsigned)blocks), dim3(64), 0, st, g, s,
-                     scratch, tr, bdf, be, diffsolvetype == 1 ? 1 : 0, Teinf);
+  if (diffsolvetype == 1)
+    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, true>), grd, blk, 0, st, g, s, scratch, swbits, be,
+                       bdf, Teinf);
+  else
+    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false>), grd, blk, 0, st, g, s, scratch, swbits, tr,
+                       bdf, Teinf);
   timing_end("trbdf2", st);
   timing_begin("floor_ghosts", st);
   launch_floor_ghosts(g, s.Ts, st);
