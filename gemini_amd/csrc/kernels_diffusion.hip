@@ -70,6 +70,9 @@ constexpr int LP = 65;         // padded LDS row (conflict-free transposed acces
 #ifndef DIFF_CHUNK
 #define DIFF_CHUNK 8           // rows per chunk: 4 x 8 x 65 doubles of LDS per wave
 #endif
+#ifndef DIFF_SPLIT
+#define DIFF_SPLIT 0           // 1: TRBDF2 as two launches (passes 1-2, passes 3-4)
+#endif
 #ifndef DIFF_BATCH_I
 #define DIFF_BATCH_I 8         // ion cells per lane whose inputs are loaded together
 #endif
@@ -205,9 +208,10 @@ __device__ __forceinline__ void elim_mat(const Group& q, Fact& f, int j, int sj,
   q.swb[f.word * 64] = f.bits;
 }
 
-template <int R, bool BE, bool ELEC>
+template <int R, bool BE, bool ELEC, int PH>
 __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState& s,
-                                                double* scratch, int* swbits, const Coef& k1,
+                                                double* scratch, int* swbits, int* lst,
+                                                const Coef& k1,
                                                 const Coef& bdf, double Teinf, int isp,
                                                 long cg, long gps, double (*sT)[LP],
                                                 double (*sC)[LP], double (*sB)[LP],
@@ -309,43 +313,6 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     }
   };
 
-  // pass 1
-  Elim e{1.0, 0.0, Tmin, false};
-  Fact f{1.0, 0.0, false, 0, 0};
-  rows([&](int ro, bool real, double t1f, double t2f, double t3f, double Tm, double Tc,
-           double Tp) {
-    const int j = ro - 1, sj = clampi(j, 1, N - 1);
-    double a, b, c, d;
-    first_row(t1f, t2f, t3f, Tm, Tc, Tp, a, b, c, d);
-    elim_rhs(q, e, j, sj, real, a, b, c, d);
-    if (!BE) {                                       // PDEparabolic.f90:146-216 matrix
-      elim_mat(q, f, j, sj, real, -t1f + t2f, (bdf.K + t3f) + t1f, -t3f - t2f);
-      S2(q, clampi(ro, 1, N), SAT) = -((bdf.c13 * Tc) / bdf.dt3);
-    }
-  });
-  elim_rhs(q, e, N - 1, N - 1, true, 0.0, 1.0, 0.0, Tmax);       // Dirichlet row N
-  if (e.r0 == 0.0) e.singular = true;
-  S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SU2) = 0.0; S2(q, N, SY) = e.rb;
-  if (!BE) {
-    elim_mat(q, f, N - 1, N - 1, true, 0.0, 1.0, 0.0);
-    if (f.r0 == 0.0) f.singular = true;
-    S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0; S2(q, N, SV2) = 0.0;
-  }
-  // a singular factorisation leaves dgbsv's right-hand side as the solution: for the
-  // lanes concerned, store the first solve's right-hand side in the y slot (a slow pass
-  // that never runs on physical inputs)
-  if (__any(e.singular)) {
-    const bool mine = e.singular;
-    if (mine) S2(q, 1, SY) = Tmin;
-    rows([&](int ro, bool real, double t1f, double t2f, double t3f, double Tm, double Tc,
-             double Tp) {
-      double a, b, c, d;
-      first_row(t1f, t2f, t3f, Tm, Tc, Tp, a, b, c, d);
-      if (mine && real) S2(q, ro, SY) = d;
-    });
-    if (mine) S2(q, N, SY) = Tmax;
-  }
-
   // dtbsv over stored pivot rows (slots s0 s1 s2, rhs sy), column-oriented: row i takes
   // x(i+2)*U2 off first, then x(i+1)*U1, and divides by U0; a zero x(j) contributes
   // nothing.  Rows are taken in the aligned chunks [kR+1, kR+R] from the top; each
@@ -402,6 +369,48 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     __builtin_amdgcn_wave_barrier();
   };
 
+  // the BDF2 factorisation's singular flag, kept between the launches of a split solve
+  int* mysing = lst + (((long)(isp - 1) * gps + cg) * 64 + lane);
+  Elim e{1.0, 0.0, Tmin, false};
+  Fact f{1.0, 0.0, false, 0, 0};
+  if (PH == 2) {
+    f.singular = *mysing != 0;
+  } else {
+  // pass 1
+  rows([&](int ro, bool real, double t1f, double t2f, double t3f, double Tm, double Tc,
+           double Tp) {
+    const int j = ro - 1, sj = clampi(j, 1, N - 1);
+    double a, b, c, d;
+    first_row(t1f, t2f, t3f, Tm, Tc, Tp, a, b, c, d);
+    elim_rhs(q, e, j, sj, real, a, b, c, d);
+    if (!BE) {                                       // PDEparabolic.f90:146-216 matrix
+      elim_mat(q, f, j, sj, real, -t1f + t2f, (bdf.K + t3f) + t1f, -t3f - t2f);
+      S2(q, clampi(ro, 1, N), SAT) = -((bdf.c13 * Tc) / bdf.dt3);
+    }
+  });
+  elim_rhs(q, e, N - 1, N - 1, true, 0.0, 1.0, 0.0, Tmax);       // Dirichlet row N
+  if (e.r0 == 0.0) e.singular = true;
+  S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SU2) = 0.0; S2(q, N, SY) = e.rb;
+  if (!BE) {
+    elim_mat(q, f, N - 1, N - 1, true, 0.0, 1.0, 0.0);
+    if (f.r0 == 0.0) f.singular = true;
+    S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0; S2(q, N, SV2) = 0.0;
+  }
+  // a singular factorisation leaves dgbsv's right-hand side as the solution: for the
+  // lanes concerned, store the first solve's right-hand side in the y slot (a slow pass
+  // that never runs on physical inputs)
+  if (__any(e.singular)) {
+    const bool mine = e.singular;
+    if (mine) S2(q, 1, SY) = Tmin;
+    rows([&](int ro, bool real, double t1f, double t2f, double t3f, double Tm, double Tc,
+             double Tp) {
+      double a, b, c, d;
+      first_row(t1f, t2f, t3f, Tm, Tc, Tp, a, b, c, d);
+      if (mine && real) S2(q, ro, SY) = d;
+    });
+    if (mine) S2(q, N, SY) = Tmax;
+  }
+
   // pass 2
   __builtin_amdgcn_wave_barrier();
   if (BE) {
@@ -410,6 +419,12 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   }
   backsub(SU0, SU1, SU2, SY, e.singular, [&](int i, int u, double x) { S2(q, i, STR) = x; },
           [&](int) {});
+
+  if (PH == 1) {
+    *mysing = f.singular ? 1 : 0;
+    return;
+  }
+  }  // passes 1 and 2
 
   // pass 3: d(r) = AT(r) + ((4/3) TR(r))/(dt/3) for rows 2..N-1 (Tmin, Tmax at the ends),
   // through the interchanges and multipliers of pass 1; singular lanes keep d itself
@@ -452,9 +467,12 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   backsub(SV0, SV1, SV2, SY, f.singular, put_out, flush_out);
 }
 
-template <int R, bool BE>
+// PH 0: the whole solve; for a solve split in two launches, PH 1 runs passes 1-2 and
+// PH 2 passes 3-4 (shorter waves, so the last wave round leaves less of the GPU idle)
+template <int R, bool BE, int PH>
 __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double* scratch,
-                                                  int* swbits, Coef k1, Coef bdf, double Teinf) {
+                                                  int* swbits, int* lst, Coef k1, Coef bdf,
+                                                  double Teinf) {
   __shared__ double sT[R][LP], sC[R][LP], sB[R][LP], sD[R][LP];
   const long ncol = (long)g.lx2 * g.lx3;
   const long gps = (ncol + 63) / 64;                 // column groups per species
@@ -463,9 +481,11 @@ __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double*
   if (cg >= gps) return;
   const int isp = (int)(slot % LSP) + 1;
   if (isp == LSP)
-    diffusion_group<R, BE, true>(g, s, scratch, swbits, k1, bdf, Teinf, isp, cg, gps, sT, sC, sB, sD);
+    diffusion_group<R, BE, true, PH>(g, s, scratch, swbits, lst, k1, bdf, Teinf, isp, cg, gps, sT,
+                                     sC, sB, sD);
   else
-    diffusion_group<R, BE, false>(g, s, scratch, swbits, k1, bdf, Teinf, isp, cg, gps, sT, sC, sB, sD);
+    diffusion_group<R, BE, false, PH>(g, s, scratch, swbits, lst, k1, bdf, Teinf, isp, cg, gps, sT,
+                                      sC, sB, sD);
 }
 
 }  // namespace
@@ -477,11 +497,18 @@ long diffusion_groups(const DevGrid& g) { return ((long)g.lx2 * g.lx3 + 63) / 64
 size_t factor_bytes(const DevGrid& g) {
   return (size_t)diffusion_groups(g) * (size_t)g.lx1 * NSLOT * 64 * sizeof(double);
 }
+size_t swap_bytes(const DevGrid& g);
 }  // namespace
 
-// pivot rows, right-hand sides and solutions, then the BDF2 interchange bits
+namespace {
+size_t swap_bytes(const DevGrid& g) {
+  return (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
+}
+}  // namespace
+
+// pivot rows, right-hand sides and solutions, the BDF2 interchange bits, one flag per lane
 size_t diffusion_scratch_bytes(const DevGrid& g) {
-  return factor_bytes(g) + (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
+  return factor_bytes(g) + swap_bytes(g) + (size_t)diffusion_groups(g) * 64 * sizeof(int);
 }
 
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
@@ -494,14 +521,21 @@ void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int
   const long ncol = (long)g.lx2 * g.lx3;
   const long blocks = ((ncol + 63) / 64 + 7) / 8 * 8 * LSP;   // XCD-major numbering
   int* swbits = (int*)((char*)scratch + factor_bytes(g));
+  int* lst = (int*)((char*)swbits + swap_bytes(g));
   const dim3 grd((unsigned)blocks), blk(64);
   timing_begin("trbdf2", st);
-  if (diffsolvetype == 1)
-    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, true>), grd, blk, 0, st, g, s, scratch, swbits, be,
-                       bdf, Teinf);
-  else
-    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false>), grd, blk, 0, st, g, s, scratch, swbits, tr,
-                       bdf, Teinf);
+  if (diffsolvetype == 1) {
+    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, true, 0>), grd, blk, 0, st, g, s, scratch, swbits,
+                       lst, be, bdf, Teinf);
+  } else if (DIFF_SPLIT) {
+    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false, 1>), grd, blk, 0, st, g, s, scratch, swbits,
+                       lst, tr, bdf, Teinf);
+    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false, 2>), grd, blk, 0, st, g, s, scratch, swbits,
+                       lst, tr, bdf, Teinf);
+  } else {
+    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false, 0>), grd, blk, 0, st, g, s, scratch, swbits,
+                       lst, tr, bdf, Teinf);
+  }
   timing_end("trbdf2", st);
   timing_begin("floor_ghosts", st);
   launch_floor_ghosts(g, s.Ts, st);
