@@ -50,6 +50,7 @@ struct Ctx {
   const double* pushed_fluid = nullptr;
   int error_mode = 0;
   std::string last_error;
+  Tuning tune;
   bool timing = false;
   std::vector<TimedLaunch> pending;
   std::vector<hipEvent_t> pool;
@@ -57,10 +58,20 @@ struct Ctx {
   std::map<std::string, hipEvent_t> open;
 };
 
-Ctx& ctx() {
+// The reference API acts on one implicit context (handle 0).  Ensembles create more
+// (gemhip_context_create) and select one at a time; every call acts on the selected one.
+Ctx& default_ctx() {
   static Ctx c;
   return c;
 }
+std::vector<Ctx*>& contexts() {          // handle h >= 1 is contexts()[h - 1] (null once freed)
+  static std::vector<Ctx*> v;
+  return v;
+}
+Ctx* g_current = nullptr;
+int g_current_handle = 0;
+
+Ctx& ctx() { return g_current ? *g_current : default_ctx(); }
 
 void fail(const char* fmt, ...) {
   char buf[1024];
@@ -244,7 +255,7 @@ void sweep_entry(double** fv, double** av, int* xtype, double* dt, int dir, int 
   if ((fields & F_MASS) && !check_host(fv, "fluidvars")) return;
   if ((fields & (F_MOMENTUM | F_ENERGY)) && !check_host(av, "fluidauxvars")) return;
   for (Var v : vars) if (!upload(io, {v})) return;
-  launch_sweep(c.g, c.s, dir, fields, *dt, false, c.stream);
+  launch_sweep(c.g, c.s, dir, fields, *dt, false, c.stream, c.tune);
   if (!hipok(hipGetLastError(), "sweep launch")) return;
   for (Var v : vars) if (!download(io, {v})) return;
 }
@@ -269,7 +280,7 @@ bool run_energy_diffusion(double dt) {
   }
   if (c.lx1 < 4) { fail("energy diffusion needs lx1 >= 4 (got %d)", c.lx1); return false; }
   if (!ensure_scratch()) return false;
-  launch_energy_diffusion(c.g, c.s, dt, c.diffsolvetype, c.Teinf, c.d_scratch, c.stream);
+  launch_energy_diffusion(c.g, c.s, dt, c.diffsolvetype, c.Teinf, c.d_scratch, c.stream, c.tune);
   return hipok(hipGetLastError(), "energy diffusion launch");
 }
 
@@ -556,10 +567,10 @@ int gemhip_transport_step(double dt) {
   launch_set_global_boundaries(g, s, LSP, st);           // :168
   halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :169
   launch_save_v1_boundaries(g, s, st);                   // :170 (x1 planes kept for next step)
-  launch_sweep(g, s, 3, F_ALL, dt, true, st);            // :174-176
-  launch_sweep(g, s, 1, F_ALL, dt, true, st);            // :178-180
+  launch_sweep(g, s, 3, F_ALL, dt, true, st, c.tune);   // :174-176
+  launch_sweep(g, s, 1, F_ALL, dt, true, st, c.tune);   // :178-180
   halo_fields({NS, RHOVS1, RHOES});                      // :181
-  launch_sweep(g, s, 2, F_ALL, dt, true, st);            // :183-185
+  launch_sweep(g, s, 2, F_ALL, dt, true, st, c.tune);   // :183-185
   launch_post_convert(g, s, st);                         // :186, then multifluid.f90:179-189
   if (!hipok(hipGetLastError(), "advection launch")) return 3;
   c.vsi_current = false;
@@ -759,6 +770,136 @@ void halo_interface_vels_allspec_C(int* xtype, void** xC, double** fluidvars, in
   double* f[2] = {dev_var(VS2), dev_var(VS3)};
   if (*lsp > 0) launch_halo_periodic(c.g, f, 2, 1, *lsp, c.stream);
   download(io, {VS2, VS3});
+}
+
+// ---------------- contexts, tuning and ensembles ----------------
+
+int gemhip_context_create(void) {
+  contexts().push_back(new Ctx());
+  return (int)contexts().size();
+}
+
+int gemhip_context_select(int handle) {
+  if (handle == 0) { g_current = nullptr; g_current_handle = 0; return 0; }
+  if (handle < 0 || handle > (int)contexts().size() || !contexts()[handle - 1]) {
+    fail("no library context %d", handle);
+    return 1;
+  }
+  g_current = contexts()[handle - 1];
+  g_current_handle = handle;
+  return 0;
+}
+
+int gemhip_context_current(void) { return g_current_handle; }
+
+int gemhip_context_destroy(int handle) {
+  if (handle <= 0 || handle > (int)contexts().size() || !contexts()[handle - 1]) return 1;
+  const int keep = g_current_handle == handle ? 0 : g_current_handle;
+  gemhip_context_select(handle);
+  gemhip_finalize();
+  delete contexts()[handle - 1];
+  contexts()[handle - 1] = nullptr;
+  gemhip_context_select(keep);
+  return 0;
+}
+
+int gemhip_set_tuning(const char* knob, int value) {
+  Ctx& c = ctx();
+  const std::string k = knob ? knob : "";
+  if (k == "sweep_pf" && (value == 1 || value == 2)) { c.tune.sweep_pf = value; return 0; }
+  if (k == "diff_chunk" && (value == 4 || value == 8)) { c.tune.diff_chunk = value; return 0; }
+  fail("unknown tuning knob or value: %s = %d", k.c_str(), value);
+  return 1;
+}
+
+int gemhip_get_tuning(const char* knob) {
+  Ctx& c = ctx();
+  const std::string k = knob ? knob : "";
+  if (k == "sweep_pf") return c.tune.sweep_pf;
+  if (k == "diff_chunk") return c.tune.diff_chunk;
+  return -1;
+}
+
+namespace {
+struct EnsembleGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t origin = nullptr;
+  std::vector<hipEvent_t> events;
+};
+std::vector<EnsembleGraph*>& graphs() {
+  static std::vector<EnsembleGraph*> v;
+  return v;
+}
+
+void release_graph(EnsembleGraph* e) {
+  if (e->exec) hipGraphExecDestroy(e->exec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  for (auto ev : e->events) hipEventDestroy(ev);
+  if (e->origin) hipStreamDestroy(e->origin);
+  delete e;
+}
+}  // namespace
+
+// Capture nsteps transport steps of every member (each on its own context's stream) into
+// one graph: a fork event on an origin stream, each member's kernels on its stream, a join
+// back onto the origin.  Members must be in residency mode 1.  Returns a graph id >= 1.
+int gemhip_ensemble_capture(int n, const int* handles, const double* dt, int nsteps) {
+  const int keep = g_current_handle;
+  if (n <= 0 || !handles || !dt || nsteps <= 0) { fail("bad ensemble arguments"); return -1; }
+  for (int m = 0; m < n; ++m) {               // everything a step allocates, before capture
+    if (gemhip_context_select(handles[m])) return -1;
+    Ctx& c = ctx();
+    if (!ready(true) || c.residency != 1) {
+      fail("ensemble member %d is not set up with its state in HBM (residency mode 1)", m);
+      gemhip_context_select(keep);
+      return -1;
+    }
+    if (c.timing) { fail("disable kernel timing before capturing an ensemble"); return -1; }
+    if (!ensure_scratch()) { gemhip_context_select(keep); return -1; }
+    if (!hipok(hipStreamSynchronize(c.stream), "ensemble member sync")) return -1;
+  }
+  auto* e = new EnsembleGraph();
+  bool ok = hipok(hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking), "origin stream");
+  e->events.resize(n + 1, nullptr);
+  for (int m = 0; ok && m <= n; ++m)
+    ok = hipok(hipEventCreateWithFlags(&e->events[m], hipEventDisableTiming), "ensemble event");
+  ok = ok && hipok(hipStreamBeginCapture(e->origin, hipStreamCaptureModeGlobal), "begin capture");
+  ok = ok && hipok(hipEventRecord(e->events[n], e->origin), "fork");
+  for (int m = 0; ok && m < n; ++m) {
+    gemhip_context_select(handles[m]);
+    Ctx& c = ctx();
+    ok = hipok(hipStreamWaitEvent(c.stream, e->events[n], 0), "fork wait");
+    for (int k = 0; ok && k < nsteps; ++k) ok = gemhip_transport_step(dt[m]) == 0;
+    ok = ok && hipok(hipEventRecord(e->events[m], c.stream), "join");
+    ok = ok && hipok(hipStreamWaitEvent(e->origin, e->events[m], 0), "join wait");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t endc = hipStreamEndCapture(e->origin, &graph);
+  e->graph = graph;
+  ok = ok && hipok(endc, "end capture");
+  ok = ok && hipok(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0), "instantiate");
+  gemhip_context_select(keep);
+  if (!ok) { release_graph(e); return -1; }
+  graphs().push_back(e);
+  return (int)graphs().size();
+}
+
+// Launch a captured ensemble graph (asynchronously, then wait when `wait`).
+int gemhip_ensemble_launch(int id, int wait) {
+  if (id <= 0 || id > (int)graphs().size() || !graphs()[id - 1]) { fail("no ensemble graph %d", id); return 1; }
+  EnsembleGraph* e = graphs()[id - 1];
+  if (!hipok(hipGraphLaunch(e->exec, e->origin), "graph launch")) return 2;
+  if (wait && !hipok(hipStreamSynchronize(e->origin), "graph wait")) return 3;
+  return 0;
+}
+
+int gemhip_ensemble_release(int id) {
+  if (id <= 0 || id > (int)graphs().size() || !graphs()[id - 1]) return 1;
+  hipStreamSynchronize(graphs()[id - 1]->origin);
+  release_graph(graphs()[id - 1]);
+  graphs()[id - 1] = nullptr;
+  return 0;
 }
 
 int gemhip_eval_math(int op, const double* x, double* y, long n) {

@@ -38,6 +38,14 @@ __host__ __device__ inline long gix(const DevGrid& g, int i1, int i2, int i3) {
   return (long)(i1 + 1) + (long)g.L1 * ((long)(i2 + 1) + (long)g.L2 * (long)(i3 + 1));
 }
 
+// Kernel variants a context may select (compiled alongside each other; an autotuner picks
+// per grid size): the x2/x3 sweep pipeline depth for three-field launches (1 or 2) and
+// the TRBDF2 chunk height (4 or 8 rows).
+struct Tuning {
+  int sweep_pf = 2;
+  int diff_chunk = 8;
+};
+
 // field-selection bits for the sweeps (multifluid.f90:49-126)
 enum : int { F_MASS = 1, F_MOMENTUM = 2, F_ENERGY = 4, F_ALL = 7 };
 
@@ -55,12 +63,13 @@ void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t 
 // dir 1,2,3; fields = F_* bits; onthefly: interface velocities from the cell velocities
 // (bit-identical to the materialised ones) instead of intvars%vs*i.
 void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, double dt,
-                  bool onthefly, hipStream_t st);
+                  bool onthefly, hipStream_t st, const Tuning& tune = Tuning{});
 
 // ---- launchers (kernels_diffusion.hip) ----
 size_t diffusion_scratch_bytes(const DevGrid& g);
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
-                             double Teinf, double* scratch, hipStream_t st);
+                             double Teinf, double* scratch, hipStream_t st,
+                             const Tuning& tune = Tuning{});
 
 // kernel-timing hooks implemented by the API layer
 void timing_begin(const char* name, hipStream_t st);

@@ -696,7 +696,7 @@ void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t 
 }
 
 namespace {
-template <int NQ>
+template <int NQ, int PFM = SWEEP_PF>
 void sweep_launch(const DevGrid& g, const DevState& s, int dir, const SweepFields& F, int isp0,
                   int nsp, double dt, bool otf, hipStream_t st) {
   const dim3 blk(64, nsp);
@@ -705,11 +705,11 @@ void sweep_launch(const DevGrid& g, const DevState& s, int dir, const SweepField
     if (otf) hipLaunchKernelGGL((k_sweep1<NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
     else hipLaunchKernelGGL((k_sweep1<NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
   } else if (dir == 2) {
-    constexpr int PF = NQ == 1 ? SWEEP_PF1 : SWEEP_PF;
+    constexpr int PF = NQ == 1 ? SWEEP_PF1 : PFM;
     if (otf) hipLaunchKernelGGL((k_sweep23<2, PF, NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
     else hipLaunchKernelGGL((k_sweep23<2, PF, NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
   } else {
-    constexpr int PF = NQ == 1 ? SWEEP_PF1 : SWEEP_PF;
+    constexpr int PF = NQ == 1 ? SWEEP_PF1 : PFM;
     if (otf) hipLaunchKernelGGL((k_sweep23<3, PF, NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
     else hipLaunchKernelGGL((k_sweep23<3, PF, NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
   }
@@ -717,7 +717,7 @@ void sweep_launch(const DevGrid& g, const DevState& s, int dir, const SweepField
 }  // namespace
 
 void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, double dt,
-                  bool onthefly, hipStream_t st) {
+                  bool onthefly, hipStream_t st, const Tuning& tune) {
   const int lx = dir == 1 ? g.lx1 : dir == 2 ? g.lx2 : g.lx3;
   if (lx <= 1) return;                       // advec1D_MC_curv leaves lx = 1 lines alone
   SweepFields A{};
@@ -733,7 +733,8 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
   } else if (na == 3 && dir != 1 && SWEEP_E7_IN_BLOCK) {
     // x2/x3, all fields: the electron wave joins the block and does energy only
     A.elec_energy = 1;
-    sweep_launch<3>(g, s, dir, A, 1, LSP, dt, onthefly, st);
+    if (tune.sweep_pf == 1) sweep_launch<3, 1>(g, s, dir, A, 1, LSP, dt, onthefly, st);
+    else sweep_launch<3, 2>(g, s, dir, A, 1, LSP, dt, onthefly, st);
   } else {                                   // (x1: six-wave blocks pack the CUs better)
     if (na == 1) sweep_launch<1>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);
     else if (na == 2) sweep_launch<2>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);

@@ -511,31 +511,41 @@ size_t diffusion_scratch_bytes(const DevGrid& g) {
   return factor_bytes(g) + swap_bytes(g) + (size_t)diffusion_groups(g) * 64 * sizeof(int);
 }
 
-void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
-                             double Teinf, double* scratch, hipStream_t st) {
-  Coef tr{}, bdf{}, be{};
-  tr.dt2 = dt / 2.0; tr.K = 1.0 / tr.dt2; tr.dt = dt;
-  bdf.dt3 = dt / 3.0; bdf.K = 1.0 / bdf.dt3; bdf.dt = dt;
-  bdf.c13 = 1.0 / 3.0; bdf.c43 = 4.0 / 3.0;
-  be.dt = dt; be.K = 1.0 / dt;
+namespace {
+template <int R>
+void diffusion_launch(const DevGrid& g, const DevState& s, int diffsolvetype, double Teinf,
+                      double* scratch, const Coef& tr, const Coef& bdf, const Coef& be,
+                      hipStream_t st) {
   const long ncol = (long)g.lx2 * g.lx3;
   const long blocks = ((ncol + 63) / 64 + 7) / 8 * 8 * LSP;   // XCD-major numbering
   int* swbits = (int*)((char*)scratch + factor_bytes(g));
   int* lst = (int*)((char*)swbits + swap_bytes(g));
   const dim3 grd((unsigned)blocks), blk(64);
-  timing_begin("trbdf2", st);
   if (diffsolvetype == 1) {
-    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, true, 0>), grd, blk, 0, st, g, s, scratch, swbits,
-                       lst, be, bdf, Teinf);
+    hipLaunchKernelGGL((k_diffusion<R, true, 0>), grd, blk, 0, st, g, s, scratch, swbits, lst, be,
+                       bdf, Teinf);
   } else if (DIFF_SPLIT) {
-    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false, 1>), grd, blk, 0, st, g, s, scratch, swbits,
-                       lst, tr, bdf, Teinf);
-    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false, 2>), grd, blk, 0, st, g, s, scratch, swbits,
-                       lst, tr, bdf, Teinf);
+    hipLaunchKernelGGL((k_diffusion<R, false, 1>), grd, blk, 0, st, g, s, scratch, swbits, lst, tr,
+                       bdf, Teinf);
+    hipLaunchKernelGGL((k_diffusion<R, false, 2>), grd, blk, 0, st, g, s, scratch, swbits, lst, tr,
+                       bdf, Teinf);
   } else {
-    hipLaunchKernelGGL((k_diffusion<DIFF_CHUNK, false, 0>), grd, blk, 0, st, g, s, scratch, swbits,
-                       lst, tr, bdf, Teinf);
+    hipLaunchKernelGGL((k_diffusion<R, false, 0>), grd, blk, 0, st, g, s, scratch, swbits, lst, tr,
+                       bdf, Teinf);
   }
+}
+}  // namespace
+
+void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
+                             double Teinf, double* scratch, hipStream_t st, const Tuning& tune) {
+  Coef tr{}, bdf{}, be{};
+  tr.dt2 = dt / 2.0; tr.K = 1.0 / tr.dt2; tr.dt = dt;
+  bdf.dt3 = dt / 3.0; bdf.K = 1.0 / bdf.dt3; bdf.dt = dt;
+  bdf.c13 = 1.0 / 3.0; bdf.c43 = 4.0 / 3.0;
+  be.dt = dt; be.K = 1.0 / dt;
+  timing_begin("trbdf2", st);
+  if (tune.diff_chunk == 4) diffusion_launch<4>(g, s, diffsolvetype, Teinf, scratch, tr, bdf, be, st);
+  else diffusion_launch<DIFF_CHUNK>(g, s, diffsolvetype, Teinf, scratch, tr, bdf, be, st);
   timing_end("trbdf2", st);
   timing_begin("floor_ghosts", st);
   launch_floor_ghosts(g, s.Ts, st);

@@ -1,0 +1,89 @@
+"""Ensembles of independent grids advanced together on one GPU.
+
+Each member is a :class:`gemini_amd.transport.Transport` with a library context of its
+own (its grid, state in HBM, scratch and stream).  ``Ensemble.run`` captures the members'
+transport steps into one HIP graph, in which the members run concurrently, and launches
+it; ``Ensemble.run_plain`` issues the same steps member by member.  Results are the same
+bit for bit: the graph only changes the order in which independent kernels are issued.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .transport import Transport, GemhipError, load_library
+
+
+class Ensemble:
+    def __init__(self, members, device: int = 0, diffsolvetype: int = 2, Teinf: float = 1500.0):
+        """members: sequence of (grid, state, dt).  States are pushed to HBM (residency
+        mode 1); use pull(i) to read one back."""
+        self.lib = load_library()
+        self.grids = [m[0] for m in members]
+        self.dts = np.array([float(m[2]) for m in members], dtype=np.float64)
+        self.trs = []
+        for g, st, dt in members:
+            tr = Transport(g, st.nn, st.Tn, device=device, diffsolvetype=diffsolvetype,
+                           Teinf=Teinf, own_context=True)
+            tr.set_residency(1)
+            tr.push(st)
+            self.trs.append(tr)
+        self._graphs = {}
+
+    def __len__(self):
+        return len(self.trs)
+
+    def _handles(self):
+        return (C.c_int * len(self.trs))(*[t.handle for t in self.trs])
+
+    def capture(self, nsteps: int) -> int:
+        """Capture nsteps steps of every member into one graph (cached per nsteps)."""
+        if nsteps in self._graphs:
+            return self._graphs[nsteps]
+        self.trs[0]._sel()
+        gid = self.lib.gemhip_ensemble_capture(len(self.trs), self._handles(),
+                                               self.dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                               int(nsteps))
+        if gid <= 0:
+            msgs = [t.lib.gemhip_last_error().decode() for t in self.trs]
+            raise GemhipError("ensemble capture failed: " + "; ".join(m for m in msgs if m))
+        self._graphs[nsteps] = gid
+        return gid
+
+    def run(self, nsteps: int, wait: bool = True):
+        """Advance every member nsteps steps with one graph launch."""
+        gid = self.capture(nsteps)
+        if self.lib.gemhip_ensemble_launch(gid, int(wait)) != 0:
+            raise GemhipError("ensemble graph launch failed")
+
+    def run_plain(self, nsteps: int):
+        """The same steps issued member after member, without a graph."""
+        for _ in range(nsteps):
+            for tr, dt in zip(self.trs, self.dts):
+                tr.transport_step(float(dt))
+        for tr in self.trs:
+            tr.synchronize()
+
+    def autotune(self, states, steps: int = 3):
+        """Choose each member's kernel variants by measurement, then re-push `states`."""
+        from .autotune import autotune
+        return autotune(self, states, steps)
+
+    def synchronize(self):
+        for tr in self.trs:
+            tr.synchronize()
+
+    def push(self, i: int, st):
+        self.trs[i].push(st)
+
+    def pull(self, i: int, st):
+        self.trs[i].pull(st)
+
+    def close(self):
+        for gid in self._graphs.values():
+            self.lib.gemhip_ensemble_release(gid)
+        self._graphs.clear()
+        for tr in self.trs:
+            tr.close()
+        self.trs = []

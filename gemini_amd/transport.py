@@ -45,6 +45,12 @@ def load_library(path: str = LIB_PATH):
                  "gemhip_synchronize", "gemhip_kernel_timing", "gemhip_kernel_time",
                  "gemhip_kernel_timing_reset", "gemhip_eval_math"):
         getattr(lib, name).restype = C.c_int
+    for name in ("gemhip_context_create", "gemhip_context_select", "gemhip_context_current",
+                 "gemhip_context_destroy", "gemhip_set_tuning", "gemhip_get_tuning",
+                 "gemhip_ensemble_capture", "gemhip_ensemble_launch", "gemhip_ensemble_release"):
+        getattr(lib, name).restype = C.c_int
+    lib.gemhip_set_tuning.argtypes = [C.c_char_p, C.c_int]
+    lib.gemhip_get_tuning.argtypes = [C.c_char_p]
     lib.gemhip_last_error.restype = C.c_char_p
     lib.gemhip_stream.restype = C.c_void_p
     lib.gemhip_transport_step.argtypes = [C.c_double]
@@ -88,12 +94,17 @@ class Transport:
             "h1x3i", "h2x3i", "h3x3i")
 
     def __init__(self, grid: Grid, nn, Tn, device: int = 0, diffsolvetype: int = 2,
-                 Teinf: float = 1500.0, geometry_on_device: bool = False):
+                 Teinf: float = 1500.0, geometry_on_device: bool = False,
+                 own_context: bool = False):
         """nn (lx1,lx2,lx3,>=3) and Tn (lx1,lx2,lx3): Fortran numpy arrays, or torch
         tensors holding the same bytes.  With geometry_on_device the x3-invariant metric
-        planes are expanded to 3-D arrays in HBM by torch instead of on the host."""
+        planes are expanded to 3-D arrays in HBM by torch instead of on the host.  With
+        own_context the instance gets a library context of its own (ensemble members);
+        otherwise it uses the implicit one the reference API acts on."""
         self.lib = load_library()
         self.g = grid
+        self.handle = self.lib.gemhip_context_create() if own_context else 0
+        self._sel()
         self.lib.gemhip_set_error_mode(1)
         self._call("gemhip_init", C.c_int(device))
         self._call("gemhip_set_grid_size", grid.lx1, grid.lx2, grid.lx3)
@@ -115,6 +126,23 @@ class Transport:
         self._call("gemhip_set_residency", 0)
 
     # ------------------------------------------------------------------ plumbing
+    def _sel(self):
+        if self.lib.gemhip_context_select(int(self.handle)) != 0:
+            raise GemhipError(f"library context {self.handle} is gone")
+
+    def close(self):
+        """Release an own context (HBM buffers, stream)."""
+        if self.handle:
+            self.lib.gemhip_context_destroy(int(self.handle))
+            self.handle = None
+
+    def set_tuning(self, knob: str, value: int):
+        self._call("gemhip_set_tuning", knob.encode(), int(value))
+
+    def get_tuning(self, knob: str) -> int:
+        self._sel()
+        return int(self.lib.gemhip_get_tuning(knob.encode()))
+
     def _check(self, rc=0):
         msg = self.lib.gemhip_last_error().decode()
         if msg or rc:
@@ -122,6 +150,7 @@ class Transport:
             raise GemhipError(msg or f"error code {rc}")
 
     def _call(self, name, *args):
+        self._sel()
         rc = getattr(self.lib, name)(*args)
         self._check(rc)
         return rc
@@ -170,6 +199,7 @@ class Transport:
 
     @property
     def stream(self) -> int:
+        self._sel()
         return self.lib.gemhip_stream()
 
     def device_state(self):
@@ -190,6 +220,7 @@ class Transport:
 
     def kernel_time(self, name: str):
         ms, n = C.c_double(), C.c_long()
+        self._sel()
         self.lib.gemhip_kernel_time(name.encode(), C.byref(ms), C.byref(n))
         return ms.value, n.value
 
@@ -201,6 +232,7 @@ class Transport:
         return _Ref(st.fluidvars), _Ref(st.fluidauxvars)
 
     def _void(self, name, *args):
+        self._sel()
         getattr(self.lib, name)(*args)
         self._check()
 

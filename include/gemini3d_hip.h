@@ -153,6 +153,29 @@ int gemhip_kernel_timing_reset(void);
    3 the device library pow(x, 2.5).  x and y are host arrays of n doubles. */
 int gemhip_eval_math(int op, const double* x, double* y, long n);
 
+/* ---------------- part 3: contexts, tuning and ensembles (no counterpart in the reference) ---------------- */
+/* The calls above act on the selected library context.  Handle 0 is the implicit context
+   the reference API uses; gemhip_context_create makes another (then gemhip_init it), for
+   ensembles of independent grids.  Selection is process-wide and not thread-safe. */
+int gemhip_context_create(void);
+int gemhip_context_select(int handle);
+int gemhip_context_current(void);
+int gemhip_context_destroy(int handle);
+
+/* Kernel variants of the selected context: "sweep_pf" (1 or 2, x2/x3 sweep pipeline
+   depth) and "diff_chunk" (4 or 8, TRBDF2 rows per chunk).  Results do not depend on
+   them; only speed does. */
+int gemhip_set_tuning(const char* knob, int value);
+int gemhip_get_tuning(const char* knob);
+
+/* Capture nsteps transport steps (gemhip_transport_step with dt[m]) of each of the n
+   contexts handles[m] (residency mode 1, state pushed) into one HIP graph whose members
+   run concurrently, each on its own context's stream.  Returns a graph id > 0, or -1. */
+int gemhip_ensemble_capture(int n, const int* handles, const double* dt, int nsteps);
+/* Launch a captured graph; with wait != 0, return after it has finished. */
+int gemhip_ensemble_launch(int id, int wait);
+int gemhip_ensemble_release(int id);
+
 #ifdef __cplusplus
 }
 #endif
