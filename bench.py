@@ -47,6 +47,12 @@ KERNELS = ["pre_convert", "ghost_fill", "halo_periodic", "save_v1b", "sweep3", "
            "sweep2", "post_convert", "rhov12v1", "rhoe2T", "trbdf2", "floor_ghosts"]
 
 
+ENSEMBLE = dict(members=64, steps=20, seed=1, verify=3,
+                desc="64 independent Cartesian grids, lx1 from {64,96,...,512}, lx2 x lx3 "
+                     "from 16x16 to 128x96 (seeded draws), 7 species, fp64, 20 steps in one "
+                     "HIP graph launch")
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -109,12 +115,115 @@ def load_traffic(workload, kernel):
         return None
 
 
+def ensemble_sizes(seed, n):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice(np.arange(64, 513, 32))), int(rng.integers(16, 129)),
+             int(rng.integers(16, 97))) for _ in range(n)]
+
+
+def ensemble_member(shape, seed, own_context=True):
+    """(grid, Transport with the seeded state pushed, dt) for one ensemble member."""
+    import gemini_amd as ga
+    from gemini_amd.synth_gpu import synthetic_state_device, cfl_dt_device
+    from gemini_amd.transport import Transport
+    g = ga.cartesian_grid(*shape)
+    st = synthetic_state_device(g, seed=seed, device="cuda:0")
+    dt = cfl_dt_device(g, st)
+    tr = Transport(g, st["nn"], st["Tn"], device=0, geometry_on_device=True,
+                   own_context=own_context)
+    tr.set_residency(1)
+    tr.push_blocks(st["fluidvars"], st["aux"], st["electrovars"])
+    return g, tr, dt
+
+
+def bench_ensemble(args):
+    """Batched form: every member advanced `steps` steps by one graph launch; a few members
+    re-run alone with the plain API and compared bit for bit."""
+    import torch
+    from gemini_amd.ensemble import Ensemble
+    from gemini_amd.autotune import best_variant
+    E = ENSEMBLE
+    sizes = ensemble_sizes(E["seed"], E["members"])
+    seeds = [1000 * E["seed"] + m for m in range(len(sizes))]
+    t0 = time.perf_counter()
+    made = [ensemble_member(sz, sd) for sz, sd in zip(sizes, seeds)]
+    grids, trs, dts = zip(*made)
+    ens = Ensemble.from_transports(trs, grids, dts)
+    log(f"{len(sizes)} members ready in {time.perf_counter() - t0:.1f} s")
+    # autotune: every combination of kernel variants timed on each member, fastest kept
+    t0 = time.perf_counter()
+    choice = {}
+    for tr, dt in zip(trs, dts):
+        c, _ = best_variant(tr, dt, steps=1)
+        for k, v in c.items():
+            tr.set_tuning(k, v)
+        choice[(c["sweep_pf"], c["diff_chunk"])] = choice.get((c["sweep_pf"], c["diff_chunk"]), 0) + 1
+    log(f"autotuned in {time.perf_counter() - t0:.1f} s: {choice}")
+    for i, (sz, sd) in enumerate(zip(sizes, seeds)):           # back to the seeded states
+        from gemini_amd.synth_gpu import synthetic_state_device
+        st = synthetic_state_device(grids[i], seed=sd, device="cuda:0")
+        trs[i].push_blocks(st["fluidvars"], st["aux"], st["electrovars"])
+        trs[i].set_interface_vels_zero()
+    torch.cuda.synchronize()
+    steps = E["steps"]
+    t0 = time.perf_counter()
+    ens.capture(steps)
+    log(f"graph of {steps} steps x {len(sizes)} members captured in {time.perf_counter() - t0:.1f} s")
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.run(steps)
+    graph_s = time.perf_counter() - t0
+    # keep a few members' results, then time the same work issued without a graph
+    keep = list(range(0, len(sizes), max(1, len(sizes) // E["verify"])))[:E["verify"]]
+    kept = {}
+    for i in keep:
+        n = grids[i].padded[0] * grids[i].padded[1] * grids[i].padded[2]
+        fv = torch.empty(35 * n, dtype=torch.float64, device="cuda:0")
+        aux = torch.empty(14 * n, dtype=torch.float64, device="cuda:0")
+        trs[i].pull_blocks(fv, aux)
+        kept[i] = (fv, aux)
+    t0 = time.perf_counter()
+    ens.run_plain(steps)
+    plain_s = time.perf_counter() - t0
+    ens.close()
+    verified = 0
+    for i in keep:                                              # plain solve of member i alone
+        g, tr, dt = ensemble_member(sizes[i], seeds[i], own_context=True)
+        tr.set_interface_vels_zero()
+        for _ in range(steps):
+            tr.transport_step(dt)
+        n = g.padded[0] * g.padded[1] * g.padded[2]
+        fv = torch.empty(35 * n, dtype=torch.float64, device="cuda:0")
+        aux = torch.empty(14 * n, dtype=torch.float64, device="cuda:0")
+        tr.pull_blocks(fv, aux)
+        same = torch.equal(fv, kept[i][0]) and torch.equal(aux, kept[i][1])
+        verified += int(same)
+        if not same:
+            log(f"member {i} differs from its plain solve")
+        tr.close()
+    cells = sum(a * b * c for a, b, c in sizes)
+    value = cells * 7 * steps / graph_s
+    out = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(1e3 * graph_s / steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded profiles generated in HBM)",
+        "config": {"workload": E["desc"], "members": len(sizes), "cells": cells,
+                   "graph_s": round(graph_s, 4), "plain_issue_s": round(plain_s, 4),
+                   "autotune_choices": {f"sweep_pf={k[0]},diff_chunk={k[1]}": v
+                                        for k, v in choice.items()},
+                   "members_verified_bit_identical": f"{verified}/{len(keep)}"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dipole512", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="dipole512", choices=sorted(WORKLOADS) + ["ensemble64"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -140,6 +249,9 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on a single rank", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble64":
+        bench_ensemble(args)
+        return
     w = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else w["steps"]
 

@@ -31,6 +31,17 @@ class Ensemble:
             self.trs.append(tr)
         self._graphs = {}
 
+    @classmethod
+    def from_transports(cls, trs, grids, dts):
+        """Members already set up (own contexts, residency mode 1, state pushed)."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.grids = list(grids)
+        self.dts = np.array([float(d) for d in dts], dtype=np.float64)
+        self.trs = list(trs)
+        self._graphs = {}
+        return self
+
     def __len__(self):
         return len(self.trs)
 

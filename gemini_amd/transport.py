@@ -191,6 +191,14 @@ class Transport:
     def get_interface_vels(self, st: State):
         self._call("gemhip_get_interface_vels", _p(st.vs1i), _p(st.vs2i), _p(st.vs3i))
 
+    def set_interface_vels_zero(self):
+        """Interface velocities all zero, as the reference allocates them (libgemini.f90:288)."""
+        n1 = (self.g.lx1 + 1) * self.g.lx2 * self.g.lx3 * 7
+        n2 = self.g.lx1 * (self.g.lx2 + 1) * self.g.lx3 * 7
+        n3 = self.g.lx1 * self.g.lx2 * (self.g.lx3 + 1) * 7
+        z = [np.zeros(n, order="F") for n in (n1, n2, n3)]
+        self._call("gemhip_set_interface_vels", *[_p(a) for a in z])
+
     def set_interface_vels(self, st: State):
         self._call("gemhip_set_interface_vels", _p(st.vs1i), _p(st.vs2i), _p(st.vs3i))
 
