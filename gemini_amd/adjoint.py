@@ -1,0 +1,138 @@
+"""Gradients of a run of transport steps through the library's hand-written HIP adjoint.
+
+The fitting extension gives the electron energy path two per-column parameters: a scale
+``kappa`` on the electron thermal conductivity and a downward heat flux ``qtop`` [W/m^2]
+through the top face (gridflag 2).  :class:`TransportRun` advances a fixed initial state
+``nsteps`` steps for given parameters and returns the final electron temperature;
+``backward`` returns dL/dkappa and dL/dqtop for any dL/dTe of that temperature.
+
+Checkpointing is two-level: ``forward`` stores a restart checkpoint (fluidvars and the x1
+boundary interface velocities, everything the next step reads) every ``every`` steps;
+``backward`` walks the segments from the last, restores the segment's checkpoint,
+re-runs its steps while recording the adjoint tape in HBM, then runs the adjoint steps
+in reverse.  Memory: nsteps/every checkpoints plus one segment's tape; cost: one extra
+forward step per step plus the adjoint kernels.  The replayed steps are the same kernels
+on the same inputs, so the recorded trajectory is bit-identical to the forward one.
+
+:func:`electron_temperature` is the ``torch.autograd.Function`` entry point.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .grid import Grid
+from .state import State
+from .transport import GemhipError, Transport
+
+TE_FIELD = 34           # fluidvars field of the electron temperature (Ts, species 7)
+
+
+class TransportRun:
+    """nsteps transport steps of one initial state, differentiable in kappa and qtop.
+
+    kappa and qtop are torch float64 tensors on the run's device with lx2*lx3 elements,
+    x2 fastest (e.g. shape (lx3, lx2)); qtop may be None (the reference's Dirichlet top).
+    """
+
+    def __init__(self, grid: Grid, state: State, dt: float, nsteps: int, every: int = 10,
+                 device: int = 0, Teinf: float = 1500.0):
+        if nsteps < 1 or every < 1:
+            raise ValueError("nsteps and every must be positive")
+        self.g, self.dt, self.nsteps, self.every = grid, float(dt), int(nsteps), int(every)
+        self.dev = torch.device(f"cuda:{device}")
+        self.tr = Transport(grid, state.nn, state.Tn, device=device, Teinf=Teinf,
+                            own_context=True)
+        self.tr.set_residency(1)
+        self.tr.push(state)
+        self.N = grid.padded[0] * grid.padded[1] * grid.padded[2]
+        self.ncol = grid.lx2 * grid.lx3
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.init = torch.empty(self.tr.checkpoint_size(), **f64)
+        self.tr.checkpoint_save(self.init)
+        self.nseg = math.ceil(self.nsteps / self.every)
+        self.ckpt = torch.empty((self.nseg, self.init.numel()), **f64)
+        self.tsize = self.tr.adjoint_tape_size()
+        self.tape = torch.empty(min(self.every, self.nsteps) * self.tsize, **f64)
+        self._final = torch.empty_like(self.init)
+        self.generation = 0
+        self.has_q = False
+
+    def close(self):
+        self.tr.close()
+
+    def tape_bytes(self) -> int:
+        return self.tape.numel() * 8
+
+    def _params(self, kappa, qtop):
+        torch.cuda.synchronize(self.dev)
+        for name, t in (("kappa", kappa), ("qtop", qtop)):
+            if t is not None and (t.numel() != self.ncol or t.dtype != torch.float64
+                                  or t.device != self.dev):
+                raise GemhipError(f"{name} must be {self.ncol} float64 values on {self.dev}")
+        self.tr.set_conductivity_scale(None if kappa is None else kappa.detach().contiguous())
+        self.tr.set_top_heat_flux(None if qtop is None else qtop.detach().contiguous())
+        self.has_q = qtop is not None
+
+    def forward(self, kappa, qtop=None) -> torch.Tensor:
+        """Electron temperature after nsteps (ghosted field, N values, x1 fastest)."""
+        self._params(kappa, qtop)
+        self.tr.checkpoint_load(self.init)
+        for k in range(self.nsteps):
+            if k % self.every == 0:
+                self.tr.checkpoint_save(self.ckpt[k // self.every])
+            self.tr.transport_step(self.dt)
+        self.tr.checkpoint_save(self._final)       # synchronises the library stream
+        self.generation += 1
+        return self._final[TE_FIELD * self.N:(TE_FIELD + 1) * self.N].clone()
+
+    def backward(self, grad_Te: torch.Tensor):
+        """dL/dkappa, dL/dqtop (lx2*lx3 each; None without a heat flux) for the last
+        forward, given dL/dTe of its result."""
+        lam = grad_Te.detach().to(device=self.dev, dtype=torch.float64).contiguous().clone()
+        if lam.numel() != self.N:
+            raise GemhipError(f"dL/dTe must have {self.N} values")
+        gk = torch.zeros(self.ncol, dtype=torch.float64, device=self.dev)
+        gq = torch.zeros(self.ncol, dtype=torch.float64, device=self.dev)
+        torch.cuda.synchronize(self.dev)
+        base, step = self.tape.data_ptr(), self.tsize * 8
+        for seg in reversed(range(self.nseg)):
+            k0, k1 = seg * self.every, min((seg + 1) * self.every, self.nsteps)
+            self.tr.checkpoint_load(self.ckpt[seg])
+            for k in range(k0, k1):
+                self.tr.record_step(self.dt, base + (k - k0) * step)
+            for k in reversed(range(k0, k1)):
+                self.tr.adjoint_step(self.dt, base + (k - k0) * step, lam.data_ptr(),
+                                     gk.data_ptr(), gq.data_ptr())
+        self.tr.synchronize()
+        self.grad_Te0 = lam                        # dL/dTe of the initial state
+        return gk, (gq if self.has_q else None)
+
+
+class _ElectronTemperature(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, kappa, qtop, run):
+        Te = run.forward(kappa, qtop)
+        ctx.run, ctx.gen = run, run.generation
+        ctx.kshape = kappa.shape
+        ctx.qshape = None if qtop is None else qtop.shape
+        return Te
+
+    @staticmethod
+    def backward(ctx, grad_Te):
+        if ctx.run.generation != ctx.gen:
+            raise RuntimeError("the run was advanced again before this backward")
+        gk, gq = ctx.run.backward(grad_Te)
+        return gk.view(ctx.kshape), (None if gq is None else gq.view(ctx.qshape)), None
+
+
+def electron_temperature(run: TransportRun, kappa, qtop=None) -> torch.Tensor:
+    """Final electron temperature of ``run`` as a differentiable function of kappa, qtop."""
+    return _ElectronTemperature.apply(kappa, qtop, run)
+
+
+def interior(run: TransportRun, Te: torch.Tensor) -> torch.Tensor:
+    """(lx3, lx2, lx1) interior view of a ghosted field returned by the run."""
+    L1, L2, L3 = run.g.padded
+    return Te.view(L3, L2, L1)[2:-2, 2:-2, 2:-2]

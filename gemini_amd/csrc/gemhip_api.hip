@@ -45,6 +45,10 @@ struct Ctx {
   double* d_neut = nullptr;
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  double* d_kappa = nullptr;    // fitting extension (lx2*lx3 each)
+  double* d_qtop = nullptr;
+  double* d_adj = nullptr;      // adjoint work: a second field and the column scratch
+  size_t adj_bytes = 0;
   bool vsi_current = true;      // full intvars%vs*i materialised
   int residency = 0;
   const double* pushed_fluid = nullptr;
@@ -124,11 +128,13 @@ void collect_timings() {
 void free_all() {
   Ctx& c = ctx();
   for (double** p : {&c.d_geom, &c.d_fluid, &c.d_aux, &c.d_J1, &c.d_vsi, &c.d_v1b, &c.d_neut,
-                     &c.d_scratch}) {
+                     &c.d_scratch, &c.d_kappa, &c.d_qtop, &c.d_adj}) {
     if (*p) hipFree(*p);
     *p = nullptr;
   }
   c.scratch_bytes = 0;
+  c.adj_bytes = 0;
+  c.s.kappa = c.s.qtop = nullptr;
   c.sized = c.grid_ok = c.neutrals_ok = false;
 }
 
@@ -279,6 +285,14 @@ bool run_energy_diffusion(double dt) {
     return false;
   }
   if (c.lx1 < 4) { fail("energy diffusion needs lx1 >= 4 (got %d)", c.lx1); return false; }
+  if ((c.s.kappa || c.s.qtop) && c.diffsolvetype != 2) {
+    fail("the conductivity scale and top heat flux need the TRBDF2 solver (diffsolvetype 2)");
+    return false;
+  }
+  if (c.s.qtop && c.g.gridflag != 2) {
+    fail("a top heat flux needs gridflag 2 (top at x1 = lx1), got %d", c.g.gridflag);
+    return false;
+  }
   if (!ensure_scratch()) return false;
   launch_energy_diffusion(c.g, c.s, dt, c.diffsolvetype, c.Teinf, c.d_scratch, c.stream, c.tune);
   return hipok(hipGetLastError(), "energy diffusion launch");
@@ -556,26 +570,157 @@ int gemhip_set_interface_vels(const double* vs1i, const double* vs2i, const doub
   return 0;
 }
 
-int gemhip_transport_step(double dt) {
+}  // extern "C"
+
+namespace gemhip {
+namespace {
+// tape of one recorded step (doubles): NE0, ES3, ES1, ES2, NE1, TD (ghosted electron
+// fields), then the electron interface velocities V1, V2, V3
+long tape_doubles() { return 6L * ctx().g.N + nv1i() + nv2i() + nv3i(); }
+
+AdjTape tape_view(const double* t) {
+  const long N = ctx().g.N;
+  AdjTape v;
+  v.NE0 = t; v.ES3 = t + N; v.ES1 = t + 2 * N; v.ES2 = t + 3 * N; v.NE1 = t + 4 * N;
+  v.TD = t + 5 * N; v.V1 = t + 6 * N; v.V2 = v.V1 + nv1i(); v.V3 = v.V2 + nv2i();
+  return v;
+}
+
+// One transport step on the HBM state.  With a tape, the electron fields the adjoint needs
+// are copied out on the way (device-to-device, on the library stream); the step itself is
+// the same sequence of kernels either way.
+int run_step(double dt, double* tape) {
   if (!ready(true)) return 1;
   Ctx& c = ctx();
   if (c.residency != 1) { fail("gemhip_transport_step needs residency mode 1"); return 2; }
   const DevGrid& g = c.g;
   const DevState& s = c.s;
   hipStream_t st = c.stream;
+  const long eo = (long)(LSP - 1) * g.N;             // electron field offset
+  const size_t fb = (size_t)g.N * sizeof(double);
+  auto keep = [&](long slot, const double* src) {
+    hipMemcpyAsync(tape + slot * g.N, src + eo, fb, hipMemcpyDeviceToDevice, st);
+  };
   launch_v12rhov1_T2rhoe(g, s, true, true, st);          // libgemini.cpp:155-156
+  if (tape) keep(0, s.ns);
   launch_set_global_boundaries(g, s, LSP, st);           // :168
   halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :169
   launch_save_v1_boundaries(g, s, st);                   // :170 (x1 planes kept for next step)
+  if (tape) {
+    keep(1, s.rhoes);
+    const AdjTape v = tape_view(tape);
+    launch_tape_vels(g, s, const_cast<double*>(v.V1), const_cast<double*>(v.V2),
+                     const_cast<double*>(v.V3), st);
+  }
   launch_sweep(g, s, 3, F_ALL, dt, true, st, c.tune);   // :174-176
+  if (tape) keep(2, s.rhoes);
   launch_sweep(g, s, 1, F_ALL, dt, true, st, c.tune);   // :178-180
   halo_fields({NS, RHOVS1, RHOES});                      // :181
+  if (tape) keep(3, s.rhoes);
   launch_sweep(g, s, 2, F_ALL, dt, true, st, c.tune);   // :183-185
   launch_post_convert(g, s, st);                         // :186, then multifluid.f90:179-189
+  if (tape) { keep(4, s.ns); keep(5, s.Ts); }
   if (!hipok(hipGetLastError(), "advection launch")) return 3;
   c.vsi_current = false;
   if (!run_energy_diffusion(dt)) return 4;               // :208
   return 0;
+}
+}  // namespace
+}  // namespace gemhip
+
+extern "C" {
+
+int gemhip_transport_step(double dt) { return run_step(dt, nullptr); }
+
+// ---- part 4: parameter fitting ----
+int gemhip_register_conductivity_scale(const double* kappa) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  hipStreamSynchronize(c.stream);
+  if (!kappa) { c.s.kappa = nullptr; return 0; }
+  const size_t n = (size_t)c.lx2 * c.lx3;
+  if (!c.d_kappa && !hipok(hipMalloc((void**)&c.d_kappa, n * sizeof(double)), "kappa")) return 2;
+  if (!hipok(hipMemcpy(c.d_kappa, kappa, n * sizeof(double), hipMemcpyDefault), "kappa upload"))
+    return 3;
+  c.s.kappa = c.d_kappa;
+  return 0;
+}
+
+int gemhip_register_top_heat_flux(const double* qtop) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  hipStreamSynchronize(c.stream);
+  if (!qtop) { c.s.qtop = nullptr; return 0; }
+  if (c.g.gridflag != 2) {
+    fail("a top heat flux needs gridflag 2 (top at x1 = lx1), got %d", c.g.gridflag);
+    return 2;
+  }
+  const size_t n = (size_t)c.lx2 * c.lx3;
+  if (!c.d_qtop && !hipok(hipMalloc((void**)&c.d_qtop, n * sizeof(double)), "qtop")) return 3;
+  if (!hipok(hipMemcpy(c.d_qtop, qtop, n * sizeof(double), hipMemcpyDefault), "qtop upload"))
+    return 4;
+  c.s.qtop = c.d_qtop;
+  return 0;
+}
+
+long gemhip_checkpoint_size(void) {
+  Ctx& c = ctx();
+  return c.sized ? 35L * c.g.N + 14L * c.lx2 * c.lx3 : 0;
+}
+
+int gemhip_checkpoint_save(double* dst) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  if (!dst) { fail("null checkpoint pointer"); return 2; }
+  const size_t nf = 35 * (size_t)c.g.N, nb = 14 * (size_t)c.lx2 * c.lx3;
+  bool ok = hipok(hipMemcpyAsync(dst, c.d_fluid, nf * sizeof(double), hipMemcpyDefault,
+                                 c.stream), "checkpoint fluidvars") &&
+            hipok(hipMemcpyAsync(dst + nf, c.d_v1b, nb * sizeof(double), hipMemcpyDefault,
+                                 c.stream), "checkpoint v1b") &&
+            hipok(hipStreamSynchronize(c.stream), "checkpoint");
+  return ok ? 0 : 3;
+}
+
+int gemhip_checkpoint_load(const double* src) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  if (!src) { fail("null checkpoint pointer"); return 2; }
+  const size_t nf = 35 * (size_t)c.g.N, nb = 14 * (size_t)c.lx2 * c.lx3;
+  bool ok = hipok(hipMemcpyAsync(c.d_fluid, src, nf * sizeof(double), hipMemcpyDefault,
+                                 c.stream), "restore fluidvars") &&
+            hipok(hipMemcpyAsync(c.d_v1b, src + nf, nb * sizeof(double), hipMemcpyDefault,
+                                 c.stream), "restore v1b") &&
+            hipok(hipStreamSynchronize(c.stream), "restore");
+  c.vsi_current = false;
+  return ok ? 0 : 3;
+}
+
+long gemhip_adjoint_tape_size(void) { return ctx().sized ? tape_doubles() : 0; }
+
+int gemhip_adjoint_record_step(double dt, double* tape) {
+  if (!tape) { fail("null tape pointer"); return 5; }
+  return run_step(dt, tape);
+}
+
+int gemhip_adjoint_step(double dt, const double* tape, double* lam, double* gkappa,
+                        double* gq) {
+  if (!ready(true)) return 1;
+  Ctx& c = ctx();
+  if (!tape || !lam) { fail("null tape or adjoint pointer"); return 2; }
+  if (c.diffsolvetype != 2) { fail("the adjoint needs the TRBDF2 solver (diffsolvetype 2)"); return 3; }
+  if (c.g.flagper && c.lx3 < 2) { fail("the adjoint of the periodic wrap needs lx3 >= 2"); return 4; }
+  const size_t need = (size_t)c.g.N * sizeof(double) + adjoint_scratch_bytes(c.g);
+  if (!c.d_adj || c.adj_bytes < need) {
+    hipStreamSynchronize(c.stream);
+    if (c.d_adj) hipFree(c.d_adj);
+    c.d_adj = nullptr;
+    c.adj_bytes = 0;
+    if (!hipok(hipMalloc((void**)&c.d_adj, need), "adjoint work")) return 5;
+    c.adj_bytes = need;
+  }
+  launch_adjoint_step(c.g, c.s, tape_view(tape), dt, c.Teinf, lam, c.d_adj, gkappa,
+                      c.s.qtop ? gq : nullptr, c.d_adj + c.g.N, c.stream);
+  return hipok(hipGetLastError(), "adjoint launch") ? 0 : 6;
 }
 
 int gemhip_device_state(double** fluidvars, double** rhovs1_rhoes, double** J1p) {

@@ -32,10 +32,32 @@ struct DevState {
   double *vs1i, *vs2i, *vs3i;   // materialised interface velocities (intvars%vs*i)
   double *v1b;                  // v1i(1,:,:,:) and v1i(lx1+1,:,:,:): (2, lx2, lx3, 7)
   const double *nn, *Tn;        // nn(lx1,lx2,lx3,1:3), Tn(lx1,lx2,lx3)
+  // parameter-fitting extension, null unless registered: electron conductivity scale and
+  // downward top heat flux [W/m^2] per (x2,x3) column, lx2*lx3 each
+  const double *kappa, *qtop;
 };
 
 __host__ __device__ inline long gix(const DevGrid& g, int i1, int i2, int i3) {
   return (long)(i1 + 1) + (long)g.L1 * ((long)(i2 + 1) + (long)g.L2 * (long)(i3 + 1));
+}
+
+// advec.f90:171-196: x1 interface velocity k (1..lx1+1) of line (i2,i3) from the cell
+// velocities u(-1..lx1+2) of that line.
+__device__ __forceinline__ double v1i_at(const DevGrid& g, const double* u, int k) {
+  const int lx1 = g.lx1;
+  if (k >= 2 && k <= lx1) return 0.5 * (u[k - 1] + u[k]);
+  if (k == 1) {
+    if (g.isglobal[0]) {
+      if (g.gridflag == 1) return fmin_f(0.5 * (u[1] + u[2]), 0.0);
+      return u[1];
+    }
+    return 0.5 * (u[0] + u[1]);
+  }
+  if (g.isglobal[1]) {
+    if (g.gridflag == 2) return fmax_f(0.5 * (u[lx1 - 1] + u[lx1]), 0.0);
+    return u[lx1];
+  }
+  return 0.5 * (u[lx1] + u[lx1 + 1]);
 }
 
 // Kernel variants a context may select (compiled alongside each other; an autotuner picks
@@ -70,6 +92,23 @@ size_t diffusion_scratch_bytes(const DevGrid& g);
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
                              double Teinf, double* scratch, hipStream_t st,
                              const Tuning& tune = Tuning{});
+
+// ---- adjoint of the electron energy path (kernels_adjoint.hip) ----
+// What one recorded forward step keeps for the adjoint (electron fields only): densities at
+// the step start and at the diffusion, the energy density entering each sweep, the
+// temperature entering the diffusion and the three interface-velocity arrays.
+struct AdjTape {
+  const double *NE0, *ES3, *ES1, *ES2, *NE1, *TD;   // ghosted fields, N each
+  const double *V1, *V2, *V3;                       // (lx1+1,lx2,lx3), (lx1,lx2+1,lx3), (lx1,lx2,lx3+1)
+};
+size_t adjoint_scratch_bytes(const DevGrid& g);
+void launch_tape_vels(const DevGrid& g, const DevState& s, double* v1, double* v2, double* v3,
+                      hipStream_t st);
+// lam (ghosted field): dL/dTe after the step in, dL/dTe before it out; gk, gq (lx2*lx3,
+// either may be null) accumulate dL/dkappa and dL/dq; work is a second field.
+void launch_adjoint_step(const DevGrid& g, const DevState& s, const AdjTape& t, double dt,
+                         double Teinf, double* lam, double* work, double* gk, double* gq,
+                         double* scratch, hipStream_t st);
 
 // kernel-timing hooks implemented by the API layer
 void timing_begin(const char* name, hipStream_t st);

@@ -209,25 +209,6 @@ __global__ void k_halo_periodic(DevGrid g, FieldList fl, int lhalo) {
 }
 
 // ---------------------------------------------------------------- interface velocities
-// advec.f90:171-196: x1 interface velocity k (1..lx1+1) of line (i2,i3) from the cell
-// velocities u(-1..lx1+2) of that line.
-__device__ __forceinline__ double v1i_at(const DevGrid& g, const double* u, int k) {
-  const int lx1 = g.lx1;
-  if (k >= 2 && k <= lx1) return 0.5 * (u[k - 1] + u[k]);
-  if (k == 1) {
-    if (g.isglobal[0]) {
-      if (g.gridflag == 1) return fmin_f(0.5 * (u[1] + u[2]), 0.0);
-      return u[1];
-    }
-    return 0.5 * (u[0] + u[1]);
-  }
-  if (g.isglobal[1]) {
-    if (g.gridflag == 2) return fmax_f(0.5 * (u[lx1 - 1] + u[lx1]), 0.0);
-    return u[lx1];
-  }
-  return 0.5 * (u[lx1] + u[lx1 + 1]);
-}
-
 __global__ void k_vs1i(DevGrid g, DevState s) {
   const int isp = blockIdx.y + 1;
   const long n = (long)(g.lx1 + 1) * g.lx2 * g.lx3;

@@ -19,7 +19,7 @@
 // reference arithmetic apart from x**2.5 (correctly rounded here, libm pow there).
 #include <type_traits>
 
-#include "gemhip_internal.h"
+#include "diffusion_cell.h"
 
 namespace gemhip {
 
@@ -106,47 +106,6 @@ __device__ __forceinline__ void column_of(const DevGrid& g, int gcol, int& i2, i
   i3 = gcol / g.lx2 + 1;
 }
 
-// inputs of one cell (neutral densities and J1 for electrons only)
-template <bool ELEC>
-struct CellIn {
-  double T, n, h1, h2, h3;
-  double n1, n2, n3, J;
-};
-
-template <bool ELEC>
-__device__ __forceinline__ void cell_load(const DevGrid& g, const DevState& s, long so, long cb,
-                                          long kb, long ni, int r, CellIn<ELEC>& c) {
-  c.T = s.Ts[so + cb + r];
-  c.n = s.ns[so + cb + r];
-  c.h1 = g.h1[cb + r]; c.h2 = g.h2[cb + r]; c.h3 = g.h3[cb + r];
-  if (ELEC) {
-    c.n1 = s.nn[kb + r - 1]; c.n2 = s.nn[ni + kb + r - 1]; c.n3 = s.nn[2 * ni + kb + r - 1];
-    c.J = s.J1[cb + r];
-  }
-}
-
-// conductivity (collisions.f90:162-199) and parabolic coefficients (diffusion.f90)
-template <bool ELEC>
-__device__ __forceinline__ void cell_coeffs(const CellIn<ELEC>& c, int isp, double gm1kB,
-                                            double& Cr, double& Br, double& Dr) {
-  const double T = c.T;
-  double lambda, beta;
-  if (!ELEC) {
-    lambda = thermal_coeff(isp) * pow25(T);
-    beta = 0.0;
-  } else {
-    const double sq = sqrt(T);
-    const double br = ((c.n1 * 1.1e-16) * (1.0 + 5.7e-4 * T) + ((c.n2 * 2.82e-17) * sq) * (1.0 - 1.21e-4 * T))
-                      + (c.n3 * 2.2e-16) * (1.0 + 3.6e-2 * sq);
-    const double den = 1.0 + ((3.22e4 * (T * T)) / c.n) * br;
-    lambda = (((elchrg * 100.0) * 7.7e5) * pow25(T)) / den;
-    beta = (((5.0 / 2.0) * kB) / elchrg) * c.J;
-  }
-  Cr = (gm1kB / fmax_f(c.n, mindensdiv)) / ((c.h1 * c.h2) * c.h3);
-  Br = (Cr * beta) / c.h1;
-  Dr = ((lambda * c.h2) * c.h3) / c.h1;
-}
-
 // Pivoted elimination step j (dgbtf2 with dgbtrs's forward solve) bringing in row j+1 =
 // (a, b, cc | d), applied to the state only when `real`.  The row at pivot position j is
 // stored at row sj (a step that is not real stores where the real step sj stores later).
@@ -208,7 +167,7 @@ __device__ __forceinline__ void elim_mat(const Group& q, Fact& f, int j, int sj,
   q.swb[f.word * 64] = f.bits;
 }
 
-template <int R, bool BE, bool ELEC, int PH>
+template <int R, bool BE, bool ELEC, int PH, bool EXT>
 __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState& s,
                                                 double* scratch, int* swbits, int* lst,
                                                 const Coef& k1,
@@ -239,6 +198,17 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     else if (g.gridflag == 1) { Tmax = TnN; Tmin = Teinf; }
     else { Tmin = Tn1; Tmax = Teinf; }
   }
+  // row N: the reference's Dirichlet row, or under the extension with a top heat flux the
+  // Neumann row (T(N) - T(N-1))/dx1(N) = q h1(N)/lambda(N)
+  double aN = 0.0, bN = 1.0;
+  if (EXT && ELEC && s.qtop) {
+    CellIn<ELEC> top;
+    cell_load<ELEC, EXT>(g, s, so, gix(g, 0, col % g.lx2 + 1, col / g.lx2 + 1), kb, ni, N, top,
+                         col);
+    aN = -1.0 / dx1[N];
+    bN = 1.0 / dx1[N];
+    Tmax = (s.qtop[col] * top.h1) / cell_lambda<ELEC, EXT>(top, isp);
+  }
   // cooperative mapping of a chunk: lane -> (row lr, column sub-index cs); R loads per lane
   const int lr = lane % R, cs = lane / R;
   constexpr int CPI = 64 / R;                        // columns per load instruction
@@ -247,6 +217,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   constexpr int NB = R < BATCH ? R : BATCH;
   int cbk[R];                                        // gix(0, i2, i3) of the columns this lane loads
   int kbk[R];                                        // and their interior offsets lx1*(column)
+  int gck[R];                                        // and column numbers (extension only)
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int gcol = (int)min(colbase0 + k * CPI + cs, ncol - 1);
@@ -254,6 +225,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     column_of(g, gcol, a2, a3);
     cbk[k] = (int)gix(g, 0, a2, a3);
     kbk[k] = g.lx1 * gcol;
+    gck[k] = gcol;
   }
 
   // load and convert the rows r0..r0+R-1 of the 64 columns into LDS (rows past N repeat N)
@@ -263,11 +235,12 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     for (int k0 = 0; k0 < R; k0 += NB) {
       CellIn<ELEC> in[NB];
 #pragma unroll
-      for (int k = 0; k < NB; ++k) cell_load<ELEC>(g, s, so, cbk[k0 + k], kbk[k0 + k], ni, r, in[k]);
+      for (int k = 0; k < NB; ++k)
+        cell_load<ELEC, EXT>(g, s, so, cbk[k0 + k], kbk[k0 + k], ni, r, in[k], gck[k0 + k]);
 #pragma unroll
       for (int k = 0; k < NB; ++k) {
         double C_, B_, D_;
-        cell_coeffs<ELEC>(in[k], isp, gm1kB, C_, B_, D_);
+        cell_coeffs<ELEC, EXT>(in[k], isp, gm1kB, C_, B_, D_);
         const int cc = (k0 + k) * CPI + cs;
         sT[lr][cc] = in[k].T; sC[lr][cc] = C_; sB[lr][cc] = B_; sD[lr][cc] = D_;
       }
@@ -388,11 +361,11 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
       S2(q, clampi(ro, 1, N), SAT) = -((bdf.c13 * Tc) / bdf.dt3);
     }
   });
-  elim_rhs(q, e, N - 1, N - 1, true, 0.0, 1.0, 0.0, Tmax);       // Dirichlet row N
+  elim_rhs(q, e, N - 1, N - 1, true, aN, bN, 0.0, Tmax);         // boundary row N
   if (e.r0 == 0.0) e.singular = true;
   S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SU2) = 0.0; S2(q, N, SY) = e.rb;
   if (!BE) {
-    elim_mat(q, f, N - 1, N - 1, true, 0.0, 1.0, 0.0);
+    elim_mat(q, f, N - 1, N - 1, true, aN, bN, 0.0);
     if (f.r0 == 0.0) f.singular = true;
     S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0; S2(q, N, SV2) = 0.0;
   }
@@ -469,7 +442,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 
 // PH 0: the whole solve; for a solve split in two launches, PH 1 runs passes 1-2 and
 // PH 2 passes 3-4 (shorter waves, so the last wave round leaves less of the GPU idle)
-template <int R, bool BE, int PH>
+template <int R, bool BE, int PH, bool EXT = false>
 __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double* scratch,
                                                   int* swbits, int* lst, Coef k1, Coef bdf,
                                                   double Teinf) {
@@ -481,11 +454,11 @@ __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double*
   if (cg >= gps) return;
   const int isp = (int)(slot % LSP) + 1;
   if (isp == LSP)
-    diffusion_group<R, BE, true, PH>(g, s, scratch, swbits, lst, k1, bdf, Teinf, isp, cg, gps, sT,
-                                     sC, sB, sD);
+    diffusion_group<R, BE, true, PH, EXT>(g, s, scratch, swbits, lst, k1, bdf, Teinf, isp, cg,
+                                          gps, sT, sC, sB, sD);
   else
-    diffusion_group<R, BE, false, PH>(g, s, scratch, swbits, lst, k1, bdf, Teinf, isp, cg, gps, sT,
-                                      sC, sB, sD);
+    diffusion_group<R, BE, false, PH, EXT>(g, s, scratch, swbits, lst, k1, bdf, Teinf, isp, cg,
+                                           gps, sT, sC, sB, sD);
 }
 
 }  // namespace
@@ -524,6 +497,9 @@ void diffusion_launch(const DevGrid& g, const DevState& s, int diffsolvetype, do
   if (diffsolvetype == 1) {
     hipLaunchKernelGGL((k_diffusion<R, true, 0>), grd, blk, 0, st, g, s, scratch, swbits, lst, be,
                        bdf, Teinf);
+  } else if (s.kappa || s.qtop) {                     // fitting extension (TRBDF2 only)
+    hipLaunchKernelGGL((k_diffusion<R, false, 0, true>), grd, blk, 0, st, g, s, scratch, swbits,
+                       lst, tr, bdf, Teinf);
   } else if (DIFF_SPLIT) {
     hipLaunchKernelGGL((k_diffusion<R, false, 1>), grd, blk, 0, st, g, s, scratch, swbits, lst, tr,
                        bdf, Teinf);

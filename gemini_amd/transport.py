@@ -49,6 +49,15 @@ def load_library(path: str = LIB_PATH):
                  "gemhip_context_destroy", "gemhip_set_tuning", "gemhip_get_tuning",
                  "gemhip_ensemble_capture", "gemhip_ensemble_launch", "gemhip_ensemble_release"):
         getattr(lib, name).restype = C.c_int
+    for name in ("gemhip_register_conductivity_scale", "gemhip_register_top_heat_flux",
+                 "gemhip_checkpoint_save", "gemhip_checkpoint_load",
+                 "gemhip_adjoint_record_step", "gemhip_adjoint_step"):
+        getattr(lib, name).restype = C.c_int
+    for name in ("gemhip_checkpoint_size", "gemhip_adjoint_tape_size"):
+        getattr(lib, name).restype = C.c_long
+    lib.gemhip_adjoint_record_step.argtypes = [C.c_double, C.c_void_p]
+    lib.gemhip_adjoint_step.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
     lib.gemhip_set_tuning.argtypes = [C.c_char_p, C.c_int]
     lib.gemhip_get_tuning.argtypes = [C.c_char_p]
     lib.gemhip_last_error.restype = C.c_char_p
@@ -322,3 +331,51 @@ class Transport:
     def transport_step(self, dt: float):
         """Fused step on the HBM-resident state (residency mode 1)."""
         self._call("gemhip_transport_step", C.c_double(dt))
+
+    # ------------------------------------------------------------------ fitting extension
+    def _column_param(self, a):
+        if a is None:
+            return None
+        if isinstance(a, np.ndarray):
+            a = np.asfortranarray(np.broadcast_to(a, (self.g.lx2, self.g.lx3)), dtype=np.float64)
+            return a.reshape(-1, order="F")
+        return a.contiguous()
+
+    def set_conductivity_scale(self, kappa):
+        """Per-column scale (lx2, lx3) of the electron thermal conductivity; None removes it.
+        A torch tensor is read as lx2*lx3 values with x2 fastest."""
+        k = self._column_param(kappa)
+        self._call("gemhip_register_conductivity_scale", None if k is None else _p(k))
+
+    def set_top_heat_flux(self, qtop):
+        """Per-column downward electron heat flux [W/m^2] through the top face (gridflag 2),
+        replacing the Dirichlet top row; None restores it."""
+        q = self._column_param(qtop)
+        self._call("gemhip_register_top_heat_flux", None if q is None else _p(q))
+
+    def checkpoint_size(self) -> int:
+        """doubles in a restart checkpoint (fluidvars and the x1 boundary velocities)."""
+        self._sel()
+        return int(self.lib.gemhip_checkpoint_size())
+
+    def checkpoint_save(self, dst):
+        self._call("gemhip_checkpoint_save", _p(dst))
+
+    def checkpoint_load(self, src):
+        self._call("gemhip_checkpoint_load", _p(src))
+
+    def adjoint_tape_size(self) -> int:
+        """doubles one recorded step keeps for the adjoint."""
+        self._sel()
+        return int(self.lib.gemhip_adjoint_tape_size())
+
+    def record_step(self, dt: float, tape_ptr: int):
+        """transport_step that also writes the adjoint tape at device address tape_ptr."""
+        self._call("gemhip_adjoint_record_step", C.c_double(dt), C.c_void_p(tape_ptr))
+
+    def adjoint_step(self, dt: float, tape_ptr: int, lam_ptr: int, gk_ptr: int = 0,
+                     gq_ptr: int = 0):
+        """dL/dTe after a recorded step -> before it (in place at lam_ptr); accumulates
+        dL/dkappa and dL/dq per column at gk_ptr, gq_ptr (0: not wanted)."""
+        self._call("gemhip_adjoint_step", C.c_double(dt), C.c_void_p(tape_ptr),
+                   C.c_void_p(lam_ptr), C.c_void_p(gk_ptr or None), C.c_void_p(gq_ptr or None))

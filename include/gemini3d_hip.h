@@ -176,6 +176,35 @@ int gemhip_ensemble_capture(int n, const int* handles, const double* dt, int nst
 int gemhip_ensemble_launch(int id, int wait);
 int gemhip_ensemble_release(int id);
 
+/* ---------------- part 4: parameter fitting and the discrete adjoint (no counterpart in the reference) ---------------- */
+/* Extension of the electron energy diffusion for fitting: a per-column scale kappa on the
+   electron thermal conductivity (collisions.f90:185-195 times kappa(i2,i3)) and, for
+   gridflag 2, a downward electron heat flux q [W/m^2] through the top face, imposed as the
+   Neumann row (T(lx1)-T(lx1-1))/dx1(lx1) = q h1(lx1)/lambda(lx1) that PDEparabolic.f90:321-326
+   carries commented out, in both TRBDF2 stages.  Arrays are lx2*lx3 values, x2 fastest,
+   host or HBM; NULL removes the parameter (the reference's behaviour).  TRBDF2 only. */
+int gemhip_register_conductivity_scale(const double* kappa);
+int gemhip_register_top_heat_flux(const double* qtop);
+
+/* Restart checkpoint of the resident state: fluidvars (35 fields) and the x1 boundary
+   interface velocities the next step's ghost fill reads.  Size in doubles; dst/src host
+   or HBM. */
+long gemhip_checkpoint_size(void);
+int gemhip_checkpoint_save(double* dst);
+int gemhip_checkpoint_load(const double* src);
+
+/* Discrete adjoint of the electron energy path of one transport step (the only part of
+   the state the fitting parameters reach).  gemhip_adjoint_record_step runs
+   gemhip_transport_step and writes, at the HBM address tape (gemhip_adjoint_tape_size()
+   doubles), the electron fields the adjoint needs.  gemhip_adjoint_step takes lam (HBM,
+   one ghosted field) = dL/dTe after that step, overwrites it with dL/dTe before the step,
+   and adds dL/dkappa and dL/dq per column into gkappa and gq (HBM, lx2*lx3 each; either
+   may be NULL).  Limiter, upwind and floor branches are those of the recorded step.
+   Needs diffsolvetype 2, and lx3 >= 2 on periodic grids. */
+long gemhip_adjoint_tape_size(void);
+int gemhip_adjoint_record_step(double dt, double* tape);
+int gemhip_adjoint_step(double dt, const double* tape, double* lam, double* gkappa, double* gq);
+
 #ifdef __cplusplus
 }
 #endif

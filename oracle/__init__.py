@@ -43,7 +43,8 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.orc_minmod.restype = C.c_double
         _lib.orc_minmod.argtypes = [C.c_double, C.c_double]
-        for name in ("orc_gbsv", "orc_energy_diffusion", "orc_transport_step", "orc_num_threads"):
+        for name in ("orc_gbsv", "orc_energy_diffusion", "orc_transport_step", "orc_num_threads",
+                     "orc_energy_diffusion_ext", "orc_transport_step_ext"):
             getattr(_lib, name).restype = C.c_int
     return _lib
 
@@ -82,13 +83,27 @@ def _f(a):
     return ptr(a)
 
 
-def transport_step(g, st, dt, diffsolvetype=2, Teinf=1500.0):
-    """Advance ``st`` (in place) by one oracle transport step; returns st."""
+def _opt(a):
+    """double* of an optional per-column parameter array (lx2, lx3), None -> NULL."""
+    if a is None:
+        return None, None
+    a = np.asfortranarray(a, dtype=np.float64)
+    return a, ptr(a)
+
+
+def transport_step(g, st, dt, diffsolvetype=2, Teinf=1500.0, kappa=None, qtop=None):
+    """Advance ``st`` (in place) by one oracle transport step; returns st.
+
+    kappa (lx2, lx3) scales the electron conductivity per column and qtop (lx2, lx3) is
+    a downward electron heat flux through the top face [W/m^2] (the fitting extension);
+    None keeps the reference behaviour."""
     og = OracleGrid(g)
-    rc = lib().orc_transport_step(og.ref, C.c_double(dt), _f(st.fluidvars), _f(st.fluidauxvars),
-                                  _f(st.electrovars), _f(st.vs1i), _f(st.vs2i), _f(st.vs3i),
-                                  _f(st.nn), _f(st.Tn), C.c_int(diffsolvetype),
-                                  C.c_double(Teinf))
+    ka, kp = _opt(kappa)
+    qa, qp = _opt(qtop)
+    rc = lib().orc_transport_step_ext(og.ref, C.c_double(dt), _f(st.fluidvars),
+                                      _f(st.fluidauxvars), _f(st.electrovars), _f(st.vs1i),
+                                      _f(st.vs2i), _f(st.vs3i), _f(st.nn), _f(st.Tn),
+                                      C.c_int(diffsolvetype), C.c_double(Teinf), kp, qp)
     if rc != 0:
         raise RuntimeError(f"oracle transport step failed ({rc})")
     return st
@@ -187,6 +202,17 @@ def trbdf2_1d(Ts, A, B, Cc, D, E, Tmin, Tmax, dt, dx1, dx1i):
     return out
 
 
+def trbdf2_1d_flux(Ts, A, B, Cc, D, E, Tmin, gtop, dt, dx1, dx1i):
+    """TRBDF2 with the Neumann top row (T(lx1)-T(lx1-1))/dx1(lx1) = gtop."""
+    lx1 = len(Ts)
+    out = np.zeros(lx1)
+    a = [_col(v) for v in (Ts, A, B, Cc, D, E, dx1, dx1i)]
+    lib().orc_TRBDF21D_flux(C.c_int(lx1), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]),
+                            ptr(a[5]), C.c_double(Tmin), C.c_double(gtop), C.c_double(dt),
+                            ptr(a[6]), ptr(a[7]), ptr(out))
+    return out
+
+
 def backeuler_1d(Ts, A, B, Cc, D, E, Tmin, Tmax, dt, dx1, dx1i):
     lx1 = len(Ts)
     out = np.zeros(lx1)
@@ -223,7 +249,7 @@ def _views(st):
 
 
 def op(g, st, name, dt=None, lsp=7, direction=None, which=None, diffsolvetype=2,
-       Teinf=1500.0):
+       Teinf=1500.0, kappa=None, qtop=None):
     """Apply one reference operation of the transport path to ``st`` in place.
 
     Names follow the reference's C entry points: v12rhov1, T2rhoe, rhoe2T, rhov12v1,
@@ -265,9 +291,11 @@ def op(g, st, name, dt=None, lsp=7, direction=None, which=None, diffsolvetype=2,
             L.orc_sweep_allspec(og.ref, C.c_int(direction), v[field], _f(vsi), C.c_double(dt),
                                 C.c_int(frank), C.c_int(nsp))
     elif name == "energy_diffusion":
-        rc = L.orc_energy_diffusion(og.ref, C.c_double(dt), v["ns"], v["Ts"], v["J1"],
-                                    _f(st.nn), _f(st.Tn), C.c_int(diffsolvetype),
-                                    C.c_double(Teinf))
+        ka, kp = _opt(kappa)
+        qa, qp = _opt(qtop)
+        rc = L.orc_energy_diffusion_ext(og.ref, C.c_double(dt), v["ns"], v["Ts"], v["J1"],
+                                        _f(st.nn), _f(st.Tn), C.c_int(diffsolvetype),
+                                        C.c_double(Teinf), kp, qp)
         if rc != 0:
             raise RuntimeError(f"oracle energy diffusion failed ({rc})")
     else:

@@ -593,10 +593,13 @@ int orc_gbsv(int n, int kl, double *ab, double *b, int *ipiv) {
 
 /* ---- PDEparabolic.f90:19-218 ----
    Ts,A..E(1:lx1) -> arrays [0..lx1-1]; dx1(0:lx1+2) -> [0..lx1+2]; dx1i(1:lx1) -> [0..]. */
-void orc_TRBDF21D(int lx1, const double *Ts, const double *A, const double *B,
-                  const double *C, const double *D, const double *E, double Tsminx1,
-                  double Tsmaxx1, double dt, const double *dx1, const double *dx1i,
-                  double *out) {
+/* top_flux = 0: the reference's Dirichlet row lx1 (T = Tsmaxx1).  top_flux = 1: the
+   Neumann row that PDEparabolic.f90:321-326 carries commented out, (T(lx1) - T(lx1-1)) /
+   dx1(lx1) = Tsmaxx1, in both stages (an extension for a top heat flux). */
+static void trbdf2_core(int lx1, const double *Ts, const double *A, const double *B,
+                        const double *C, const double *D, const double *E, double Tsminx1,
+                        double Tsmaxx1, double dt, const double *dx1, const double *dx1i,
+                        double *out, int top_flux) {
   const int ll = 2;
   double *M = (double *)calloc((size_t)7 * (size_t)lx1, sizeof(double));
   double *Dh = (double *)malloc(sizeof(double) * (size_t)lx1);
@@ -636,7 +639,9 @@ void orc_TRBDF21D(int lx1, const double *Ts, const double *A, const double *B,
     TR[i - 1] = r;
   }
   i = lx1;
-  MM(ll + 5, i - 2) = 0.0; MM(ll + 4, i - 1) = 0.0; MM(ll + 3, i) = 1.0;
+  MM(ll + 5, i - 2) = 0.0;
+  if (top_flux) { MM(ll + 4, i - 1) = -1.0 / DX(i); MM(ll + 3, i) = 1.0 / DX(i); }
+  else { MM(ll + 4, i - 1) = 0.0; MM(ll + 3, i) = 1.0; }
   TR[i - 1] = Tsmaxx1;
   orc_gbsv(lx1, 2, M, TR, ipiv);
 
@@ -656,10 +661,26 @@ void orc_TRBDF21D(int lx1, const double *Ts, const double *A, const double *B,
     out[i - 1] = (E_(i) - ((1.0 / 3.0) * T_(i)) / (dt / 3.0)) + ((4.0 / 3.0) * TR[i - 1]) / (dt / 3.0);
   }
   i = lx1;
-  MM(ll + 5, i - 2) = 0.0; MM(ll + 4, i - 1) = 0.0; MM(ll + 3, i) = 1.0;
+  MM(ll + 5, i - 2) = 0.0;
+  if (top_flux) { MM(ll + 4, i - 1) = -1.0 / DX(i); MM(ll + 3, i) = 1.0 / DX(i); }
+  else { MM(ll + 4, i - 1) = 0.0; MM(ll + 3, i) = 1.0; }
   out[i - 1] = Tsmaxx1;
   orc_gbsv(lx1, 2, M, out, ipiv);
   free(M); free(Dh); free(TR); free(ipiv);
+}
+
+void orc_TRBDF21D(int lx1, const double *Ts, const double *A, const double *B,
+                  const double *C, const double *D, const double *E, double Tsminx1,
+                  double Tsmaxx1, double dt, const double *dx1, const double *dx1i,
+                  double *out) {
+  trbdf2_core(lx1, Ts, A, B, C, D, E, Tsminx1, Tsmaxx1, dt, dx1, dx1i, out, 0);
+}
+
+void orc_TRBDF21D_flux(int lx1, const double *Ts, const double *A, const double *B,
+                       const double *C, const double *D, const double *E, double Tsminx1,
+                       double gtop, double dt, const double *dx1, const double *dx1i,
+                       double *out) {
+  trbdf2_core(lx1, Ts, A, B, C, D, E, Tsminx1, gtop, dt, dx1, dx1i, out, 1);
 }
 
 /* ---- PDEparabolic.f90:221-346 (coeffs(lx1,3) and rhs(lx1) optional outputs) ---- */
@@ -716,10 +737,13 @@ void orc_backEuler1D(int lx1, const double *Ts, const double *A, const double *B
    Ghost cells of Ts, which the reference leaves undefined (the TRBDF23D result is only
    assigned on 1:lx1 x 1:lx2 x 1:lx3, diffusion.f90:136,149), keep their input values
    and are floored at 100 like every other cell. */
-int orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double *Ts,
-                         const double *J1, const double *nn, const double *Tn,
-                         int diffsolvetype, double Teinf) {
+static int energy_diffusion_core(const orc_grid *g, double dt, const double *ns, double *Ts,
+                                 const double *J1, const double *nn, const double *Tn,
+                                 int diffsolvetype, double Teinf, const double *kappa,
+                                 const double *qtop) {
   if (diffsolvetype != 1 && diffsolvetype != 2) return -1;
+  if ((kappa || qtop) && diffsolvetype != 2) return -2;
+  if (qtop && g->gridflag != 2) return -3;
   dims_t d = dims_of(g);
   const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
   const long ni = (long)lx1 * lx2 * lx3;
@@ -734,14 +758,20 @@ int orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double 
     for (int i3 = 1; i3 <= lx3; ++i3)
       for (int i2 = 1; i2 <= lx2; ++i2) {
         double Tc[lx1], A[lx1], B[lx1], C[lx1], D[lx1], E[lx1], res[lx1];
+        const long col = (i2 - 1) + (long)lx2 * (i3 - 1);
+        const int elec = isp == LSP;
+        double lamN = 0.0;
         for (int i1 = 1; i1 <= lx1; ++i1) {
           long c = GIX(d, i1, i2, i3);
           long k = (i1 - 1) + (long)lx1 * ((i2 - 1) + (long)lx2 * (i3 - 1));
           double h1 = g->h1[c], h2 = g->h2[c], h3 = g->h3[c];
+          double lam = lambda[k];
+          if (elec && kappa) lam = lam * kappa[col];
+          if (i1 == lx1) lamN = lam;
           A[i1 - 1] = 0.0;
           C[i1 - 1] = (gm1kB / fmax_f(n[c], mindensdiv)) / ((h1 * h2) * h3);
           B[i1 - 1] = (C[i1 - 1] * beta[k]) / h1;
-          D[i1 - 1] = ((lambda[k] * h2) * h3) / h1;
+          D[i1 - 1] = ((lam * h2) * h3) / h1;
           E[i1 - 1] = 0.0;
           Tc[i1 - 1] = T[c];
         }
@@ -750,7 +780,11 @@ int orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double 
         if (g->gridflag == 0) { Tmin = Tn[kb]; Tmax = Tn[kb + lx1 - 1]; }
         else if (g->gridflag == 1) { Tmax = Tn[kb + lx1 - 1]; Tmin = Teinf; }
         else { Tmin = Tn[kb]; Tmax = Teinf; }
-        if (diffsolvetype == 2)
+        if (elec && qtop) {
+          /* downward heat flux q through the top face: lambda/h1 dT/dx1 = q */
+          const double gtop = (qtop[col] * g->h1[GIX(d, lx1, i2, i3)]) / lamN;
+          orc_TRBDF21D_flux(lx1, Tc, A, B, C, D, E, Tmin, gtop, dt, g->dx1, g->dx1i, res);
+        } else if (diffsolvetype == 2)
           orc_TRBDF21D(lx1, Tc, A, B, C, D, E, Tmin, Tmax, dt, g->dx1, g->dx1i, res);
         else
           orc_backEuler1D(lx1, Tc, A, B, C, D, E, Tmin, Tmax, dt, g->dx1, g->dx1i, res, 0, 0);
@@ -763,6 +797,19 @@ int orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double 
   return 0;
 }
 
+int orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double *Ts,
+                         const double *J1, const double *nn, const double *Tn,
+                         int diffsolvetype, double Teinf) {
+  return energy_diffusion_core(g, dt, ns, Ts, J1, nn, Tn, diffsolvetype, Teinf, 0, 0);
+}
+
+int orc_energy_diffusion_ext(const orc_grid *g, double dt, const double *ns, double *Ts,
+                             const double *J1, const double *nn, const double *Tn,
+                             int diffsolvetype, double Teinf, const double *kappa,
+                             const double *qtop) {
+  return energy_diffusion_core(g, dt, ns, Ts, J1, nn, Tn, diffsolvetype, Teinf, kappa, qtop);
+}
+
 /* ---- one transport step in the order of src/libgemini.cpp:155-186 and :208
    (app/main.f90:277-355): v12rhov1, T2rhoe, set_global_boundaries_allspec,
    halo_fluidvars, interface_vels_allspec, sweep3 (mass, momentum, energy), sweep1,
@@ -771,9 +818,11 @@ int orc_energy_diffusion(const orc_grid *g, double dt, const double *ns, double 
    artificial-viscosity and clean_param substeps between them are outside this path.
    Block layout: fluidvars(...,35) = ns 1-7, vs1 8-14, vs2 15-21, vs3 22-28, Ts 29-35;
    fluidauxvars(...,>=14) = rhovs1 1-7, rhoes 8-14; electrovars(...,4) = J1. */
-int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *fluidauxvars,
-                       const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
-                       const double *nn, const double *Tn, int diffsolvetype, double Teinf) {
+int orc_transport_step_ext(const orc_grid *g, double dt, double *fluidvars,
+                           double *fluidauxvars, const double *electrovars, double *vs1i,
+                           double *vs2i, double *vs3i, const double *nn, const double *Tn,
+                           int diffsolvetype, double Teinf, const double *kappa,
+                           const double *qtop) {
   dims_t d = dims_of(g);
   double *ns = fluidvars, *vs1 = fluidvars + 7 * d.N, *vs2 = fluidvars + 14 * d.N,
          *vs3 = fluidvars + 21 * d.N, *Ts = fluidvars + 28 * d.N;
@@ -802,5 +851,12 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
   orc_sweep_allspec(g, 2, rhoes, vs2i, dt, 0, 7);
   orc_rhov12v1(g, ns, rhovs1, vs1);
   orc_rhoe2T(g, ns, rhoes, Ts);
-  return orc_energy_diffusion(g, dt, ns, Ts, J1, nn, Tn, diffsolvetype, Teinf);
+  return energy_diffusion_core(g, dt, ns, Ts, J1, nn, Tn, diffsolvetype, Teinf, kappa, qtop);
+}
+
+int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *fluidauxvars,
+                       const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
+                       const double *nn, const double *Tn, int diffsolvetype, double Teinf) {
+  return orc_transport_step_ext(g, dt, fluidvars, fluidauxvars, electrovars, vs1i, vs2i, vs3i,
+                                nn, Tn, diffsolvetype, Teinf, 0, 0);
 }

@@ -85,6 +85,26 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
                        const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
                        const double *nn, const double *Tn, int diffsolvetype, double Teinf);
 
+/* Extension for parameter fitting (no counterpart in the reference): kappa(lx2*lx3)
+   scales the electron thermal conductivity of each (x2,x3) column; qtop(lx2*lx3), for
+   gridflag 2 only, replaces the electron Dirichlet top row by the Neumann row
+   (T(lx1)-T(lx1-1))/dx1(lx1) = qtop*h1(lx1)/lambda(lx1), a downward heat flux qtop
+   [W/m^2] through the top face.  NULL leaves the reference behaviour.  TRBDF2 only:
+   returns -2 for diffsolvetype 1 with either array, -3 for qtop on gridflag 0/1. */
+void orc_TRBDF21D_flux(int lx1, const double *Ts, const double *A, const double *B,
+                       const double *C, const double *D, const double *E, double Tsminx1,
+                       double gtop, double dt, const double *dx1, const double *dx1i,
+                       double *out);
+int orc_energy_diffusion_ext(const orc_grid *g, double dt, const double *ns, double *Ts,
+                             const double *J1, const double *nn, const double *Tn,
+                             int diffsolvetype, double Teinf, const double *kappa,
+                             const double *qtop);
+int orc_transport_step_ext(const orc_grid *g, double dt, double *fluidvars,
+                           double *fluidauxvars, const double *electrovars, double *vs1i,
+                           double *vs2i, double *vs3i, const double *nn, const double *Tn,
+                           int diffsolvetype, double Teinf, const double *kappa,
+                           const double *qtop);
+
 /* 0: x**2.5 by libm pow, as the reference (default); 1: correctly rounded x**2.5 */
 void orc_set_pow_mode(int mode);
 void orc_eval_pow25(const double *x, double *y, long n, int mode);
