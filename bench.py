@@ -8,7 +8,10 @@ order): state conversions, global ghost fill, x3/x1/x2 MC advection sweeps of th
 advected field-species, rhov12v1, rhoe2T and the TRBDF2 energy diffusion of all 7
 species.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload dipole512|cart256|cart128]
+    python bench.py [--steps K] [--warmup W] [--workload dipole512|cart256|cart128|ensemble64|adjoint]
+
+The default line also carries the differentiation form's adjoint/forward cost ratio,
+measured on its own configuration (ADJOINT below) with a few central-difference checks.
 """
 from __future__ import annotations
 
@@ -51,6 +54,15 @@ ENSEMBLE = dict(members=64, steps=20, seed=1, verify=3,
                 desc="64 independent Cartesian grids, lx1 from {64,96,...,512}, lx2 x lx3 "
                      "from 16x16 to 128x96 (seeded draws), 7 species, fp64, 20 steps in one "
                      "HIP graph launch")
+
+
+ADJOINT = dict(shape=(256, 128, 96), steps=50, every=10, seed=1, fd_dirs=32, eps=1e-6,
+               qmax=1e-5, fd_sweep_dirs=3, fd_sweep_eps=(1e-4, 1e-5, 1e-6, 1e-7, 1e-8),
+               alts_km=(150, 200, 250, 300, 400, 500, 650, 800),
+               desc="3-D Cartesian 256x128x96 (3.1M cells), 50 advection+TRBDF2 steps, "
+                    "gradient of the squared Te mismatch at 8 altitudes against a seeded "
+                    "target run w.r.t. the per-column electron conductivity scale and top "
+                    "heat flux, checkpoints every 10 steps")
 
 
 def log(*a):
@@ -218,12 +230,111 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(fd_dirs):
+    """Differentiation form: forward run with checkpoints, then the hand-written adjoint
+    (replay of each segment with the tape, adjoint kernels in reverse), through the
+    torch.autograd front end.  Returns the measurements as a dict."""
+    import torch
+    from gemini_amd.synth_gpu import synthetic_state_device, cfl_dt_device
+    from gemini_amd.adjoint import TransportRun, electron_temperature, interior
+    A = ADJOINT
+    t0 = time.perf_counter()
+    g = make_grid("cart", A["shape"])
+    st = synthetic_state_device(g, seed=A["seed"], device="cuda:0")
+    dt = cfl_dt_device(g, st)
+    run = TransportRun(g, st, dt, A["steps"], every=A["every"], geometry_on_device=True)
+    del st
+    torch.cuda.empty_cache()
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev).manual_seed(A["seed"])
+    sh = (g.lx3, g.lx2)
+    rnd = lambda: torch.rand(sh, generator=gen, device=dev, dtype=torch.float64)
+    k0, q0 = 1.0 + 0.2 * (2.0 * rnd() - 1.0), A["qmax"] * rnd()
+    kt, qt = 1.0 + 0.2 * (2.0 * rnd() - 1.0), A["qmax"] * rnd()
+    z = g.x1[2:g.lx1 + 2]
+    alts = torch.tensor([int(abs(z - 1e3 * a).argmin()) for a in A["alts_km"]], device=dev)
+    target = interior(run, run.forward(kt, qt))[..., alts].clone()
+
+    def loss(k, q):
+        Te = interior(run, electron_temperature(run, k, q))
+        return ((Te[..., alts] - target) ** 2).sum()
+
+    # warm-up through the same path (autograd's backward runs on its own device thread;
+    # the first library calls there and the adjoint work buffers are set up here)
+    kw = k0.clone().requires_grad_(True)
+    loss(kw, q0.clone().requires_grad_(True)).backward()
+    log(f"adjoint workload ready in {time.perf_counter() - t0:.1f} s, dt = {dt:.4g} s")
+
+    k = k0.clone().requires_grad_(True)
+    q = q0.clone().requires_grad_(True)
+    run.tr.kernel_timing_reset()
+    run.tr.kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L = loss(k, q)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()                          # the same backward called directly
+    run.backward(torch.zeros(run.N, dtype=torch.float64, device=dev))
+    torch.cuda.synchronize()
+    t_adj_direct = time.perf_counter() - t0
+    run.tr.kernel_timing(False)
+    kern = {}
+    for name in ("adj_trbdf2", "adj_advection", "trbdf2", "sweep1", "sweep2", "sweep3"):
+        tot, n = run.tr.kernel_time(name)
+        if n:
+            kern[name] = {"ms": round(tot / n, 4), "launches": n}
+    gk, gq = k.grad.detach().clone(), q.grad.detach().clone()
+    rels, sweep = [], {}
+    with torch.no_grad():
+        for d in range(fd_dirs):
+            dk = torch.randn(sh, generator=gen, device=dev, dtype=torch.float64)
+            dq = A["qmax"] * torch.randn(sh, generator=gen, device=dev, dtype=torch.float64)
+            ad = ((gk * dk).sum() + (gq * dq).sum()).item()
+            # the configured eps on every direction; on the first few, smaller and larger
+            # steps too: the minmod limiter makes the loss piecewise smooth, and the number
+            # of limiter switches a +-eps perturbation crosses grows with eps
+            for e in ([A["eps"]] if d >= A["fd_sweep_dirs"] else A["fd_sweep_eps"]):
+                fd = (loss(k0 + e * dk, q0 + e * dq) - loss(k0 - e * dk, q0 - e * dq)).item() / (2 * e)
+                r = abs(fd - ad) / abs(fd)
+                if e == A["eps"]:
+                    rels.append(r)
+                if d < A["fd_sweep_dirs"]:
+                    sweep.setdefault(f"{e:.0e}", []).append(float(f"{r:.3e}"))
+    run.close()
+    rels.sort()
+    res = {"forward_ms_per_step": round(1e3 * t_fwd / A["steps"], 4),
+           "adjoint_ms_per_step": round(1e3 * t_adj / A["steps"], 4),
+           "ratio": round(t_adj / t_fwd, 3), "loss": L.item(),
+           "adjoint_ms_per_step_direct_call": round(1e3 * t_adj_direct / A["steps"], 4),
+           "checkpoints": run.nseg, "tape_GB": round(run.tape_bytes() / 1e9, 2),
+           "kernels_ms": kern}
+    if rels:
+        res["fd_check"] = {"directions": len(rels), "eps": A["eps"],
+                           "max_rel_err": float(f"{rels[-1]:.3e}"),
+                           "median_rel_err": float(f"{rels[len(rels) // 2]:.3e}"),
+                           "eps_sweep_rel_err": sweep}
+    log(f"adjoint: forward {res['forward_ms_per_step']} ms/step, adjoint "
+        f"{res['adjoint_ms_per_step']} ms/step, ratio {res['ratio']}" +
+        (f", FD max rel err {rels[-1]:.2e} over {len(rels)} directions" if rels else ""))
+    return res, g
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dipole512", choices=sorted(WORKLOADS) + ["ensemble64"])
+    ap.add_argument("--workload", default="dipole512",
+                    choices=sorted(WORKLOADS) + ["ensemble64", "adjoint"])
+    ap.add_argument("--adjoint-fd", type=int, default=None,
+                    help="central-difference directions checked by the adjoint measurement "
+                         "(default 4 in the default run, 32 in the adjoint workload)")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -251,6 +362,20 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble64":
         bench_ensemble(args)
+        return
+    if args.workload == "adjoint":
+        import torch
+        res, g = bench_adjoint(ADJOINT["fd_dirs"] if args.adjoint_fd is None else args.adjoint_fd)
+        steps = ADJOINT["steps"]
+        ms = res["adjoint_ms_per_step"]
+        print(json.dumps({
+            "metric": METRIC, "value": g.ncells * 7 / (res["forward_ms_per_step"] / 1e3),
+            "unit": UNIT, "n_gpus": 1, "steps": steps, "warmup": 1,
+            "ms_per_step": res["forward_ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded profiles generated in HBM)",
+            "config": {"workload": ADJOINT["desc"]},
+            "adjoint_forward_ratio": res["ratio"], "adjoint": res}), flush=True)
         return
     w = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else w["steps"]
@@ -319,6 +444,12 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle on host cores)")
         cpu = cpu_baseline(w, dt, args.cpu_seconds)
+    adj = None
+    if not args.no_adjoint:
+        del tr
+        torch.cuda.empty_cache()
+        log("adjoint measurement (" + ADJOINT["desc"] + ")")
+        adj, _ = bench_adjoint(4 if args.adjoint_fd is None else args.adjoint_fd)
     out = {
         "metric": METRIC,
         "value": value,
@@ -345,7 +476,8 @@ def main():
                      "frac": round(step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kern,
         "cpu_baseline": cpu,
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": None if adj is None else adj["ratio"],
+        "adjoint": None if adj is None else dict(adj, workload=ADJOINT["desc"]),
     }
     print(json.dumps(out), flush=True)
 

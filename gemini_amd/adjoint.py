@@ -36,16 +36,25 @@ class TransportRun:
     x2 fastest (e.g. shape (lx3, lx2)); qtop may be None (the reference's Dirichlet top).
     """
 
-    def __init__(self, grid: Grid, state: State, dt: float, nsteps: int, every: int = 10,
-                 device: int = 0, Teinf: float = 1500.0):
+    def __init__(self, grid: Grid, state, dt: float, nsteps: int, every: int = 10,
+                 device: int = 0, Teinf: float = 1500.0, geometry_on_device: bool = False):
+        """state: a :class:`State`, or a mapping of device blocks (fluidvars, aux,
+        electrovars, nn, Tn) laid out like the reference's, whose interface velocities
+        start at zero as the reference allocates them."""
         if nsteps < 1 or every < 1:
             raise ValueError("nsteps and every must be positive")
         self.g, self.dt, self.nsteps, self.every = grid, float(dt), int(nsteps), int(every)
         self.dev = torch.device(f"cuda:{device}")
-        self.tr = Transport(grid, state.nn, state.Tn, device=device, Teinf=Teinf,
-                            own_context=True)
+        blocks = not isinstance(state, State)
+        nn, Tn = (state["nn"], state["Tn"]) if blocks else (state.nn, state.Tn)
+        self.tr = Transport(grid, nn, Tn, device=device, Teinf=Teinf, own_context=True,
+                            geometry_on_device=geometry_on_device)
         self.tr.set_residency(1)
-        self.tr.push(state)
+        if blocks:
+            self.tr.push_blocks(state["fluidvars"], state["aux"], state["electrovars"])
+            self.tr.set_interface_vels_zero()
+        else:
+            self.tr.push(state)
         self.N = grid.padded[0] * grid.padded[1] * grid.padded[2]
         self.ncol = grid.lx2 * grid.lx3
         f64 = dict(dtype=torch.float64, device=self.dev)

@@ -39,6 +39,8 @@ inline unsigned grid_for(long n) {
 }
 inline unsigned grid_exact(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
 // ------------------------------------------------------------------ tape helpers
 // electron interface velocities (advec.f90:155-229), the values the forward sweeps use
 __global__ void k_tape_vels(DevGrid g, const double* vs1, const double* vs2, const double* vs3,
@@ -357,43 +359,93 @@ __device__ __forceinline__ bool adj_elim(const Col& q, int j, int s0, double& r0
   return sing;
 }
 
-// dtbsv (upper, no transpose) over stored rows s0..s0+2 with rhs slot sy, into slot sx
-__device__ __forceinline__ void adj_backsub(const Col& q, int N, int s0, int sy, int sx) {
-  double x1 = 0.0, x2 = 0.0;                         // x(i+1), x(i+2)
-  for (int i = N; i >= 1; --i) {
-    double acc = q(i, sy);
-    if (i + 2 <= N && x2 != 0.0) acc = acc - x2 * q(i, s0 + 2);
-    if (i + 1 <= N && x1 != 0.0) acc = acc - x1 * q(i, s0 + 1);
-    if (acc != 0.0) acc = acc / q(i, s0);
-    q(i, sx) = acc;
-    x2 = x1;
-    x1 = acc;
+// Every pass below walks the rows of a column in order with a loop-carried recurrence.
+// Row data are fetched P rows ahead into a register ring, so the loads of later rows are
+// in flight while the recurrence runs (the loop over a full ring is unrolled, its ring
+// slots are registers); the last count % P rows are fetched as they come.
+template <int P, int DIR, class Raw, class Fetch, class Body>
+__device__ __forceinline__ void stream_rows(int first, int count, Fetch fetch, Body body) {
+  Raw ring[P];
+#pragma unroll
+  for (int u = 0; u < P; ++u) fetch(first + DIR * u, ring[u]);
+  const int nfull = count / P;
+  for (int c = 0; c < nfull; ++c) {
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+      const int i = first + DIR * (c * P + u);
+      const Raw v = ring[u];
+      fetch(i + DIR * P, ring[u]);
+      body(i, v);
+    }
+  }
+  for (int k = nfull * P; k < count; ++k) {
+    const int i = first + DIR * k;
+    Raw v;
+    fetch(i, v);
+    body(i, v);
   }
 }
 
-// M^T w = rhs for a matrix factored by adj_elim into slots s0..s0+5; the rhs arrives in
-// slot s0+3 (set by the caller), which ends holding w.  U^T z = rhs top-down, then the
-// eliminations transposed bottom-up: z(j) -= l_j z(j+1), then the interchange.
-__device__ __forceinline__ void adj_solveT(const Col& q, int N, int s0) {
+constexpr int PF = 8;            // ring depth of the light passes
+constexpr int PFI = 2;           // ring depth of the passes that read the cell inputs
+
+struct R4 { double a, b, c, d; };
+struct R5 { double a, b, c, d, e; };
+
+// dtbsv (upper, no transpose) over stored rows s0..s0+2 with rhs slot sy, into slot sx;
+// with mask_lam, also mu(i) = dL/dx(i) through the output floor into slot sy
+__device__ __forceinline__ void adj_backsub(const Col& q, int N, int s0, int sy, int sx,
+                                            const double* lam = nullptr) {
+  double x1 = 0.0, x2 = 0.0;                         // x(i+1), x(i+2)
+  auto fetch = [&](int i, R5& r) {
+    const int c = clampi(i, 1, N);
+    r.a = q(c, s0); r.b = q(c, s0 + 1); r.c = q(c, s0 + 2); r.d = q(c, sy);
+    r.e = lam ? lam[c] : 0.0;
+  };
+  stream_rows<PF, -1, R5>(N, N, fetch, [&](int i, const R5& r) {
+    double acc = r.d;
+    if (i + 2 <= N && x2 != 0.0) acc = acc - x2 * r.c;
+    if (i + 1 <= N && x1 != 0.0) acc = acc - x1 * r.b;
+    if (acc != 0.0) acc = acc / r.a;
+    q(i, sx) = acc;
+    if (lam) q(i, sy) = acc >= 100.0 ? r.e : 0.0;
+    x2 = x1;
+    x1 = acc;
+  });
+}
+
+// M^T w = rhs for a matrix factored by adj_elim into slots s0..s0+5.  The rhs is slot
+// s0+3, or with cw != 0 the interior rows of cw * (slot BY); slot s0+3 ends holding w.
+// U^T z = rhs top-down, then the eliminations transposed bottom-up: z(j) -= l_j z(j+1),
+// then the interchange.
+__device__ __forceinline__ void adj_solveT(const Col& q, int N, int s0, double cw = 0.0) {
   double zm1 = 0.0, zm2 = 0.0;                       // z(i-1), z(i-2)
   double u1m1 = 0.0, u2m1 = 0.0, u2m2 = 0.0;         // U1(i-1), U2(i-1), U2(i-2)
-  for (int i = 1; i <= N; ++i) {
-    const double z = ((q(i, s0 + 3) - u1m1 * zm1) - u2m2 * zm2) / q(i, s0);
+  const int sr = cw != 0.0 ? BY : s0 + 3;
+  stream_rows<PF, 1, R4>(1, N, [&](int i, R4& r) {
+    const int c = clampi(i, 1, N);
+    r.a = q(c, s0); r.b = q(c, s0 + 1); r.c = q(c, s0 + 2); r.d = q(c, sr);
+  }, [&](int i, const R4& r) {
+    const double rhs = cw == 0.0 ? r.d : (i >= 2 && i <= N - 1 ? cw * r.d : 0.0);
+    const double z = ((rhs - u1m1 * zm1) - u2m2 * zm2) / r.a;
     q(i, s0 + 3) = z;
     zm2 = zm1; zm1 = z;
     u2m2 = u2m1;
-    u1m1 = q(i, s0 + 1); u2m1 = q(i, s0 + 2);
-  }
+    u1m1 = r.b; u2m1 = r.c;
+  });
   double carry = q(N, s0 + 3);
-  for (int j = N - 1; j >= 1; --j) {
-    const double a = q(j, s0 + 3) - q(j, s0 + 4) * carry;
-    if (q(j, s0 + 5) != 0.0) {
+  stream_rows<PF, -1, R4>(N - 1, N - 1, [&](int j, R4& r) {
+    const int c = clampi(j, 1, N);
+    r.a = q(c, s0 + 3); r.b = q(c, s0 + 4); r.c = q(c, s0 + 5);
+  }, [&](int j, const R4& r) {
+    const double a = r.a - r.b * carry;
+    if (r.c != 0.0) {
       q(j + 1, s0 + 3) = a;
     } else {
       q(j + 1, s0 + 3) = carry;
       carry = a;
     }
-  }
+  });
   q(1, s0 + 3) = carry;
 }
 
@@ -420,16 +472,20 @@ __global__ __launch_bounds__(64) void k_adj_trbdf2(DevGrid g, DevState s, AdjArg
   const double kap = s.kappa ? s.kappa[col] : 1.0;
   const bool flux = s.qtop != nullptr;
 
-  // per-row inputs: T, coefficients C, B, D, conductivity without and with kappa
-  struct Row { double T, C, B, D, lam_e, h1, h23; CellIn<true> in; };
-  auto load_row = [&](int r, Row& w) {
-    cell_load<true, true>(g, se, 0, cb, kb, ni, r, w.in, (int)col);
+  // per-row inputs: T, coefficients C, B, D, conductivity without kappa; in pass I also
+  // both solutions and both adjoint solutions
+  struct Row { double T, C, B, D, lam_e, h1, h23, x1, x2, w1, w2; CellIn<true> in; };
+  auto fetch_in = [&](int r, CellIn<true>& c) {
+    cell_load<true, true>(g, se, 0, cb, kb, ni, clampi(r, 1, N), c, (int)col);
+  };
+  auto make_row = [&](const CellIn<true>& c, Row& w) {
+    w.in = c;
     w.in.kap = kap;
-    w.T = w.in.T;
+    w.T = c.T;
     w.lam_e = cell_lambda<true, false>(w.in, LSP);
     cell_coeffs<true, true>(w.in, LSP, gm1kB, w.C, w.B, w.D);
-    w.h1 = w.in.h1;
-    w.h23 = w.in.h2 * w.in.h3;
+    w.h1 = c.h1;
+    w.h23 = c.h2 * c.h3;
   };
   double Tmin, Tmax;
   {
@@ -440,42 +496,45 @@ __global__ __launch_bounds__(64) void k_adj_trbdf2(DevGrid g, DevState s, AdjArg
   }
   double aN = 0.0, bN = 1.0, lamN = 0.0, h1N = 0.0;
   if (flux) {
+    CellIn<true> c;
     Row top;
-    load_row(N, top);
+    fetch_in(N, c);
+    make_row(c, top);
     lamN = top.lam_e * kap;
     h1N = top.h1;
     aN = -1.0 / dx1[N];
     bN = 1.0 / dx1[N];
     Tmax = (s.qtop[col] * h1N) / lamN;
   }
+  auto coeffs = [&](int i, const Row& rm, const Row& rc, const Row& rp, double& s1, double& s2,
+                    double& s3) {
+    const double Dh = 0.5 * (rm.D + rc.D), Dh1 = 0.5 * (rc.D + rp.D);
+    s1 = ((rc.C * Dh) / dx1i[i]) / dx1[i];
+    s3 = ((rc.C * Dh1) / dx1i[i]) / dx1[i + 1];
+    s2 = rc.B / (dx1[i + 1] + dx1[i]);
+  };
 
-  // pass A (top-down): both factorisations, the TR right-hand side eliminated alongside
+  // pass A (top-down): both factorisations, the TR right-hand side eliminated alongside;
+  // row r's arrival completes row r-1
   bool sing = false;
   double e0 = 1.0, e1 = 0.0, eb = Tmin;              // TR pending row (row 1: identity)
   double f0 = 1.0, f1 = 0.0;                         // BDF2 pending row
   Row rm, rc, rp;
-  load_row(1, rc);
-  load_row(2, rp);
-  rm = rc;
-  for (int i = 2; i <= N; ++i) {
-    double a, b, cc, d, a2, b2, c2;
-    if (i <= N - 1) {
-      rm = rc; rc = rp;
-      load_row(i + 1, rp);
-      const double Dh = 0.5 * (rm.D + rc.D), Dh1 = 0.5 * (rc.D + rp.D);
-      const double s1 = ((rc.C * Dh) / dx1i[i]) / dx1[i];
-      const double s3 = ((rc.C * Dh1) / dx1i[i]) / dx1[i + 1];
-      const double s2 = rc.B / (dx1[i + 1] + dx1[i]);
-      const double t1 = s1 / 2.0, t2 = s2 / 2.0, t3 = s3 / 2.0;
-      a = -t1 + t2; b = (K1 + t3) + t1; cc = -t3 - t2;
-      d = ((rc.T / dt2 - a * rm.T) - (t3 + t1) * rc.T) - cc * rp.T;
-      a2 = -s1 + s2; b2 = (K2 + s3) + s1; c2 = -s3 - s2;
-    } else {
-      a = a2 = aN; b = b2 = bN; cc = c2 = 0.0; d = Tmax;
-    }
+  stream_rows<PFI, 1, CellIn<true>>(1, N, fetch_in, [&](int r, const CellIn<true>& c) {
+    rm = rc; rc = rp;
+    make_row(c, rp);
+    if (r < 3) return;
+    const int i = r - 1;
+    double s1, s2, s3;
+    coeffs(i, rm, rc, rp, s1, s2, s3);
+    const double t1 = s1 / 2.0, t2 = s2 / 2.0, t3 = s3 / 2.0;
+    const double a = -t1 + t2, b = (K1 + t3) + t1, cc = -t3 - t2;
+    const double d = ((rc.T / dt2 - a * rm.T) - (t3 + t1) * rc.T) - cc * rp.T;
     sing |= adj_elim(q, i - 1, A0, e0, e1, &eb, a, b, cc, d);
-    sing |= adj_elim(q, i - 1, B0, f0, f1, nullptr, a2, b2, c2, 0.0);
-  }
+    sing |= adj_elim(q, i - 1, B0, f0, f1, nullptr, -s1 + s2, (K2 + s3) + s1, -s3 - s2, 0.0);
+  });
+  sing |= adj_elim(q, N - 1, A0, e0, e1, &eb, aN, bN, 0.0, Tmax);   // boundary row N
+  sing |= adj_elim(q, N - 1, B0, f0, f1, nullptr, aN, bN, 0.0, 0.0);
   q(N, A0) = e0; q(N, A1) = 0.0; q(N, A2) = 0.0; q(N, AY) = eb;
   q(N, B0) = f0; q(N, B1) = 0.0; q(N, B2) = 0.0;
   sing |= e0 == 0.0 || f0 == 0.0;
@@ -489,26 +548,26 @@ __global__ __launch_bounds__(64) void k_adj_trbdf2(DevGrid g, DevState s, AdjArg
   // pass C: BDF2 right-hand side through the stored eliminations (row 1: Tmin, row N: Tmax)
   {
     double rb = Tmin;
-    for (int j = 1; j <= N - 1; ++j) {
-      const int r = j + 1;
-      const double d = r >= N ? Tmax : -((c13 * A.TD[cb + r]) / dt3) + (c43 * q(r, X1)) / dt3;
-      const bool swap = q(j, BS) != 0.0;
+    stream_rows<PF, 1, R4>(1, N - 1, [&](int j, R4& r) {
+      const int c = clampi(j, 1, N - 1);
+      r.a = A.TD[cb + c + 1]; r.b = q(c + 1, X1); r.c = q(c, BS); r.d = q(c, BL);
+    }, [&](int j, const R4& r) {
+      const double d = j + 1 >= N ? Tmax : -((c13 * r.a) / dt3) + (c43 * r.b) / dt3;
+      const bool swap = r.c != 0.0;
       const double pd = swap ? d : rb;
       double od = swap ? rb : d;
-      if (pd != 0.0) od = od + q(j, BL) * (-pd);
+      if (pd != 0.0) od = od + r.d * (-pd);
       q(j, BY) = pd;
       rb = od;
-    }
+    });
     q(N, BY) = rb;
   }
-  // pass D: BDF2 solution; mu = dL/dx2 through the output floor
-  adj_backsub(q, N, B0, BY, X2);
-  for (int i = 1; i <= N; ++i) q(i, BY) = q(i, X2) >= 100.0 ? A.lam[cb + i] : 0.0;
+  // pass D: BDF2 solution, and mu = dL/dx2 through the output floor into BY
+  adj_backsub(q, N, B0, BY, X2, A.lam + cb);
   // passes E-F: w2
   adj_solveT(q, N, B0);
   // passes G-H: w1 from dL/dx1 = (4/3)/(dt/3) w2 on the interior rows
-  for (int i = 1; i <= N; ++i) q(i, AY) = (i >= 2 && i <= N - 1) ? (c43 / dt3) * q(i, BY) : 0.0;
-  adj_solveT(q, N, A0);
+  adj_solveT(q, N, A0, c43 / dt3);
 
   // pass I (top-down): chain w1, w2 through the matrix and rhs entries of rows 2..N-1
   //   dL/dM1(i,k) = -w1(i) (x1(k) + T(k))   (M1 x1 = (2/(dt/2) - M1) T on those rows)
@@ -518,37 +577,36 @@ __global__ __launch_bounds__(64) void k_adj_trbdf2(DevGrid g, DevState s, AdjArg
   double gk = 0.0, gqc = 0.0;
   double gDh_p = 0.0, gDh_c = 0.0, gDh_n = 0.0;     // dL/dDh(i-1), (i), (i+1) so far
   double gT_m = 0.0, gT_c = 0.0, gT_n = 0.0;         // dL/dT(i-1), (i), (i+1) so far
-  double x1m = 0.0, x1c = q(1, X1), x1p = q(2, X1);
-  double x2m = 0.0, x2c = q(1, X2), x2p = q(2, X2);
-  load_row(1, rc);
-  load_row(2, rp);
-  rm = rc;
   // row j complete: D(j) = lambda(j) h2 h3 / h1 enters Dh(j) and Dh(j+1)
   auto finish = [&](int j, const Row& w, double gDh_j, double gDh_j1, double gT_j) {
     const double gD = 0.5 * (gDh_j + gDh_j1);
     double glam = gD * (w.h23 / w.h1);
     if (flux && j == N) {                            // row N rhs: q h1(N) / lambda(N)
-      const double gg = q(N, AY) + q(N, BY);
+      const double gg = w.w1 + w.w2;
       glam += -gg * (Tmax / lamN);
       gqc += gg * (h1N / lamN);
     }
     gk += glam * w.lam_e;
     A.lam[cb + j] = gT_j + glam * (kap * lambda_e_dT(w.in, w.lam_e));
   };
-  for (int i = 2; i <= N - 1; ++i) {
+  struct RawI { CellIn<true> in; double x1, x2, w1, w2; };
+  stream_rows<PFI, 1, RawI>(1, N, [&](int r, RawI& v) {
+    const int c = clampi(r, 1, N);
+    fetch_in(c, v.in);
+    v.x1 = q(c, X1); v.x2 = q(c, X2); v.w1 = q(c, AY); v.w2 = q(c, BY);
+  }, [&](int r, const RawI& v) {
     rm = rc; rc = rp;
-    load_row(i + 1, rp);
-    x1m = x1c; x1c = x1p; x1p = q(i + 1, X1);
-    x2m = x2c; x2c = x2p; x2p = q(i + 1, X2);
-    const double w1 = q(i, AY), w2 = q(i, BY);
-    const double Dh = 0.5 * (rm.D + rc.D), Dh1 = 0.5 * (rc.D + rp.D);
-    const double s1 = ((rc.C * Dh) / dx1i[i]) / dx1[i];
-    const double s3 = ((rc.C * Dh1) / dx1i[i]) / dx1[i + 1];
-    const double s2 = rc.B / (dx1[i + 1] + dx1[i]);
+    make_row(v.in, rp);
+    rp.x1 = v.x1; rp.x2 = v.x2; rp.w1 = v.w1; rp.w2 = v.w2;
+    if (r < 3) return;
+    const int i = r - 1;
+    double s1, s2, s3;
+    coeffs(i, rm, rc, rp, s1, s2, s3);
     const double t1 = s1 / 2.0, t2 = s2 / 2.0, t3 = s3 / 2.0;
     const double a = -t1 + t2, cc = -t3 - t2;
-    const double G1a = -w1 * (x1m + rm.T), G1b = -w1 * (x1c + rc.T), G1c = -w1 * (x1p + rp.T);
-    const double G2a = -w2 * x2m, G2b = -w2 * x2c, G2c = -w2 * x2p;
+    const double w1 = rc.w1, w2 = rc.w2;
+    const double G1a = -w1 * (rm.x1 + rm.T), G1b = -w1 * (rc.x1 + rc.T), G1c = -w1 * (rp.x1 + rp.T);
+    const double G2a = -w2 * rm.x2, G2b = -w2 * rc.x2, G2c = -w2 * rp.x2;
     const double gs1 = 0.5 * (G1b - G1a) + (G2b - G2a);
     const double gs3 = 0.5 * (G1b - G1c) + (G2b - G2c);
     gDh_c += gs1 * ((rc.C / dx1i[i]) / dx1[i]);
@@ -559,7 +617,7 @@ __global__ __launch_bounds__(64) void k_adj_trbdf2(DevGrid g, DevState s, AdjArg
     finish(i - 1, rm, gDh_p, gDh_c, gT_m);
     gDh_p = gDh_c; gDh_c = gDh_n; gDh_n = 0.0;
     gT_m = gT_c; gT_c = gT_n; gT_n = 0.0;
-  }
+  });
   finish(N - 1, rc, gDh_p, gDh_c, gT_m);
   finish(N, rp, gDh_c, 0.0, gT_c);
   if (A.gk) A.gk[col] += gk;

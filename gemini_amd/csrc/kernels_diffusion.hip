@@ -497,8 +497,9 @@ void diffusion_launch(const DevGrid& g, const DevState& s, int diffsolvetype, do
   if (diffsolvetype == 1) {
     hipLaunchKernelGGL((k_diffusion<R, true, 0>), grd, blk, 0, st, g, s, scratch, swbits, lst, be,
                        bdf, Teinf);
-  } else if (s.kappa || s.qtop) {                     // fitting extension (TRBDF2 only)
-    hipLaunchKernelGGL((k_diffusion<R, false, 0, true>), grd, blk, 0, st, g, s, scratch, swbits,
+  } else if (s.kappa || s.qtop) {                     // fitting extension (TRBDF2 only; the
+    // 4-row chunk keeps two waves per SIMD with the extension's extra registers)
+    hipLaunchKernelGGL((k_diffusion<4, false, 0, true>), grd, blk, 0, st, g, s, scratch, swbits,
                        lst, tr, bdf, Teinf);
   } else if (DIFF_SPLIT) {
     hipLaunchKernelGGL((k_diffusion<R, false, 1>), grd, blk, 0, st, g, s, scratch, swbits, lst, tr,
