@@ -15,7 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgemhip.so")
-SOURCES = ["gemhip_api.hip", "kernels_advec.hip", "kernels_diffusion.hip", "kernels_adjoint.hip"]
+SOURCES = ["gemhip_api.hip", "kernels_advec.hip", "kernels_diffusion.hip", "kernels_adjoint.hip",
+           "kernels_block.hip"]
 ARCH = os.environ.get("GEMHIP_ARCH", "gfx950")
 
 

@@ -45,6 +45,8 @@ struct Ctx {
   double* d_neut = nullptr;
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  double* d_Q = nullptr;        // artificial viscosity (intvars%Q), lx1*lx2*lx3*7
+  int* d_null = nullptr;        // null-cell mask, lx1*lx2*lx3 (null when none registered)
   double* d_kappa = nullptr;    // fitting extension (lx2*lx3 each)
   double* d_qtop = nullptr;
   double* d_adj = nullptr;      // adjoint work: a second field and the column scratch
@@ -128,10 +130,12 @@ void collect_timings() {
 void free_all() {
   Ctx& c = ctx();
   for (double** p : {&c.d_geom, &c.d_fluid, &c.d_aux, &c.d_J1, &c.d_vsi, &c.d_v1b, &c.d_neut,
-                     &c.d_scratch, &c.d_kappa, &c.d_qtop, &c.d_adj}) {
+                     &c.d_scratch, &c.d_kappa, &c.d_qtop, &c.d_adj, &c.d_Q}) {
     if (*p) hipFree(*p);
     *p = nullptr;
   }
+  if (c.d_null) hipFree(c.d_null);
+  c.d_null = nullptr;
   c.scratch_bytes = 0;
   c.adj_bytes = 0;
   c.s.kappa = c.s.qtop = nullptr;
@@ -401,7 +405,8 @@ int gemhip_set_grid_size(int lx1, int lx2, int lx3) {
             dev_alloc(&c.d_J1, (size_t)g.N, "J1") &&
             dev_alloc(&c.d_vsi, 7L * (nv1i() + nv2i() + nv3i()), "interface velocities") &&
             dev_alloc(&c.d_v1b, 14L * lx2 * lx3, "x1 boundary velocities") &&
-            dev_alloc(&c.d_neut, 4L * ni, "neutrals");
+            dev_alloc(&c.d_neut, 4L * ni, "neutrals") &&
+            dev_alloc(&c.d_Q, 7L * ni, "artificial viscosity");
   if (!ok) { free_all(); return 3; }
   DevState& s = c.s;
   s = DevState{};
@@ -625,12 +630,131 @@ int run_step(double dt, double* tape) {
   if (!run_energy_diffusion(dt)) return 4;               // :208
   return 0;
 }
+// The transport block of src/libgemini.cpp:155-210: the advection part of run_step
+// (the energy density stays as the sweeps leave it), rhov12v1, clean_param 1 and 2,
+// VNRicht_artvisc, halo_fluidvars, RK2_global_boundary_allspec, compression, rhoe2T,
+// clean_param 3, energy_diffusion, clean_param 3, T2rhoe.
+int run_block(double dt) {
+  if (!ready(true)) return 1;
+  Ctx& c = ctx();
+  if (c.residency != 1) { fail("gemhip_fluid_adv_block needs residency mode 1"); return 2; }
+  const DevGrid& g = c.g;
+  const DevState& s = c.s;
+  hipStream_t st = c.stream;
+  launch_v12rhov1_T2rhoe(g, s, true, true, st);          // libgemini.cpp:155-156
+  launch_set_global_boundaries(g, s, LSP, st);           // :168
+  halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :169
+  launch_save_v1_boundaries(g, s, st);                   // :170
+  launch_sweep(g, s, 3, F_ALL, dt, true, st, c.tune);   // :174-176
+  launch_sweep(g, s, 1, F_ALL, dt, true, st, c.tune);   // :178-180
+  halo_fields({NS, RHOVS1, RHOES});                      // :181
+  launch_sweep(g, s, 2, F_ALL, dt, true, st, c.tune);   // :183-185
+  launch_rhov12v1(g, s, st);                             // :186
+  const int* nul = c.d_null;
+  launch_clean_param(g, s, 1, nul, st);                  // :187-188
+  launch_clean_param(g, s, 2, nul, st);
+  launch_artvisc(g, s, c.d_Q, st);                       // :191
+  halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :199
+  launch_rk2_global_boundary(g, s, st);                  // :200
+  launch_compression(g, s, c.d_Q, dt, st);               // :203
+  launch_rhoe2T(g, s, st);                               // :204
+  launch_clean_param(g, s, 3, nul, st);                  // :205
+  if (!hipok(hipGetLastError(), "transport block launch")) return 3;
+  c.vsi_current = false;
+  if (!run_energy_diffusion(dt)) return 4;               // :208
+  launch_clean_param(g, s, 3, nul, st);                  // :209
+  launch_v12rhov1_T2rhoe(g, s, false, true, st);         // :210
+  return hipok(hipGetLastError(), "transport block launch") ? 0 : 3;
+}
 }  // namespace
 }  // namespace gemhip
 
 extern "C" {
 
 int gemhip_transport_step(double dt) { return run_step(dt, nullptr); }
+
+int gemhip_fluid_adv_block(double dt) { return run_block(dt); }
+
+int gemhip_register_null_cells(const int* nullpts) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  hipStreamSynchronize(c.stream);
+  if (!nullpts) {
+    if (c.d_null) hipFree(c.d_null);
+    c.d_null = nullptr;
+    return 0;
+  }
+  const size_t n = (size_t)c.lx1 * c.lx2 * c.lx3;
+  if (!c.d_null && !hipok(hipMalloc((void**)&c.d_null, n * sizeof(int)), "null cells")) return 2;
+  if (!hipok(hipMemcpy(c.d_null, nullpts, n * sizeof(int), hipMemcpyDefault), "null cells upload"))
+    return 3;
+  return 0;
+}
+
+void clean_param_C(int* iparm, int* xtype, void** xC, double** fluidvars) {
+  (void)xC;
+  if (!ready() || !check_xtype(xtype) || !check_host(fluidvars, "fluidvars")) return;
+  if (!iparm || *iparm < 1 || *iparm > 3) {     // libgemini.f90:1050
+    fail("  libgemini:clean_params_C(); invalid parameter selected");
+    return;
+  }
+  Ctx& c = ctx();
+  const Var v = *iparm == 1 ? NS : *iparm == 2 ? VS1 : TS;
+  Io io{fluidvars, nullptr, nullptr};
+  if (!upload(io, {v})) return;
+  launch_clean_param(c.g, c.s, *iparm, c.d_null, c.stream);
+  if (!hipok(hipGetLastError(), "clean_param launch")) return;
+  download(io, {v});
+}
+
+void VNRicht_artvisc_C(double** fluidvars, void** intvars) {
+  (void)intvars;
+  if (!ready() || !check_host(fluidvars, "fluidvars")) return;
+  Ctx& c = ctx();
+  Io io{fluidvars, nullptr, nullptr};
+  if (!upload(io, {NS, VS1})) return;
+  launch_artvisc(c.g, c.s, c.d_Q, c.stream);
+  if (c.residency == 0) hipok(hipStreamSynchronize(c.stream), "artificial viscosity");
+}
+
+void compression_C(double** fluidvars, double** fluidauxvars, void** intvars, int* xtype,
+                   void** xC, double* dt) {
+  (void)intvars; (void)xC;
+  if (!ready() || !check_xtype(xtype) || !check_host(fluidvars, "fluidvars") ||
+      !check_host(fluidauxvars, "fluidauxvars"))
+    return;
+  if (!dt) { fail("dt is a null pointer"); return; }
+  Ctx& c = ctx();
+  Io io{fluidvars, fluidauxvars, nullptr};
+  if (!upload(io, {VS1, VS2, VS3, RHOES})) return;
+  launch_compression(c.g, c.s, c.d_Q, *dt, c.stream);
+  if (!hipok(hipGetLastError(), "compression launch")) return;
+  download(io, {RHOES});
+}
+
+void RK2_prep_mpi_allspec_C(int* xtype, void** xC, double** fluidvars) {
+  (void)xC;
+  if (!ready() || !check_xtype(xtype)) return;
+  Ctx& c = ctx();
+  if (!c.g.flagper) return;                    // one process, aperiodic: nothing to exchange
+  if (!check_host(fluidvars, "fluidvars")) return;
+  Io io{fluidvars, nullptr, nullptr};
+  if (!upload(io, {VS1, VS2, VS3})) return;
+  double* f[3] = {dev_var(VS1), dev_var(VS2), dev_var(VS3)};
+  launch_halo_periodic(c.g, f, 3, 1, LSP, c.stream);   // sources_mpi.f90:14-50, depth 1
+  download(io, {VS1, VS2, VS3});
+}
+
+void RK2_global_boundary_allspec_C(int* xtype, void** xC, double** fluidvars) {
+  (void)xC;
+  if (!ready() || !check_xtype(xtype) || !check_host(fluidvars, "fluidvars")) return;
+  Ctx& c = ctx();
+  Io io{fluidvars, nullptr, nullptr};
+  if (!upload(io, {VS1, VS2, VS3})) return;
+  launch_rk2_global_boundary(c.g, c.s, c.stream);
+  if (!hipok(hipGetLastError(), "RK2 boundary launch")) return;
+  download(io, {VS1, VS2, VS3});
+}
 
 // ---- part 4: parameter fitting ----
 int gemhip_register_conductivity_scale(const double* kappa) {

@@ -93,6 +93,16 @@ void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int
                              double Teinf, double* scratch, hipStream_t st,
                              const Tuning& tune = Tuning{});
 
+// ---- the rest of fluid_adv's transport block (kernels_block.hip) ----
+// iparm 1 density, 2 velocity (vs1), 3 temperature; nul: null-cell mask (lx1*lx2*lx3,
+// nonzero = null) or null for none.  Q is (lx1, lx2, lx3, 7).
+void launch_clean_param(const DevGrid& g, const DevState& s, int iparm, const int* nul,
+                        hipStream_t st);
+void launch_artvisc(const DevGrid& g, const DevState& s, double* Q, hipStream_t st);
+void launch_rk2_global_boundary(const DevGrid& g, const DevState& s, hipStream_t st);
+void launch_compression(const DevGrid& g, const DevState& s, const double* Q, double dt,
+                        hipStream_t st);
+
 // ---- adjoint of the electron energy path (kernels_adjoint.hip) ----
 // What one recorded forward step keeps for the adjoint (electron fields only): densities at
 // the step start and at the diffusion, the energy density entering each sweep, the

@@ -18,12 +18,14 @@ module gemhip_bind
             sweep1_allspec_mass_C, sweep1_allspec_momentum_C, sweep1_allspec_energy_C, &
             sweep2_allparams_C, sweep2_allspec_mass_C, sweep2_allspec_momentum_C, &
             sweep2_allspec_energy_C, energy_diffusion_C, halo_interface_vels_allspec_C, &
-            halo_allparams_C, halo_fluidvars_C
+            halo_allparams_C, halo_fluidvars_C, clean_param_C, VNRicht_artvisc_C, &
+            compression_C, RK2_prep_mpi_allspec_C, RK2_global_boundary_allspec_C
   public :: gemhip_init, gemhip_finalize, gemhip_set_error_mode, gemhip_clear_error, &
             gemhip_set_grid_size, gemhip_register_grid, gemhip_register_neutrals, &
             gemhip_register_config, gemhip_set_residency, gemhip_state_push, &
             gemhip_state_pull, gemhip_get_interface_vels, gemhip_set_interface_vels, &
-            gemhip_transport_step, gemhip_synchronize, gemhip_error_message
+            gemhip_transport_step, gemhip_synchronize, gemhip_error_message, &
+            gemhip_register_null_cells, gemhip_fluid_adv_block
 
   ! ------------------------------------------------ part 1: reference entry points
   interface
@@ -162,6 +164,33 @@ module gemhip_bind
       integer(c_int) :: xtype
       type(c_ptr) :: xC, fluidvars, fluidauxvars
     end subroutine
+    subroutine clean_param_C(iparm, xtype, xC, fluidvars) bind(C, name="clean_param_C")
+      import :: c_ptr, c_int
+      integer(c_int) :: iparm, xtype
+      type(c_ptr) :: xC, fluidvars
+    end subroutine
+    subroutine VNRicht_artvisc_C(fluidvars, intvars) bind(C, name="VNRicht_artvisc_C")
+      import :: c_ptr
+      type(c_ptr) :: fluidvars, intvars
+    end subroutine
+    subroutine compression_C(fluidvars, fluidauxvars, intvars, xtype, xC, dt) &
+        bind(C, name="compression_C")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: fluidvars, fluidauxvars, intvars, xC
+      integer(c_int) :: xtype
+      real(c_double) :: dt
+    end subroutine
+    subroutine RK2_prep_mpi_allspec_C(xtype, xC, fluidvars) bind(C, name="RK2_prep_mpi_allspec_C")
+      import :: c_ptr, c_int
+      integer(c_int) :: xtype
+      type(c_ptr) :: xC, fluidvars
+    end subroutine
+    subroutine RK2_global_boundary_allspec_C(xtype, xC, fluidvars) &
+        bind(C, name="RK2_global_boundary_allspec_C")
+      import :: c_ptr, c_int
+      integer(c_int) :: xtype
+      type(c_ptr) :: xC, fluidvars
+    end subroutine
   end interface
 
   ! ------------------------------------------------ part 2: library set-up
@@ -227,6 +256,15 @@ module gemhip_bind
         bind(C, name="gemhip_set_interface_vels")
       import :: c_int, c_double
       real(c_double), intent(in) :: vs1i(*), vs2i(*), vs3i(*)
+    end function
+    integer(c_int) function gemhip_register_null_cells(nullpts) &
+        bind(C, name="gemhip_register_null_cells")
+      import :: c_int
+      integer(c_int), intent(in) :: nullpts(*)
+    end function
+    integer(c_int) function gemhip_fluid_adv_block(dt) bind(C, name="gemhip_fluid_adv_block")
+      import :: c_int, c_double
+      real(c_double), value :: dt
     end function
     integer(c_int) function gemhip_transport_step(dt) bind(C, name="gemhip_transport_step")
       import :: c_int, c_double

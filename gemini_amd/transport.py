@@ -49,6 +49,9 @@ def load_library(path: str = LIB_PATH):
                  "gemhip_context_destroy", "gemhip_set_tuning", "gemhip_get_tuning",
                  "gemhip_ensemble_capture", "gemhip_ensemble_launch", "gemhip_ensemble_release"):
         getattr(lib, name).restype = C.c_int
+    for name in ("gemhip_register_null_cells", "gemhip_fluid_adv_block"):
+        getattr(lib, name).restype = C.c_int
+    lib.gemhip_fluid_adv_block.argtypes = [C.c_double]
     for name in ("gemhip_register_conductivity_scale", "gemhip_register_top_heat_flux",
                  "gemhip_checkpoint_save", "gemhip_checkpoint_load",
                  "gemhip_adjoint_record_step", "gemhip_adjoint_step"):
@@ -309,6 +312,69 @@ class Transport:
         d = C.c_double(dt)
         self._void("energy_diffusion_C", C.byref(self._null), C.byref(self._xt),
                    C.byref(self._null), f.ref, e.ref, C.byref(self._null), C.byref(d))
+
+    def clean_param(self, st, iparm: int):
+        """clean_param_C: 1 densities, 2 vs1, 3 temperatures."""
+        f = _Ref(st.fluidvars)
+        i = C.c_int(iparm)
+        self._void("clean_param_C", C.byref(i), C.byref(self._xt), C.byref(self._null), f.ref)
+
+    def VNRicht_artvisc(self, st):
+        f = _Ref(st.fluidvars)
+        self._void("VNRicht_artvisc_C", f.ref, C.byref(self._null))
+
+    def compression(self, st, dt: float):
+        f, a = self._fa(st)
+        d = C.c_double(dt)
+        self._void("compression_C", f.ref, a.ref, C.byref(self._null), C.byref(self._xt),
+                   C.byref(self._null), C.byref(d))
+
+    def RK2_prep_mpi_allspec(self, st):
+        f = _Ref(st.fluidvars)
+        self._void("RK2_prep_mpi_allspec_C", C.byref(self._xt), C.byref(self._null), f.ref)
+
+    def RK2_global_boundary_allspec(self, st):
+        f = _Ref(st.fluidvars)
+        self._void("RK2_global_boundary_allspec_C", C.byref(self._xt), C.byref(self._null), f.ref)
+
+    def register_null_cells(self, nullpts):
+        """Null-cell mask (lx1, lx2, lx3) (nonzero: null), or None for none."""
+        if nullpts is None:
+            self._call("gemhip_register_null_cells", None)
+            return
+        m = np.asfortranarray(nullpts, dtype=np.int32)
+        self._call("gemhip_register_null_cells", m.ctypes.data_as(C.POINTER(C.c_int)))
+
+    def fluid_adv_block(self, dt: float):
+        """The whole transport block of src/libgemini.cpp:155-210 on the HBM-resident state
+        (residency mode 1)."""
+        self._call("gemhip_fluid_adv_block", C.c_double(dt))
+
+    def fluid_adv_block_calls(self, st, dt: float):
+        """The reference's calls of src/libgemini.cpp:155-210 one by one."""
+        self.v12rhov1(st)
+        self.T2rhoe(st)
+        self.set_global_boundaries_allspec(st, 7)
+        self.halo_fluidvars(st)
+        self.interface_vels_allspec(st, 7)
+        for d in (3, 1):
+            for w in ("mass", "momentum", "energy"):
+                self.sweep(st, d, w, dt)
+        self.halo_allparams(st)
+        for w in ("mass", "momentum", "energy"):
+            self.sweep(st, 2, w, dt)
+        self.rhov12v1(st)
+        self.clean_param(st, 1)
+        self.clean_param(st, 2)
+        self.VNRicht_artvisc(st)
+        self.halo_fluidvars(st)
+        self.RK2_global_boundary_allspec(st)
+        self.compression(st, dt)
+        self.rhoe2T(st)
+        self.clean_param(st, 3)
+        self.energy_diffusion(st, dt)
+        self.clean_param(st, 3)
+        self.T2rhoe(st)
 
     def fluid_adv_hotpath(self, st, dt: float):
         """The hot-path calls of src/libgemini.cpp:155-208 one by one, then rhoe2T before

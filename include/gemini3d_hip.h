@@ -81,6 +81,22 @@ void energy_diffusion_C(void** cfgC, int* xtype, void** xC, double** fluidvars,
 void halo_interface_vels_allspec_C(int* xtype, void** xC, double** fluidvars, int* lsp);
 void halo_allparams_C(int* xtype, void** xC, double** fluidvars, double** fluidauxvars);
 void halo_fluidvars_C(int* xtype, void** xC, double** fluidvars, double** fluidauxvars);
+/* The rest of fluid_adv's transport block (src/libgemini.cpp:187-210). */
+/* gemini3d.h:80, libgemini_c.f90:811-823 -> libgemini.f90:1037-1056 -> multifluid.f90:553-688:
+   iparm 1 densities, 2 vs1, 3 temperatures; null cells are those registered with
+   gemhip_register_null_cells (the reference's x%nullpts, meshobj.f90:586-619). */
+void clean_param_C(int* iparm, int* xtype, void** xC, double** fluidvars);
+/* gemini3d.h:81, libgemini_c.f90:762-771 -> multifluid.f90:237-268; the library holds
+   intvars%Q, which compression_C reads. */
+void VNRicht_artvisc_C(double** fluidvars, void** intvars);
+/* gemini3d.h:82, libgemini_c.f90:775-793 -> multifluid.f90:274-302 with div3D_curv_23
+   (div.f90:118-206). */
+void compression_C(double** fluidvars, double** fluidauxvars, void** intvars, int* xtype,
+                   void** xC, double* dt);
+/* gemini3d.h:138-139, libgemini_mpi_c.f90:461-490 -> sources_mpi.f90:14-112.  One process:
+   the depth-1 periodic x3 wrap of vs1, vs2, vs3, and the zero-order-hold boundaries. */
+void RK2_prep_mpi_allspec_C(int* xtype, void** xC, double** fluidvars);
+void RK2_global_boundary_allspec_C(int* xtype, void** xC, double** fluidvars);
 
 /* ---------------- part 2: library set-up (no counterpart in the reference ABI) ---------------- */
 /* Array arguments of part 2 may be host or HBM addresses: copies use hipMemcpyDefault. */
@@ -135,6 +151,16 @@ int gemhip_set_interface_vels(const double* vs1i, const double* vs2i, const doub
    bit-identical (interface velocities are formed inside the sweeps).  Requires
    residency mode 1.  Returns 0 or an error code. */
 int gemhip_transport_step(double dt);
+
+/* Null cells (the reference's x%nullpts, computed by meshobj.f90:586-619 as alt < 80 km):
+   lx1*lx2*lx3 ints, nonzero = null, host or HBM; NULL: none (the default). */
+int gemhip_register_null_cells(const int* nullpts);
+
+/* The whole transport block of src/libgemini.cpp:155-210 on the HBM state (mode 1): the
+   advection of gemhip_transport_step, then rhov12v1, clean_param 1 and 2,
+   VNRicht_artvisc, halo_fluidvars, RK2_global_boundary_allspec, compression, rhoe2T,
+   clean_param 3, energy_diffusion, clean_param 3 and T2rhoe.  Returns 0 or an error code. */
+int gemhip_fluid_adv_block(double dt);
 
 /* HBM pointers of the state blocks (mode 1), for zero-copy callers. */
 int gemhip_device_state(double** fluidvars, double** rhovs1_rhoes, double** J1);

@@ -44,7 +44,8 @@ def lib():
         _lib.orc_minmod.restype = C.c_double
         _lib.orc_minmod.argtypes = [C.c_double, C.c_double]
         for name in ("orc_gbsv", "orc_energy_diffusion", "orc_transport_step", "orc_num_threads",
-                     "orc_energy_diffusion_ext", "orc_transport_step_ext"):
+                     "orc_energy_diffusion_ext", "orc_transport_step_ext", "orc_clean_param",
+                     "orc_fluid_adv_block"):
             getattr(_lib, name).restype = C.c_int
     return _lib
 
@@ -248,8 +249,35 @@ def _views(st):
                 J1=st.electrovars[..., 3])
 
 
+def _mask(nullpts):
+    """int* of a null-cell mask (lx1, lx2, lx3), None -> NULL."""
+    if nullpts is None:
+        return None
+    m = np.asfortranarray(nullpts, dtype=np.int32)
+    _mask.keep = m
+    return m.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def fluid_adv_block(g, st, dt, nullpts=None, Q=None, diffsolvetype=2, Teinf=1500.0):
+    """The transport block of src/libgemini.cpp:155-210 on ``st`` (in place); Q is the
+    (lx1, lx2, lx3, 7) artificial-viscosity workspace (allocated when None)."""
+    og = OracleGrid(g)
+    if Q is None:
+        Q = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F")
+    rc = lib().orc_fluid_adv_block(og.ref, C.c_double(dt), _f(st.fluidvars), _f(st.fluidauxvars),
+                                   _f(st.electrovars), _f(st.vs1i), _f(st.vs2i), _f(st.vs3i),
+                                   _f(st.nn), _f(st.Tn), C.c_int(diffsolvetype),
+                                   C.c_double(Teinf), _mask(nullpts), _f(Q))
+    if rc != 0:
+        raise RuntimeError(f"oracle transport block failed ({rc})")
+    # libgemini.cpp:210 converts back to energy density for the sources step
+    lib().orc_T2rhoe(og.ref, _f(st.fluidvars[..., 0:7]), _f(st.fluidvars[..., 28:35]),
+                     _f(st.fluidauxvars[..., 7:14]))
+    return st
+
+
 def op(g, st, name, dt=None, lsp=7, direction=None, which=None, diffsolvetype=2,
-       Teinf=1500.0, kappa=None, qtop=None):
+       Teinf=1500.0, kappa=None, qtop=None, nullpts=None, Q=None):
     """Apply one reference operation of the transport path to ``st`` in place.
 
     Names follow the reference's C entry points: v12rhov1, T2rhoe, rhoe2T, rhov12v1,
@@ -290,6 +318,20 @@ def op(g, st, name, dt=None, lsp=7, direction=None, which=None, diffsolvetype=2,
         for field, frank, nsp in todo:
             L.orc_sweep_allspec(og.ref, C.c_int(direction), v[field], _f(vsi), C.c_double(dt),
                                 C.c_int(frank), C.c_int(nsp))
+    elif name == "clean_param":
+        field = {1: "ns", 2: "vs1", 3: "Ts"}.get(which, "ns")
+        rc = L.orc_clean_param(og.ref, C.c_int(which), v[field], _mask(nullpts))
+        if rc != 0:
+            raise RuntimeError(f"oracle clean_param failed ({rc})")
+    elif name == "VNRicht_artvisc":
+        L.orc_VNRicht_artvisc(og.ref, v["ns"], v["vs1"], _f(Q))
+    elif name == "RK2_prep_mpi_allspec":
+        for k in ("vs1", "vs2", "vs3"):
+            L.orc_halo_periodic(og.ref, v[k], C.c_int(7), C.c_int(1))
+    elif name == "RK2_global_boundary_allspec":
+        L.orc_RK2_global_boundary_allspec(og.ref, v["vs1"], v["vs2"], v["vs3"])
+    elif name == "compression":
+        L.orc_compression(og.ref, C.c_double(dt), v["vs1"], v["vs2"], v["vs3"], _f(Q), v["rhoes"])
     elif name == "energy_diffusion":
         ka, kp = _opt(kappa)
         qa, qp = _opt(qtop)

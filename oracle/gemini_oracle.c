@@ -810,6 +810,224 @@ int orc_energy_diffusion_ext(const orc_grid *g, double dt, const double *ns, dou
   return energy_diffusion_core(g, dt, ns, Ts, J1, nn, Tn, diffsolvetype, Teinf, kappa, qtop);
 }
 
+/* ---- multifluid.f90:553-688 clean_param (paramflag 1 density, 2 velocity vs1,
+   3 temperature) over the ghosted 7-species field.  nullpts(lx1,lx2,lx3) (nonzero: null
+   cell, meshobj.f90:586-619; NULL: no null cells). */
+static const double mindens = 1.0e-100, mindensnull = 1.0e-20;   /* phys_consts.in.f90:82-83 */
+int orc_clean_param(const orc_grid *g, int iparm, double *param, const int *nullpts) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+#define P(isp, i1, i2, i3) param[(long)((isp) - 1) * d.N + GIX(d, i1, i2, i3)]
+#define NUL(i1, i2, i3) (nullpts && nullpts[((i1) - 1) + (long)lx1 * (((i2) - 1) + (long)lx2 * ((i3) - 1))])
+  if (iparm == 1) {
+    for (int isp = 1; isp <= LSP - 1; ++isp)
+      for (long k = 0; k < d.N; ++k) {
+        double *v = param + (long)(isp - 1) * d.N + k;
+        *v = fmax_f(*v, mindens);
+      }
+    for (long k = 0; k < d.N; ++k) {
+      double sacc = 0.0;
+      for (int isp = 1; isp <= LSP - 1; ++isp) sacc = sacc + param[(long)(isp - 1) * d.N + k];
+      param[(long)(LSP - 1) * d.N + k] = sacc;
+    }
+    for (int isp = 1; isp <= LSP; ++isp)
+      for (int i3 = 1; i3 <= lx3; ++i3)
+        for (int i2 = 1; i2 <= lx2; ++i2)
+          for (int i1 = 1; i1 <= lx1; ++i1)
+            if (NUL(i1, i2, i3)) P(isp, i1, i2, i3) = isp == 1 ? mindensnull * 1e-2 : mindensnull;
+  } else if (iparm == 2) {
+    for (int isp = 1; isp <= LSP; ++isp)
+      for (int i3 = 1; i3 <= lx3; ++i3)
+        for (int i2 = 1; i2 <= lx2; ++i2)
+          for (int i1 = 1; i1 <= lx1; ++i1)
+            if (NUL(i1, i2, i3)) P(isp, i1, i2, i3) = 0.0;
+    if (g->gridflag == 0) {
+      for (int isp = 1; isp <= LSP; ++isp)
+        for (int i3 = 1; i3 <= lx3; ++i3)
+          for (int i2 = 1; i2 <= lx2; ++i2) {
+            int beg = 1;
+            while (!NUL(beg, i2, i3) && beg < lx1) ++beg;
+            int end = beg;
+            while (NUL(end, i2, i3) && end < lx1) ++end;
+            if (beg != lx1) P(isp, beg, i2, i3) = P(isp, beg + 1, i2, i3);
+            if (end != lx1) P(isp, end, i2, i3) = P(isp, end - 1, i2, i3);
+          }
+    } else if (g->gridflag == 1) {
+      for (int isp = 1; isp <= LSP; ++isp)
+        for (int i3 = 1; i3 <= lx3; ++i3)
+          for (int i2 = 1; i2 <= lx2; ++i2) {
+            int end = 1;
+            while (!NUL(end, i2, i3) && end < lx1) ++end;
+            if (end != lx1) P(isp, end, i2, i3) = P(isp, end - 1, i2, i3);
+          }
+    }
+    return 0;
+  } else if (iparm == 3) {
+    for (long k = 0; k < (long)LSP * d.N; ++k) param[k] = fmax_f(param[k], 100.0);
+    for (int isp = 1; isp <= LSP; ++isp)
+      for (int i3 = 1; i3 <= lx3; ++i3)
+        for (int i2 = 1; i2 <= lx2; ++i2)
+          for (int i1 = 1; i1 <= lx1; ++i1)
+            if (NUL(i1, i2, i3)) P(isp, i1, i2, i3) = 100.0;
+  } else {
+    return -1;
+  }
+  /* ghost slabs (paramflag 1 and 3): mindensdiv or 100 */
+  const double gv = iparm == 1 ? mindensdiv : 100.0;
+  for (int isp = 1; isp <= LSP; ++isp)
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1)
+          if (i1 < 1 || i1 > lx1 || i2 < 1 || i2 > lx2 || i3 < 1 || i3 > lx3) P(isp, i1, i2, i3) = gv;
+#undef P
+#undef NUL
+  return 0;
+}
+
+/* ---- multifluid.f90:237-268 VNRicht_artvisc (xicon = 3, multifluid.f90:43):
+   Q(lx1,lx2,lx3,7) interior array */
+void orc_VNRicht_artvisc(const orc_grid *g, const double *ns, const double *vs1, double *Q) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  const double xicon = 3.0;
+  for (int isp = 1; isp <= LSP; ++isp)
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1) {
+          const long k = (i1 - 1) + (long)lx1 * ((i2 - 1) + (long)lx2 * (i3 - 1)) + (isp - 1) * ni;
+          if (isp == LSP) { Q[k] = 0.0; continue; }
+          const double *u = vs1 + (long)(isp - 1) * d.N;
+          const double vl = 0.5 * (u[GIX(d, i1 - 1, i2, i3)] + u[GIX(d, i1, i2, i3)]);
+          const double vr = 0.5 * (u[GIX(d, i1, i2, i3)] + u[GIX(d, i1 + 1, i2, i3)]);
+          const double m = fmin_f(vr - vl, 0.0);
+          Q[k] = (((ns[(long)(isp - 1) * d.N + GIX(d, i1, i2, i3)] * ms_(isp)) * 0.25) *
+                  (xicon * xicon)) * (m * m);
+        }
+}
+
+/* ---- sources_mpi.f90:55-112 RK2_global_boundary_allspec on one process: zero-order hold
+   of vs1 into the x1 ghosts 0 and lx1+1, then of vs1, vs2, vs3 into the x2 ghosts and
+   (unless periodic) the x3 ghosts, each over the whole extent of the other indices */
+void orc_RK2_global_boundary_allspec(const orc_grid *g, double *vs1, double *vs2, double *vs3) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  for (int isp = 1; isp <= LSP; ++isp) {
+    double *f[3] = {vs1 + (long)(isp - 1) * d.N, vs2 + (long)(isp - 1) * d.N,
+                    vs3 + (long)(isp - 1) * d.N};
+    for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+      for (int i2 = -1; i2 <= lx2 + 2; ++i2) {
+        f[0][GIX(d, 0, i2, i3)] = f[0][GIX(d, 1, i2, i3)];
+        f[0][GIX(d, lx1 + 1, i2, i3)] = f[0][GIX(d, lx1, i2, i3)];
+      }
+    for (int q = 0; q < 3; ++q)
+      for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1) f[q][GIX(d, i1, 0, i3)] = f[q][GIX(d, i1, 1, i3)];
+    for (int q = 0; q < 3; ++q)
+      for (int i3 = -1; i3 <= lx3 + 2; ++i3)
+        for (int i1 = -1; i1 <= lx1 + 2; ++i1)
+          f[q][GIX(d, i1, lx2 + 1, i3)] = f[q][GIX(d, i1, lx2, i3)];
+    if (!g->flagper) {
+      for (int q = 0; q < 3; ++q)
+        for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+          for (int i1 = -1; i1 <= lx1 + 2; ++i1) f[q][GIX(d, i1, i2, 0)] = f[q][GIX(d, i1, i2, 1)];
+      for (int q = 0; q < 3; ++q)
+        for (int i2 = -1; i2 <= lx2 + 2; ++i2)
+          for (int i1 = -1; i1 <= lx1 + 2; ++i1)
+            f[q][GIX(d, i1, i2, lx3 + 1)] = f[q][GIX(d, i1, i2, lx3)];
+    }
+  }
+}
+
+/* ---- multifluid.f90:274-302 compression with div3D_curv_23 (div.f90:118-206) over
+   (0:lx1+1, 0:lx2+1, 0:lx3+1), of which the interior, centred differences, is used */
+void orc_compression(const orc_grid *g, double dt, const double *vs1, const double *vs2,
+                     const double *vs3, const double *Q, double *rhoes) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  const double *h1 = g->h1, *h2 = g->h2, *h3 = g->h3;
+  for (int isp = 1; isp <= LSP; ++isp) {
+    const long o = (long)(isp - 1) * d.N;
+    const double gm1 = gammas_(isp) - 1.0;
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1) {
+          const long c = GIX(d, i1, i2, i3);
+          long a = GIX(d, i1 + 1, i2, i3), b = GIX(d, i1 - 1, i2, i3);
+          double dv = ((h2[a] * h3[a]) * vs1[o + a] - (h2[b] * h3[b]) * vs1[o + b]) /
+                      (g->dx1[i1 + 1] + g->dx1[i1]);
+          if (lx2 > 1) {
+            a = GIX(d, i1, i2 + 1, i3); b = GIX(d, i1, i2 - 1, i3);
+            dv = dv + ((h1[a] * h3[a]) * vs2[o + a] - (h1[b] * h3[b]) * vs2[o + b]) /
+                          (g->dx2[i2 + 1] + g->dx2[i2]);
+          }
+          if (lx3 > 1) {
+            a = GIX(d, i1, i2, i3 + 1); b = GIX(d, i1, i2, i3 - 1);
+            dv = dv + ((h1[a] * h2[a]) * vs3[o + a] - (h1[b] * h2[b]) * vs3[o + b]) /
+                          (g->dx3[i3 + 1] + g->dx3[i3]);
+          }
+          dv = dv / ((h1[c] * h2[c]) * h3[c]);
+          const double q = Q[(i1 - 1) + (long)lx1 * ((i2 - 1) + (long)lx2 * (i3 - 1)) + (isp - 1) * ni];
+          const double e0 = rhoes[o + c];
+          const double half = e0 - ((dt / 2.0) * (e0 * gm1 + q)) * dv;
+          rhoes[o + c] = e0 - (dt * (half * gm1 + q)) * dv;
+        }
+  }
+}
+
+/* ---- the transport block of src/libgemini.cpp:155-209 (app/main.f90:277-341): the
+   advection step above, then clean_param 1 and 2, VNRicht_artvisc, halo_fluidvars,
+   RK2_global_boundary_allspec, compression, rhoe2T, clean_param 3, energy_diffusion,
+   clean_param 3.  Q(lx1,lx2,lx3,7) is the caller's workspace (intvars%Q). */
+int orc_fluid_adv_block(const orc_grid *g, double dt, double *fluidvars, double *fluidauxvars,
+                        const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
+                        const double *nn, const double *Tn, int diffsolvetype, double Teinf,
+                        const int *nullpts, double *Q) {
+  dims_t d = dims_of(g);
+  double *ns = fluidvars, *vs1 = fluidvars + 7 * d.N, *vs2 = fluidvars + 14 * d.N,
+         *vs3 = fluidvars + 21 * d.N, *Ts = fluidvars + 28 * d.N;
+  double *rhovs1 = fluidauxvars, *rhoes = fluidauxvars + 7 * d.N;
+  const double *J1 = electrovars + 3 * d.N;
+  orc_v12rhov1(g, ns, vs1, rhovs1);
+  orc_T2rhoe(g, ns, Ts, rhoes);
+  orc_set_global_boundaries_allspec(g, ns, rhovs1, vs1, vs2, vs3, rhoes, vs1i, LSP);
+  orc_halo_periodic(g, ns, LSP, 2);
+  orc_halo_periodic(g, rhovs1, LSP, 2);
+  orc_halo_periodic(g, rhoes, LSP, 2);
+  orc_halo_periodic(g, vs2, LSP, 2);
+  orc_halo_periodic(g, vs3, LSP, 2);
+  orc_interface_vels_allspec(g, vs1, vs2, vs3, vs1i, vs2i, vs3i, LSP);
+  orc_sweep_allspec(g, 3, ns, vs3i, dt, 0, 6);
+  orc_sweep_allspec(g, 3, rhovs1, vs3i, dt, 1, 6);
+  orc_sweep_allspec(g, 3, rhoes, vs3i, dt, 0, 7);
+  orc_sweep_allspec(g, 1, ns, vs1i, dt, 0, 6);
+  orc_sweep_allspec(g, 1, rhovs1, vs1i, dt, 0, 6);
+  orc_sweep_allspec(g, 1, rhoes, vs1i, dt, 0, 7);
+  orc_halo_periodic(g, ns, LSP, 2);
+  orc_halo_periodic(g, rhovs1, LSP, 2);
+  orc_halo_periodic(g, rhoes, LSP, 2);
+  orc_sweep_allspec(g, 2, ns, vs2i, dt, 0, 6);
+  orc_sweep_allspec(g, 2, rhovs1, vs2i, dt, 1, 6);
+  orc_sweep_allspec(g, 2, rhoes, vs2i, dt, 0, 7);
+  orc_rhov12v1(g, ns, rhovs1, vs1);
+  orc_clean_param(g, 1, ns, nullpts);
+  orc_clean_param(g, 2, vs1, nullpts);
+  orc_VNRicht_artvisc(g, ns, vs1, Q);
+  orc_halo_periodic(g, ns, LSP, 2);
+  orc_halo_periodic(g, rhovs1, LSP, 2);
+  orc_halo_periodic(g, rhoes, LSP, 2);
+  orc_halo_periodic(g, vs2, LSP, 2);
+  orc_halo_periodic(g, vs3, LSP, 2);
+  orc_RK2_global_boundary_allspec(g, vs1, vs2, vs3);
+  orc_compression(g, dt, vs1, vs2, vs3, Q, rhoes);
+  orc_rhoe2T(g, ns, rhoes, Ts);
+  orc_clean_param(g, 3, Ts, nullpts);
+  int rc = orc_energy_diffusion(g, dt, ns, Ts, J1, nn, Tn, diffsolvetype, Teinf);
+  orc_clean_param(g, 3, Ts, nullpts);
+  return rc;
+}
+
 /* ---- one transport step in the order of src/libgemini.cpp:155-186 and :208
    (app/main.f90:277-355): v12rhov1, T2rhoe, set_global_boundaries_allspec,
    halo_fluidvars, interface_vels_allspec, sweep3 (mass, momentum, energy), sweep1,

@@ -85,6 +85,20 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
                        const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
                        const double *nn, const double *Tn, int diffsolvetype, double Teinf);
 
+/* The rest of the fluid_adv transport block (multifluid.f90:237-302, 553-688,
+   sources_mpi.f90:55-112).  nullpts(lx1,lx2,lx3): nonzero marks a null cell (NULL: none).
+   orc_clean_param returns -1 for a paramflag other than 1, 2, 3 (the reference's error
+   stop).  Q is (lx1,lx2,lx3,7). */
+int orc_clean_param(const orc_grid *g, int iparm, double *param, const int *nullpts);
+void orc_VNRicht_artvisc(const orc_grid *g, const double *ns, const double *vs1, double *Q);
+void orc_RK2_global_boundary_allspec(const orc_grid *g, double *vs1, double *vs2, double *vs3);
+void orc_compression(const orc_grid *g, double dt, const double *vs1, const double *vs2,
+                     const double *vs3, const double *Q, double *rhoes);
+int orc_fluid_adv_block(const orc_grid *g, double dt, double *fluidvars, double *fluidauxvars,
+                        const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
+                        const double *nn, const double *Tn, int diffsolvetype, double Teinf,
+                        const int *nullpts, double *Q);
+
 /* Extension for parameter fitting (no counterpart in the reference): kappa(lx2*lx3)
    scales the electron thermal conductivity of each (x2,x3) column; qtop(lx2*lx3), for
    gridflag 2 only, replaces the electron Dirichlet top row by the Neumann row
