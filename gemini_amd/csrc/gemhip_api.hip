@@ -649,21 +649,17 @@ int run_block(double dt) {
   launch_sweep(g, s, 1, F_ALL, dt, true, st, c.tune);   // :178-180
   halo_fields({NS, RHOVS1, RHOES});                      // :181
   launch_sweep(g, s, 2, F_ALL, dt, true, st, c.tune);   // :183-185
-  launch_rhov12v1(g, s, st);                             // :186
   const int* nul = c.d_null;
-  launch_clean_param(g, s, 1, nul, st);                  // :187-188
-  launch_clean_param(g, s, 2, nul, st);
+  launch_rhov12v1_clean_density(g, s, nul, st);          // :186-187, one pass
+  launch_clean_param(g, s, 2, nul, st);                  // :188
   launch_artvisc(g, s, c.d_Q, st);                       // :191
   halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :199
   launch_rk2_global_boundary(g, s, st);                  // :200
-  launch_compression(g, s, c.d_Q, dt, st);               // :203
-  launch_rhoe2T(g, s, st);                               // :204
-  launch_clean_param(g, s, 3, nul, st);                  // :205
+  launch_compression_rhoe2T_clean(g, s, c.d_Q, dt, nul, st);   // :203-205, one pass
   if (!hipok(hipGetLastError(), "transport block launch")) return 3;
   c.vsi_current = false;
   if (!run_energy_diffusion(dt)) return 4;               // :208
-  launch_clean_param(g, s, 3, nul, st);                  // :209
-  launch_v12rhov1_T2rhoe(g, s, false, true, st);         // :210
+  launch_clean_T2rhoe(g, s, nul, st);                    // :209-210, one pass
   return hipok(hipGetLastError(), "transport block launch") ? 0 : 3;
 }
 }  // namespace
@@ -685,8 +681,15 @@ int gemhip_register_null_cells(const int* nullpts) {
     return 0;
   }
   const size_t n = (size_t)c.lx1 * c.lx2 * c.lx3;
+  std::vector<int> h(n);                          // a mask without null cells is no mask
+  if (!hipok(hipMemcpy(h.data(), nullpts, n * sizeof(int), hipMemcpyDefault), "null cells"))
+    return 3;
+  size_t count = 0;
+  for (int v : h) count += v != 0;
+  if (count == 0) return gemhip_register_null_cells(nullptr);
   if (!c.d_null && !hipok(hipMalloc((void**)&c.d_null, n * sizeof(int)), "null cells")) return 2;
-  if (!hipok(hipMemcpy(c.d_null, nullpts, n * sizeof(int), hipMemcpyDefault), "null cells upload"))
+  if (!hipok(hipMemcpy(c.d_null, h.data(), n * sizeof(int), hipMemcpyHostToDevice),
+             "null cells upload"))
     return 3;
   return 0;
 }

@@ -102,6 +102,13 @@ void launch_artvisc(const DevGrid& g, const DevState& s, double* Q, hipStream_t 
 void launch_rk2_global_boundary(const DevGrid& g, const DevState& s, hipStream_t st);
 void launch_compression(const DevGrid& g, const DevState& s, const double* Q, double dt,
                         hipStream_t st);
+// the same passes fused per cell for the block step: compression + rhoe2T + clean_param(3),
+// and clean_param(3) + T2rhoe
+void launch_compression_rhoe2T_clean(const DevGrid& g, const DevState& s, const double* Q,
+                                     double dt, const int* nul, hipStream_t st);
+void launch_clean_T2rhoe(const DevGrid& g, const DevState& s, const int* nul, hipStream_t st);
+void launch_rhov12v1_clean_density(const DevGrid& g, const DevState& s, const int* nul,
+                                   hipStream_t st);
 
 // ---- adjoint of the electron energy path (kernels_adjoint.hip) ----
 // What one recorded forward step keeps for the adjoint (electron fields only): densities at

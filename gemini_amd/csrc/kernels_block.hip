@@ -190,7 +190,135 @@ __global__ void k_compression(DevGrid g, double dt, const double* vs1, const dou
   }
 }
 
+// rhov12v1 (multifluid.f90:218-233) then clean_param(1), fused per cell
+// (src/libgemini.cpp:186-187): velocities from the advected momentum densities with the
+// densities as the sweeps left them, then the density clean-up.
+__global__ void k_rhov12v1_clean_density(DevGrid g, double* ns, const double* rhovs1,
+                                         double* vs1, const int* nul) {
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < g.N;
+       c += (long)gridDim.x * blockDim.x) {
+    double n[LSP - 1];
+    double chrgflux = 0.0, sum = 0.0;
+#pragma unroll
+    for (int isp = 1; isp <= LSP - 1; ++isp) {
+      const long o = (isp - 1) * g.N + c;
+      n[isp - 1] = ns[o];
+      const double v = rhovs1[o] / (ms(isp) * fmax_f(n[isp - 1], mindensdiv));
+      vs1[o] = v;
+      chrgflux = chrgflux + (n[isp - 1] * qs(isp)) * v;
+    }
+#pragma unroll
+    for (int isp = 0; isp < LSP - 1; ++isp) sum = sum + n[isp];
+    vs1[(LSP - 1) * g.N + c] = -(((1.0 / fmax_f(sum, mindensdiv)) / qs(LSP)) * chrgflux);
+    int i1, i2, i3;
+    cell_of(g, c, i1, i2, i3);
+    if (!interior(g, i1, i2, i3)) {
+      for (int isp = 0; isp < LSP; ++isp) ns[isp * g.N + c] = mindensdiv;
+    } else if (is_null(g, nul, i1, i2, i3)) {
+      for (int isp = 0; isp < LSP; ++isp) ns[isp * g.N + c] = isp == 0 ? mindensnull * 1e-2 : mindensnull;
+    } else {
+      double s = 0.0;
+#pragma unroll
+      for (int isp = 0; isp < LSP - 1; ++isp) {
+        const double v = fmax_f(n[isp], mindens);
+        ns[isp * g.N + c] = v;
+        s = s + v;
+      }
+      ns[(LSP - 1) * g.N + c] = s;
+    }
+  }
+}
+
+// The tail of the compression substep fused per cell (src/libgemini.cpp:203-205):
+// compression of the interior energy density, rhoe2T over every cell, clean_param(3).
+// Ghost cells end at 100 K whatever rhoe2T gives them, and keep their energy density.
+__global__ void k_compression_rhoe2T_clean(DevGrid g, double dt, const double* ns,
+                                           const double* vs1, const double* vs2,
+                                           const double* vs3, const double* Q, double* rhoes,
+                                           double* Ts, const int* nul) {
+  const long ni = (long)g.lx1 * g.lx2 * g.lx3;
+  const long s2 = g.L1, s3 = (long)g.L1 * g.L2;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < g.N;
+       c += (long)gridDim.x * blockDim.x) {
+    int i1, i2, i3;
+    cell_of(g, c, i1, i2, i3);
+    if (!interior(g, i1, i2, i3)) {
+      for (int isp = 0; isp < LSP; ++isp) Ts[isp * g.N + c] = 100.0;
+      continue;
+    }
+    const bool isnull = is_null(g, nul, i1, i2, i3);
+    const long k = (i1 - 1) + (long)g.lx1 * ((i2 - 1) + (long)g.lx2 * (i3 - 1));
+    const double* h1 = g.h1;
+    const double* h2 = g.h2;
+    const double* h3 = g.h3;
+    const double m1a = h2[c + 1] * h3[c + 1], m1b = h2[c - 1] * h3[c - 1];
+    const double d1 = g.dx1[i1 + 1] + g.dx1[i1];
+    double m2a = 0.0, m2b = 0.0, d2 = 1.0, m3a = 0.0, m3b = 0.0, d3 = 1.0;
+    if (g.lx2 > 1) { m2a = h1[c + s2] * h3[c + s2]; m2b = h1[c - s2] * h3[c - s2]; d2 = g.dx2[i2 + 1] + g.dx2[i2]; }
+    if (g.lx3 > 1) { m3a = h1[c + s3] * h2[c + s3]; m3b = h1[c - s3] * h2[c - s3]; d3 = g.dx3[i3 + 1] + g.dx3[i3]; }
+    const double hc = (h1[c] * h2[c]) * h3[c];
+    for (int isp = 1; isp <= LSP; ++isp) {
+      const long o = (isp - 1) * g.N;
+      double dv = (m1a * vs1[o + c + 1] - m1b * vs1[o + c - 1]) / d1;
+      if (g.lx2 > 1) dv = dv + (m2a * vs2[o + c + s2] - m2b * vs2[o + c - s2]) / d2;
+      if (g.lx3 > 1) dv = dv + (m3a * vs3[o + c + s3] - m3b * vs3[o + c - s3]) / d3;
+      dv = dv / hc;
+      const double gm1 = gammas(isp) - 1.0;
+      const double q = Q[(isp - 1) * ni + k];
+      const double e0 = rhoes[o + c];
+      const double half = e0 - ((dt / 2.0) * (e0 * gm1 + q)) * dv;
+      const double e = e0 - (dt * (half * gm1 + q)) * dv;
+      rhoes[o + c] = e;
+      const double t = fmax_f(((gm1 / kB) * e) / fmax_f(ns[o + c], mindensdiv), 100.0);
+      Ts[o + c] = isnull ? 100.0 : fmax_f(t, 100.0);
+    }
+  }
+}
+
+// The end of the block fused per cell (src/libgemini.cpp:209-210): clean_param(3), then
+// T2rhoe over every cell.
+__global__ void k_clean_T2rhoe(DevGrid g, const double* ns, double* Ts, double* rhoes,
+                               const int* nul) {
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < g.N;
+       c += (long)gridDim.x * blockDim.x) {
+    int i1, i2, i3;
+    cell_of(g, c, i1, i2, i3);
+    const bool fill = !interior(g, i1, i2, i3) || is_null(g, nul, i1, i2, i3);
+    for (int isp = 1; isp <= LSP; ++isp) {
+      const long o = (isp - 1) * g.N + c;
+      const double t = fill ? 100.0 : fmax_f(Ts[o], 100.0);
+      Ts[o] = t;
+      rhoes[o] = ((ns[o] * kB) * t) / (gammas(isp) - 1.0);
+    }
+  }
+}
+
 }  // namespace
+
+void launch_compression_rhoe2T_clean(const DevGrid& g, const DevState& s, const double* Q,
+                                     double dt, const int* nul, hipStream_t st) {
+  timing_begin("compression", st);
+  hipLaunchKernelGGL(k_compression_rhoe2T_clean,
+                     dim3(grid_exact(g.N) > 8192 ? 8192 : grid_exact(g.N)), dim3(TPB), 0, st, g,
+                     dt, s.ns, s.vs1, s.vs2, s.vs3, Q, s.rhoes, s.Ts, nul);
+  timing_end("compression", st);
+}
+
+void launch_rhov12v1_clean_density(const DevGrid& g, const DevState& s, const int* nul,
+                                   hipStream_t st) {
+  timing_begin("rhov12v1", st);
+  hipLaunchKernelGGL(k_rhov12v1_clean_density,
+                     dim3(grid_exact(g.N) > 8192 ? 8192 : grid_exact(g.N)), dim3(TPB), 0, st, g,
+                     s.ns, s.rhovs1, s.vs1, nul);
+  timing_end("rhov12v1", st);
+}
+
+void launch_clean_T2rhoe(const DevGrid& g, const DevState& s, const int* nul, hipStream_t st) {
+  timing_begin("clean_param", st);
+  hipLaunchKernelGGL(k_clean_T2rhoe, dim3(grid_exact(g.N) > 8192 ? 8192 : grid_exact(g.N)),
+                     dim3(TPB), 0, st, g, s.ns, s.Ts, s.rhoes, nul);
+  timing_end("clean_param", st);
+}
 
 void launch_clean_param(const DevGrid& g, const DevState& s, int iparm, const int* nul,
                         hipStream_t st) {
