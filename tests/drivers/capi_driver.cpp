@@ -3,7 +3,8 @@
 // take the address of each state-block pointer, sizes come from library state.  Test
 // program: reads a case written by the test suite (case.txt, in.bin), runs nsteps steps
 // and writes out.bin.  mode 0 calls the entry points one by one on host blocks; mode 1
-// pushes the state once and runs gemhip_transport_step on HBM.
+// pushes the state once and runs gemhip_transport_step on HBM; modes 2 and 3 do the same
+// for the whole transport block (src/libgemini.cpp:155-210, gemhip_fluid_adv_block).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -80,11 +81,40 @@ int main(int argc, char** argv) {
   void* intvars = nullptr;  // contents were registered above
   void* cfgC = nullptr;
   int lsp = 7;
-  if (mode == 1) {
+  int one = 1, two = 2, three = 3;
+  if (mode == 1 || mode == 3) {
     check(gemhip_set_residency(1), "set_residency");
     check(gemhip_state_push(fluidvars, fluidauxvars, electrovars), "state_push");
-    for (int k = 0; k < nsteps; ++k) check(gemhip_transport_step(dt), "transport_step");
+    for (int k = 0; k < nsteps; ++k)
+      check(mode == 1 ? gemhip_transport_step(dt) : gemhip_fluid_adv_block(dt), "fused step");
     check(gemhip_state_pull(fluidvars, fluidauxvars), "state_pull");
+  } else if (mode == 2) {
+    for (int k = 0; k < nsteps; ++k) {
+      // src/libgemini.cpp:155-210, the whole transport block
+      v12rhov1_C(&fluidvars, &fluidauxvars);
+      T2rhoe_C(&fluidvars, &fluidauxvars);
+      set_global_boundaries_allspec_C(&xtype, &xC, &fluidvars, &fluidauxvars, &intvars, &lsp);
+      halo_fluidvars_C(&xtype, &xC, &fluidvars, &fluidauxvars);
+      interface_vels_allspec_C(&xtype, &xC, &fluidvars, &intvars, &lsp);
+      sweep3_allparams_C(&fluidvars, &fluidauxvars, &intvars, &xtype, &xC, &dt);
+      sweep1_allparams_C(&fluidvars, &fluidauxvars, &intvars, &xtype, &xC, &dt);
+      halo_allparams_C(&xtype, &xC, &fluidvars, &fluidauxvars);
+      sweep2_allparams_C(&fluidvars, &fluidauxvars, &intvars, &xtype, &xC, &dt);
+      rhov12v1_C(&fluidvars, &fluidauxvars);
+      clean_param_C(&one, &xtype, &xC, &fluidvars);
+      clean_param_C(&two, &xtype, &xC, &fluidvars);
+      VNRicht_artvisc_C(&fluidvars, &intvars);
+      halo_fluidvars_C(&xtype, &xC, &fluidvars, &fluidauxvars);
+      RK2_global_boundary_allspec_C(&xtype, &xC, &fluidvars);
+      compression_C(&fluidvars, &fluidauxvars, &intvars, &xtype, &xC, &dt);
+      rhoe2T_C(&fluidvars, &fluidauxvars);
+      clean_param_C(&three, &xtype, &xC, &fluidvars);
+      energy_diffusion_C(&cfgC, &xtype, &xC, &fluidvars, &electrovars, &intvars, &dt);
+      clean_param_C(&three, &xtype, &xC, &fluidvars);
+      T2rhoe_C(&fluidvars, &fluidauxvars);
+      if (gemhip_last_error()[0]) check(1, "step");
+    }
+    check(gemhip_get_interface_vels(vs1i.data(), vs2i.data(), vs3i.data()), "get_interface_vels");
   } else {
     for (int k = 0; k < nsteps; ++k) {
       // src/libgemini.cpp:155-186

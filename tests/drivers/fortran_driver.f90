@@ -61,14 +61,43 @@ program fortran_driver
   intvars = c_null_ptr
   cfgC = c_null_ptr
   lsp = 7
-  if (mode == 1) then
+  if (mode == 1 .or. mode == 3) then
     rc = gemhip_set_residency(1_c_int)
     rc = rc + gemhip_state_push(fluidvars, fluidauxvars, electrovars)
     do k = 1, nsteps
-      rc = rc + gemhip_transport_step(dt)
+      if (mode == 1) then
+        rc = rc + gemhip_transport_step(dt)
+      else
+        rc = rc + gemhip_fluid_adv_block(dt)
+      end if
     end do
     rc = rc + gemhip_state_pull(fluidvars, fluidauxvars)
     if (rc /= 0) error stop 'fused step failed: '//gemhip_error_message()
+  else if (mode == 2) then
+    do k = 1, nsteps
+      call v12rhov1_C(fvp, fap)
+      call T2rhoe_C(fvp, fap)
+      call set_global_boundaries_allspec_C(xtype, xC, fvp, fap, intvars, lsp)
+      call halo_fluidvars_C(xtype, xC, fvp, fap)
+      call interface_vels_allspec_C(xtype, xC, fvp, intvars, lsp)
+      call sweep3_allparams_C(fvp, fap, intvars, xtype, xC, dt)
+      call sweep1_allparams_C(fvp, fap, intvars, xtype, xC, dt)
+      call halo_allparams_C(xtype, xC, fvp, fap)
+      call sweep2_allparams_C(fvp, fap, intvars, xtype, xC, dt)
+      call rhov12v1_C(fvp, fap)
+      call clean_param_C(1_c_int, xtype, xC, fvp)
+      call clean_param_C(2_c_int, xtype, xC, fvp)
+      call VNRicht_artvisc_C(fvp, intvars)
+      call halo_fluidvars_C(xtype, xC, fvp, fap)
+      call RK2_global_boundary_allspec_C(xtype, xC, fvp)
+      call compression_C(fvp, fap, intvars, xtype, xC, dt)
+      call rhoe2T_C(fvp, fap)
+      call clean_param_C(3_c_int, xtype, xC, fvp)
+      call energy_diffusion_C(cfgC, xtype, xC, fvp, evp, intvars, dt)
+      call clean_param_C(3_c_int, xtype, xC, fvp)
+      call T2rhoe_C(fvp, fap)
+      if (len(gemhip_error_message()) > 0) error stop 'step failed: '//gemhip_error_message()
+    end do
   else
     do k = 1, nsteps
       call v12rhov1_C(fvp, fap)
@@ -92,7 +121,7 @@ program fortran_driver
       if (len(gemhip_error_message()) > 0) error stop 'step failed: '//gemhip_error_message()
     end do
   end if
-  if (mode == 0) rc = gemhip_get_interface_vels(vs1i, vs2i, vs3i)   ! mode 1 leaves them as read
+  if (mode == 0 .or. mode == 2) rc = gemhip_get_interface_vels(vs1i, vs2i, vs3i)
   open(newunit=w, file=trim(dir)//'/out.bin', access='stream', form='unformatted', status='replace')
   write(w) fluidvars, fluidauxvars(:,:,:,1:14), vs1i, vs2i, vs3i
   close(w)

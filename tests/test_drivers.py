@@ -1,7 +1,8 @@
 """Host drivers in the reference's host languages, on the GPU: a Fortran program through
 the iso_c_binding module gemini_amd/fortran/gemhip_bind.f90 and a C++ program through
 include/gemini3d_hip.h, each calling the reference-named entry points in the order of
-src/libgemini.cpp:155-208 (mode 0) or the fused step (mode 1).  Their results must be
+src/libgemini.cpp:155-208 (mode 0) or the fused step (mode 1), and the same for the whole
+transport block of src/libgemini.cpp:155-210 (modes 2 and 3).  Their results must be
 bit-identical to the oracle's (with the GPU's x**2.5)."""
 import os
 import subprocess
@@ -21,7 +22,8 @@ ALL = ("ns", "vs1", "vs2", "vs3", "Ts", "rhovs1", "rhoes")
 
 @pytest.mark.parametrize("driver", ["fortran_driver", "capi_driver"])
 @pytest.mark.parametrize("kind,shape,mode", [("cart", (16, 6, 5), 0), ("dipole", (20, 9, 5), 0),
-                                             ("dipole", (20, 9, 5), 1)])
+                                             ("dipole", (20, 9, 5), 1), ("cart", (16, 6, 5), 2),
+                                             ("dipole", (20, 9, 5), 3)])
 def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
     exe = os.path.join(BIN, driver)
     if not os.path.exists(exe):
@@ -38,8 +40,11 @@ def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
     ref = st.copy()
     with oracle.pow_mode(1):
         for _ in range(nsteps):
-            oracle.transport_step(g, ref, dt)
+            if mode >= 2:
+                oracle.fluid_adv_block(g, ref, dt)
+            else:
+                oracle.transport_step(g, ref, dt)
     compare(got, ref, exact=ALL)
-    if mode == 0:
+    if mode in (0, 2):
         for a in ("vs1i", "vs2i", "vs3i"):
             assert np.array_equal(getattr(got, a), getattr(ref, a)), a
