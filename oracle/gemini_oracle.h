@@ -87,6 +87,9 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
 
 /* The rest of the fluid_adv transport block (multifluid.f90:237-302, 553-688,
    sources_mpi.f90:55-112).  nullpts(lx1,lx2,lx3): nonzero marks a null cell (NULL: none).
+   Parity unpinned: the dipole border copy of clean_param 2 (multifluid.f90:623-667) is
+   restated line for line with no independent check; the rest is pinned in
+   tests/test_block.py. */
    orc_clean_param returns -1 for a paramflag other than 1, 2, 3 (the reference's error
    stop).  Q is (lx1,lx2,lx3,7). */
 int orc_clean_param(const orc_grid *g, int iparm, double *param, const int *nullpts);
