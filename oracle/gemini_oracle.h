@@ -87,11 +87,11 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
 
 /* The rest of the fluid_adv transport block (multifluid.f90:237-302, 553-688,
    sources_mpi.f90:55-112).  nullpts(lx1,lx2,lx3): nonzero marks a null cell (NULL: none).
+   orc_clean_param returns -1 for a paramflag other than 1, 2, 3 (the reference's error
+   stop).  Q is (lx1,lx2,lx3,7).
    Parity unpinned: the dipole border copy of clean_param 2 (multifluid.f90:623-667) is
    restated line for line with no independent check; the rest is pinned in
    tests/test_block.py. */
-   orc_clean_param returns -1 for a paramflag other than 1, 2, 3 (the reference's error
-   stop).  Q is (lx1,lx2,lx3,7). */
 int orc_clean_param(const orc_grid *g, int iparm, double *param, const int *nullpts);
 void orc_VNRicht_artvisc(const orc_grid *g, const double *ns, const double *vs1, double *Q);
 void orc_RK2_global_boundary_allspec(const orc_grid *g, double *vs1, double *vs2, double *vs3);
@@ -101,6 +101,14 @@ int orc_fluid_adv_block(const orc_grid *g, double dt, double *fluidvars, double 
                         const double *electrovars, double *vs1i, double *vs2i, double *vs3i,
                         const double *nn, const double *Tn, int diffsolvetype, double Teinf,
                         const int *nullpts, double *Q);
+
+/* Time step (temporal_mpi.f90:27-165, libgemini_mpi.f90:450-483) on one process.
+   rates[3*(isp-1)+k]: species isp's largest grid rate along x(k+1). */
+double orc_dt_calc(const orc_grid *g, const double *Ts, const double *vs1, const double *vs2,
+                   const double *vs3, double tcfl, double *rates);
+void orc_dt_select(const orc_grid *g, const double *Ts, const double *vs1, const double *vs2,
+                   const double *vs3, double tcfl, int flagglow, double t, double tout,
+                   double tglowout, int it, double *dt);
 
 /* Extension for parameter fitting (no counterpart in the reference): kappa(lx2*lx3)
    scales the electron thermal conductivity of each (x2,x3) column; qtop(lx2*lx3), for
