@@ -46,6 +46,10 @@ struct Ctx {
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   double* d_Q = nullptr;        // artificial viscosity (intvars%Q), lx1*lx2*lx3*7
+  double* d_rates = nullptr;    // dt_calc partial maxima and results
+  double tcfl = 0.9;            // cfg%tcfl
+  int flagglow = 0;             // cfg%flagglow
+  int dt_calls = 0;             // the reference's iteration count as dt_select sees it
   int* d_null = nullptr;        // null-cell mask, lx1*lx2*lx3 (null when none registered)
   double* d_kappa = nullptr;    // fitting extension (lx2*lx3 each)
   double* d_qtop = nullptr;
@@ -130,7 +134,7 @@ void collect_timings() {
 void free_all() {
   Ctx& c = ctx();
   for (double** p : {&c.d_geom, &c.d_fluid, &c.d_aux, &c.d_J1, &c.d_vsi, &c.d_v1b, &c.d_neut,
-                     &c.d_scratch, &c.d_kappa, &c.d_qtop, &c.d_adj, &c.d_Q}) {
+                     &c.d_scratch, &c.d_kappa, &c.d_qtop, &c.d_adj, &c.d_Q, &c.d_rates}) {
     if (*p) hipFree(*p);
     *p = nullptr;
   }
@@ -670,6 +674,61 @@ extern "C" {
 int gemhip_transport_step(double dt) { return run_step(dt, nullptr); }
 
 int gemhip_fluid_adv_block(double dt) { return run_block(dt); }
+
+int gemhip_register_cfl(double tcfl, int flagglow) {
+  Ctx& c = ctx();
+  c.tcfl = tcfl;
+  c.flagglow = flagglow;
+  c.dt_calls = 0;
+  return 0;
+}
+
+int gemhip_grid_rates(double* rates) {
+  if (!ready()) return 1;
+  Ctx& c = ctx();
+  if (!rates) { fail("null rates pointer"); return 2; }
+  const int nb = gridrate_blocks(c.g);
+  if (!c.d_rates && !hipok(hipMalloc((void**)&c.d_rates, (size_t)(nb + 1) * 3 * LSP * sizeof(double)),
+                           "grid rates"))
+    return 3;
+  launch_gridrates(c.g, c.s, c.d_rates, c.stream);
+  if (!hipok(hipMemcpyAsync(rates, c.d_rates + (long)nb * 3 * LSP, 3 * LSP * sizeof(double),
+                            hipMemcpyDefault, c.stream), "grid rates") ||
+      !hipok(hipStreamSynchronize(c.stream), "grid rates"))
+    return 4;
+  return 0;
+}
+
+// libgemini_mpi_c.f90:289-308 -> libgemini_mpi.f90:450-483, temporal_mpi.f90:27-165 on one
+// process.  The reference's iteration count (get_it) is the number of dt_select calls.
+void dt_select_C(void** cfgC, int* xtype, void** xC, double** fluidvars, double** fluidauxvars,
+                 double* t, double* tout, double* tglowout, double* dt) {
+  (void)cfgC; (void)xC; (void)fluidauxvars;
+  if (!ready() || !check_xtype(xtype) || !check_host(fluidvars, "fluidvars")) return;
+  if (!t || !tout || !tglowout || !dt) { fail("null time argument"); return; }
+  Ctx& c = ctx();
+  Io io{fluidvars, nullptr, nullptr};
+  if (!upload(io, {TS, VS1, VS2, VS3})) return;
+  double r[3 * LSP];
+  if (gemhip_grid_rates(r) != 0) return;
+  const double dtprev = *dt;
+  double m = 0.0;
+  for (int k = 0; k < 3; ++k) {                  // min(minval(1/gridrate1), ..2, ..3)
+    double mk = 0.0;
+    for (int isp = 1; isp <= LSP; ++isp) {
+      const double inv = 1.0 / fmax_f(r[3 * (isp - 1) + k], 1e-10);
+      if (isp == 1 || inv < mk) mk = inv;
+    }
+    if (k == 0 || mk < m) m = mk;
+  }
+  double v = c.tcfl * m;
+  if (c.flagglow != 0 && *t + v > *tglowout) v = *tglowout - *t;
+  if (*t + v > *tout) v = *tout - *t;
+  v = fmax_f(v, 1e-6);
+  c.dt_calls += 1;
+  if (c.dt_calls > 1 && v / dtprev > 2.0) v = 2.0 * dtprev;
+  *dt = v;
+}
 
 int gemhip_register_null_cells(const int* nullpts) {
   if (!ready()) return 1;

@@ -110,6 +110,11 @@ void launch_clean_T2rhoe(const DevGrid& g, const DevState& s, const int* nul, hi
 void launch_rhov12v1_clean_density(const DevGrid& g, const DevState& s, const int* nul,
                                    hipStream_t st);
 
+// dt_calc's 21 grid rates (species-major, x1 x2 x3): work holds gridrate_blocks(g) * 21
+// partial maxima followed by the 21 results.
+int gridrate_blocks(const DevGrid& g);
+void launch_gridrates(const DevGrid& g, const DevState& s, double* work, hipStream_t st);
+
 // ---- adjoint of the electron energy path (kernels_adjoint.hip) ----
 // What one recorded forward step keeps for the adjoint (electron fields only): densities at
 // the step start and at the diffusion, the energy density entering each sweep, the

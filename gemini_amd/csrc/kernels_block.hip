@@ -293,7 +293,78 @@ __global__ void k_clean_T2rhoe(DevGrid g, const double* ns, double* Ts, double* 
   }
 }
 
+// dt_calc's grid rates (temporal_mpi.f90:115-131): per species the largest
+// (vsnd + |vs1|)/dl1, |vs2|/dl2, |vs3|/dl3 over the interior, dl_k = dxki h_k
+// (meshobj.f90:436-442).  Maxima are exact, so the grid-stride partial maxima, the
+// 64-lane shuffle tree and the final pass give the reference's values in any order.
+constexpr int NRATE = 3 * LSP;
+
+__global__ __launch_bounds__(TPB) void k_gridrate(DevGrid g, const double* Ts, const double* vs1,
+                                                  const double* vs2, const double* vs3,
+                                                  double* part) {
+  double r[NRATE];
+#pragma unroll
+  for (int k = 0; k < NRATE; ++k) r[k] = 0.0;
+  const long ni = (long)g.lx1 * g.lx2 * g.lx3;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < ni;
+       k += (long)gridDim.x * blockDim.x) {
+    const int i1 = (int)(k % g.lx1) + 1;
+    const long rr = k / g.lx1;
+    const int i2 = (int)(rr % g.lx2) + 1, i3 = (int)(rr / g.lx2) + 1;
+    const long c = gix(g, i1, i2, i3);
+    const double dl1 = g.dx1i[i1 - 1] * g.h1[c], dl2 = g.dx2i[i2 - 1] * g.h2[c],
+                 dl3 = g.dx3i[i3 - 1] * g.h3[c];
+    const double Te = Ts[(LSP - 1) * g.N + c];
+#pragma unroll
+    for (int isp = 1; isp <= LSP; ++isp) {
+      const long o = (isp - 1) * g.N + c;
+      const double vsnd = isp < LSP
+          ? sqrt((kB * Ts[o]) / ms(isp) + (((5.0 / 3.0) * kB) * Te) / ms(isp)) : 0.0;
+      r[3 * (isp - 1)] = fmax_f((vsnd + fabs(vs1[o])) / dl1, r[3 * (isp - 1)]);
+      r[3 * (isp - 1) + 1] = fmax_f(fabs(vs2[o]) / dl2, r[3 * (isp - 1) + 1]);
+      r[3 * (isp - 1) + 2] = fmax_f(fabs(vs3[o]) / dl3, r[3 * (isp - 1) + 2]);
+    }
+  }
+  __shared__ double red[TPB / 64][NRATE];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+  for (int k = 0; k < NRATE; ++k) {
+    double v = r[k];
+    for (int off = 32; off >= 1; off >>= 1) v = fmax_f(__shfl_xor(v, off), v);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NRATE) {
+    double m = red[0][threadIdx.x];
+    for (int w = 1; w < TPB / 64; ++w) m = fmax_f(red[w][threadIdx.x], m);
+    part[(long)blockIdx.x * NRATE + threadIdx.x] = m;
+  }
+}
+
+__global__ void k_gridrate_final(const double* part, int nblocks, double* out) {
+  const int k = threadIdx.x;
+  if (k >= NRATE) return;
+  double m = 0.0;
+  for (int b = 0; b < nblocks; ++b) m = fmax_f(part[(long)b * NRATE + k], m);
+  out[k] = m;
+}
+
 }  // namespace
+
+int gridrate_blocks(const DevGrid& g) {
+  const long ni = (long)g.lx1 * g.lx2 * g.lx3;
+  const long b = (ni + TPB - 1) / TPB;
+  return (int)(b > 1024 ? 1024 : b);
+}
+
+void launch_gridrates(const DevGrid& g, const DevState& s, double* work, hipStream_t st) {
+  const int nb = gridrate_blocks(g);
+  timing_begin("gridrate", st);
+  hipLaunchKernelGGL(k_gridrate, dim3(nb), dim3(TPB), 0, st, g, s.Ts, s.vs1, s.vs2, s.vs3, work);
+  hipLaunchKernelGGL(k_gridrate_final, dim3(1), dim3(64), 0, st, work, nb,
+                     work + (long)nb * NRATE);
+  timing_end("gridrate", st);
+}
 
 void launch_compression_rhoe2T_clean(const DevGrid& g, const DevState& s, const double* Q,
                                      double dt, const int* nul, hipStream_t st) {

@@ -19,13 +19,13 @@ module gemhip_bind
             sweep2_allparams_C, sweep2_allspec_mass_C, sweep2_allspec_momentum_C, &
             sweep2_allspec_energy_C, energy_diffusion_C, halo_interface_vels_allspec_C, &
             halo_allparams_C, halo_fluidvars_C, clean_param_C, VNRicht_artvisc_C, &
-            compression_C, RK2_prep_mpi_allspec_C, RK2_global_boundary_allspec_C
+            compression_C, RK2_prep_mpi_allspec_C, RK2_global_boundary_allspec_C, dt_select_C
   public :: gemhip_init, gemhip_finalize, gemhip_set_error_mode, gemhip_clear_error, &
             gemhip_set_grid_size, gemhip_register_grid, gemhip_register_neutrals, &
             gemhip_register_config, gemhip_set_residency, gemhip_state_push, &
             gemhip_state_pull, gemhip_get_interface_vels, gemhip_set_interface_vels, &
             gemhip_transport_step, gemhip_synchronize, gemhip_error_message, &
-            gemhip_register_null_cells, gemhip_fluid_adv_block
+            gemhip_register_null_cells, gemhip_fluid_adv_block, gemhip_register_cfl
 
   ! ------------------------------------------------ part 1: reference entry points
   interface
@@ -191,6 +191,13 @@ module gemhip_bind
       integer(c_int) :: xtype
       type(c_ptr) :: xC, fluidvars
     end subroutine
+    subroutine dt_select_C(cfgC, xtype, xC, fluidvars, fluidauxvars, t, tout, tglowout, dt) &
+        bind(C, name="dt_select_C")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: cfgC, xC, fluidvars, fluidauxvars
+      integer(c_int) :: xtype
+      real(c_double) :: t, tout, tglowout, dt
+    end subroutine
   end interface
 
   ! ------------------------------------------------ part 2: library set-up
@@ -261,6 +268,11 @@ module gemhip_bind
         bind(C, name="gemhip_register_null_cells")
       import :: c_int
       integer(c_int), intent(in) :: nullpts(*)
+    end function
+    integer(c_int) function gemhip_register_cfl(tcfl, flagglow) bind(C, name="gemhip_register_cfl")
+      import :: c_int, c_double
+      real(c_double), value :: tcfl
+      integer(c_int), value :: flagglow
     end function
     integer(c_int) function gemhip_fluid_adv_block(dt) bind(C, name="gemhip_fluid_adv_block")
       import :: c_int, c_double

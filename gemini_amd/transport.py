@@ -49,9 +49,11 @@ def load_library(path: str = LIB_PATH):
                  "gemhip_context_destroy", "gemhip_set_tuning", "gemhip_get_tuning",
                  "gemhip_ensemble_capture", "gemhip_ensemble_launch", "gemhip_ensemble_release"):
         getattr(lib, name).restype = C.c_int
-    for name in ("gemhip_register_null_cells", "gemhip_fluid_adv_block"):
+    for name in ("gemhip_register_null_cells", "gemhip_fluid_adv_block", "gemhip_register_cfl",
+                 "gemhip_grid_rates"):
         getattr(lib, name).restype = C.c_int
     lib.gemhip_fluid_adv_block.argtypes = [C.c_double]
+    lib.gemhip_register_cfl.argtypes = [C.c_double, C.c_int]
     for name in ("gemhip_register_conductivity_scale", "gemhip_register_top_heat_flux",
                  "gemhip_checkpoint_save", "gemhip_checkpoint_load",
                  "gemhip_adjoint_record_step", "gemhip_adjoint_step"):
@@ -344,6 +346,23 @@ class Transport:
             return
         m = np.asfortranarray(nullpts, dtype=np.int32)
         self._call("gemhip_register_null_cells", m.ctypes.data_as(C.POINTER(C.c_int)))
+
+    def register_cfl(self, tcfl: float, flagglow: int = 0):
+        self._call("gemhip_register_cfl", float(tcfl), int(flagglow))
+
+    def grid_rates(self) -> np.ndarray:
+        """dt_calc's 21 grid rates of the resident state, shape (7, 3)."""
+        r = np.zeros(21)
+        self._call("gemhip_grid_rates", r.ctypes.data_as(_dp))
+        return r.reshape(7, 3)
+
+    def dt_select(self, st, t: float, tout: float, tglowout: float, dt: float) -> float:
+        """dt_select_C: the next time step from dt (the previous one)."""
+        f, a = self._fa(st)
+        args = [C.c_double(v) for v in (t, tout, tglowout, dt)]
+        self._void("dt_select_C", C.byref(self._null), C.byref(self._xt), C.byref(self._null),
+                   f.ref, a.ref, *[C.byref(v) for v in args])
+        return args[3].value
 
     def fluid_adv_block(self, dt: float):
         """The whole transport block of src/libgemini.cpp:155-210 on the HBM-resident state

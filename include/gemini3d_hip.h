@@ -97,6 +97,13 @@ void compression_C(double** fluidvars, double** fluidauxvars, void** intvars, in
    the depth-1 periodic x3 wrap of vs1, vs2, vs3, and the zero-order-hold boundaries. */
 void RK2_prep_mpi_allspec_C(int* xtype, void** xC, double** fluidvars);
 void RK2_global_boundary_allspec_C(int* xtype, void** xC, double** fluidvars);
+/* gemini3d.h:140, libgemini_mpi_c.f90:289-308 -> libgemini_mpi.f90:450-483 and
+   temporal_mpi.f90:27-165 on one process: the CFL-limited dt from Ts, vs1, vs2, vs3 (an
+   on-device max reduction), clamped to the next output time, floored at 1e-6 and, from
+   the second call on, limited to twice the previous dt (*dt in and out).  cfg%tcfl and
+   cfg%flagglow come from gemhip_register_cfl. */
+void dt_select_C(void** cfgC, int* xtype, void** xC, double** fluidvars, double** fluidauxvars,
+                 double* t, double* tout, double* tglowout, double* dt);
 
 /* ---------------- part 2: library set-up (no counterpart in the reference ABI) ---------------- */
 /* Array arguments of part 2 may be host or HBM addresses: copies use hipMemcpyDefault. */
@@ -151,6 +158,13 @@ int gemhip_set_interface_vels(const double* vs1i, const double* vs2i, const doub
    bit-identical (interface velocities are formed inside the sweeps).  Requires
    residency mode 1.  Returns 0 or an error code. */
 int gemhip_transport_step(double dt);
+
+/* cfg%tcfl (config.f90:15; 0.9 until registered) and cfg%flagglow for dt_select_C; also
+   restarts its iteration count. */
+int gemhip_register_cfl(double tcfl, int flagglow);
+/* dt_calc's grid rates of the current state: rates[3*(isp-1)+k] is species isp's largest
+   grid rate along x(k+1) (host or HBM array of 21 doubles). */
+int gemhip_grid_rates(double* rates);
 
 /* Null cells (the reference's x%nullpts, computed by meshobj.f90:586-619 as alt < 80 km):
    lx1*lx2*lx3 ints, nonzero = null, host or HBM; NULL: none (the default). */

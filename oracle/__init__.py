@@ -276,6 +276,28 @@ def fluid_adv_block(g, st, dt, nullpts=None, Q=None, diffsolvetype=2, Teinf=1500
     return st
 
 
+def dt_calc(g, st, tcfl):
+    """temporal_mpi.f90 dt_calc on one process: (dt, rates (7, 3))."""
+    og = OracleGrid(g)
+    r = np.zeros(21)
+    f = lib().orc_dt_calc
+    f.restype = C.c_double
+    v = _views(st)
+    dt = f(og.ref, _f(v["Ts"]), _f(v["vs1"]), _f(v["vs2"]), _f(v["vs3"]), C.c_double(tcfl), ptr(r))
+    return dt, r.reshape(7, 3)
+
+
+def dt_select(g, st, tcfl, flagglow, t, tout, tglowout, it, dt):
+    """dt_select with dt_comm on one process; it is the iteration count (get_it)."""
+    og = OracleGrid(g)
+    v = _views(st)
+    d = C.c_double(dt)
+    lib().orc_dt_select(og.ref, _f(v["Ts"]), _f(v["vs1"]), _f(v["vs2"]), _f(v["vs3"]),
+                        C.c_double(tcfl), C.c_int(flagglow), C.c_double(t), C.c_double(tout),
+                        C.c_double(tglowout), C.c_int(it), C.byref(d))
+    return d.value
+
+
 def op(g, st, name, dt=None, lsp=7, direction=None, which=None, diffsolvetype=2,
        Teinf=1500.0, kappa=None, qtop=None, nullpts=None, Q=None):
     """Apply one reference operation of the transport path to ``st`` in place.

@@ -976,6 +976,59 @@ void orc_compression(const orc_grid *g, double dt, const double *vs1, const doub
   }
 }
 
+/* ---- temporal_mpi.f90:86-165 dt_calc: per species the largest grid rates
+   (vsnd + |vs1|)/dl1, |vs2|/dl2, |vs3|/dl3 over the interior, dl_k = dxki * h_k
+   (meshobj.f90:436-442); rates[3*(isp-1)+k].  Returns dt = tcfl * min(1/rate). */
+double orc_dt_calc(const orc_grid *g, const double *Ts, const double *vs1, const double *vs2,
+                   const double *vs3, double tcfl, double *rates) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  for (int k = 0; k < 3 * LSP; ++k) rates[k] = 0.0;
+  for (int isp = 1; isp <= LSP; ++isp) {
+    const long o = (long)(isp - 1) * d.N, oe = (long)(LSP - 1) * d.N;
+    double r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1) {
+          const long c = GIX(d, i1, i2, i3);
+          double vsnd = 0.0;
+          if (isp < LSP)
+            vsnd = sqrt((kB * Ts[o + c]) / ms_(isp) + (((5.0 / 3.0) * kB) * Ts[oe + c]) / ms_(isp));
+          r1 = fmax_f((vsnd + fabs(vs1[o + c])) / (g->dx1i[i1 - 1] * g->h1[c]), r1);
+          r2 = fmax_f(fabs(vs2[o + c]) / (g->dx2i[i2 - 1] * g->h2[c]), r2);
+          r3 = fmax_f(fabs(vs3[o + c]) / (g->dx3i[i3 - 1] * g->h3[c]), r3);
+        }
+    rates[3 * (isp - 1)] = r1; rates[3 * (isp - 1) + 1] = r2; rates[3 * (isp - 1) + 2] = r3;
+  }
+  double m = 0.0;
+  for (int k = 0; k < 3; ++k) {               /* min(minval(1/gridrate1), minval(..2), ..3) */
+    double mk = 0.0;
+    for (int isp = 1; isp <= LSP; ++isp) {
+      const double r = fmax_f(rates[3 * (isp - 1) + k], 1e-10);
+      const double inv = 1.0 / r;
+      if (isp == 1 || inv < mk) mk = inv;
+    }
+    if (k == 0 || mk < m) m = mk;
+  }
+  return tcfl * m;
+}
+
+/* ---- libgemini_mpi.f90:450-483 dt_select with temporal_mpi.f90:27-83 dt_comm on one
+   process: dt_calc, the output-time clamps, the 1e-6 floor, then from the second
+   iteration on the growth limit dt <= 2 dtprev.  dt is in (previous) and out. */
+void orc_dt_select(const orc_grid *g, const double *Ts, const double *vs1, const double *vs2,
+                   const double *vs3, double tcfl, int flagglow, double t, double tout,
+                   double tglowout, int it, double *dt) {
+  double rates[3 * LSP];
+  const double dtprev = *dt;
+  double v = orc_dt_calc(g, Ts, vs1, vs2, vs3, tcfl, rates);
+  if (flagglow != 0 && t + v > tglowout) v = tglowout - t;
+  if (t + v > tout) v = tout - t;
+  v = fmax_f(v, 1e-6);
+  if (it > 1 && v / dtprev > 2.0) v = 2.0 * dtprev;
+  *dt = v;
+}
+
 /* ---- the transport block of src/libgemini.cpp:155-209 (app/main.f90:277-341): the
    advection step above, then clean_param 1 and 2, VNRicht_artvisc, halo_fluidvars,
    RK2_global_boundary_allspec, compression, rhoe2T, clean_param 3, energy_diffusion,

@@ -163,3 +163,42 @@ def test_rk2_boundary_is_zero_order_hold():
         assert np.array_equal(v[:, 1], v[:, 2]) and np.array_equal(v[:, lx2 + 2], v[:, lx2 + 1])
         assert np.array_equal(v[:, :, 1], v[:, :, 2]) and np.array_equal(v[:, :, lx3 + 2], v[:, :, lx3 + 1])
     assert np.array_equal(v1[1], v1[2]) and np.array_equal(v1[lx1 + 2], v1[lx1 + 1])
+
+
+def test_dt_calc_closed_form():
+    """Uniform temperatures and velocities: the largest grid rate of each species is its
+    signal speed over the smallest cell length, with the ion sound speed
+    sqrt(kB (Ti + 5/3 Te) / ms) and none for electrons."""
+    g = ga.cartesian_grid(20, 5, 4)
+    st = ga.synthetic_state(g, seed=1)
+    Ti, Te, v1, v2, v3 = 1000.0, 2500.0, 300.0, -200.0, 150.0
+    st.fluidvars[..., 28:34] = Ti
+    st.fluidvars[..., 34] = Te
+    st.fluidvars[..., 7:14] = v1
+    st.fluidvars[..., 14:21] = v2
+    st.fluidvars[..., 21:28] = v3
+    c = oracle.constants()
+    dt, rates = oracle.dt_calc(g, st, 0.9)
+    dl = [np.min(np.asarray(g.dx1i)), np.min(np.asarray(g.dx2i)), np.min(np.asarray(g.dx3i))]
+    for isp in range(7):
+        cs = np.sqrt(c["kB"] * (Ti + 5.0 / 3.0 * Te) / c["ms"][isp]) if isp < 6 else 0.0
+        want = [(cs + abs(v1)) / dl[0], abs(v2) / dl[1], abs(v3) / dl[2]]
+        assert np.allclose(rates[isp], want, rtol=1e-14, atol=0.0)
+    assert np.isclose(dt, 0.9 / rates.max(), rtol=1e-15)
+
+
+def test_dt_select_rules():
+    """Output-time clamps, the 1e-6 floor and the factor-2 growth limit from the second
+    iteration (libgemini_mpi.f90:467-482, temporal_mpi.f90:53-66)."""
+    g = ga.cartesian_grid(16, 4, 3)
+    st = ga.synthetic_state(g, seed=2)
+    free, _ = oracle.dt_calc(g, st, 0.9)
+    sel = lambda **k: oracle.dt_select(g, st, 0.9, k.get("glow", 0), k.get("t", 0.0),
+                                       k.get("tout", 1e9), k.get("tglow", 1e9), k.get("it", 1),
+                                       k.get("dt", 1e-3))
+    assert sel() == free                                   # first iteration: no growth limit
+    assert sel(it=2, dt=free / 10) == 2 * (free / 10)      # later: at most twice the last dt
+    assert sel(t=10.0, tout=10.0 + free / 3) == (10.0 + free / 3) - 10.0
+    assert sel(glow=1, t=1.0, tglow=1.0 + free / 5) == (1.0 + free / 5) - 1.0
+    assert sel(glow=0, t=1.0, tglow=1.0 + free / 5) == free
+    assert sel(t=5.0, tout=5.0) == 1e-6

@@ -113,3 +113,25 @@ def test_block_errors_are_reported():
         tr.clean_param(s, 4)
     with pytest.raises(GemhipError, match="residency mode 1"):
         tr.fluid_adv_block(0.1)
+
+
+@pytest.mark.parametrize("kind", ["cart", "dipole"])
+def test_dt_select_matches_oracle(kind):
+    """dt_select_C reduces the grid rates on the device; maxima are exact, so the rates and
+    every dt of a sequence of calls are bit-identical to the oracle's."""
+    g, st, nul = case(kind)
+    tr = Transport(g, st.nn, st.Tn)
+    tr.register_cfl(0.8, 1)
+    tr.set_residency(1)
+    tr.push(st.copy())
+    _, ref_rates = oracle.dt_calc(g, st, 0.8)
+    assert np.array_equal(tr.grid_rates(), ref_rates)
+    tr.set_residency(0)
+    dt_gpu = dt_ref = 1e-3
+    free, _ = oracle.dt_calc(g, st, 0.8)
+    for it, (t, tout, tglow) in enumerate([(0.0, 1e9, 1e9), (1.0, 1e9, 1e9),
+                                           (2.0, 2.0 + free / 3, 1e9), (3.0, 1e9, 3.0 + free / 4),
+                                           (4.0, 4.0, 1e9)], start=1):
+        dt_gpu = tr.dt_select(st, t, tout, tglow, dt_gpu)
+        dt_ref = oracle.dt_select(g, st, 0.8, 1, t, tout, tglow, it, dt_ref)
+        assert dt_gpu == dt_ref, (it, dt_gpu, dt_ref)
