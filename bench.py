@@ -355,6 +355,9 @@ def main():
                     help="central-difference directions checked by the adjoint measurement "
                          "(default 4 in the default run, 32 in the adjoint workload)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--host-state", action="store_true",
+                    help="PCIe-inclusive rate: the state lives in pinned host memory and each "
+                         "timed step pushes it to HBM, steps and pulls it back")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -427,6 +430,21 @@ def main():
         del nul
         step = tr.fluid_adv_block
         args.no_adjoint = True
+    if args.host_state:
+        # the caller keeps fluidvars, fluidauxvars and J1 on the host (pinned); every step
+        # crosses PCIe twice, as with residency mode 0 around a fused step
+        n = g.padded[0] * g.padded[1] * g.padded[2]
+        hf = torch.empty(35 * n, dtype=torch.float64, pin_memory=True)
+        ha = torch.empty(14 * n, dtype=torch.float64, pin_memory=True)
+        tr.pull_blocks(hf, ha)
+        inner = step
+
+        def step(dt):
+            tr.push_blocks(hf, ha)
+            inner(dt)
+            tr.pull_blocks(hf, ha)
+        args.no_adjoint = True
+        args.no_cpu_baseline = True
     log("state in HBM; warm-up")
     for _ in range(args.warmup):
         step(dt)

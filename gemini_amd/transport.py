@@ -191,9 +191,11 @@ class Transport:
         self.set_interface_vels(st)
         self._pushed = st
 
-    def push_blocks(self, fluidvars, fluidauxvars, electrovars):
-        """Push raw blocks (numpy or torch) laid out like the reference's."""
-        self._call("gemhip_state_push", _p(fluidvars), _p(fluidauxvars), _p(electrovars))
+    def push_blocks(self, fluidvars, fluidauxvars=None, electrovars=None):
+        """Push raw blocks (numpy or torch) laid out like the reference's; None leaves that
+        block's copy in HBM as it is."""
+        opt = lambda a: None if a is None else _p(a)
+        self._call("gemhip_state_push", _p(fluidvars), opt(fluidauxvars), opt(electrovars))
 
     def pull_blocks(self, fluidvars, fluidauxvars):
         """Copy the resident state into raw blocks (numpy or torch, host or HBM)."""
