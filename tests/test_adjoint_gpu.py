@@ -24,6 +24,14 @@ DEV = "cuda:0"
 
 
 def case(kind, seed=3):
+    if kind == "cart2d":
+        g = ga.cartesian_grid(32, 8, 1)
+        st = ga.synthetic_state(g, seed=seed)
+        dt = min(ga.cfl_dt(g, st), 0.5)
+        rng = np.random.default_rng(seed)
+        kappa = 1.0 + 0.2 * rng.uniform(-1.0, 1.0, (g.lx2, g.lx3))
+        qtop = 1e-5 * rng.uniform(0.0, 1.0, (g.lx2, g.lx3))
+        return g, st, dt, kappa, qtop, rng
     if kind == "cart":
         g = ga.cartesian_grid(24, 6, 5)
     elif kind == "periodic":
@@ -90,7 +98,7 @@ def test_record_step_is_the_transport_step():
     tr.set_residency(0)
 
 
-@pytest.mark.parametrize("kind", ["cart", "periodic", "dipole"])
+@pytest.mark.parametrize("kind", ["cart", "periodic", "dipole", "cart2d"])
 def test_adjoint_matches_numpy_adjoint(kind):
     print(f"case {kind}", flush=True)
     g, st, dt, kappa, qtop, rng = case(kind)

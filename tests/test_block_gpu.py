@@ -13,13 +13,15 @@ pytestmark = pytest.mark.gpu
 
 
 def case(kind, seed=2):
-    if kind == "cart":
+    if kind == "cart2d":
+        g = ga.cartesian_grid(64, 16, 1)
+    elif kind == "cart":
         g = ga.cartesian_grid(24, 10, 7)
     elif kind == "periodic":
         g = ga.cartesian_grid(20, 6, 8, flagper=True)
     else:
         g = ga.dipole_grid(32, 12, 6)
-    st = ga.synthetic_state(g, seed=seed, perturb=True)
+    st = ga.synthetic_state(g, seed=seed, perturb=g.lx3 > 1)
     alt = np.asarray(g.planes["alt"][0])[2:-2, 2:-2].T          # (lx1, lx2)
     nul = np.ascontiguousarray(np.broadcast_to((alt < 200e3)[:, :, None],
                                                (g.lx1, g.lx2, g.lx3))).astype(np.int32)
@@ -72,7 +74,7 @@ def test_each_block_entry_point(kind):
     tr.register_null_cells(None)
 
 
-@pytest.mark.parametrize("kind", ["cart", "periodic", "dipole"])
+@pytest.mark.parametrize("kind", ["cart", "periodic", "dipole", "cart2d"])
 def test_block_matches_oracle(kind):
     print(f"case {kind}", flush=True)
     g, st, nul = case(kind)
