@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgemhip.so")
 SOURCES = ["gemhip_api.hip", "kernels_advec.hip", "kernels_diffusion.hip", "kernels_adjoint.hip",
-           "kernels_block.hip"]
+           "kernels_block.hip", "kernels_sources.hip"]
 ARCH = os.environ.get("GEMHIP_ARCH", "gfx950")
 
 

@@ -39,13 +39,16 @@ struct Ctx {
   double* d_geom = nullptr;
   double* d_fluid = nullptr;
   double* d_aux = nullptr;
-  double* d_J1 = nullptr;
+  double* d_J1 = nullptr;        // J1 (electrovars field 4), then E1 (field 1): 2 fields
   double* d_vsi = nullptr;
   double* d_v1b = nullptr;
   double* d_neut = nullptr;
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   double* d_Q = nullptr;        // artificial viscosity (intvars%Q), lx1*lx2*lx3*7
+  double* d_src = nullptr;      // source/loss inputs (src_layout), allocated on first use
+  bool atmos_ok = false;        // gemhip_register_atmosphere called
+  bool gravity_ok = false;      // gemhip_register_gravity called
   double* d_rates = nullptr;    // dt_calc partial maxima and results
   double tcfl = 0.9;            // cfg%tcfl
   int flagglow = 0;             // cfg%flagglow
@@ -134,7 +137,7 @@ void collect_timings() {
 void free_all() {
   Ctx& c = ctx();
   for (double** p : {&c.d_geom, &c.d_fluid, &c.d_aux, &c.d_J1, &c.d_vsi, &c.d_v1b, &c.d_neut,
-                     &c.d_scratch, &c.d_kappa, &c.d_qtop, &c.d_adj, &c.d_Q, &c.d_rates}) {
+                     &c.d_scratch, &c.d_kappa, &c.d_qtop, &c.d_adj, &c.d_Q, &c.d_rates, &c.d_src}) {
     if (*p) hipFree(*p);
     *p = nullptr;
   }
@@ -144,6 +147,7 @@ void free_all() {
   c.adj_bytes = 0;
   c.s.kappa = c.s.qtop = nullptr;
   c.sized = c.grid_ok = c.neutrals_ok = false;
+  c.atmos_ok = c.gravity_ok = false;
 }
 
 bool dev_alloc(double** p, size_t n, const char* what) {
@@ -183,7 +187,7 @@ bool check_xtype(const int* xtype) {
 }
 
 // ---- host <-> HBM movement of whole variables (7 species fields each) ----
-enum Var { NS = 0, VS1 = 1, VS2 = 2, VS3 = 3, TS = 4, RHOVS1 = 5, RHOES = 6, J1 = 7 };
+enum Var { NS = 0, VS1 = 1, VS2 = 2, VS3 = 3, TS = 4, RHOVS1 = 5, RHOES = 6, J1 = 7, E1 = 8 };
 
 double* dev_var(Var v) {
   Ctx& c = ctx();
@@ -196,6 +200,7 @@ double* dev_var(Var v) {
     case TS: return c.d_fluid + 4 * n7;
     case RHOVS1: return c.d_aux;
     case RHOES: return c.d_aux + n7;
+    case E1: return c.d_J1 + c.g.N;
     default: return c.d_J1;
   }
 }
@@ -207,6 +212,7 @@ const double* host_var(Var v, double* const* fluidvars, double* const* aux, doub
     case NS: case VS1: case VS2: case VS3: case TS: return *fluidvars + (long)v * n7;
     case RHOVS1: return *aux;
     case RHOES: return *aux + n7;
+    case E1: return *ev;
     default: return *ev + 3L * c.g.N;
   }
 }
@@ -232,7 +238,7 @@ bool upload(const Io& io, std::initializer_list<Var> vars) {
     return true;
   }
   for (Var v : vars) {
-    const size_t n = (v == J1 ? 1L : 7L) * c.g.N;
+    const size_t n = (v == J1 || v == E1 ? 1L : 7L) * c.g.N;
     if (!hipok(hipMemcpyAsync(dev_var(v), host_var(v, io.fv, io.av, io.ev), n * sizeof(double),
                               hipMemcpyDefault, c.stream), "upload"))
       return false;
@@ -406,7 +412,7 @@ int gemhip_set_grid_size(int lx1, int lx2, int lx3) {
   const long ni = (long)lx1 * lx2 * lx3;
   bool ok = dev_alloc(&c.d_fluid, 35L * g.N, "fluidvars") &&
             dev_alloc(&c.d_aux, 14L * g.N, "fluidauxvars") &&
-            dev_alloc(&c.d_J1, (size_t)g.N, "J1") &&
+            dev_alloc(&c.d_J1, 2 * (size_t)g.N, "J1 and E1") &&
             dev_alloc(&c.d_vsi, 7L * (nv1i() + nv2i() + nv3i()), "interface velocities") &&
             dev_alloc(&c.d_v1b, 14L * lx2 * lx3, "x1 boundary velocities") &&
             dev_alloc(&c.d_neut, 4L * ni, "neutrals") &&
@@ -513,7 +519,9 @@ int gemhip_state_push(const double* fluidvars, const double* fluidauxvars,
                               hipMemcpyDefault, c.stream), "push fluidauxvars");
   if (ok && electrovars)
     ok = hipok(hipMemcpyAsync(c.d_J1, electrovars + 3 * N, N * sizeof(double),
-                              hipMemcpyDefault, c.stream), "push J1");
+                              hipMemcpyDefault, c.stream), "push J1") &&
+         hipok(hipMemcpyAsync(c.d_J1 + N, electrovars, N * sizeof(double), hipMemcpyDefault,
+                              c.stream), "push E1");
   if (ok) ok = hipok(hipStreamSynchronize(c.stream), "push");
   c.pushed_fluid = fluidvars;
   return ok ? 0 : 3;
@@ -666,6 +674,89 @@ int run_block(double dt) {
   launch_clean_T2rhoe(g, s, nul, st);                    // :209-210, one pass
   return hipok(hipGetLastError(), "transport block launch") ? 0 : 3;
 }
+
+// d_src layout (ni = lx1*lx2*lx3): nn 6 ni | vn1 vn2 vn3 | Prionize 6 ni | Qeionize | g1 |
+// alt (N, ghosted)
+constexpr long SRC_NI = 18;
+bool src_alloc() {
+  Ctx& c = ctx();
+  if (c.d_src) return true;
+  const long ni = (long)c.lx1 * c.lx2 * c.lx3;
+  return dev_alloc(&c.d_src, (size_t)(SRC_NI * ni + c.g.N), "source/loss inputs");
+}
+
+DevSources sources() {
+  Ctx& c = ctx();
+  const long ni = (long)c.lx1 * c.lx2 * c.lx3;
+  DevSources a;
+  a.nn = c.d_src;
+  a.vn1 = c.d_src + 6 * ni;
+  a.vn2 = c.d_src + 7 * ni;
+  a.vn3 = c.d_src + 8 * ni;
+  a.Prionize = c.d_src + 9 * ni;
+  a.Qeionize = c.d_src + 15 * ni;
+  a.g1 = c.d_src + 16 * ni;
+  a.alt = c.d_src + SRC_NI * ni;
+  a.E1 = c.d_J1 + c.g.N;
+  a.Q = c.d_Q;
+  return a;
+}
+
+// which: 1 energy, 2 momentum, 4 mass
+bool sources_ready(int which) {
+  Ctx& c = ctx();
+  if (!c.atmos_ok) { fail("neutral atmosphere not registered (gemhip_register_atmosphere)"); return false; }
+  if ((which & 2) && !c.gravity_ok) {
+    fail("altitude and gravity not registered (gemhip_register_gravity)");
+    return false;
+  }
+  if ((which & 2) && c.lx1 < 2) { fail("the momentum sources need lx1 >= 2"); return false; }
+  return true;
+}
+
+void launch_sources(int which, double dt) {
+  Ctx& c = ctx();
+  const DevSources a = sources();
+  if (which & 1) launch_source_loss_energy(c.g, c.s, a, dt, c.stream);
+  if (which & 2) launch_source_loss_momentum(c.g, c.s, a, dt, c.stream);
+  if (which & 4) launch_source_loss_mass(c.g, c.s, a, dt, c.stream);
+}
+
+// src/libgemini.cpp:223-232 after the ionization rates are known
+int run_sources_block(double dt) {
+  if (!ready(true) || !sources_ready(7)) return 1;
+  Ctx& c = ctx();
+  if (c.residency != 1) { fail("gemhip_sources_block needs residency mode 1"); return 2; }
+  const DevGrid& g = c.g;
+  const DevState& s = c.s;
+  hipStream_t st = c.stream;
+  const int* nul = c.d_null;
+  launch_set_global_boundaries(g, s, LSP, st);           // :223
+  launch_sources(7, dt);                                 // :225-227
+  launch_clean_param(g, s, 3, nul, st);                  // :229-231
+  launch_clean_param(g, s, 2, nul, st);
+  launch_clean_param(g, s, 1, nul, st);
+  return hipok(hipGetLastError(), "sources block launch") ? 0 : 3;
+}
+
+// one source_loss_*_C call: 1 energy, 2 momentum, 4 mass, 7 all three in that order
+void source_entry(double** fv, double** av, double** ev, int* xtype, double* dt, int which) {
+  if (!ready(true) || !check_xtype(xtype) || !check_host(fv, "fluidvars") ||
+      !check_host(av, "fluidauxvars") || !sources_ready(which))
+    return;
+  if (!dt) { fail("dt is a null pointer"); return; }
+  if ((which & 2) && !check_host(ev, "electrovars")) return;
+  Io io{fv, av, ev};
+  bool ok = upload(io, {NS, VS1, VS2, VS3, TS});
+  if (ok && (which & 1)) ok = upload(io, {RHOES});
+  if (ok && (which & 2)) ok = upload(io, {RHOVS1, E1});
+  if (!ok) return;
+  launch_sources(which, *dt);
+  if (!hipok(hipGetLastError(), "source/loss launch")) return;
+  if (which & 1) download(io, {TS, RHOES});
+  if (which & 2) download(io, {VS1, RHOVS1});
+  if (which & 4) download(io, {NS});
+}
 }  // namespace
 }  // namespace gemhip
 
@@ -674,6 +765,95 @@ extern "C" {
 int gemhip_transport_step(double dt) { return run_step(dt, nullptr); }
 
 int gemhip_fluid_adv_block(double dt) { return run_block(dt); }
+
+int gemhip_sources_block(double dt) { return run_sources_block(dt); }
+
+int gemhip_register_atmosphere(const double* nn, const double* Tn, const double* vn1,
+                               const double* vn2, const double* vn3) {
+  Ctx& c = ctx();
+  if (!ready()) return 1;
+  if (!nn || !Tn || !vn1 || !vn2 || !vn3) { fail("null atmosphere pointer"); return 2; }
+  if (!src_alloc()) return 3;
+  const long ni = (long)c.lx1 * c.lx2 * c.lx3;
+  const size_t b = (size_t)ni * sizeof(double);
+  const DevSources a = sources();
+  bool ok = hipok(hipMemcpy(c.d_src, nn, 6 * b, hipMemcpyDefault), "nn upload") &&
+            hipok(hipMemcpy(const_cast<double*>(a.vn1), vn1, b, hipMemcpyDefault), "vn1 upload") &&
+            hipok(hipMemcpy(const_cast<double*>(a.vn2), vn2, b, hipMemcpyDefault), "vn2 upload") &&
+            hipok(hipMemcpy(const_cast<double*>(a.vn3), vn3, b, hipMemcpyDefault), "vn3 upload") &&
+            hipok(hipMemcpy(c.d_neut, nn, 3 * b, hipMemcpyDefault), "nn upload") &&
+            hipok(hipMemcpy(c.d_neut + 3 * ni, Tn, b, hipMemcpyDefault), "Tn upload");
+  if (!ok) return 4;
+  c.atmos_ok = c.neutrals_ok = true;
+  return 0;
+}
+
+int gemhip_register_gravity(const double* alt, const double* g1) {
+  Ctx& c = ctx();
+  if (!ready()) return 1;
+  if (!alt || !g1) { fail("null altitude or gravity pointer"); return 2; }
+  if (!src_alloc()) return 3;
+  const long ni = (long)c.lx1 * c.lx2 * c.lx3;
+  const DevSources a = sources();
+  bool ok = hipok(hipMemcpy(const_cast<double*>(a.alt), alt, (size_t)c.g.N * sizeof(double),
+                            hipMemcpyDefault), "alt upload") &&
+            hipok(hipMemcpy(const_cast<double*>(a.g1), g1, (size_t)ni * sizeof(double),
+                            hipMemcpyDefault), "g1 upload");
+  if (!ok) return 4;
+  c.gravity_ok = true;
+  return 0;
+}
+
+int gemhip_set_ionization_rates(const double* Prionize, const double* Qeionize) {
+  Ctx& c = ctx();
+  if (!ready()) return 1;
+  if (!Prionize || !Qeionize) { fail("null ionization-rate pointer"); return 2; }
+  if (!src_alloc()) return 3;
+  const long ni = (long)c.lx1 * c.lx2 * c.lx3;
+  const DevSources a = sources();
+  bool ok = hipok(hipMemcpyAsync(const_cast<double*>(a.Prionize), Prionize,
+                                 6 * (size_t)ni * sizeof(double), hipMemcpyDefault, c.stream),
+                  "Prionize upload") &&
+            hipok(hipMemcpyAsync(const_cast<double*>(a.Qeionize), Qeionize,
+                                 (size_t)ni * sizeof(double), hipMemcpyDefault, c.stream),
+                  "Qeionize upload") &&
+            hipok(hipStreamSynchronize(c.stream), "ionization rates");
+  return ok ? 0 : 4;
+}
+
+void clear_ionization_arrays_C(void** intvars) {
+  (void)intvars;
+  if (!ready() || !src_alloc()) return;
+  Ctx& c = ctx();
+  const long ni = (long)c.lx1 * c.lx2 * c.lx3;
+  hipok(hipMemsetAsync(const_cast<double*>(sources().Prionize), 0, 7 * (size_t)ni * sizeof(double),
+                       c.stream), "clear ionization arrays");
+  if (c.residency == 0) hipok(hipStreamSynchronize(c.stream), "clear ionization arrays");
+}
+
+void source_loss_energy_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                          void** intvars, int* xtype, void** xC, double* dt) {
+  (void)intvars; (void)xC;
+  source_entry(fluidvars, fluidauxvars, electrovars, xtype, dt, 1);
+}
+
+void source_loss_momentum_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                            void** intvars, int* xtype, void** xC, double* dt) {
+  (void)intvars; (void)xC;
+  source_entry(fluidvars, fluidauxvars, electrovars, xtype, dt, 2);
+}
+
+void source_loss_mass_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                        void** intvars, int* xtype, void** xC, double* dt) {
+  (void)intvars; (void)xC;
+  source_entry(fluidvars, fluidauxvars, electrovars, xtype, dt, 4);
+}
+
+void source_loss_allparams_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                             void** intvars, int* xtype, void** xC, double* dt) {
+  (void)intvars; (void)xC;
+  source_entry(fluidvars, fluidauxvars, electrovars, xtype, dt, 7);
+}
 
 int gemhip_register_cfl(double tcfl, int flagglow) {
   Ctx& c = ctx();

@@ -132,6 +132,21 @@ void launch_adjoint_step(const DevGrid& g, const DevState& s, const AdjTape& t, 
                          double Teinf, double* lam, double* work, double* gk, double* gq,
                          double* scratch, hipStream_t st);
 
+// ---- the source/loss substep (kernels_sources.hip) ----
+// The reference's intvars%atmos, ionization rates and the mesh's altitude and gravity, in
+// HBM: nn (lx1,lx2,lx3,6) O, N2, O2, H, N, NO; vn1-3, Qeionize, g1 (lx1,lx2,lx3); Prionize
+// (lx1,lx2,lx3,6); alt and E1 ghosted fields; Q the artificial viscosity (lx1,lx2,lx3,7).
+// The neutral temperature is DevState::Tn.
+struct DevSources {
+  const double *nn, *vn1, *vn2, *vn3, *Prionize, *Qeionize, *g1, *alt, *E1, *Q;
+};
+void launch_source_loss_energy(const DevGrid& g, const DevState& s, const DevSources& a,
+                               double dt, hipStream_t st);
+void launch_source_loss_momentum(const DevGrid& g, const DevState& s, const DevSources& a,
+                                 double dt, hipStream_t st);
+void launch_source_loss_mass(const DevGrid& g, const DevState& s, const DevSources& a,
+                             double dt, hipStream_t st);
+
 // kernel-timing hooks implemented by the API layer
 void timing_begin(const char* name, hipStream_t st);
 void timing_end(const char* name, hipStream_t st);

@@ -83,4 +83,15 @@ GEMHIP_HD double pow25(double x) {
   return ph + pl;
 }
 
+// x**1.5 rounded to nearest in the same way: x * sqrt(x) in double-double, one rounding.
+GEMHIP_HD double pow15(double x) {
+  if (!(x > 0x1p-400 && x < 0x1p+400)) return pow(x, 1.5);
+  const double s = sqrt(x);
+  const double e = fma_exact(-s, s, x);
+  const double sl = e / (2.0 * s);
+  const double ph = x * s;
+  const double pl = fma_exact(x, s, -ph) + x * sl;
+  return ph + pl;
+}
+
 }  // namespace gemhip

@@ -1,0 +1,505 @@
+// kernels_sources.hip -- the source/loss substep of a time step (src/libgemini.cpp:223-232):
+// source_loss_energy, source_loss_momentum and source_loss_mass (multifluid.f90:130-175,
+// 472-549) with their collisional and chemical rates (sources.f90, collisions.f90:58-159)
+// and the exponential time differencing of calculus.in.f90:174-196.
+//
+// Each substep is one pass over the padded grid, one thread per cell.  The interior cells
+// form their production and loss rates in registers and advance the conserved quantity;
+// every cell (ghosts included, as the reference's whole-array statements do) then converts
+// back to temperature / velocity / electron density.  The rates never touch HBM.
+//
+// Arithmetic follows the reference expression by expression (-ffp-contract=off, true
+// divides).  The transcendental functions (exp, log, log10, erf, tanh, sin, and x**p for
+// the fitted exponents) are the device library's, which agree with glibc to about an ulp,
+// so these passes match the reference to rounding rather than bit for bit; x**0.5 is the
+// correctly rounded sqrt and x**1.5 is rounded correctly in double-double (pow15).
+//
+// Ion-ion (Coulomb) terms: Teff, Wsj, Phisj and Psisj of coulomb_colln are symmetric in
+// the species pair, so each unordered pair is evaluated once and feeds both species.  The
+// pairs run in lexicographic order, which adds each species' terms in the reference's
+// isp2 order; self-collision terms are exact zeros and are skipped.
+#include "gemhip_internal.h"
+
+namespace gemhip {
+namespace {
+
+constexpr int TPB = 256;
+constexpr double PI = 3.141592653589793;   // 4*atan(1.0_wp), phys_consts.in.f90:20
+
+// collisions.f90:11-47: default-real literals widened to real64
+__device__ __forceinline__ double tab_csn(int isp, int isp2) {
+  constexpr float t[7][4] = {
+      {-1, 6.82e-10f, 6.64e-10f, -1},        {2.44e-10f, 4.34e-10f, 4.27e-10f, 0.69e-10f},
+      {2.58e-10f, -1, 4.49e-10f, 0.74e-10f}, {2.31e-10f, 4.13e-10f, -1, 0.65e-10f},
+      {4.42e-10f, 7.47e-10f, 7.25e-10f, 1.45e-10f}, {-1, 33.6e-10f, 32.0e-10f, -1},
+      {-1, -1, -1, -1}};
+  return (double)t[isp - 1][isp2 - 1];
+}
+__device__ __forceinline__ double tab_c2sn1(int isp, int isp2) {
+  constexpr float t[7][4] = {
+      {3.67e-11f, 0, 0, 4.63e-12f}, {0, 0, 0, 0}, {0, 5.14e-11f, 0, 0}, {0, 0, 2.59e-11f, 0},
+      {0, 0, 0, 0}, {6.61e-11f, 0, 0, 2.65e-10f}, {-1, -1, -1, -1}};
+  return (double)t[isp - 1][isp2 - 1];
+}
+__device__ __forceinline__ double tab_c2sn2(int isp, int isp2) {
+  constexpr float t[7][4] = {
+      {0.064f, 0, 0, -1}, {0, 0, 0, 0}, {0, 0.069f, 0, 0}, {0, 0, 0.073f, 0},
+      {0, 0, 0, 0}, {0.047f, 0, 0, 0.083f}, {-1, -1, -1, -1}};
+  return (double)t[isp - 1][isp2 - 1];
+}
+__device__ __forceinline__ double tab_csj(int isp, int isp2) {
+  constexpr float t[7][7] = {
+      {0.22f, 0.26f, 0.25f, 0.26f, 0.22f, 0.077f, 1.87e-3f},
+      {0.14f, 0.16f, 0.16f, 0.17f, 0.13f, 0.042f, 9.97e-4f},
+      {0.15f, 0.17f, 0.17f, 0.18f, 0.14f, 0.045f, 1.07e-3f},
+      {0.13f, 0.16f, 0.15f, 0.16f, 0.12f, 0.039f, 9.347e-4f},
+      {0.25f, 0.28f, 0.28f, 0.28f, 0.24f, 0.088f, 2.136e-3f},
+      {1.23f, 1.25f, 1.25f, 1.25f, 1.23f, 0.90f, 29.7e-3f},
+      {54.5f, 54.5f, 54.5f, 54.5f, 54.5f, 54.5f, 38.537f}};
+  return (double)t[isp - 1][isp2 - 1];
+}
+// phys_consts.in.f90:60: mn = [16,28,32,1]*amu
+__device__ __forceinline__ double mn(int isp2) {
+  return (isp2 == 1 ? 16.0 : isp2 == 2 ? 28.0 : isp2 == 3 ? 32.0 : 1.0) * amu;
+}
+
+// calculus.in.f90:174-196
+__device__ __forceinline__ double etd(double f, double P, double L, double dt) {
+  const double Ldt = L * dt;
+  const double expL = exp(-1.0 * Ldt);
+  if (Ldt > 1e-10) return f * expL + (P / L) * (1.0 - expL);
+  return f + P * dt;
+}
+
+// collisions.f90:58-114 at one cell (nn2 = nn(isp2), Tsp = Ts(isp))
+__device__ __forceinline__ double maxwell(int isp, int isp2, double nn2, double Tn, double Tsp) {
+  if (isp < LSP) {
+    const double csn = tab_csn(isp, isp2);
+    if (csn < 0) {
+      const double c1 = tab_c2sn1(isp, isp2);
+      if (isp == 1 && isp2 == 4) {
+        const double Teff = Tn + Tsp / 16.0;
+        return ((c1 * sqrt(Teff)) * nn2) * 1e-6;
+      }
+      const double Teff = 0.5 * (Tn + Tsp);
+      const double b = 1.0 - tab_c2sn2(isp, isp2) * log10(Teff);
+      return (((c1 * (b * b)) * sqrt(Teff)) * nn2) * 1e-6;
+    }
+    return (csn * nn2) * 1e-6;
+  }
+  const double T = Tsp;
+  switch (isp2) {
+    case 1: return (((8.9e-11 * (1.0 + 5.7e-4 * T)) * sqrt(T)) * nn2) * 1e-6;
+    case 2: return (((2.33e-11 * (1.0 - 1.21e-4 * T)) * T) * nn2) * 1e-6;
+    case 3: return (((1.82e-10 * (1.0 + 3.6e-2 * sqrt(T))) * sqrt(T)) * nn2) * 1e-6;
+    default: return (((4.5e-9 * (1.0 - 1.35e-4 * T)) * sqrt(T)) * nn2) * 1e-6;
+  }
+}
+
+// the pair-symmetric part of coulomb_colln (collisions.f90:138-157) for species a < b:
+// Teff**1.5, Phisj, Psisj
+struct Pair { double T15, Phi, Psi; };
+__device__ __forceinline__ Pair coulomb_pair(int a, int b, double Ta, double Tb, double va,
+                                             double vb) {
+  const double ma = ms(a), mb = ms(b);
+  const double Teff = (mb * Ta + ma * Tb) / (mb + ma);
+  const double mred = (ma * mb) / (ma + mb);
+  const double W = fabs(va - vb) / sqrt(((2.0 * kB) * Teff) / mred);
+  Pair p;
+  p.T15 = pow15(Teff);
+  p.Psi = exp(-(W * W));
+  p.Phi = W < 0.1 ? 1.0
+                  : (((3.0 / 4.0) * sqrt(PI)) * erf(W)) / ((W * W) * W) - ((3.0 / 2.0) / (W * W)) * p.Psi;
+  return p;
+}
+
+struct Cell {
+  int i1, i2, i3;
+  bool interior;
+  long k;          // interior index (i1-1) + lx1*((i2-1) + lx2*(i3-1))
+};
+__device__ __forceinline__ Cell decode(const DevGrid& g, long c) {
+  Cell q;
+  q.i1 = (int)(c % g.L1) - 1;
+  const long r = c / g.L1;
+  q.i2 = (int)(r % g.L2) - 1;
+  q.i3 = (int)(r / g.L2) - 1;
+  q.interior = q.i1 >= 1 && q.i1 <= g.lx1 && q.i2 >= 1 && q.i2 <= g.lx2 && q.i3 >= 1 && q.i3 <= g.lx3;
+  q.k = (long)(q.i1 - 1) + (long)g.lx1 * ((long)(q.i2 - 1) + (long)g.lx2 * (long)(q.i3 - 1));
+  return q;
+}
+
+// ---- source_loss_energy: srcsEnergy (sources.f90:292-374), Qeionize, ETD, rhoe2T ----
+__global__ __launch_bounds__(TPB) void k_src_energy(DevGrid g, DevState s, DevSources a, double dt) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const Cell q = decode(g, c);
+  double n[LSP], e[LSP];
+#pragma unroll
+  for (int i = 0; i < LSP; ++i) {
+    n[i] = s.ns[i * g.N + c];
+    e[i] = s.rhoes[i * g.N + c];
+  }
+  if (q.interior) {
+    const long ni = (long)g.lx1 * g.lx2 * g.lx3, k = q.k;
+    double T[LSP], v1[LSP], v2[LSP], v3[LSP], P[LSP], L[LSP];
+#pragma unroll
+    for (int i = 0; i < LSP; ++i) {
+      T[i] = s.Ts[i * g.N + c];
+      v1[i] = s.vs1[i * g.N + c];
+      v2[i] = s.vs2[i * g.N + c];
+      v3[i] = s.vs3[i * g.N + c];
+    }
+    const double Tn = s.Tn[k], vn1 = a.vn1[k], vn2 = a.vn2[k], vn3 = a.vn3[k];
+    double nnk[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) nnk[j] = a.nn[j * ni + k];
+    // ion-neutral and electron-neutral: heat transfer and friction
+#pragma unroll
+    for (int isp = 1; isp <= LSP; ++isp) {
+      const double m = ms(isp), gm1 = gammas(isp) - 1.0, nm = n[isp - 1] * m;
+      const double d1 = v1[isp - 1] - vn1, d2 = v2[isp - 1] - vn2, d3 = v3[isp - 1] - vn3;
+      const double dv2 = ((d1 * d1) + (d2 * d2)) + (d3 * d3);
+      double Pp = 0.0, Ll = 0.0;
+#pragma unroll
+      for (int isp2 = 1; isp2 <= 4; ++isp2) {
+        const double nu = maxwell(isp, isp2, nnk[isp2 - 1], Tn, T[isp - 1]);
+        double fact = (2.0 * nu) / (m + mn(isp2));
+        Pp = Pp + (((nm * kB) / gm1) * fact) * Tn;
+        Ll = Ll + m * fact;
+        fact = (fact * mn(isp2)) / 3.0;
+        Pp = Pp + ((nm / gm1) * dv2) * fact;
+      }
+      P[isp - 1] = Pp;
+      L[isp - 1] = Ll;
+    }
+    // ion-ion and ion-electron (Coulomb)
+#pragma unroll
+    for (int ia = 1; ia <= LSP; ++ia) {
+#pragma unroll
+      for (int ib = ia + 1; ib <= LSP; ++ib) {
+        const Pair p = coulomb_pair(ia, ib, T[ia - 1], T[ib - 1], v1[ia - 1], v1[ib - 1]);
+        const double d1 = v1[ia - 1] - v1[ib - 1], d2 = v2[ia - 1] - v2[ib - 1],
+                     d3 = v3[ia - 1] - v3[ib - 1];
+        const double dv2 = ((d1 * d1) + (d2 * d2)) + (d3 * d3);
+        const double msum = ms(ia) + ms(ib);
+        // species ia with partner ib, then ib with partner ia
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int x = side ? ib : ia, y = side ? ia : ib;
+          const double m = ms(x), gm1 = gammas(x) - 1.0, nm = n[x - 1] * m;
+          const double nu = ((tab_csj(x, y) * n[y - 1]) * 1e-6) / p.T15;
+          double fact = ((2.0 * nu) * p.Psi) / msum;
+          P[x - 1] = P[x - 1] + (((nm * kB) / gm1) * fact) * T[y - 1];
+          L[x - 1] = L[x - 1] + m * fact;
+          fact = ((((2.0 * nu) * p.Phi) / msum) * ms(y)) / 3.0;
+          P[x - 1] = P[x - 1] + ((nm / gm1) * dv2) * fact;
+        }
+      }
+    }
+    // inelastic electron-neutral collisions (sources.f90:350-373)
+    const double sfact = (elchrg / kB) * (gammas(LSP) - 1.0);
+    const double Te = T[LSP - 1], nn2 = nnk[1], nn3 = nnk[2];
+    double nu = (((sfact * 6.9e-14) * nn3) * 1e-6) / sqrt(Te);
+    double iePT = nu * Tn, ieLT = nu;
+    nu = (((sfact * 2.9e-14) * nn2) * 1e-6) / sqrt(Te);
+    iePT = iePT + nu * Tn;
+    ieLT = ieLT + nu;
+    double f = 1.06e4 + 7.51e3 * tanh(1.10e-3 * (Te - 1800.0));
+    const double gg = (3300.0 + 1.233 * (Te - 1000.0)) - (2.056e-4 * (Te - 1000.0)) * (Te - 4000.0);
+    double fact = ((((sfact * 2.99e-12) * nn2) * 1e-6) * exp(((f * (Te - 2000.0)) / Te) / 2000.0)) *
+                  (exp((-(gg * (Te - Tn)) / Te) / Tn) - 1.0);
+    iePT = iePT - fmax_f(fact, 0.0);
+    f = 3300.0 - 839.0 * sin(1.91e-4 * (Te - 2700.0));
+    fact = ((((sfact * 5.196e-13) * nn3) * 1e-6) * exp(((f * (Te - 700.0)) / Te) / 700.0)) *
+           (exp((-(2770.0 * (Te - Tn)) / Te) / Tn) - 1.0);
+    iePT = iePT - fmax_f(fact, 0.0);
+    P[LSP - 1] = P[LSP - 1] + (((iePT * n[LSP - 1]) * kB) / (gammas(LSP) - 1.0));
+    L[LSP - 1] = L[LSP - 1] + ieLT;
+    // energy_source_loss_solve (multifluid.f90:474-495)
+    P[LSP - 1] = P[LSP - 1] + a.Qeionize[k];
+#pragma unroll
+    for (int i = 0; i < LSP; ++i) {
+      e[i] = etd(e[i], P[i], L[i], dt);
+      s.rhoes[i * g.N + c] = e[i];
+    }
+  }
+#pragma unroll
+  for (int isp = 1; isp <= LSP; ++isp) {
+    const double t = (((gammas(isp) - 1.0) / kB) * e[isp - 1]) / fmax_f(n[isp - 1], mindensdiv);
+    s.Ts[(isp - 1) * g.N + c] = fmax_f(t, 100.0);
+  }
+}
+
+// d(log p)/dx1 / h1 of species isp at interior cell c (gradient.f90:136-146): centred over
+// the ghost cells when the subdomain touches neither x1 end, otherwise over the interior
+// with one-sided differences at i1 = 1 and lx1 (sources.f90:259-270)
+__device__ __forceinline__ double dlogp(const DevGrid& g, const DevState& s, int isp, long c,
+                                        int i1, double lp0, bool ghosted) {
+  const long o = (long)(isp - 1) * g.N;
+  auto lp = [&](long cc) { return log((s.ns[o + cc] * kB) * s.Ts[o + cc]); };
+  if (!ghosted) {
+    if (i1 == 1) return ((lp(c + 1) - lp0) / g.dx1[1]) / g.h1[c];
+    if (i1 == g.lx1) return ((lp0 - lp(c - 1)) / g.dx1[g.lx1]) / g.h1[c];
+  }
+  return ((lp(c + 1) - lp(c - 1)) / (g.dx1[i1 + 1] + g.dx1[i1])) / g.h1[c];
+}
+
+// ---- source_loss_momentum: srcsMomentum_curv (sources.f90:228-289), ETD, velocities ----
+__global__ __launch_bounds__(TPB) void k_src_momentum(DevGrid g, DevState s, DevSources a, double dt) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const Cell q = decode(g, c);
+  double n[LSP], r[LSP - 1];
+#pragma unroll
+  for (int i = 0; i < LSP; ++i) n[i] = s.ns[i * g.N + c];
+#pragma unroll
+  for (int i = 0; i < LSP - 1; ++i) r[i] = s.rhovs1[i * g.N + c];
+  if (q.interior) {
+    const long ni = (long)g.lx1 * g.lx2 * g.lx3, k = q.k;
+    const int i1 = q.i1;
+    const bool ghosted = !(g.isglobal[0] || g.isglobal[1]);
+    double T[LSP], v1[LSP], P[LSP - 1], L[LSP - 1];
+#pragma unroll
+    for (int i = 0; i < LSP; ++i) {
+      T[i] = s.Ts[i * g.N + c];
+      v1[i] = s.vs1[i * g.N + c];
+    }
+    const double Tn = s.Tn[k], vn1 = a.vn1[k];
+    double nnk[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) nnk[j] = a.nn[j * ni + k];
+#pragma unroll
+    for (int isp = 1; isp <= LSP - 1; ++isp) {
+      const double nm = n[isp - 1] * ms(isp);
+      double Pp = 0.0, Ll = 0.0;
+#pragma unroll
+      for (int isp2 = 1; isp2 <= 4; ++isp2) {
+        const double nu = maxwell(isp, isp2, nnk[isp2 - 1], Tn, T[isp - 1]);
+        Ll = Ll + nu;
+        Pp = Pp + (nm * nu) * vn1;
+      }
+      P[isp - 1] = Pp;
+      L[isp - 1] = Ll;
+    }
+#pragma unroll
+    for (int ia = 1; ia <= LSP - 1; ++ia) {
+#pragma unroll
+      for (int ib = ia + 1; ib <= LSP; ++ib) {
+        const Pair p = coulomb_pair(ia, ib, T[ia - 1], T[ib - 1], v1[ia - 1], v1[ib - 1]);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int x = side ? ib : ia, y = side ? ia : ib;
+          if (x == LSP) continue;
+          const double nu = ((tab_csj(x, y) * n[y - 1]) * 1e-6) / p.T15;
+          L[x - 1] = L[x - 1] + nu * p.Phi;
+          P[x - 1] = P[x - 1] + (((n[x - 1] * ms(x)) * nu) * p.Phi) * v1[y - 1];
+        }
+      }
+    }
+    // geometric factors, ambipolar field, gravity (sources.f90:248-288)
+    const double h1 = g.h1[c], h2 = g.h2[c], h3 = g.h3[c];
+    const double h1h2h3 = (h1 * h2) * h3;
+    const double dsum = g.dx1[i1 + 1] + g.dx1[i1];
+    const double dh2dx1 = ((g.h2[c + 1] - g.h2[c - 1]) / dsum) / h1;
+    const double dh3dx1 = ((g.h3[c + 1] - g.h3[c - 1]) / dsum) / h1;
+    const double Te = T[LSP - 1];
+    const double lpe = log((n[LSP - 1] * kB) * Te);
+    const double Epol1 = ((kB * Te) / qs(LSP)) * dlogp(g, s, LSP, c, i1, lpe, ghosted);
+    const double E1f = a.alt[c] < 100e3 ? 0.0 : a.E1[c];
+    const double g1 = a.g1[k];
+#pragma unroll
+    for (int isp = 1; isp <= LSP - 1; ++isp) {
+      const double nsp = n[isp - 1], m = ms(isp);
+      const double pr = (nsp * kB) * T[isp - 1];
+      const double ionpress = pr * dlogp(g, s, isp, c, i1, log(pr), ghosted);
+      const double* Qs = a.Q + (long)(isp - 1) * ni;
+      double gQ;
+      if (i1 == 1) gQ = ((Qs[k + 1] - Qs[k]) / g.dx1[1]) / h1;
+      else if (i1 == g.lx1) gQ = ((Qs[k] - Qs[k - 1]) / g.dx1[g.lx1]) / h1;
+      else gQ = ((Qs[k + 1] - Qs[k - 1]) / dsum) / h1;
+      const double w2 = s.vs2[(isp - 1) * g.N + c], w3 = s.vs3[(isp - 1) * g.N + c];
+      const double geom = (((((w2 * w2) * h3) * dh2dx1 + ((w3 * w3) * h2) * dh3dx1) * nsp) * m) / h1h2h3;
+      const double Pp = ((((P[isp - 1] + (nsp * qs(isp)) * (E1f + Epol1)) - ionpress) - gQ) + geom) +
+                        (nsp * m) * g1;
+      r[isp - 1] = etd(r[isp - 1], Pp, L[isp - 1], dt);
+      s.rhovs1[(isp - 1) * g.N + c] = r[isp - 1];
+    }
+  }
+  // momentum_source_loss_solve (multifluid.f90:510-527), whole padded grid
+  double chrgflux = 0.0;
+#pragma unroll
+  for (int isp = 1; isp <= LSP - 1; ++isp) {
+    const double v = r[isp - 1] / (ms(isp) * fmax_f(n[isp - 1], mindensdiv));
+    s.vs1[(isp - 1) * g.N + c] = v;
+    chrgflux = chrgflux + (n[isp - 1] * qs(isp)) * v;
+  }
+  const double ve = -(((1.0 / fmax_f(n[LSP - 1], mindensdiv)) / qs(LSP)) * chrgflux);
+  s.vs1[(LSP - 1) * g.N + c] = ve;
+  s.rhovs1[(LSP - 1) * g.N + c] = (n[LSP - 1] * ms(LSP)) * ve;
+}
+
+// kreac fits of srcsContinuity: ((((c0 s1 c1 t) s2 c2 t^2) s3 c3 t^3) s4 c4 t^4)
+__device__ __forceinline__ double poly4(double t, double c0, double c1, double c2, double c3,
+                                        double c4, bool p1, bool p2, bool p3, bool p4) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
+  double r = c0;
+  r = p1 ? r + c1 * t : r - c1 * t;
+  r = p2 ? r + c2 * t2 : r - c2 * t2;
+  r = p3 ? r + c3 * t3 : r - c3 * t3;
+  r = p4 ? r + c4 * t4 : r - c4 * t4;
+  return r;
+}
+
+// ---- source_loss_mass: srcsContinuity (sources.f90:18-225), Prionize, ETD, ne ----
+__global__ __launch_bounds__(TPB) void k_src_mass(DevGrid g, DevState s, DevSources a, double dt) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const Cell q = decode(g, c);
+  double n[LSP];
+#pragma unroll
+  for (int i = 0; i < LSP; ++i) n[i] = s.ns[i * g.N + c];
+  if (q.interior) {
+    const long ni = (long)g.lx1 * g.lx2 * g.lx3, k = q.k;
+    double P[LSP] = {0, 0, 0, 0, 0, 0, 0}, L[LSP] = {0, 0, 0, 0, 0, 0, 0};
+    const double Tn = s.Tn[k];
+    const double T1 = s.Ts[c], T3 = s.Ts[2 * g.N + c], T6 = s.Ts[5 * g.N + c];
+    const double Te = s.Ts[6 * g.N + c], ne = n[LSP - 1];
+    const double d1 = s.vs1[c] - a.vn1[k], d2 = s.vs2[c] - a.vn2[k], d3 = s.vs3[c] - a.vn3[k];
+    const double dv2 = ((d1 * d1) + (d2 * d2)) + (d3 * d3);
+    auto NN = [&](int j) { return a.nn[(long)(j - 1) * ni + k]; };
+    const double e07 = (double)0.7f, e085 = (double)0.85f, e039 = (double)0.39f;
+    const double e023 = (double)0.23f, em041 = -(double)0.41f, e044 = (double)0.44f;
+    const double em02 = -(double)0.2f, e070 = (double)0.70f, e056 = (double)0.56f;
+    const double X16 = (((16.0 * amu) / 3.0) / kB) * dv2, X28 = (((28.0 * amu) / 3.0) / kB) * dv2;
+    double Teff, kreac = 0.0, b;
+    // O+ + N2 -> NO+ + N
+    Teff = (28.0 / (16.0 + 28.0)) * ((X16 + T1) - Tn) + Tn;
+    if (Teff <= 3725.0)
+      kreac = poly4(Teff / 300.0, 1.71676e-12, 7.19934e-13, 1.33276e-13, 9.28213e-15, 6.39557e-16, false, true, false, true);
+    if (Teff > 3725.0 && Teff <= 30000.0)
+      kreac = poly4(Teff / 300.0, -1.52489e-11, 7.67112e-13, 1.19064e-13, 1.30858e-15, 4.67756e-18, true, true, false, true);
+    if (Teff > 30000.0)
+      kreac = poly4(100.0, -1.52489e-11, 7.67112e-13, 1.19064e-13, 1.30858e-15, 4.67756e-18, true, true, false, true);
+    b = (kreac * NN(2)) * 1e-6;
+    P[1] = P[1] + b * n[0]; L[0] = L[0] + b;
+    // O+ + O2 -> O2+ + O
+    Teff = (32.0 / (16.0 + 32.0)) * ((X16 + T1) - Tn) + Tn;
+    if (Teff <= 4800.0)
+      kreac = poly4(Teff / 300.0, 2.78932e-11, 6.92612e-12, 8.67684e-13, 3.47251e-14, 5.07097e-16, false, true, false, true);
+    if (Teff > 4800.0 && Teff <= 30000.0) {
+      const double t = Teff / 300.0;
+      kreac = ((-1.74046e-11 + (3.02328e-12 * Teff) / 300.0) - 2.39214e-15 * (t * t)) - 4.02394e-17 * ((t * t) * t);
+    }
+    if (Teff > 30000.0)
+      kreac = ((-1.74046e-11 + 3.02328e-12 * 100.0) - 2.39214e-15 * 10000.0) - 4.02394e-17 * 1000000.0;
+    b = (kreac * NN(3)) * 1e-6;
+    P[3] = P[3] + b * n[0]; L[0] = L[0] + b;
+    // O+ + NO -> NO+ + O
+    Teff = (30.0 / (16.0 + 30.0)) * ((X16 + T1) - Tn) + Tn;
+    if (Teff <= 3800.0)
+      kreac = poly4(Teff / 300.0, 6.40408e-13, 1.33888e-13, 7.65103e-14, 3.11509e-15, 6.62374e-17, false, true, false, true);
+    if (Teff > 3800.0 && Teff <= 30000.0)
+      kreac = poly4(Teff / 300.0, -7.48312e-13, 2.31502e-13, 3.07160e-14, 2.65436e-16, 7.76665e-19, true, true, false, true);
+    if (Teff > 30000.0)
+      kreac = poly4(100.0, -7.48312e-13, 2.31502e-13, 3.07160e-14, 2.65436e-16, 7.76665e-19, true, true, false, true);
+    b = (kreac * NN(6)) * 1e-6;
+    P[1] = P[1] + b * n[0]; L[0] = L[0] + b;
+    // O+ + e -> O + hv
+    b = ((3.7e-12 * pow(250.0 / Te, e07)) * ne) * 1e-6;
+    L[0] = L[0] + b;
+    // N2+ + O -> O+ + N2
+    Teff = (16.0 / (28.0 + 16.0)) * ((X28 + T3) - Tn) + Tn;
+    kreac = Teff <= 1500.0 ? 1e-11 * pow(300.0 / Teff, e023) : 3.6e-12 * pow(300.0 / Teff, em041);
+    b = (kreac * NN(1)) * 1e-6;
+    P[0] = P[0] + b * n[2]; L[2] = L[2] + b;
+    // N+ + O -> O+ + N
+    b = (5e-13 * NN(1)) * 1e-6;
+    P[0] = P[0] + b * n[4]; L[4] = L[4] + b;
+    // H+ + O -> O+ + H
+    b = (((6.e-10 * (8.0 / 9.0)) * sqrt((T6 + Tn / 4.0) / (Tn + T6 / 16.0))) * NN(1)) * 1e-6;
+    P[0] = P[0] + b * n[5]; L[5] = L[5] + b;
+    // O+ + H -> H+ + O
+    b = (6.0e-10 * NN(4)) * 1e-6;
+    P[5] = P[5] + b * n[0]; L[0] = L[0] + b;
+    // O2+ + N2 -> NO+ + NO (the reference takes nn(:,:,:,3))
+    b = (5e-16 * NN(3)) * 1e-6;
+    P[1] = P[1] + b * n[3]; L[3] = L[3] + b;
+    // O2+ + N -> NO+ + O
+    b = (1.2e-10 * NN(5)) * 1e-6;
+    P[1] = P[1] + b * n[3]; L[3] = L[3] + b;
+    // O2+ + NO -> NO+ + O2
+    b = (4.6e-10 * NN(6)) * 1e-6;
+    P[1] = P[1] + b * n[3]; L[3] = L[3] + b;
+    // N2+ + O -> NO+ + N
+    kreac = Teff <= 1500.0 ? 1.4e-10 * pow(300.0 / Teff, e044) : 5.2e-11 * pow(300.0 / Teff, em02);
+    b = (kreac * NN(1)) * 1e-6;
+    P[1] = P[1] + b * n[2]; L[2] = L[2] + b;
+    // N2+ + NO -> NO+ + N2
+    b = (4.1e-10 * NN(6)) * 1e-6;
+    P[1] = P[1] + b * n[2]; L[2] = L[2] + b;
+    // N+ + O2 -> NO+ + O
+    b = (2.6e-10 * NN(3)) * 1e-6;
+    P[1] = P[1] + b * n[4]; L[4] = L[4] + b;
+    // NO+ + e -> N + O
+    b = ((4.2e-7 * pow(300.0 / Te, e085)) * ne) * 1e-6;
+    L[1] = L[1] + b;
+    // N2+ + O2 -> O2+ + N2
+    Teff = (32.0 / (28.0 + 32.0)) * ((X28 + T3) - Tn) + Tn;
+    b = ((5e-11 * (300.0 / Teff)) * NN(3)) * 1e-6;
+    P[3] = P[3] + b * n[2]; L[2] = L[2] + b;
+    // N2+ + e -> N + N
+    b = ((1.8e-7 * pow(300.0 / Te, e039)) * ne) * 1e-6;
+    L[2] = L[2] + b;
+    // N+ + O2 -> O2+ + N
+    b = (3.1e-10 * NN(3)) * 1e-6;
+    P[3] = P[3] + b * n[4]; L[4] = L[4] + b;
+    // O2+ + e -> O + O
+    kreac = Te <= 1200.0 ? 1.95e-7 * pow(300.0 / Te, e070) : 7.38e-8 * pow(1200.0 / Te, e056);
+    b = (kreac * ne) * 1e-6;
+    L[3] = L[3] + b;
+    // N+ + H -> H+ + N
+    b = (3.6e-12 * NN(4)) * 1e-6;
+    P[5] = P[5] + b * n[4]; L[4] = L[4] + b;
+    // H+ + e -> H + hv
+    b = ((3.7e-12 * pow(250.0 / Te, e07)) * ne) * 1e-6;
+    L[5] = L[5] + b;
+    // mass_source_loss_solve (multifluid.f90:532-549)
+#pragma unroll
+    for (int i = 0; i < LSP - 1; ++i) {
+      n[i] = etd(n[i], P[i] + a.Prionize[i * ni + k], L[i], dt);
+      s.ns[i * g.N + c] = n[i];
+    }
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < LSP - 1; ++i) sum = sum + n[i];
+  s.ns[(LSP - 1) * g.N + c] = sum;
+}
+
+int blocks_of(long n) { return (int)((n + TPB - 1) / TPB); }
+
+}  // namespace
+
+void launch_source_loss_energy(const DevGrid& g, const DevState& s, const DevSources& a,
+                               double dt, hipStream_t st) {
+  timing_begin("source_energy", st);
+  hipLaunchKernelGGL(k_src_energy, dim3(blocks_of(g.N)), dim3(TPB), 0, st, g, s, a, dt);
+  timing_end("source_energy", st);
+}
+
+void launch_source_loss_momentum(const DevGrid& g, const DevState& s, const DevSources& a,
+                                 double dt, hipStream_t st) {
+  timing_begin("source_momentum", st);
+  hipLaunchKernelGGL(k_src_momentum, dim3(blocks_of(g.N)), dim3(TPB), 0, st, g, s, a, dt);
+  timing_end("source_momentum", st);
+}
+
+void launch_source_loss_mass(const DevGrid& g, const DevState& s, const DevSources& a,
+                             double dt, hipStream_t st) {
+  timing_begin("source_mass", st);
+  hipLaunchKernelGGL(k_src_mass, dim3(blocks_of(g.N)), dim3(TPB), 0, st, g, s, a, dt);
+  timing_end("source_mass", st);
+}
+
+}  // namespace gemhip

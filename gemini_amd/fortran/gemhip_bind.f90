@@ -19,13 +19,17 @@ module gemhip_bind
             sweep2_allparams_C, sweep2_allspec_mass_C, sweep2_allspec_momentum_C, &
             sweep2_allspec_energy_C, energy_diffusion_C, halo_interface_vels_allspec_C, &
             halo_allparams_C, halo_fluidvars_C, clean_param_C, VNRicht_artvisc_C, &
-            compression_C, RK2_prep_mpi_allspec_C, RK2_global_boundary_allspec_C, dt_select_C
+            compression_C, RK2_prep_mpi_allspec_C, RK2_global_boundary_allspec_C, dt_select_C, &
+            source_loss_allparams_C, source_loss_mass_C, source_loss_momentum_C, &
+            source_loss_energy_C, clear_ionization_arrays_C
   public :: gemhip_init, gemhip_finalize, gemhip_set_error_mode, gemhip_clear_error, &
             gemhip_set_grid_size, gemhip_register_grid, gemhip_register_neutrals, &
             gemhip_register_config, gemhip_set_residency, gemhip_state_push, &
             gemhip_state_pull, gemhip_get_interface_vels, gemhip_set_interface_vels, &
             gemhip_transport_step, gemhip_synchronize, gemhip_error_message, &
-            gemhip_register_null_cells, gemhip_fluid_adv_block, gemhip_register_cfl
+            gemhip_register_null_cells, gemhip_fluid_adv_block, gemhip_register_cfl, &
+            gemhip_register_atmosphere, gemhip_register_gravity, gemhip_set_ionization_rates, &
+            gemhip_sources_block
 
   ! ------------------------------------------------ part 1: reference entry points
   interface
@@ -198,6 +202,38 @@ module gemhip_bind
       integer(c_int) :: xtype
       real(c_double) :: t, tout, tglowout, dt
     end subroutine
+    subroutine source_loss_allparams_C(fluidvars, fluidauxvars, electrovars, intvars, xtype, xC, dt) &
+        bind(C, name="source_loss_allparams_C")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: fluidvars, fluidauxvars, electrovars, intvars, xC
+      integer(c_int) :: xtype
+      real(c_double) :: dt
+    end subroutine
+    subroutine source_loss_mass_C(fluidvars, fluidauxvars, electrovars, intvars, xtype, xC, dt) &
+        bind(C, name="source_loss_mass_C")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: fluidvars, fluidauxvars, electrovars, intvars, xC
+      integer(c_int) :: xtype
+      real(c_double) :: dt
+    end subroutine
+    subroutine source_loss_momentum_C(fluidvars, fluidauxvars, electrovars, intvars, xtype, xC, dt) &
+        bind(C, name="source_loss_momentum_C")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: fluidvars, fluidauxvars, electrovars, intvars, xC
+      integer(c_int) :: xtype
+      real(c_double) :: dt
+    end subroutine
+    subroutine source_loss_energy_C(fluidvars, fluidauxvars, electrovars, intvars, xtype, xC, dt) &
+        bind(C, name="source_loss_energy_C")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: fluidvars, fluidauxvars, electrovars, intvars, xC
+      integer(c_int) :: xtype
+      real(c_double) :: dt
+    end subroutine
+    subroutine clear_ionization_arrays_C(intvars) bind(C, name="clear_ionization_arrays_C")
+      import :: c_ptr
+      type(c_ptr) :: intvars
+    end subroutine
   end interface
 
   ! ------------------------------------------------ part 2: library set-up
@@ -277,6 +313,24 @@ module gemhip_bind
     integer(c_int) function gemhip_fluid_adv_block(dt) bind(C, name="gemhip_fluid_adv_block")
       import :: c_int, c_double
       real(c_double), value :: dt
+    end function
+    integer(c_int) function gemhip_sources_block(dt) bind(C, name="gemhip_sources_block")
+      import :: c_int, c_double
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function gemhip_register_atmosphere(nn, Tn, vn1, vn2, vn3) &
+        bind(C, name="gemhip_register_atmosphere")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: nn(*), Tn(*), vn1(*), vn2(*), vn3(*)
+    end function
+    integer(c_int) function gemhip_register_gravity(alt, g1) bind(C, name="gemhip_register_gravity")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: alt(*), g1(*)
+    end function
+    integer(c_int) function gemhip_set_ionization_rates(Prionize, Qeionize) &
+        bind(C, name="gemhip_set_ionization_rates")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: Prionize(*), Qeionize(*)
     end function
     integer(c_int) function gemhip_transport_step(dt) bind(C, name="gemhip_transport_step")
       import :: c_int, c_double

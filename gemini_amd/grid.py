@@ -30,7 +30,8 @@ RE = 6371.0e3  # phys_consts.in.f90:34
 
 
 PLANE_FIELDS = ("h1", "h2", "h3", "h1x1i", "h2x1i", "h3x1i", "h1x2i", "h2x2i", "h3x2i",
-                "h1x3i", "h2x3i", "h3x3i", "alt")
+                "h1x3i", "h2x3i", "h3x3i", "alt", "g1")
+GM_EARTH = 6.67408e-11 * 5.9722e24          # Gconst*Me, phys_consts.in.f90:27,36
 
 
 class Grid:
@@ -163,6 +164,9 @@ def cartesian_grid(lx1, lx2, lx3, dx2=5e3, dx3=5e3, zmin=80e3, zmax=1000e3,
     ones = lambda n2, n1, n3: (np.ones((n2, n1)), n3)
     planes = {"h1": ones(L2, L1, L3), "h2": ones(L2, L1, L3), "h3": ones(L2, L1, L3),
               "alt": (np.ascontiguousarray(np.broadcast_to(x1[None, :], (L2, L1))), L3)}
+    # gravity along x1 (up), interior cells: -G Me / r^2
+    g1 = -GM_EARTH / (RE + x1[2:lx1 + 2]) ** 2
+    planes["g1"] = (np.ascontiguousarray(np.broadcast_to(g1[None, :], (lx2, lx1))), lx3)
     for n in ("h1x1i", "h2x1i", "h3x1i"):
         planes[n] = ones(lx2, lx1 + 1, lx3)
     for n in ("h1x2i", "h2x2i", "h3x2i"):
@@ -238,6 +242,11 @@ def dipole_grid(lx1, lx2, lx3, Lmin=1.2, Lmax=5.0, altfoot=80e3, phispan=np.radi
     r, th = _dipole_r_theta(x1[None, :], x2[:, None])
     hq, hp, hphi = _h_dipole(r, th)
     planes = {"h1": (hq, L3), "h2": (hp, L3), "h3": (hphi, L3), "alt": (r - RE, L3)}
+    # gravity along x1, interior cells (meshobj_dipole.f90:265-282): -G Me / r^2 times
+    # er . e1, with e1 the unit vector along grad q = -(2 cos th er + sin th eth)/(r^3 ...)
+    ri_, thi_ = r[2:lx2 + 2, 2:lx1 + 2], th[2:lx2 + 2, 2:lx1 + 2]
+    c_ = np.cos(thi_)
+    planes["g1"] = (np.ascontiguousarray((-GM_EARTH / ri_ ** 2) * (-2.0 * c_ / np.sqrt(1 + 3 * c_ ** 2))), lx3)
     # q interfaces x1i(1:lx1+1) at interior p
     x1i = 0.5 * (x1[1:lx1 + 2] + x1[2:lx1 + 3])
     ri, thi = _dipole_r_theta(x1i[None, :], x2[2:lx2 + 2, None])

@@ -240,3 +240,30 @@ def cfl_dt(g: Grid, st: State, tcfl: float = 0.9, dtmax: float = 0.5) -> float:
                    float(np.max(np.abs(FV[14 + isp][sl]) / dl2)),
                    float(np.max(np.abs(FV[21 + isp][sl]) / dl3)))
     return min(dtmax, tcfl / rate) if rate > 0 else dtmax
+
+
+@dataclass
+class Atmosphere:
+    """The reference's intvars%atmos and ionization rates for the source/loss step, as
+    Fortran arrays: nn (lx1,lx2,lx3,6) O, N2, O2, H, N, NO; Tn, vn1, vn2, vn3, Qeionize
+    (lx1,lx2,lx3); Prionize (lx1,lx2,lx3,6)."""
+    nn: np.ndarray
+    Tn: np.ndarray
+    vn1: np.ndarray
+    vn2: np.ndarray
+    vn3: np.ndarray
+    Prionize: np.ndarray
+    Qeionize: np.ndarray
+
+
+def synthetic_atmosphere(g: Grid, st: State, seed: int = 0) -> Atmosphere:
+    """Host copy of :func:`gemini_amd.synth_gpu.synthetic_atmosphere_device` for ``st``'s
+    neutrals (generated by torch on the CPU)."""
+    import torch
+    from .synth_gpu import synthetic_atmosphere_device
+    nn3 = torch.as_tensor(np.ascontiguousarray(st.nn[..., :3].transpose(3, 2, 1, 0)))
+    Tn = torch.as_tensor(np.ascontiguousarray(st.Tn.transpose(2, 1, 0)))
+    d = synthetic_atmosphere_device(g, nn3, Tn, seed=seed, device="cpu")
+    F = lambda t: np.asfortranarray(t.numpy().transpose(*range(t.dim() - 1, -1, -1)))
+    return Atmosphere(F(d["nn"]), F(d["Tn"]), F(d["vn1"]), F(d["vn2"]), F(d["vn3"]),
+                      F(d["Prionize"]), F(d["Qeionize"]))

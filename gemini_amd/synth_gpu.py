@@ -135,3 +135,42 @@ def cfl_dt_device(g: Grid, st, tcfl: float = 0.9, dtmax: float = 0.5) -> float:
                    float((FV[14 + isp][sl].abs() / dl2).max()),
                    float((FV[21 + isp][sl].abs() / dl3).max()))
     return min(dtmax, tcfl / rate) if rate > 0 else dtmax
+
+
+def synthetic_atmosphere_device(g: Grid, nn3, Tn, seed: int = 0, device="cuda"):
+    """Neutral atmosphere and ionization rates for the source/loss step, as torch tensors.
+
+    nn3 (3, lx3, lx2, lx1) and Tn (lx3, lx2, lx1) are the state's O, N2, O2 densities and
+    neutral temperature (the ones the energy diffusion uses); H, N and NO are added as
+    exponential profiles, winds are smooth fields of tens of m/s and the ionization rates
+    a Chapman production layer (floored at 1e-5 m^-3 s^-1 as the reference floors them)
+    with electron heating proportional to it.  Returns nn (6, lx3, lx2, lx1): O, N2, O2,
+    H, N, NO; Tn, vn1, vn2, vn3, Qeionize (lx3, lx2, lx1); Prionize (6, lx3, lx2, lx1):
+    C-order bytes of the reference's Fortran intvars arrays.
+    """
+    dev = torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 7919)
+    f64 = torch.float64
+    sh = (g.lx3, g.lx2, g.lx1)
+    z = torch.as_tensor(g.alt_plane(), device=dev)[2:g.lx2 + 2, 2:g.lx1 + 2]   # (lx2, lx1)
+    nn = torch.empty((6,) + sh, device=dev, dtype=f64)
+    nn[0:3].copy_(torch.as_tensor(nn3, device=dev).reshape((3,) + sh))
+    zc = z.clamp_min(80e3)
+    for k, prof in ((3, 1e11 * torch.exp(-(zc - 500e3) / 2000e3)),        # H
+                    (4, 5e12 * torch.exp(-(zc - 150e3) / 40e3)),          # N
+                    (5, 4e14 * torch.exp(-(zc - 100e3) / 8e3))):          # NO
+        noise = torch.randn(sh, generator=gen, device=dev, dtype=f64)
+        nn[k].copy_(noise.mul_(0.05).exp_().mul_(prof[None]))
+    winds = []
+    for scale in (10.0, 60.0, 60.0):
+        w = torch.empty(sh, device=dev, dtype=f64)
+        winds.append(_smooth_field(gen, sh, w, scale, dev))
+    Prionize = torch.empty((6,) + sh, device=dev, dtype=f64)
+    q = 2e9 * _chapman(z, 140e3, 25e3)[None].expand(sh)
+    for k, frac in enumerate((0.45, 0.15, 0.25, 0.12, 0.03, 1e-4)):
+        noise = torch.randn(sh, generator=gen, device=dev, dtype=f64)
+        Prionize[k].copy_((noise.mul_(0.05).exp_() * (frac * q)).clamp_min(1e-5))
+    Qeionize = Prionize[0:6].sum(0) * 2.0e-18
+    return dict(nn=nn, Tn=torch.as_tensor(Tn, device=dev).reshape(sh).contiguous(),
+                vn1=winds[0], vn2=winds[1], vn3=winds[2], Prionize=Prionize, Qeionize=Qeionize)

@@ -53,6 +53,10 @@ def load_library(path: str = LIB_PATH):
                  "gemhip_grid_rates"):
         getattr(lib, name).restype = C.c_int
     lib.gemhip_fluid_adv_block.argtypes = [C.c_double]
+    for name in ("gemhip_sources_block", "gemhip_register_atmosphere", "gemhip_register_gravity",
+                 "gemhip_set_ionization_rates"):
+        getattr(lib, name).restype = C.c_int
+    lib.gemhip_sources_block.argtypes = [C.c_double]
     lib.gemhip_register_cfl.argtypes = [C.c_double, C.c_int]
     for name in ("gemhip_register_conductivity_scale", "gemhip_register_top_heat_flux",
                  "gemhip_checkpoint_save", "gemhip_checkpoint_load",
@@ -365,6 +369,60 @@ class Transport:
         self._void("dt_select_C", C.byref(self._null), C.byref(self._xt), C.byref(self._null),
                    f.ref, a.ref, *[C.byref(v) for v in args])
         return args[3].value
+
+    # ------------------------------------------------------------ source/loss substep
+    def register_atmosphere(self, atm):
+        """intvars%atmos of an :class:`gemini_amd.state.Atmosphere` (numpy) or a mapping of
+        torch tensors with the same bytes (nn, Tn, vn1, vn2, vn3); also the neutrals of the
+        energy diffusion."""
+        get = (lambda n: atm[n]) if isinstance(atm, dict) else (lambda n: getattr(atm, n))
+        arrs = [get(n) for n in ("nn", "Tn", "vn1", "vn2", "vn3")]
+        arrs = [np.asfortranarray(a, dtype=np.float64) if isinstance(a, np.ndarray) else a
+                for a in arrs]
+        self._call("gemhip_register_atmosphere", *[_p(a) for a in arrs])
+        self._keep_neut = tuple(arrs)
+
+    def register_gravity(self):
+        """The grid's altitude (ghosted) and gravity along x1 (interior)."""
+        alt = np.asfortranarray(self.g.alt, dtype=np.float64)
+        g1 = np.asfortranarray(self.g.g1, dtype=np.float64)
+        self._call("gemhip_register_gravity", _p(alt), _p(g1))
+
+    def set_ionization_rates(self, Prionize, Qeionize):
+        """intvars%Prionize (lx1,lx2,lx3,6) and Qeionize (lx1,lx2,lx3), numpy or torch."""
+        conv = lambda a: np.asfortranarray(a, dtype=np.float64) if isinstance(a, np.ndarray) else a
+        P, Q = conv(Prionize), conv(Qeionize)
+        self._call("gemhip_set_ionization_rates", _p(P), _p(Q))
+
+    def enable_sources(self, atm):
+        """Register everything the source/loss substep reads: atmosphere, gravity and the
+        ionization rates of ``atm``."""
+        self.register_atmosphere(atm)
+        self.register_gravity()
+        get = (lambda n: atm[n]) if isinstance(atm, dict) else (lambda n: getattr(atm, n))
+        self.set_ionization_rates(get("Prionize"), get("Qeionize"))
+
+    def clear_ionization_arrays(self):
+        self._void("clear_ionization_arrays_C", C.byref(self._null))
+
+    def source_loss(self, st, which: str, dt: float):
+        """source_loss_{energy,momentum,mass,allparams}_C."""
+        f, a, e = _Ref(st.fluidvars), _Ref(st.fluidauxvars), _Ref(st.electrovars)
+        d = C.c_double(dt)
+        self._void(f"source_loss_{which}_C", f.ref, a.ref, e.ref, C.byref(self._null),
+                   C.byref(self._xt), C.byref(self._null), C.byref(d))
+
+    def sources_block(self, dt: float):
+        """The sources substep of src/libgemini.cpp:223-232 on the HBM-resident state."""
+        self._call("gemhip_sources_block", C.c_double(dt))
+
+    def sources_block_calls(self, st, dt: float):
+        """The reference's calls of src/libgemini.cpp:223-232 one by one."""
+        self.set_global_boundaries_allspec(st, 7)
+        for w in ("energy", "momentum", "mass"):
+            self.source_loss(st, w, dt)
+        for iparm in (3, 2, 1):
+            self.clean_param(st, iparm)
 
     def fluid_adv_block(self, dt: float):
         """The whole transport block of src/libgemini.cpp:155-210 on the HBM-resident state

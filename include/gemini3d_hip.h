@@ -105,6 +105,26 @@ void RK2_global_boundary_allspec_C(int* xtype, void** xC, double** fluidvars);
 void dt_select_C(void** cfgC, int* xtype, void** xC, double** fluidvars, double** fluidauxvars,
                  double* t, double* tout, double* tglowout, double* dt);
 
+/* The sources substep (src/libgemini.cpp:216-227).  gemini3d.h:85-89, libgemini_c.f90:852-948
+   -> libgemini.f90:1120-1195 -> multifluid.f90:130-175, 472-549, sources.f90:18-374,
+   collisions.f90:58-159, calculus.in.f90:174-196.  The library holds what the reference
+   keeps in intvars and the mesh: the neutral atmosphere (gemhip_register_atmosphere), the
+   ionization and heating rates Prionize/Qeionize that impact_ionization_C and
+   solar_ionization_C accumulate (gemhip_set_ionization_rates), the altitude and gravity
+   (gemhip_register_gravity) and Q from VNRicht_artvisc_C; the momentum sources read
+   electrovars field 1 (E1).  Energy updates rhoes and Ts, momentum rhovs1 and vs1, mass ns;
+   allparams runs the three in that order (libgemini.f90:1120-1127). */
+void source_loss_allparams_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                             void** intvars, int* xtype, void** xC, double* dt);
+void source_loss_mass_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                        void** intvars, int* xtype, void** xC, double* dt);
+void source_loss_momentum_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                            void** intvars, int* xtype, void** xC, double* dt);
+void source_loss_energy_C(double** fluidvars, double** fluidauxvars, double** electrovars,
+                          void** intvars, int* xtype, void** xC, double* dt);
+/* gemini3d.h:90, libgemini.f90:1202-1207: Prionize and Qeionize to zero. */
+void clear_ionization_arrays_C(void** intvars);
+
 /* ---------------- part 2: library set-up (no counterpart in the reference ABI) ---------------- */
 /* Array arguments of part 2 may be host or HBM addresses: copies use hipMemcpyDefault. */
 
@@ -175,6 +195,22 @@ int gemhip_register_null_cells(const int* nullpts);
    VNRicht_artvisc, halo_fluidvars, RK2_global_boundary_allspec, compression, rhoe2T,
    clean_param 3, energy_diffusion, clean_param 3 and T2rhoe.  Returns 0 or an error code. */
 int gemhip_fluid_adv_block(double dt);
+
+/* Inputs of the source/loss substep.  intvars%atmos (neutral.f90): nn (lx1,lx2,lx3,6) with
+   O, N2, O2, H, N, NO; Tn, vn1, vn2, vn3 (lx1,lx2,lx3).  Also replaces the neutrals of
+   gemhip_register_neutrals (the first three densities and Tn), so diffusion and sources
+   see one atmosphere. */
+int gemhip_register_atmosphere(const double* nn, const double* Tn, const double* vn1,
+                               const double* vn2, const double* vn3);
+/* x%alt (lx1+4,lx2+4,lx3+4, ghosted) and x%g1 (lx1,lx2,lx3), meshobj.f90:135-143. */
+int gemhip_register_gravity(const double* alt, const double* g1);
+/* intvars%Prionize (lx1,lx2,lx3,6) and intvars%Qeionize (lx1,lx2,lx3): the totals the
+   reference's ionization calls leave before the sources substep (libgemini.cpp:216-219). */
+int gemhip_set_ionization_rates(const double* Prionize, const double* Qeionize);
+/* The sources substep of src/libgemini.cpp:223-232 on the HBM state (mode 1), after
+   gemhip_fluid_adv_block and the ionization rates: set_global_boundaries_allspec,
+   source_loss_energy, _momentum, _mass, then clean_param 3, 2 and 1. */
+int gemhip_sources_block(double dt);
 
 /* HBM pointers of the state blocks (mode 1), for zero-copy callers. */
 int gemhip_device_state(double** fluidvars, double** rhovs1_rhoes, double** J1);

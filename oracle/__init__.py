@@ -365,3 +365,112 @@ def op(g, st, name, dt=None, lsp=7, direction=None, which=None, diffsolvetype=2,
     else:
         raise KeyError(name)
     return st
+
+
+# ---------------------------------------------------------------- source/loss step
+class _OrcAtmos(C.Structure):
+    _fields_ = [(n, _dp) for n in ("nn", "Tn", "vn1", "vn2", "vn3", "Prionize", "Qeionize",
+                                   "alt", "g1")]
+
+
+class OracleAtmos:
+    """orc_atmos over an :class:`gemini_amd.state.Atmosphere` and the grid's alt and g1."""
+
+    def __init__(self, g, atm):
+        s = _OrcAtmos()
+        self._keep = []
+        arrays = dict(nn=atm.nn, Tn=atm.Tn, vn1=atm.vn1, vn2=atm.vn2, vn3=atm.vn3,
+                      Prionize=atm.Prionize, Qeionize=atm.Qeionize, alt=g.alt, g1=g.g1)
+        for n, a in arrays.items():
+            a = np.asfortranarray(a, dtype=np.float64)
+            self._keep.append(a)
+            setattr(s, n, ptr(a))
+        self.s = s
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def etd(f, P, L, dt):
+    fn = lib().orc_ETD_uncoupled
+    fn.restype = C.c_double
+    return fn(C.c_double(f), C.c_double(P), C.c_double(L), C.c_double(dt))
+
+
+def maxwell_colln(isp, isp2, nn2, Tn, Tsp):
+    fn = lib().orc_maxwell_colln
+    fn.restype = C.c_double
+    return fn(C.c_int(isp), C.c_int(isp2), C.c_double(nn2), C.c_double(Tn), C.c_double(Tsp))
+
+
+def coulomb_colln(isp, isp2, ns2, Ts1, Ts2, v1, v2):
+    """(nusj, Phisj, Psisj) at one cell."""
+    out = np.zeros(3)
+    lib().orc_coulomb_colln(C.c_int(isp), C.c_int(isp2), C.c_double(ns2), C.c_double(Ts1),
+                            C.c_double(Ts2), C.c_double(v1), C.c_double(v2), ptr(out))
+    return tuple(out)
+
+
+def grad3D1(g, f):
+    """grad3D1_curv_23 of an interior (lx1, lx2, lx3) array over bounds 1..lx1."""
+    og = OracleGrid(g)
+    f = np.asfortranarray(f, dtype=np.float64)
+    out = np.zeros_like(f, order="F")
+    lib().orc_grad3D1_interior(og.ref, ptr(f), ptr(out))
+    return out
+
+
+def srcs(g, st, atm, which, Q=None):
+    """Pr, Lo (lx1, lx2, lx3, 7) of srcsEnergy / srcsMomentum / srcsContinuity."""
+    og, oa = OracleGrid(g), OracleAtmos(g, atm)
+    v = {k: _f(a) for k, a in _views(st).items()}
+    Pr = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F")
+    Lo = np.zeros_like(Pr, order="F")
+    L = lib()
+    if which == "energy":
+        L.orc_srcs_energy(og.ref, oa.ref, v["ns"], v["vs1"], v["vs2"], v["vs3"], v["Ts"], ptr(Pr), ptr(Lo))
+    elif which == "momentum":
+        Q = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F") if Q is None else Q
+        L.orc_srcs_momentum(og.ref, oa.ref, v["ns"], v["vs1"], v["vs2"], v["vs3"], v["Ts"],
+                            _f(st.electrovars[..., 0]), _f(Q), ptr(Pr), ptr(Lo))
+    elif which == "mass":
+        L.orc_srcs_continuity(og.ref, oa.ref, v["ns"], v["vs1"], v["vs2"], v["vs3"], v["Ts"],
+                              ptr(Pr), ptr(Lo))
+    else:
+        raise KeyError(which)
+    return Pr, Lo
+
+
+def source_loss(g, st, atm, dt, which, Q=None):
+    """source_loss_{energy,momentum,mass} (multifluid.f90:130-175) on ``st`` in place; Q is
+    the (lx1, lx2, lx3, 7) artificial viscosity the momentum sources read."""
+    og, oa = OracleGrid(g), OracleAtmos(g, atm)
+    v = {k: _f(a) for k, a in _views(st).items()}
+    L = lib()
+    if which == "energy":
+        L.orc_source_loss_energy(og.ref, oa.ref, C.c_double(dt), v["ns"], v["vs1"], v["vs2"],
+                                 v["vs3"], v["Ts"], v["rhoes"])
+    elif which == "momentum":
+        Q = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F") if Q is None else Q
+        L.orc_source_loss_momentum(og.ref, oa.ref, C.c_double(dt), v["ns"], v["vs1"], v["vs2"],
+                                   v["vs3"], v["Ts"], _f(st.electrovars[..., 0]), _f(Q), v["rhovs1"])
+    elif which == "mass":
+        L.orc_source_loss_mass(og.ref, oa.ref, C.c_double(dt), v["ns"], v["vs1"], v["vs2"],
+                               v["vs3"], v["Ts"])
+    else:
+        raise KeyError(which)
+    return st
+
+
+def sources_block(g, st, atm, dt, Q=None, nullpts=None):
+    """The sources substep of src/libgemini.cpp:223-232 on ``st`` in place."""
+    og, oa = OracleGrid(g), OracleAtmos(g, atm)
+    Q = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F") if Q is None else Q
+    fn = lib().orc_sources_block
+    fn.restype = C.c_int
+    rc = fn(og.ref, oa.ref, C.c_double(dt), _f(st.fluidvars), _f(st.fluidauxvars),
+            _f(st.electrovars), _f(st.vs1i), _f(Q), _mask(nullpts))
+    if rc != 0:
+        raise RuntimeError(f"oracle sources block failed ({rc})")
+    return st

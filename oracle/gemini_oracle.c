@@ -1131,3 +1131,491 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
   return orc_transport_step_ext(g, dt, fluidvars, fluidauxvars, electrovars, vs1i, vs2i, vs3i,
                                 nn, Tn, diffsolvetype, Teinf, 0, 0);
 }
+
+/* =======================================================================================
+ * Source/loss step (multifluid.f90:130-175, 472-549; sources.f90; collisions.f90:58-159;
+ * calculus.in.f90:174-196; gradient.f90:79-149).  Pr/Lo are (lx1,lx2,lx3,7) interior
+ * arrays; the atmosphere and ionization rates are interior arrays (orc_atmos).
+ * ===================================================================================== */
+
+/* collisions.f90:11-47.  [real(wp) :: 6.82e-10, ...] converts default-real (single
+   precision) literals, so every coefficient is the float nearest the decimal. */
+static const float Csn_f[LSP][4] = {
+    {-1, 6.82e-10f, 6.64e-10f, -1},        {2.44e-10f, 4.34e-10f, 4.27e-10f, 0.69e-10f},
+    {2.58e-10f, -1, 4.49e-10f, 0.74e-10f}, {2.31e-10f, 4.13e-10f, -1, 0.65e-10f},
+    {4.42e-10f, 7.47e-10f, 7.25e-10f, 1.45e-10f}, {-1, 33.6e-10f, 32.0e-10f, -1},
+    {-1, -1, -1, -1}};
+static const float C2sn1_f[LSP][4] = {
+    {3.67e-11f, 0, 0, 4.63e-12f}, {0, 0, 0, 0}, {0, 5.14e-11f, 0, 0}, {0, 0, 2.59e-11f, 0},
+    {0, 0, 0, 0}, {6.61e-11f, 0, 0, 2.65e-10f}, {-1, -1, -1, -1}};
+static const float C2sn2_f[LSP][4] = {
+    {0.064f, 0, 0, -1}, {0, 0, 0, 0}, {0, 0.069f, 0, 0}, {0, 0, 0.073f, 0},
+    {0, 0, 0, 0}, {0.047f, 0, 0, 0.083f}, {-1, -1, -1, -1}};
+static const float Csj_f[LSP][LSP] = {
+    {0.22f, 0.26f, 0.25f, 0.26f, 0.22f, 0.077f, 1.87e-3f},
+    {0.14f, 0.16f, 0.16f, 0.17f, 0.13f, 0.042f, 9.97e-4f},
+    {0.15f, 0.17f, 0.17f, 0.18f, 0.14f, 0.045f, 1.07e-3f},
+    {0.13f, 0.16f, 0.15f, 0.16f, 0.12f, 0.039f, 9.347e-4f},
+    {0.25f, 0.28f, 0.28f, 0.28f, 0.24f, 0.088f, 2.136e-3f},
+    {1.23f, 1.25f, 1.25f, 1.25f, 1.23f, 0.90f, 29.7e-3f},
+    {54.5f, 54.5f, 54.5f, 54.5f, 54.5f, 54.5f, 38.537f}};
+static double mn_(int isp2) {              /* phys_consts.in.f90:60 [16,28,32,1]*amu */
+  static const double m[4] = {16.0, 28.0, 32.0, 1.0};
+  return m[isp2 - 1] * amu;
+}
+static const double pi_ = 3.141592653589793;   /* 4*atan(1.0_wp), phys_consts.in.f90:20 */
+
+/* ---- calculus.in.f90:174-196 ETD_uncoupled, one element ---- */
+double orc_ETD_uncoupled(double f, double P, double L, double dt) {
+  const double Ldt = L * dt;
+  const double expL = exp(-1.0 * Ldt);
+  if (Ldt > 1e-10) return f * expL + (P / L) * (1.0 - expL);
+  return f + P * dt;
+}
+
+/* ---- collisions.f90:58-114 maxwell_colln at one cell: nn2 = nn(isp2), Tsp = Ts(isp) ---- */
+double orc_maxwell_colln(int isp, int isp2, double nn2, double Tn, double Tsp) {
+  if (isp < LSP) {
+    const double csn = (double)Csn_f[isp - 1][isp2 - 1];
+    if (csn < 0) {
+      const double c1 = (double)C2sn1_f[isp - 1][isp2 - 1];
+      if (isp == 1 && isp2 == 4) {
+        const double Teff = Tn + Tsp / 16.0;
+        return ((c1 * pow(Teff, 0.5)) * nn2) * 1e-6;
+      }
+      const double Teff = 0.5 * (Tn + Tsp);
+      const double b = 1.0 - (double)C2sn2_f[isp - 1][isp2 - 1] * log10(Teff);
+      return (((c1 * (b * b)) * pow(Teff, 0.5)) * nn2) * 1e-6;
+    }
+    return (csn * nn2) * 1e-6;
+  }
+  const double T = Tsp;
+  switch (isp2) {
+    case 1: return (((8.9e-11 * (1.0 + 5.7e-4 * T)) * pow(T, 0.5)) * nn2) * 1e-6;
+    case 2: return (((2.33e-11 * (1.0 - 1.21e-4 * T)) * T) * nn2) * 1e-6;
+    case 3: return (((1.82e-10 * (1.0 + 3.6e-2 * pow(T, 0.5))) * pow(T, 0.5)) * nn2) * 1e-6;
+    default: return (((4.5e-9 * (1.0 - 1.35e-4 * T)) * pow(T, 0.5)) * nn2) * 1e-6;
+  }
+}
+
+/* ---- collisions.f90:117-159 coulomb_colln at one cell: out = {nusj, Phisj, Psisj} ---- */
+void orc_coulomb_colln(int isp, int isp2, double ns2, double Ts1, double Ts2, double v1,
+                       double v2, double *out) {
+  if (isp == isp2) { out[0] = out[1] = out[2] = 0.0; return; }
+  const double m1 = ms_(isp), m2 = ms_(isp2);
+  const double Teff = (m2 * Ts1 + m1 * Ts2) / (m2 + m1);
+  out[0] = (((double)Csj_f[isp - 1][isp2 - 1] * ns2) * 1e-6) / pow(Teff, 1.5);
+  const double mred = (m1 * m2) / (m1 + m2);
+  const double W = fabs(v1 - v2) / sqrt(((2.0 * kB) * Teff) / mred);
+  const double Psi = exp(-(W * W));
+  out[2] = Psi;
+  if (W < 0.1) out[1] = 1.0;
+  else out[1] = (((3.0 / 4.0) * sqrt(pi_)) * erf(W)) / ((W * W) * W) - ((3.0 / 2.0) / (W * W)) * Psi;
+}
+
+#define IIX(i1, i2, i3) ((long)((i1) - 1) + (long)lx1 * ((long)((i2) - 1) + (long)lx2 * (long)((i3) - 1)))
+
+/* ---- sources.f90:292-374 srcsEnergy ---- */
+void orc_srcs_energy(const orc_grid *g, const orc_atmos *a, const double *ns, const double *vs1,
+                     const double *vs2, const double *vs3, const double *Ts, double *Pr,
+                     double *Lo) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  const double sfact = (elchrg / kB) * (gammas_(LSP) - 1.0);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i3 = 1; i3 <= lx3; ++i3)
+    for (int i2 = 1; i2 <= lx2; ++i2)
+      for (int i1 = 1; i1 <= lx1; ++i1) {
+        const long c = GIX(d, i1, i2, i3), k = IIX(i1, i2, i3);
+        const double Tn = a->Tn[k], vn1 = a->vn1[k], vn2 = a->vn2[k], vn3 = a->vn3[k];
+#define S(arr, isp) (arr)[(long)((isp) - 1) * d.N + c]
+        for (int isp = 1; isp <= LSP; ++isp) {
+          const double n = S(ns, isp), m = ms_(isp), gm1 = gammas_(isp) - 1.0;
+          double P = 0.0, L = 0.0;
+          for (int isp2 = 1; isp2 <= 4; ++isp2) {
+            const double nu = orc_maxwell_colln(isp, isp2, a->nn[(isp2 - 1) * ni + k], Tn, S(Ts, isp));
+            double fact = (2.0 * nu) / (m + mn_(isp2));
+            P = P + ((((n * m) * kB) / gm1) * fact) * Tn;
+            L = L + m * fact;
+            fact = (fact * mn_(isp2)) / 3.0;
+            const double d1 = S(vs1, isp) - vn1, d2 = S(vs2, isp) - vn2, d3 = S(vs3, isp) - vn3;
+            P = P + (((n * m) / gm1) * (((d1 * d1) + (d2 * d2)) + (d3 * d3))) * fact;
+          }
+          for (int isp2 = 1; isp2 <= LSP; ++isp2) {
+            double cc[3];
+            orc_coulomb_colln(isp, isp2, S(ns, isp2), S(Ts, isp), S(Ts, isp2), S(vs1, isp),
+                              S(vs1, isp2), cc);
+            const double m2 = ms_(isp2);
+            double fact = ((2.0 * cc[0]) * cc[2]) / (m + m2);
+            P = P + ((((n * m) * kB) / gm1) * fact) * S(Ts, isp2);
+            L = L + m * fact;
+            fact = ((((2.0 * cc[0]) * cc[1]) / (m + m2)) * m2) / 3.0;
+            const double d1 = S(vs1, isp) - S(vs1, isp2), d2 = S(vs2, isp) - S(vs2, isp2),
+                         d3 = S(vs3, isp) - S(vs3, isp2);
+            P = P + (((n * m) / gm1) * (((d1 * d1) + (d2 * d2)) + (d3 * d3))) * fact;
+          }
+          Pr[(isp - 1) * ni + k] = P;
+          Lo[(isp - 1) * ni + k] = L;
+        }
+        /* inelastic electron collisions, sources.f90:350-373 */
+        const double Te = S(Ts, LSP), nn2 = a->nn[ni + k], nn3 = a->nn[2 * ni + k];
+        double nu = (((sfact * 6.9e-14) * nn3) * 1e-6) / sqrt(Te);
+        double iePT = nu * Tn, ieLT = nu;
+        nu = (((sfact * 2.9e-14) * nn2) * 1e-6) / sqrt(Te);
+        iePT = iePT + nu * Tn;
+        ieLT = ieLT + nu;
+        double f = 1.06e4 + 7.51e3 * tanh(1.10e-3 * (Te - 1800.0));
+        const double gg = (3300.0 + 1.233 * (Te - 1000.0)) - (2.056e-4 * (Te - 1000.0)) * (Te - 4000.0);
+        double fact = ((((sfact * 2.99e-12) * nn2) * 1e-6) * exp(((f * (Te - 2000.0)) / Te) / 2000.0)) *
+                      (exp((-(gg * (Te - Tn)) / Te) / Tn) - 1.0);
+        iePT = iePT - fmax_f(fact, 0.0);
+        f = 3300.0 - 839.0 * sin(1.91e-4 * (Te - 2700.0));
+        fact = ((((sfact * 5.196e-13) * nn3) * 1e-6) * exp(((f * (Te - 700.0)) / Te) / 700.0)) *
+               (exp((-(2770.0 * (Te - Tn)) / Te) / Tn) - 1.0);
+        iePT = iePT - fmax_f(fact, 0.0);
+        const long ke = (LSP - 1) * ni + k;
+        Pr[ke] = Pr[ke] + (((iePT * S(ns, LSP)) * kB) / (gammas_(LSP) - 1.0));
+        Lo[ke] = Lo[ke] + ieLT;
+#undef S
+      }
+}
+
+/* gradient.f90:79-149 grad3D1_curv_23 over the interior (bounds 1..lx1): one-sided at the
+   ends with dx1(1) and dx1(lx1), centred between.  f and out are (lx1,lx2,lx3). */
+void orc_grad3D1_interior(const orc_grid *g, const double *f, double *out) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const double *dx1 = g->dx1, *h1 = g->h1;
+  for (int i3 = 1; i3 <= lx3; ++i3)
+    for (int i2 = 1; i2 <= lx2; ++i2) {
+      out[IIX(1, i2, i3)] = ((f[IIX(2, i2, i3)] - f[IIX(1, i2, i3)]) / dx1[1]) / h1[GIX(d, 1, i2, i3)];
+      for (int i1 = 2; i1 <= lx1 - 1; ++i1)
+        out[IIX(i1, i2, i3)] = ((f[IIX(i1 + 1, i2, i3)] - f[IIX(i1 - 1, i2, i3)]) /
+                                (dx1[i1 + 1] + dx1[i1])) / h1[GIX(d, i1, i2, i3)];
+      out[IIX(lx1, i2, i3)] = ((f[IIX(lx1, i2, i3)] - f[IIX(lx1 - 1, i2, i3)]) / dx1[lx1]) /
+                              h1[GIX(d, lx1, i2, i3)];
+    }
+}
+
+/* centred x1 derivative at interior cell i1 of a ghosted field (grad3D1 over 0..lx1+1) */
+static double dx1_centred(const orc_grid *g, dims_t d, const double *f, int i1, int i2, int i3) {
+  return ((f[GIX(d, i1 + 1, i2, i3)] - f[GIX(d, i1 - 1, i2, i3)]) / (g->dx1[i1 + 1] + g->dx1[i1])) /
+         g->h1[GIX(d, i1, i2, i3)];
+}
+
+/* ---- sources.f90:228-289 srcsMomentum_curv (ions; the electron row is not used) ---- */
+void orc_srcs_momentum(const orc_grid *g, const orc_atmos *a, const double *ns, const double *vs1,
+                       const double *vs2, const double *vs3, const double *Ts, const double *E1,
+                       const double *Q, double *Pr, double *Lo) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  const int ghosted = !(g->isglobal[0] || g->isglobal[1]);
+  /* log pressure of every species: ghosted (x1 ghosts 0 and lx1+1 used) or interior */
+  double *lp = malloc(sizeof(double) * (ghosted ? d.N : ni) * LSP);
+  double *glp = malloc(sizeof(double) * ni * LSP);
+  double *gQ = malloc(sizeof(double) * ni * LSP);
+  for (int isp = 1; isp <= LSP; ++isp) {
+    const long o = (long)(isp - 1) * d.N;
+    double *l = lp + (isp - 1) * (ghosted ? d.N : ni), *gl = glp + (isp - 1) * ni;
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = ghosted ? 0 : 1; i1 <= (ghosted ? lx1 + 1 : lx1); ++i1) {
+          const long c = GIX(d, i1, i2, i3);
+          const double p = (ns[o + c] * kB) * Ts[o + c];
+          l[ghosted ? c : IIX(i1, i2, i3)] = log(p);
+        }
+    if (ghosted) {
+      for (int i3 = 1; i3 <= lx3; ++i3)
+        for (int i2 = 1; i2 <= lx2; ++i2)
+          for (int i1 = 1; i1 <= lx1; ++i1) gl[IIX(i1, i2, i3)] = dx1_centred(g, d, l, i1, i2, i3);
+    } else {
+      orc_grad3D1_interior(g, l, gl);
+    }
+    if (isp < LSP) orc_grad3D1_interior(g, Q + (isp - 1) * ni, gQ + (isp - 1) * ni);
+  }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i3 = 1; i3 <= lx3; ++i3)
+    for (int i2 = 1; i2 <= lx2; ++i2)
+      for (int i1 = 1; i1 <= lx1; ++i1) {
+        const long c = GIX(d, i1, i2, i3), k = IIX(i1, i2, i3);
+#define S(arr, isp) (arr)[(long)((isp) - 1) * d.N + c]
+        const double h1h2h3 = (g->h1[c] * g->h2[c]) * g->h3[c];
+        const double dh2dx1 = dx1_centred(g, d, g->h2, i1, i2, i3);
+        const double dh3dx1 = dx1_centred(g, d, g->h3, i1, i2, i3);
+        const double Te = S(Ts, LSP);
+        const double Epol1 = ((kB * Te) / qs_(LSP)) * glp[(LSP - 1) * ni + k];
+        const double E1f = a->alt[c] < 100e3 ? 0.0 : E1[c];
+        const double Tn = a->Tn[k];
+        for (int isp = 1; isp <= LSP - 1; ++isp) {
+          const double n = S(ns, isp), m = ms_(isp);
+          double P = 0.0, L = 0.0;
+          for (int isp2 = 1; isp2 <= 4; ++isp2) {
+            const double nu = orc_maxwell_colln(isp, isp2, a->nn[(isp2 - 1) * ni + k], Tn, S(Ts, isp));
+            L = L + nu;
+            P = P + ((n * m) * nu) * a->vn1[k];
+          }
+          for (int isp2 = 1; isp2 <= LSP; ++isp2) {
+            double cc[3];
+            orc_coulomb_colln(isp, isp2, S(ns, isp2), S(Ts, isp), S(Ts, isp2), S(vs1, isp),
+                              S(vs1, isp2), cc);
+            L = L + cc[0] * cc[1];
+            P = P + (((n * m) * cc[0]) * cc[1]) * S(vs1, isp2);
+          }
+          const double pr = (n * kB) * S(Ts, isp);
+          const double ionpress = pr * glp[(isp - 1) * ni + k];
+          const double v2 = S(vs2, isp), v3 = S(vs3, isp);
+          const double geom = (((((v2 * v2) * g->h3[c]) * dh2dx1 + ((v3 * v3) * g->h2[c]) * dh3dx1) * n) * m) / h1h2h3;
+          P = ((((P + (n * qs_(isp)) * (E1f + Epol1)) - ionpress) - gQ[(isp - 1) * ni + k]) + geom) +
+              (n * m) * a->g1[k];
+          Pr[(isp - 1) * ni + k] = P;
+          Lo[(isp - 1) * ni + k] = L;
+        }
+        Pr[(LSP - 1) * ni + k] = 0.0;
+        Lo[(LSP - 1) * ni + k] = 0.0;
+#undef S
+      }
+  free(lp); free(glp); free(gQ);
+}
+
+/* kreac polynomial c0 + c1 t + c2 t^2 + c3 t^3 + c4 t^4 with the reference's signs folded
+   into the coefficients' use: ((((c0 s1 c1*t) s2 c2*t^2) s3 c3*t^3) s4 c4*t^4) */
+static double poly4(double t, double c0, double c1, double c2, double c3, double c4, int s1,
+                    int s2, int s3, int s4) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
+  double r = c0;
+  r = s1 > 0 ? r + c1 * t : r - c1 * t;
+  r = s2 > 0 ? r + c2 * t2 : r - c2 * t2;
+  r = s3 > 0 ? r + c3 * t3 : r - c3 * t3;
+  r = s4 > 0 ? r + c4 * t4 : r - c4 * t4;
+  return r;
+}
+
+/* ---- sources.f90:18-225 srcsContinuity ---- */
+void orc_srcs_continuity(const orc_grid *g, const orc_atmos *a, const double *ns,
+                         const double *vs1, const double *vs2, const double *vs3,
+                         const double *Ts, double *Pr, double *Lo) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  const double e07 = (double)0.7f, e085 = (double)0.85f, e039 = (double)0.39f;
+  const double e023 = (double)0.23f, em041 = -(double)0.41f, e044 = (double)0.44f;
+  const double em02 = -(double)0.2f, e070 = (double)0.70f, e056 = (double)0.56f;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i3 = 1; i3 <= lx3; ++i3)
+    for (int i2 = 1; i2 <= lx2; ++i2)
+      for (int i1 = 1; i1 <= lx1; ++i1) {
+        const long c = GIX(d, i1, i2, i3), k = IIX(i1, i2, i3);
+#define S(arr, isp) (arr)[(long)((isp) - 1) * d.N + c]
+#define NN(j) a->nn[(long)((j) - 1) * ni + k]
+        double P[LSP + 1] = {0}, L[LSP + 1] = {0};
+        const double Tn = a->Tn[k], Te = S(Ts, LSP), ne = S(ns, LSP);
+        const double d1 = S(vs1, 1) - a->vn1[k], d2 = S(vs2, 1) - a->vn2[k], d3 = S(vs3, 1) - a->vn3[k];
+        const double dv2 = ((d1 * d1) + (d2 * d2)) + (d3 * d3);
+        double Teff, kreac = 0.0, b;
+        /* O+ + N2 -> NO+ + N */
+        Teff = (28.0 / (16.0 + 28.0)) * (((((16.0 * amu) / 3.0) / kB) * dv2 + S(Ts, 1)) - Tn) + Tn;
+        if (Teff <= 3725.0)
+          kreac = poly4(Teff / 300.0, 1.71676e-12, 7.19934e-13, 1.33276e-13, 9.28213e-15, 6.39557e-16, -1, 1, -1, 1);
+        if (Teff > 3725.0 && Teff <= 30000.0)
+          kreac = poly4(Teff / 300.0, -1.52489e-11, 7.67112e-13, 1.19064e-13, 1.30858e-15, 4.67756e-18, 1, 1, -1, 1);
+        if (Teff > 30000.0)
+          kreac = poly4(100.0, -1.52489e-11, 7.67112e-13, 1.19064e-13, 1.30858e-15, 4.67756e-18, 1, 1, -1, 1);
+        b = (kreac * NN(2)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 1); L[1] = L[1] + b;
+        /* O+ + O2 -> O2+ + O */
+        Teff = (32.0 / (16.0 + 32.0)) * (((((16.0 * amu) / 3.0) / kB) * dv2 + S(Ts, 1)) - Tn) + Tn;
+        if (Teff <= 4800.0)
+          kreac = poly4(Teff / 300.0, 2.78932e-11, 6.92612e-12, 8.67684e-13, 3.47251e-14, 5.07097e-16, -1, 1, -1, 1);
+        if (Teff > 4800.0 && Teff <= 30000.0) {
+          const double t = Teff / 300.0;
+          kreac = ((-1.74046e-11 + (3.02328e-12 * Teff) / 300.0) - 2.39214e-15 * (t * t)) - 4.02394e-17 * ((t * t) * t);
+        }
+        if (Teff > 30000.0)
+          kreac = ((-1.74046e-11 + 3.02328e-12 * 100.0) - 2.39214e-15 * 10000.0) - 4.02394e-17 * 1000000.0;
+        b = (kreac * NN(3)) * 1e-6;
+        P[4] = P[4] + b * S(ns, 1); L[1] = L[1] + b;
+        /* O+ + NO -> NO+ + O */
+        Teff = (30.0 / (16.0 + 30.0)) * (((((16.0 * amu) / 3.0) / kB) * dv2 + S(Ts, 1)) - Tn) + Tn;
+        if (Teff <= 3800.0)
+          kreac = poly4(Teff / 300.0, 6.40408e-13, 1.33888e-13, 7.65103e-14, 3.11509e-15, 6.62374e-17, -1, 1, -1, 1);
+        if (Teff > 3800.0 && Teff <= 30000.0)
+          kreac = poly4(Teff / 300.0, -7.48312e-13, 2.31502e-13, 3.07160e-14, 2.65436e-16, 7.76665e-19, 1, 1, -1, 1);
+        if (Teff > 30000.0)
+          kreac = poly4(100.0, -7.48312e-13, 2.31502e-13, 3.07160e-14, 2.65436e-16, 7.76665e-19, 1, 1, -1, 1);
+        b = (kreac * NN(6)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 1); L[1] = L[1] + b;
+        /* O+ + e -> O + hv */
+        b = ((3.7e-12 * pow(250.0 / Te, e07)) * ne) * 1e-6;
+        L[1] = L[1] + b;
+        /* N2+ + O -> O+ + N2 */
+        Teff = (16.0 / (28.0 + 16.0)) * (((((28.0 * amu) / 3.0) / kB) * dv2 + S(Ts, 3)) - Tn) + Tn;
+        kreac = Teff <= 1500.0 ? 1e-11 * pow(300.0 / Teff, e023) : 3.6e-12 * pow(300.0 / Teff, em041);
+        b = (kreac * NN(1)) * 1e-6;
+        P[1] = P[1] + b * S(ns, 3); L[3] = L[3] + b;
+        /* N+ + O -> O+ + N */
+        b = (5e-13 * NN(1)) * 1e-6;
+        P[1] = P[1] + b * S(ns, 5); L[5] = L[5] + b;
+        /* H+ + O -> O+ + H */
+        Teff = S(Ts, 6);
+        b = (((6.e-10 * (8.0 / 9.0)) * pow((Teff + Tn / 4.0) / (Tn + Teff / 16.0), 0.5)) * NN(1)) * 1e-6;
+        P[1] = P[1] + b * S(ns, 6); L[6] = L[6] + b;
+        /* O+ + H -> H+ + O */
+        b = (6.0e-10 * NN(4)) * 1e-6;
+        P[6] = P[6] + b * S(ns, 1); L[1] = L[1] + b;
+        /* O2+ + N2 -> NO+ + NO (the reference takes nn(:,:,:,3)) */
+        b = (5e-16 * NN(3)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 4); L[4] = L[4] + b;
+        /* O2+ + N -> NO+ + O */
+        b = (1.2e-10 * NN(5)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 4); L[4] = L[4] + b;
+        /* O2+ + NO -> NO+ + O2 */
+        b = (4.6e-10 * NN(6)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 4); L[4] = L[4] + b;
+        /* N2+ + O -> NO+ + N */
+        Teff = (16.0 / (28.0 + 16.0)) * (((((28.0 * amu) / 3.0) / kB) * dv2 + S(Ts, 3)) - Tn) + Tn;
+        kreac = Teff <= 1500.0 ? 1.4e-10 * pow(300.0 / Teff, e044) : 5.2e-11 * pow(300.0 / Teff, em02);
+        b = (kreac * NN(1)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 3); L[3] = L[3] + b;
+        /* N2+ + NO -> NO+ + N2 */
+        b = (4.1e-10 * NN(6)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 3); L[3] = L[3] + b;
+        /* N+ + O2 -> NO+ + O */
+        b = (2.6e-10 * NN(3)) * 1e-6;
+        P[2] = P[2] + b * S(ns, 5); L[5] = L[5] + b;
+        /* NO+ + e -> N + O */
+        b = ((4.2e-7 * pow(300.0 / Te, e085)) * ne) * 1e-6;
+        L[2] = L[2] + b;
+        /* N2+ + O2 -> O2+ + N2 */
+        Teff = (32.0 / (28.0 + 32.0)) * (((((28.0 * amu) / 3.0) / kB) * dv2 + S(Ts, 3)) - Tn) + Tn;
+        b = ((5e-11 * (300.0 / Teff)) * NN(3)) * 1e-6;
+        P[4] = P[4] + b * S(ns, 3); L[3] = L[3] + b;
+        /* N2+ + e -> N + N */
+        b = ((1.8e-7 * pow(300.0 / Te, e039)) * ne) * 1e-6;
+        L[3] = L[3] + b;
+        /* N+ + O2 -> O2+ + N */
+        b = (3.1e-10 * NN(3)) * 1e-6;
+        P[4] = P[4] + b * S(ns, 5); L[5] = L[5] + b;
+        /* O2+ + e -> O + O */
+        kreac = Te <= 1200.0 ? 1.95e-7 * pow(300.0 / Te, e070) : 7.38e-8 * pow(1200.0 / Te, e056);
+        b = (kreac * ne) * 1e-6;
+        L[4] = L[4] + b;
+        /* N+ + H -> H+ + N */
+        b = (3.6e-12 * NN(4)) * 1e-6;
+        P[6] = P[6] + b * S(ns, 5); L[5] = L[5] + b;
+        /* H+ + e -> H + hv */
+        b = ((3.7e-12 * pow(250.0 / Te, e07)) * ne) * 1e-6;
+        L[6] = L[6] + b;
+        for (int isp = 1; isp <= LSP; ++isp) {
+          Pr[(isp - 1) * ni + k] = P[isp];
+          Lo[(isp - 1) * ni + k] = L[isp];
+        }
+#undef NN
+#undef S
+      }
+}
+
+/* ---- multifluid.f90:160-175, 472-495: srcsEnergy, then per species rhoes <- ETD over the
+   interior (electrons with Qeionize added to Pr) and Ts <- rhoe2T over the whole padded grid */
+void orc_source_loss_energy(const orc_grid *g, const orc_atmos *a, double dt, const double *ns,
+                            const double *vs1, const double *vs2, const double *vs3,
+                            double *Ts, double *rhoes) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  double *Pr = malloc(sizeof(double) * ni * LSP), *Lo = malloc(sizeof(double) * ni * LSP);
+  orc_srcs_energy(g, a, ns, vs1, vs2, vs3, Ts, Pr, Lo);
+  for (int isp = 1; isp <= LSP; ++isp)
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1) {
+          const long k = IIX(i1, i2, i3), c = (long)(isp - 1) * d.N + GIX(d, i1, i2, i3);
+          double P = Pr[(isp - 1) * ni + k];
+          if (isp == LSP) P = P + a->Qeionize[k];
+          rhoes[c] = orc_ETD_uncoupled(rhoes[c], P, Lo[(isp - 1) * ni + k], dt);
+        }
+  orc_rhoe2T(g, ns, rhoes, Ts);
+  free(Pr); free(Lo);
+}
+
+/* ---- multifluid.f90:144-157, 498-528: srcsMomentum, then rhovs1 <- ETD (ions, interior),
+   vs1 <- rhovs1/(ms max(ns, mindensdiv)) and the electron velocity and momentum from the
+   ion charge flux, over the whole padded grid */
+void orc_source_loss_momentum(const orc_grid *g, const orc_atmos *a, double dt, const double *ns,
+                              double *vs1, const double *vs2, const double *vs3, const double *Ts,
+                              const double *E1, const double *Q, double *rhovs1) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  double *Pr = malloc(sizeof(double) * ni * LSP), *Lo = malloc(sizeof(double) * ni * LSP);
+  orc_srcs_momentum(g, a, ns, vs1, vs2, vs3, Ts, E1, Q, Pr, Lo);
+  for (int isp = 1; isp <= LSP - 1; ++isp) {
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1) {
+          const long k = IIX(i1, i2, i3), c = (long)(isp - 1) * d.N + GIX(d, i1, i2, i3);
+          rhovs1[c] = orc_ETD_uncoupled(rhovs1[c], Pr[(isp - 1) * ni + k], Lo[(isp - 1) * ni + k], dt);
+        }
+    const long o = (long)(isp - 1) * d.N;
+    for (long k = 0; k < d.N; ++k) vs1[o + k] = rhovs1[o + k] / (ms_(isp) * fmax_f(ns[o + k], mindensdiv));
+  }
+  const long o7 = (long)(LSP - 1) * d.N;
+  for (long k = 0; k < d.N; ++k) {
+    double chrgflux = 0.0;
+    for (int isp = 1; isp <= LSP - 1; ++isp) {
+      const long o = (long)(isp - 1) * d.N + k;
+      chrgflux = chrgflux + (ns[o] * qs_(isp)) * vs1[o];
+    }
+    vs1[o7 + k] = -(((1.0 / fmax_f(ns[o7 + k], mindensdiv)) / qs_(LSP)) * chrgflux);
+    rhovs1[o7 + k] = (ns[o7 + k] * ms_(LSP)) * vs1[o7 + k];
+  }
+  free(Pr); free(Lo);
+}
+
+/* ---- multifluid.f90:130-141, 531-549: srcsContinuity, Pr(:,:,:,1:6) += Prionize, ns <-
+   ETD (ions, interior), ns(lsp) <- sum of the ions over the whole padded grid */
+void orc_source_loss_mass(const orc_grid *g, const orc_atmos *a, double dt, double *ns,
+                          const double *vs1, const double *vs2, const double *vs3,
+                          const double *Ts) {
+  dims_t d = dims_of(g);
+  const int lx1 = g->lx1, lx2 = g->lx2, lx3 = g->lx3;
+  const long ni = (long)lx1 * lx2 * lx3;
+  double *Pr = malloc(sizeof(double) * ni * LSP), *Lo = malloc(sizeof(double) * ni * LSP);
+  orc_srcs_continuity(g, a, ns, vs1, vs2, vs3, Ts, Pr, Lo);
+  for (int isp = 1; isp <= LSP - 1; ++isp)
+    for (int i3 = 1; i3 <= lx3; ++i3)
+      for (int i2 = 1; i2 <= lx2; ++i2)
+        for (int i1 = 1; i1 <= lx1; ++i1) {
+          const long k = IIX(i1, i2, i3), c = (long)(isp - 1) * d.N + GIX(d, i1, i2, i3);
+          const double P = Pr[(isp - 1) * ni + k] + a->Prionize[(isp - 1) * ni + k];
+          ns[c] = orc_ETD_uncoupled(ns[c], P, Lo[(isp - 1) * ni + k], dt);
+        }
+  const long o7 = (long)(LSP - 1) * d.N;
+  for (long k = 0; k < d.N; ++k) {
+    double s = 0.0;
+    for (int isp = 1; isp <= LSP - 1; ++isp) s = s + ns[(long)(isp - 1) * d.N + k];
+    ns[o7 + k] = s;
+  }
+  free(Pr); free(Lo);
+}
+
+/* ---- libgemini.cpp:223-232: the sources substep after the ionization rates are known:
+   global-boundary ghost fill, energy, momentum and mass source/loss, clean_param 3, 2, 1 */
+int orc_sources_block(const orc_grid *g, const orc_atmos *a, double dt, double *fluidvars,
+                      double *fluidauxvars, const double *electrovars, const double *vs1i,
+                      const double *Q, const int *nullpts) {
+  dims_t d = dims_of(g);
+  const long B = (long)LSP * d.N;
+  double *ns = fluidvars, *vs1 = fluidvars + B, *vs2 = fluidvars + 2 * B, *vs3 = fluidvars + 3 * B,
+         *Ts = fluidvars + 4 * B, *rhovs1 = fluidauxvars, *rhoes = fluidauxvars + B;
+  orc_set_global_boundaries_allspec(g, ns, rhovs1, vs1, vs2, vs3, rhoes, vs1i, LSP);
+  orc_source_loss_energy(g, a, dt, ns, vs1, vs2, vs3, Ts, rhoes);
+  orc_source_loss_momentum(g, a, dt, ns, vs1, vs2, vs3, Ts, electrovars, Q, rhovs1);
+  orc_source_loss_mass(g, a, dt, ns, vs1, vs2, vs3, Ts);
+  if (orc_clean_param(g, 3, Ts, nullpts) || orc_clean_param(g, 2, vs1, nullpts) ||
+      orc_clean_param(g, 1, ns, nullpts))
+    return -1;
+  return 0;
+}
+#undef IIX

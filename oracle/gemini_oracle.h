@@ -130,6 +130,39 @@ int orc_transport_step_ext(const orc_grid *g, double dt, double *fluidvars,
                            int diffsolvetype, double Teinf, const double *kappa,
                            const double *qtop);
 
+/* Source/loss step (multifluid.f90:130-175, 472-549, sources.f90, collisions.f90:58-159,
+   calculus.in.f90:174-196).  nn(lx1,lx2,lx3,6): O, N2, O2, H, N, NO; Tn, vn1-3, Qeionize,
+   g1 (lx1,lx2,lx3); Prionize (lx1,lx2,lx3,6); alt ghosted.  Pr, Lo (lx1,lx2,lx3,7). */
+typedef struct {
+  const double *nn, *Tn, *vn1, *vn2, *vn3, *Prionize, *Qeionize, *alt, *g1;
+} orc_atmos;
+double orc_ETD_uncoupled(double f, double P, double L, double dt);
+double orc_maxwell_colln(int isp, int isp2, double nn2, double Tn, double Tsp);
+void orc_coulomb_colln(int isp, int isp2, double ns2, double Ts1, double Ts2, double v1,
+                       double v2, double *out);
+void orc_grad3D1_interior(const orc_grid *g, const double *f, double *out);
+void orc_srcs_energy(const orc_grid *g, const orc_atmos *a, const double *ns, const double *vs1,
+                     const double *vs2, const double *vs3, const double *Ts, double *Pr,
+                     double *Lo);
+void orc_srcs_momentum(const orc_grid *g, const orc_atmos *a, const double *ns, const double *vs1,
+                       const double *vs2, const double *vs3, const double *Ts, const double *E1,
+                       const double *Q, double *Pr, double *Lo);
+void orc_srcs_continuity(const orc_grid *g, const orc_atmos *a, const double *ns,
+                         const double *vs1, const double *vs2, const double *vs3,
+                         const double *Ts, double *Pr, double *Lo);
+void orc_source_loss_energy(const orc_grid *g, const orc_atmos *a, double dt, const double *ns,
+                            const double *vs1, const double *vs2, const double *vs3,
+                            double *Ts, double *rhoes);
+void orc_source_loss_momentum(const orc_grid *g, const orc_atmos *a, double dt, const double *ns,
+                              double *vs1, const double *vs2, const double *vs3, const double *Ts,
+                              const double *E1, const double *Q, double *rhovs1);
+void orc_source_loss_mass(const orc_grid *g, const orc_atmos *a, double dt, double *ns,
+                          const double *vs1, const double *vs2, const double *vs3,
+                          const double *Ts);
+int orc_sources_block(const orc_grid *g, const orc_atmos *a, double dt, double *fluidvars,
+                      double *fluidauxvars, const double *electrovars, const double *vs1i,
+                      const double *Q, const int *nullpts);
+
 /* 0: x**2.5 by libm pow, as the reference (default); 1: correctly rounded x**2.5 */
 void orc_set_pow_mode(int mode);
 void orc_eval_pow25(const double *x, double *y, long n, int mode);
