@@ -42,6 +42,13 @@ WORKLOADS = {
     "block512": dict(kind="dipole", shape=(512, 256, 192), steps=50, block=True,
                      desc="closed dipole grid 512x256x192 (25.2M cells), 7 species, fp64, "
                           "whole fluid_adv transport block", sample=(512, 32, 16)),
+    # a whole time step: the transport block, then the source/loss substep (collisional and
+    # chemical sources with exponential time differencing, libgemini.cpp:223-232) with a
+    # synthetic neutral atmosphere and ionization rates
+    "step512": dict(kind="dipole", shape=(512, 256, 192), steps=30, block=True, sources=True,
+                    desc="closed dipole grid 512x256x192 (25.2M cells), 7 species, fp64, "
+                         "whole time step: fluid_adv transport block and source/loss substep",
+                    sample=(512, 32, 16)),
 }
 
 # Algorithmic HBM bytes per cell and launch: each array the kernel must read or write,
@@ -49,11 +56,18 @@ WORKLOADS = {
 # metric arrays read; TRBDF2: Ts, ns (7 species each), h1..h3, nn(1:3), J1 read, Ts
 # written; conversions over the padded grid.
 BYTES_PER_CELL = {"sweep1": 408, "sweep2": 408, "sweep3": 408, "trbdf2": 224,
-                  "pre_convert": 280, "post_convert": 272, "rhov12v1": 160, "rhoe2T": 168}
+                  "pre_convert": 280, "post_convert": 272, "rhov12v1": 160, "rhoe2T": 168,
+                  # sources, per interior cell: energy reads ns, rhoes, Ts, vs1-3 (7 each),
+                  # nn(1:4), Tn, vn1-3, Qeionize and writes rhoes, Ts; momentum reads ns, Ts,
+                  # vs1 (7), rhovs1, vs2, vs3, Q (6), nn(1:4), Tn, vn1, E1, alt, g1, h1-h3 and
+                  # writes rhovs1, vs1; mass reads ns (7), Ts (4), vs1-3 of O+, nn(1:6),
+                  # Tn, vn1-3, Prionize (6) and writes ns
+                  "source_energy": 520, "source_momentum": 568, "source_mass": 296}
 PADDED_KERNELS = {"pre_convert", "post_convert", "rhov12v1", "rhoe2T"}
 KERNELS = ["pre_convert", "ghost_fill", "halo_periodic", "save_v1b", "sweep3", "sweep1",
            "sweep2", "post_convert", "rhov12v1", "rhoe2T", "trbdf2", "floor_ghosts",
-           "clean_param", "artvisc", "rk2_boundary", "compression"]
+           "clean_param", "artvisc", "rk2_boundary", "compression", "source_energy",
+           "source_momentum", "source_mass"]
 
 
 ENSEMBLE = dict(members=64, steps=20, seed=1, verify=3,
@@ -99,7 +113,15 @@ def cpu_baseline(w, dt, seconds=15.0, block=20):
     oracle.set_num_threads(threads)
     g = make_grid(w["kind"], w["sample"])
     st0 = ga.synthetic_state(g, seed=1, perturb=(w["kind"] == "cart" and g.lx3 > 1))
-    if w.get("block"):
+    if w.get("sources"):
+        nul = null_cells(g)
+        atm = ga.synthetic_atmosphere(g, st0, seed=1)
+
+        def step(s):
+            Q = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F")
+            oracle.fluid_adv_block(g, s, dt, nullpts=nul, Q=Q)
+            oracle.sources_block(g, s, atm, dt, Q=Q, nullpts=nul)
+    elif w.get("block"):
         nul = null_cells(g)
         step = lambda s: oracle.fluid_adv_block(g, s, dt, nullpts=nul)
     else:
@@ -420,6 +442,13 @@ def main():
     tr = Transport(g, st["nn"], st["Tn"], device=0, geometry_on_device=True)
     tr.set_residency(1)
     tr.push_blocks(st["fluidvars"], st["aux"], st["electrovars"])
+    if w.get("sources"):
+        from gemini_amd.synth_gpu import synthetic_atmosphere_device
+        L1, L2, L3 = g.padded
+        ne = st["fluidvars"][6].view(L3, L2, L1)[2:-2, 2:-2, 2:-2]
+        atm = synthetic_atmosphere_device(g, st["nn"], st["Tn"], ne, seed=args.seed, device="cuda:0")
+        tr.enable_sources(atm)
+        del atm
     del st
     torch.cuda.empty_cache()
     step = tr.transport_step
@@ -430,6 +459,10 @@ def main():
         del nul
         step = tr.fluid_adv_block
         args.no_adjoint = True
+    if w.get("sources"):
+        def step(dt):
+            tr.fluid_adv_block(dt)
+            tr.sources_block(dt)
     if args.host_state:
         # the caller keeps fluidvars, fluidauxvars and J1 on the host (pinned); every step
         # crosses PCIe twice, as with residency mode 0 around a fused step
@@ -513,7 +546,9 @@ def main():
                    "species": 7, "dt_s": dt, "diffusion": "TRBDF2",
                    "step": ("v12rhov1,T2rhoe,ghost fill,sweep3,sweep1,sweep2,rhov12v1,"
                             "clean_param 1-2,VNRicht_artvisc,RK2 boundaries,compression,rhoe2T,"
-                            "clean_param 3,TRBDF2,clean_param 3,T2rhoe") if w.get("block") else
+                            "clean_param 3,TRBDF2,clean_param 3,T2rhoe" + (
+                                ",ghost fill,source_loss energy,momentum,mass,clean_param 3,2,1"
+                                if w.get("sources") else "")) if w.get("block") else
                            "v12rhov1,T2rhoe,ghost fill,sweep3,sweep1,sweep2,rhov12v1,rhoe2T,"
                            "TRBDF2", "state_finite_after_timing": finite},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),

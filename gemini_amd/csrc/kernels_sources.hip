@@ -8,16 +8,21 @@
 // every cell (ghosts included, as the reference's whole-array statements do) then converts
 // back to temperature / velocity / electron density.  The rates never touch HBM.
 //
-// Arithmetic follows the reference expression by expression (-ffp-contract=off, true
-// divides).  The transcendental functions (exp, log, log10, erf, tanh, sin, and x**p for
-// the fitted exponents) are the device library's, which agree with glibc to about an ulp,
-// so these passes match the reference to rounding rather than bit for bit; x**0.5 is the
-// correctly rounded sqrt and x**1.5 is rounded correctly in double-double (pow15).
+// Arithmetic follows the reference's expressions (-ffp-contract=off), with two liberties
+// that cost at most an ulp per operation: a divide by a compile-time constant (a mass sum,
+// 3) is a multiply by its correctly rounded reciprocal, and within a Coulomb pair 1/Teff**1.5
+// and 1/W are formed once.  The transcendental functions (exp, log, log10, erf, tanh, sin,
+// and x**p for the fitted exponents) are the device library's, which agree with glibc to
+// about an ulp; x**0.5 is the correctly rounded sqrt and x**1.5 is rounded correctly in
+// double-double (pow15).  These passes therefore match the reference to rounding (the tests
+// hold them to 1e-12 of each field's magnitude) rather than bit for bit.
 //
 // Ion-ion (Coulomb) terms: Teff, Wsj, Phisj and Psisj of coulomb_colln are symmetric in
 // the species pair, so each unordered pair is evaluated once and feeds both species.  The
 // pairs run in lexicographic order, which adds each species' terms in the reference's
-// isp2 order; self-collision terms are exact zeros and are skipped.
+// isp2 order; self-collision terms are exact zeros and are skipped.  The pair loop is
+// rolled (one copy of exp, erf and pow15 instead of 21, per-species values selected by a
+// wave-uniform index), which keeps the energy kernel within 256 registers.
 #include "gemhip_internal.h"
 
 namespace gemhip {
@@ -45,17 +50,6 @@ __device__ __forceinline__ double tab_c2sn2(int isp, int isp2) {
   constexpr float t[7][4] = {
       {0.064f, 0, 0, -1}, {0, 0, 0, 0}, {0, 0.069f, 0, 0}, {0, 0, 0.073f, 0},
       {0, 0, 0, 0}, {0.047f, 0, 0, 0.083f}, {-1, -1, -1, -1}};
-  return (double)t[isp - 1][isp2 - 1];
-}
-__device__ __forceinline__ double tab_csj(int isp, int isp2) {
-  constexpr float t[7][7] = {
-      {0.22f, 0.26f, 0.25f, 0.26f, 0.22f, 0.077f, 1.87e-3f},
-      {0.14f, 0.16f, 0.16f, 0.17f, 0.13f, 0.042f, 9.97e-4f},
-      {0.15f, 0.17f, 0.17f, 0.18f, 0.14f, 0.045f, 1.07e-3f},
-      {0.13f, 0.16f, 0.15f, 0.16f, 0.12f, 0.039f, 9.347e-4f},
-      {0.25f, 0.28f, 0.28f, 0.28f, 0.24f, 0.088f, 2.136e-3f},
-      {1.23f, 1.25f, 1.25f, 1.25f, 1.23f, 0.90f, 29.7e-3f},
-      {54.5f, 54.5f, 54.5f, 54.5f, 54.5f, 54.5f, 38.537f}};
   return (double)t[isp - 1][isp2 - 1];
 }
 // phys_consts.in.f90:60: mn = [16,28,32,1]*amu
@@ -96,20 +90,99 @@ __device__ __forceinline__ double maxwell(int isp, int isp2, double nn2, double 
   }
 }
 
-// the pair-symmetric part of coulomb_colln (collisions.f90:138-157) for species a < b:
-// Teff**1.5, Phisj, Psisj
-struct Pair { double T15, Phi, Psi; };
-__device__ __forceinline__ Pair coulomb_pair(int a, int b, double Ta, double Tb, double va,
-                                             double vb) {
-  const double ma = ms(a), mb = ms(b);
-  const double Teff = (mb * Ta + ma * Tb) / (mb + ma);
-  const double mred = (ma * mb) / (ma + mb);
-  const double W = fabs(va - vb) / sqrt(((2.0 * kB) * Teff) / mred);
+// Tables of the rolled pair loop.  The pair index is the same on every lane of a wave, so
+// these are scalar loads; each entry is the constant expression the unrolled form would
+// fold at compile time.  Pairs run (1,2), (1,3), .., (1,7), (2,3), .., (6,7).
+__constant__ int c_pair_a[21] = {1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 5, 5, 6};
+__constant__ int c_pair_b[21] = {2, 3, 4, 5, 6, 7, 3, 4, 5, 6, 7, 4, 5, 6, 7, 5, 6, 7, 6, 7, 7};
+__constant__ double c_ms[7] = {
+    16.0 * amu, 30.0 * amu, 28.0 * amu, 32.0 * amu,
+    14.0 * amu, 1.0 * amu, 5.485799090e-4 * amu};
+__constant__ double c_ms3[7] = {
+    (16.0 * amu) / 3.0, (30.0 * amu) / 3.0, (28.0 * amu) / 3.0,
+    (32.0 * amu) / 3.0, (14.0 * amu) / 3.0, (1.0 * amu) / 3.0,
+    (5.485799090e-4 * amu) / 3.0};
+__constant__ double c_nkB_igm1[7] = {
+    kB * (1.0 / (5.0 / 3.0 - 1.0)), kB * (1.0 / (7.0 / 5.0 - 1.0)),
+    kB * (1.0 / (7.0 / 5.0 - 1.0)), kB * (1.0 / (7.0 / 5.0 - 1.0)),
+    kB * (1.0 / (5.0 / 3.0 - 1.0)), kB * (1.0 / (5.0 / 3.0 - 1.0)),
+    kB * (1.0 / (5.0 / 3.0 - 1.0))};
+__constant__ double c_igm1[7] = {
+    1.0 / (5.0 / 3.0 - 1.0), 1.0 / (7.0 / 5.0 - 1.0), 1.0 / (7.0 / 5.0 - 1.0),
+    1.0 / (7.0 / 5.0 - 1.0), 1.0 / (5.0 / 3.0 - 1.0), 1.0 / (5.0 / 3.0 - 1.0),
+    1.0 / (5.0 / 3.0 - 1.0)};
+__constant__ double c_two_msum[21] = {
+    2.0 / ((16.0 * amu) + (30.0 * amu)), 2.0 / ((16.0 * amu) + (28.0 * amu)),
+    2.0 / ((16.0 * amu) + (32.0 * amu)), 2.0 / ((16.0 * amu) + (14.0 * amu)),
+    2.0 / ((16.0 * amu) + (1.0 * amu)), 2.0 / ((16.0 * amu) + (5.485799090e-4 * amu)),
+    2.0 / ((30.0 * amu) + (28.0 * amu)), 2.0 / ((30.0 * amu) + (32.0 * amu)),
+    2.0 / ((30.0 * amu) + (14.0 * amu)), 2.0 / ((30.0 * amu) + (1.0 * amu)),
+    2.0 / ((30.0 * amu) + (5.485799090e-4 * amu)), 2.0 / ((28.0 * amu) + (32.0 * amu)),
+    2.0 / ((28.0 * amu) + (14.0 * amu)), 2.0 / ((28.0 * amu) + (1.0 * amu)),
+    2.0 / ((28.0 * amu) + (5.485799090e-4 * amu)), 2.0 / ((32.0 * amu) + (14.0 * amu)),
+    2.0 / ((32.0 * amu) + (1.0 * amu)), 2.0 / ((32.0 * amu) + (5.485799090e-4 * amu)),
+    2.0 / ((14.0 * amu) + (1.0 * amu)), 2.0 / ((14.0 * amu) + (5.485799090e-4 * amu)),
+    2.0 / ((1.0 * amu) + (5.485799090e-4 * amu))};
+__constant__ double c_inv_msum[21] = {
+    1.0 / ((30.0 * amu) + (16.0 * amu)), 1.0 / ((28.0 * amu) + (16.0 * amu)),
+    1.0 / ((32.0 * amu) + (16.0 * amu)), 1.0 / ((14.0 * amu) + (16.0 * amu)),
+    1.0 / ((1.0 * amu) + (16.0 * amu)), 1.0 / ((5.485799090e-4 * amu) + (16.0 * amu)),
+    1.0 / ((28.0 * amu) + (30.0 * amu)), 1.0 / ((32.0 * amu) + (30.0 * amu)),
+    1.0 / ((14.0 * amu) + (30.0 * amu)), 1.0 / ((1.0 * amu) + (30.0 * amu)),
+    1.0 / ((5.485799090e-4 * amu) + (30.0 * amu)), 1.0 / ((32.0 * amu) + (28.0 * amu)),
+    1.0 / ((14.0 * amu) + (28.0 * amu)), 1.0 / ((1.0 * amu) + (28.0 * amu)),
+    1.0 / ((5.485799090e-4 * amu) + (28.0 * amu)), 1.0 / ((14.0 * amu) + (32.0 * amu)),
+    1.0 / ((1.0 * amu) + (32.0 * amu)), 1.0 / ((5.485799090e-4 * amu) + (32.0 * amu)),
+    1.0 / ((1.0 * amu) + (14.0 * amu)), 1.0 / ((5.485799090e-4 * amu) + (14.0 * amu)),
+    1.0 / ((5.485799090e-4 * amu) + (1.0 * amu))};
+__constant__ double c_2kB_mred[21] = {
+    (2.0 * kB) / (((16.0 * amu) * (30.0 * amu)) / ((16.0 * amu) + (30.0 * amu))),
+    (2.0 * kB) / (((16.0 * amu) * (28.0 * amu)) / ((16.0 * amu) + (28.0 * amu))),
+    (2.0 * kB) / (((16.0 * amu) * (32.0 * amu)) / ((16.0 * amu) + (32.0 * amu))),
+    (2.0 * kB) / (((16.0 * amu) * (14.0 * amu)) / ((16.0 * amu) + (14.0 * amu))),
+    (2.0 * kB) / (((16.0 * amu) * (1.0 * amu)) / ((16.0 * amu) + (1.0 * amu))),
+    (2.0 * kB) / (((16.0 * amu) * (5.485799090e-4 * amu)) / ((16.0 * amu) + (5.485799090e-4 * amu))),
+    (2.0 * kB) / (((30.0 * amu) * (28.0 * amu)) / ((30.0 * amu) + (28.0 * amu))),
+    (2.0 * kB) / (((30.0 * amu) * (32.0 * amu)) / ((30.0 * amu) + (32.0 * amu))),
+    (2.0 * kB) / (((30.0 * amu) * (14.0 * amu)) / ((30.0 * amu) + (14.0 * amu))),
+    (2.0 * kB) / (((30.0 * amu) * (1.0 * amu)) / ((30.0 * amu) + (1.0 * amu))),
+    (2.0 * kB) / (((30.0 * amu) * (5.485799090e-4 * amu)) / ((30.0 * amu) + (5.485799090e-4 * amu))),
+    (2.0 * kB) / (((28.0 * amu) * (32.0 * amu)) / ((28.0 * amu) + (32.0 * amu))),
+    (2.0 * kB) / (((28.0 * amu) * (14.0 * amu)) / ((28.0 * amu) + (14.0 * amu))),
+    (2.0 * kB) / (((28.0 * amu) * (1.0 * amu)) / ((28.0 * amu) + (1.0 * amu))),
+    (2.0 * kB) / (((28.0 * amu) * (5.485799090e-4 * amu)) / ((28.0 * amu) + (5.485799090e-4 * amu))),
+    (2.0 * kB) / (((32.0 * amu) * (14.0 * amu)) / ((32.0 * amu) + (14.0 * amu))),
+    (2.0 * kB) / (((32.0 * amu) * (1.0 * amu)) / ((32.0 * amu) + (1.0 * amu))),
+    (2.0 * kB) / (((32.0 * amu) * (5.485799090e-4 * amu)) / ((32.0 * amu) + (5.485799090e-4 * amu))),
+    (2.0 * kB) / (((14.0 * amu) * (1.0 * amu)) / ((14.0 * amu) + (1.0 * amu))),
+    (2.0 * kB) / (((14.0 * amu) * (5.485799090e-4 * amu)) / ((14.0 * amu) + (5.485799090e-4 * amu))),
+    (2.0 * kB) / (((1.0 * amu) * (5.485799090e-4 * amu)) / ((1.0 * amu) + (5.485799090e-4 * amu)))};
+__constant__ double c_csj_1e6[7][7] = {
+    {(double)0.22f * 1e-6, (double)0.26f * 1e-6, (double)0.25f * 1e-6, (double)0.26f * 1e-6, (double)0.22f * 1e-6, (double)0.077f * 1e-6, (double)1.87e-3f * 1e-6},
+    {(double)0.14f * 1e-6, (double)0.16f * 1e-6, (double)0.16f * 1e-6, (double)0.17f * 1e-6, (double)0.13f * 1e-6, (double)0.042f * 1e-6, (double)9.97e-4f * 1e-6},
+    {(double)0.15f * 1e-6, (double)0.17f * 1e-6, (double)0.17f * 1e-6, (double)0.18f * 1e-6, (double)0.14f * 1e-6, (double)0.045f * 1e-6, (double)1.07e-3f * 1e-6},
+    {(double)0.13f * 1e-6, (double)0.16f * 1e-6, (double)0.15f * 1e-6, (double)0.16f * 1e-6, (double)0.12f * 1e-6, (double)0.039f * 1e-6, (double)9.347e-4f * 1e-6},
+    {(double)0.25f * 1e-6, (double)0.28f * 1e-6, (double)0.28f * 1e-6, (double)0.28f * 1e-6, (double)0.24f * 1e-6, (double)0.088f * 1e-6, (double)2.136e-3f * 1e-6},
+    {(double)1.23f * 1e-6, (double)1.25f * 1e-6, (double)1.25f * 1e-6, (double)1.25f * 1e-6, (double)1.23f * 1e-6, (double)0.90f * 1e-6, (double)29.7e-3f * 1e-6},
+    {(double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)38.537f * 1e-6}};
+
+// the pair-symmetric part of coulomb_colln (collisions.f90:138-157) for pair pr = (a, b),
+// a < b: 1/Teff**1.5, Phisj, Psisj
+struct Pair { double iT15, Phi, Psi; };
+__device__ __forceinline__ Pair coulomb_pair(int pr, double Ta, double Tb, double va, double vb) {
+  const int a = c_pair_a[pr], b = c_pair_b[pr];
+  const double ma = c_ms[a - 1], mb = c_ms[b - 1];
+  const double Teff = (mb * Ta + ma * Tb) * c_inv_msum[pr];
+  const double W = fabs(va - vb) / sqrt(Teff * c_2kB_mred[pr]);
   Pair p;
-  p.T15 = pow15(Teff);
+  p.iT15 = 1.0 / pow15(Teff);
   p.Psi = exp(-(W * W));
-  p.Phi = W < 0.1 ? 1.0
-                  : (((3.0 / 4.0) * sqrt(PI)) * erf(W)) / ((W * W) * W) - ((3.0 / 2.0) / (W * W)) * p.Psi;
+  if (W < 0.1) {
+    p.Phi = 1.0;
+  } else {
+    const double iW = 1.0 / W, iW2 = iW * iW;
+    p.Phi = (((3.0 / 4.0) * sqrt(PI)) * erf(W)) * (iW2 * iW) - ((3.0 / 2.0) * iW2) * p.Psi;
+  }
   return p;
 }
 
@@ -160,41 +233,40 @@ __global__ __launch_bounds__(TPB) void k_src_energy(DevGrid g, DevState s, DevSo
       const double m = ms(isp), gm1 = gammas(isp) - 1.0, nm = n[isp - 1] * m;
       const double d1 = v1[isp - 1] - vn1, d2 = v2[isp - 1] - vn2, d3 = v3[isp - 1] - vn3;
       const double dv2 = ((d1 * d1) + (d2 * d2)) + (d3 * d3);
+      const double heat = (nm * kB) * (1.0 / gm1), fric = (nm * (1.0 / gm1)) * dv2;
       double Pp = 0.0, Ll = 0.0;
 #pragma unroll
       for (int isp2 = 1; isp2 <= 4; ++isp2) {
         const double nu = maxwell(isp, isp2, nnk[isp2 - 1], Tn, T[isp - 1]);
-        double fact = (2.0 * nu) / (m + mn(isp2));
-        Pp = Pp + (((nm * kB) / gm1) * fact) * Tn;
+        double fact = nu * (2.0 / (m + mn(isp2)));
+        Pp = Pp + (heat * fact) * Tn;
         Ll = Ll + m * fact;
-        fact = (fact * mn(isp2)) / 3.0;
-        Pp = Pp + ((nm / gm1) * dv2) * fact;
+        fact = fact * (mn(isp2) / 3.0);
+        Pp = Pp + fric * fact;
       }
       P[isp - 1] = Pp;
       L[isp - 1] = Ll;
     }
-    // ion-ion and ion-electron (Coulomb)
+    // ion-ion and ion-electron (Coulomb), one unordered pair per iteration
+#pragma unroll 1
+    for (int pr = 0; pr < 21; ++pr) {
+      const int ia = c_pair_a[pr], ib = c_pair_b[pr];
+      const Pair p = coulomb_pair(pr, T[ia - 1], T[ib - 1], v1[ia - 1], v1[ib - 1]);
+      const double d1 = v1[ia - 1] - v1[ib - 1], d2 = v2[ia - 1] - v2[ib - 1],
+                   d3 = v3[ia - 1] - v3[ib - 1];
+      const double dv2 = ((d1 * d1) + (d2 * d2)) + (d3 * d3);
+      const double psi2 = p.Psi * c_two_msum[pr], phi2 = p.Phi * c_two_msum[pr];
+      // species ia with partner ib, then ib with partner ia
 #pragma unroll
-    for (int ia = 1; ia <= LSP; ++ia) {
-#pragma unroll
-      for (int ib = ia + 1; ib <= LSP; ++ib) {
-        const Pair p = coulomb_pair(ia, ib, T[ia - 1], T[ib - 1], v1[ia - 1], v1[ib - 1]);
-        const double d1 = v1[ia - 1] - v1[ib - 1], d2 = v2[ia - 1] - v2[ib - 1],
-                     d3 = v3[ia - 1] - v3[ib - 1];
-        const double dv2 = ((d1 * d1) + (d2 * d2)) + (d3 * d3);
-        const double msum = ms(ia) + ms(ib);
-        // species ia with partner ib, then ib with partner ia
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-          const int x = side ? ib : ia, y = side ? ia : ib;
-          const double m = ms(x), gm1 = gammas(x) - 1.0, nm = n[x - 1] * m;
-          const double nu = ((tab_csj(x, y) * n[y - 1]) * 1e-6) / p.T15;
-          double fact = ((2.0 * nu) * p.Psi) / msum;
-          P[x - 1] = P[x - 1] + (((nm * kB) / gm1) * fact) * T[y - 1];
-          L[x - 1] = L[x - 1] + m * fact;
-          fact = ((((2.0 * nu) * p.Phi) / msum) * ms(y)) / 3.0;
-          P[x - 1] = P[x - 1] + ((nm / gm1) * dv2) * fact;
-        }
+      for (int side = 0; side < 2; ++side) {
+        const int x = side ? ib : ia, y = side ? ia : ib;
+        const double nm = n[x - 1] * c_ms[x - 1];
+        const double nu = (c_csj_1e6[x - 1][y - 1] * n[y - 1]) * p.iT15;
+        double fact = nu * psi2;
+        P[x - 1] = P[x - 1] + ((nm * c_nkB_igm1[x - 1]) * fact) * T[y - 1];
+        L[x - 1] = L[x - 1] + c_ms[x - 1] * fact;
+        fact = (nu * phi2) * c_ms3[y - 1];
+        P[x - 1] = P[x - 1] + ((nm * c_igm1[x - 1]) * dv2) * fact;
       }
     }
     // inelastic electron-neutral collisions (sources.f90:350-373)
@@ -233,21 +305,34 @@ __global__ __launch_bounds__(TPB) void k_src_energy(DevGrid g, DevState s, DevSo
 
 // d(log p)/dx1 / h1 of species isp at interior cell c (gradient.f90:136-146): centred over
 // the ghost cells when the subdomain touches neither x1 end, otherwise over the interior
-// with one-sided differences at i1 = 1 and lx1 (sources.f90:259-270)
-__device__ __forceinline__ double dlogp(const DevGrid& g, const DevState& s, int isp, long c,
-                                        int i1, double lp0, bool ghosted) {
+// with one-sided differences at i1 = 1 and lx1 (sources.f90:259-270).  lp holds log p of
+// the block's cells (x1 neighbours are neighbouring threads); cells outside the block are
+// evaluated directly, with the same expression.
+__device__ __forceinline__ double dlogp(const DevGrid& g, const DevState& s, const double* lp,
+                                        int isp, long c, int t, int i1, bool ghosted) {
   const long o = (long)(isp - 1) * g.N;
-  auto lp = [&](long cc) { return log((s.ns[o + cc] * kB) * s.Ts[o + cc]); };
+  auto at = [&](int dt_) -> double {
+    const int u = t + dt_;
+    if (u >= 0 && u < TPB) return lp[(isp - 1) * TPB + u];
+    return log((s.ns[o + c + dt_] * kB) * s.Ts[o + c + dt_]);
+  };
   if (!ghosted) {
-    if (i1 == 1) return ((lp(c + 1) - lp0) / g.dx1[1]) / g.h1[c];
-    if (i1 == g.lx1) return ((lp0 - lp(c - 1)) / g.dx1[g.lx1]) / g.h1[c];
+    if (i1 == 1) return ((at(1) - at(0)) / g.dx1[1]) / g.h1[c];
+    if (i1 == g.lx1) return ((at(0) - at(-1)) / g.dx1[g.lx1]) / g.h1[c];
   }
-  return ((lp(c + 1) - lp(c - 1)) / (g.dx1[i1 + 1] + g.dx1[i1])) / g.h1[c];
+  return ((at(1) - at(-1)) / (g.dx1[i1 + 1] + g.dx1[i1])) / g.h1[c];
 }
 
 // ---- source_loss_momentum: srcsMomentum_curv (sources.f90:228-289), ETD, velocities ----
 __global__ __launch_bounds__(TPB) void k_src_momentum(DevGrid g, DevState s, DevSources a, double dt) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double lp[LSP * TPB];
+  const int t = threadIdx.x;
+  const long c = (long)blockIdx.x * blockDim.x + t;
+  if (c < g.N) {
+#pragma unroll
+    for (int i = 0; i < LSP; ++i) lp[i * TPB + t] = log((s.ns[i * g.N + c] * kB) * s.Ts[i * g.N + c]);
+  }
+  __syncthreads();
   if (c >= g.N) return;
   const Cell q = decode(g, c);
   double n[LSP], r[LSP - 1];
@@ -283,18 +368,16 @@ __global__ __launch_bounds__(TPB) void k_src_momentum(DevGrid g, DevState s, Dev
       L[isp - 1] = Ll;
     }
 #pragma unroll
-    for (int ia = 1; ia <= LSP - 1; ++ia) {
+    for (int pr = 0; pr < 21; ++pr) {
+      const int ia = c_pair_a[pr], ib = c_pair_b[pr];
+      const Pair p = coulomb_pair(pr, T[ia - 1], T[ib - 1], v1[ia - 1], v1[ib - 1]);
 #pragma unroll
-      for (int ib = ia + 1; ib <= LSP; ++ib) {
-        const Pair p = coulomb_pair(ia, ib, T[ia - 1], T[ib - 1], v1[ia - 1], v1[ib - 1]);
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-          const int x = side ? ib : ia, y = side ? ia : ib;
-          if (x == LSP) continue;
-          const double nu = ((tab_csj(x, y) * n[y - 1]) * 1e-6) / p.T15;
-          L[x - 1] = L[x - 1] + nu * p.Phi;
-          P[x - 1] = P[x - 1] + (((n[x - 1] * ms(x)) * nu) * p.Phi) * v1[y - 1];
-        }
+      for (int side = 0; side < 2; ++side) {
+        const int x = side ? ib : ia, y = side ? ia : ib;
+        if (x == LSP) continue;
+        const double nu = (c_csj_1e6[x - 1][y - 1] * n[y - 1]) * p.iT15;
+        L[x - 1] = L[x - 1] + nu * p.Phi;
+        P[x - 1] = P[x - 1] + (((n[x - 1] * c_ms[x - 1]) * nu) * p.Phi) * v1[y - 1];
       }
     }
     // geometric factors, ambipolar field, gravity (sources.f90:248-288)
@@ -304,15 +387,14 @@ __global__ __launch_bounds__(TPB) void k_src_momentum(DevGrid g, DevState s, Dev
     const double dh2dx1 = ((g.h2[c + 1] - g.h2[c - 1]) / dsum) / h1;
     const double dh3dx1 = ((g.h3[c + 1] - g.h3[c - 1]) / dsum) / h1;
     const double Te = T[LSP - 1];
-    const double lpe = log((n[LSP - 1] * kB) * Te);
-    const double Epol1 = ((kB * Te) / qs(LSP)) * dlogp(g, s, LSP, c, i1, lpe, ghosted);
+    const double Epol1 = ((kB * Te) / qs(LSP)) * dlogp(g, s, lp, LSP, c, t, i1, ghosted);
     const double E1f = a.alt[c] < 100e3 ? 0.0 : a.E1[c];
     const double g1 = a.g1[k];
 #pragma unroll
     for (int isp = 1; isp <= LSP - 1; ++isp) {
       const double nsp = n[isp - 1], m = ms(isp);
       const double pr = (nsp * kB) * T[isp - 1];
-      const double ionpress = pr * dlogp(g, s, isp, c, i1, log(pr), ghosted);
+      const double ionpress = pr * dlogp(g, s, lp, isp, c, t, i1, ghosted);
       const double* Qs = a.Q + (long)(isp - 1) * ni;
       double gQ;
       if (i1 == 1) gQ = ((Qs[k + 1] - Qs[k]) / g.dx1[1]) / h1;

@@ -263,7 +263,8 @@ def synthetic_atmosphere(g: Grid, st: State, seed: int = 0) -> Atmosphere:
     from .synth_gpu import synthetic_atmosphere_device
     nn3 = torch.as_tensor(np.ascontiguousarray(st.nn[..., :3].transpose(3, 2, 1, 0)))
     Tn = torch.as_tensor(np.ascontiguousarray(st.Tn.transpose(2, 1, 0)))
-    d = synthetic_atmosphere_device(g, nn3, Tn, seed=seed, device="cpu")
+    ne = torch.as_tensor(np.ascontiguousarray(st.fluidvars[2:-2, 2:-2, 2:-2, 6].transpose(2, 1, 0)))
+    d = synthetic_atmosphere_device(g, nn3, Tn, ne, seed=seed, device="cpu")
     F = lambda t: np.asfortranarray(t.numpy().transpose(*range(t.dim() - 1, -1, -1)))
     return Atmosphere(F(d["nn"]), F(d["Tn"]), F(d["vn1"]), F(d["vn2"]), F(d["vn3"]),
                       F(d["Prionize"]), F(d["Qeionize"]))

@@ -137,14 +137,16 @@ def cfl_dt_device(g: Grid, st, tcfl: float = 0.9, dtmax: float = 0.5) -> float:
     return min(dtmax, tcfl / rate) if rate > 0 else dtmax
 
 
-def synthetic_atmosphere_device(g: Grid, nn3, Tn, seed: int = 0, device="cuda"):
+def synthetic_atmosphere_device(g: Grid, nn3, Tn, ne, seed: int = 0, device="cuda"):
     """Neutral atmosphere and ionization rates for the source/loss step, as torch tensors.
 
     nn3 (3, lx3, lx2, lx1) and Tn (lx3, lx2, lx1) are the state's O, N2, O2 densities and
     neutral temperature (the ones the energy diffusion uses); H, N and NO are added as
     exponential profiles, winds are smooth fields of tens of m/s and the ionization rates
     a Chapman production layer (floored at 1e-5 m^-3 s^-1 as the reference floors them)
-    with electron heating proportional to it.  Returns nn (6, lx3, lx2, lx1): O, N2, O2,
+    with the Swartz and Nisbet (1973) thermal-electron heating the reference's eheating
+    (ionization.f90:281-306) gives for it at electron density ne (lx3, lx2, lx1, the
+    interior cells).  Returns nn (6, lx3, lx2, lx1): O, N2, O2,
     H, N, NO; Tn, vn1, vn2, vn3, Qeionize (lx3, lx2, lx1); Prionize (6, lx3, lx2, lx1):
     C-order bytes of the reference's Fortran intvars arrays.
     """
@@ -171,6 +173,9 @@ def synthetic_atmosphere_device(g: Grid, nn3, Tn, seed: int = 0, device="cuda"):
     for k, frac in enumerate((0.45, 0.15, 0.25, 0.12, 0.03, 1e-4)):
         noise = torch.randn(sh, generator=gen, device=dev, dtype=f64)
         Prionize[k].copy_((noise.mul_(0.05).exp_() * (frac * q)).clamp_min(1e-5))
-    Qeionize = Prionize[0:6].sum(0) * 2.0e-18
+    R = torch.log(torch.as_tensor(ne, device=dev).reshape(sh) / (nn[1] + nn[2] + 0.1 * nn[0]))
+    avgenergy = torch.exp(-(12.75 + 6.941 * R + 1.166 * R ** 2 + 0.08034 * R ** 3 + 0.001996 * R ** 4))
+    Qeionize = (1.60217662e-19 * avgenergy) * Prionize.sum(0)
+    del R, avgenergy
     return dict(nn=nn, Tn=torch.as_tensor(Tn, device=dev).reshape(sh).contiguous(),
                 vn1=winds[0], vn2=winds[1], vn3=winds[2], Prionize=Prionize, Qeionize=Qeionize)
