@@ -146,6 +146,40 @@ def cpu_baseline(w, dt, seconds=15.0, block=20):
                       f"from the seeded state, after 1 warm-up step"}
 
 
+def measure_whole_step(tr, g, dt, atm, steps=10, warmup=2):
+    """The whole time step -- the fluid_adv transport block, then the source/loss substep --
+    on the resident state of the headline run (same grid, HBM-resident inputs)."""
+    tr.register_null_cells(null_cells(g))
+    tr.enable_sources(atm)
+
+    def step():
+        tr.fluid_adv_block(dt)
+        tr.sources_block(dt)
+    for _ in range(warmup):
+        step()
+    tr.synchronize()
+    tr.kernel_timing_reset()
+    tr.kernel_timing(True)
+    tr.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    tr.synchronize()
+    wall = time.perf_counter() - t0
+    tr.kernel_timing(False)
+    ms = 1e3 * wall / steps
+    kern = {}
+    for name in ("source_energy", "source_momentum", "source_mass", "trbdf2"):
+        tot, n = tr.kernel_time(name)
+        if n:
+            kern[name] = round(tot / n, 4)
+    return {"workload": WORKLOADS["step512"]["desc"], "steps": steps, "warmup": warmup,
+            "ms_per_step": round(ms, 4), "value": g.ncells * 7 / (ms / 1e3), "unit": UNIT,
+            "sources_ms": round(sum(kern.get(k, 0.0) for k in
+                                    ("source_energy", "source_momentum", "source_mass")), 4),
+            "kernel_ms": kern, "state_finite_after_timing": state_is_finite(tr, g)}
+
+
 def state_is_finite(tr, g):
     """Copy the state blocks back into HBM tensors (after timing) and check them."""
     import torch
@@ -377,6 +411,8 @@ def main():
                     help="central-difference directions checked by the adjoint measurement "
                          "(default 4 in the default run, 32 in the adjoint workload)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-whole-step", action="store_true",
+                    help="skip the whole-step (transport block + sources) figure of the default run")
     ap.add_argument("--host-state", action="store_true",
                     help="PCIe-inclusive rate: the state lives in pinned host memory and each "
                          "timed step pushes it to HBM, steps and pulls it back")
@@ -442,13 +478,16 @@ def main():
     tr = Transport(g, st["nn"], st["Tn"], device=0, geometry_on_device=True)
     tr.set_residency(1)
     tr.push_blocks(st["fluidvars"], st["aux"], st["electrovars"])
-    if w.get("sources"):
+    whole_step = (args.workload == "dipole512" and not args.host_state and not args.no_whole_step)
+    atm = None
+    if w.get("sources") or whole_step:
         from gemini_amd.synth_gpu import synthetic_atmosphere_device
         L1, L2, L3 = g.padded
         ne = st["fluidvars"][6].view(L3, L2, L1)[2:-2, 2:-2, 2:-2]
         atm = synthetic_atmosphere_device(g, st["nn"], st["Tn"], ne, seed=args.seed, device="cuda:0")
+    if w.get("sources"):
         tr.enable_sources(atm)
-        del atm
+        atm = None
     del st
     torch.cuda.empty_cache()
     step = tr.transport_step
@@ -519,6 +558,12 @@ def main():
         f"{kern[dom]['ms']:.3f} ms ({achieved:.0f} GB/s)")
     # the TRBDF2 launch is one kernel for all species; report its step share as well
     step_bytes = 1448 * cells
+    whole = None
+    if atm is not None:
+        log("whole time step: transport block + source/loss substep")
+        whole = measure_whole_step(tr, g, dt, atm)
+        del atm
+        log(f"whole step {whole['ms_per_step']:.3f} ms, sources {whole['sources_ms']:.3f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle on host cores)")
@@ -560,6 +605,7 @@ def main():
                      "frac": round(step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kern,
         "cpu_baseline": cpu,
+        "whole_step": whole,
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": None if adj is None else dict(adj, workload=ADJOINT["desc"]),
     }
