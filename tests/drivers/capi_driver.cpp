@@ -4,7 +4,9 @@
 // program: reads a case written by the test suite (case.txt, in.bin), runs nsteps steps
 // and writes out.bin.  mode 0 calls the entry points one by one on host blocks; mode 1
 // pushes the state once and runs gemhip_transport_step on HBM; modes 2 and 3 do the same
-// for the whole transport block (src/libgemini.cpp:155-210, gemhip_fluid_adv_block).
+// for the whole transport block (src/libgemini.cpp:155-210, gemhip_fluid_adv_block);
+// modes 4 and 5 follow the block with the sources substep (:223-232), call by call or
+// fused (gemhip_sources_block).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -59,6 +61,12 @@ int main(int argc, char** argv) {
   auto nn = take(f, 3 * ni), Tn = take(f, ni);
   auto fluid = take(f, 35 * L), aux = take(f, 23 * L), electro = take(f, 7 * L);
   auto vs1i = take(f, 7 * n1), vs2i = take(f, 7 * n2), vs3i = take(f, 7 * n3);
+  const bool sources = mode >= 4;
+  std::vector<double> nn6, vn1, vn2, vn3, Prion, Qeion, alt, g1;
+  if (sources) {
+    nn6 = take(f, 6 * ni); vn1 = take(f, ni); vn2 = take(f, ni); vn3 = take(f, ni);
+    Prion = take(f, 6 * ni); Qeion = take(f, ni); alt = take(f, L); g1 = take(f, ni);
+  }
   fclose(f);
 
   check(gemhip_init(0), "gemhip_init");
@@ -73,6 +81,12 @@ int main(int argc, char** argv) {
   check(gemhip_register_neutrals(nn.data(), Tn.data()), "gemhip_register_neutrals");
   check(gemhip_register_config(solver, Teinf), "gemhip_register_config");
   check(gemhip_set_interface_vels(vs1i.data(), vs2i.data(), vs3i.data()), "set_interface_vels");
+  if (sources) {
+    check(gemhip_register_atmosphere(nn6.data(), Tn.data(), vn1.data(), vn2.data(), vn3.data()),
+          "gemhip_register_atmosphere");
+    check(gemhip_register_gravity(alt.data(), g1.data()), "gemhip_register_gravity");
+    check(gemhip_set_ionization_rates(Prion.data(), Qeion.data()), "set_ionization_rates");
+  }
 
   double* fluidvars = fluid.data();
   double* fluidauxvars = aux.data();
@@ -82,13 +96,15 @@ int main(int argc, char** argv) {
   void* cfgC = nullptr;
   int lsp = 7;
   int one = 1, two = 2, three = 3;
-  if (mode == 1 || mode == 3) {
+  if (mode == 1 || mode == 3 || mode == 5) {
     check(gemhip_set_residency(1), "set_residency");
     check(gemhip_state_push(fluidvars, fluidauxvars, electrovars), "state_push");
-    for (int k = 0; k < nsteps; ++k)
+    for (int k = 0; k < nsteps; ++k) {
       check(mode == 1 ? gemhip_transport_step(dt) : gemhip_fluid_adv_block(dt), "fused step");
+      if (mode == 5) check(gemhip_sources_block(dt), "sources block");
+    }
     check(gemhip_state_pull(fluidvars, fluidauxvars), "state_pull");
-  } else if (mode == 2) {
+  } else if (mode == 2 || mode == 4) {
     for (int k = 0; k < nsteps; ++k) {
       // src/libgemini.cpp:155-210, the whole transport block
       v12rhov1_C(&fluidvars, &fluidauxvars);
@@ -112,6 +128,16 @@ int main(int argc, char** argv) {
       energy_diffusion_C(&cfgC, &xtype, &xC, &fluidvars, &electrovars, &intvars, &dt);
       clean_param_C(&three, &xtype, &xC, &fluidvars);
       T2rhoe_C(&fluidvars, &fluidauxvars);
+      if (mode == 4) {
+        // src/libgemini.cpp:223-232, the sources substep
+        set_global_boundaries_allspec_C(&xtype, &xC, &fluidvars, &fluidauxvars, &intvars, &lsp);
+        source_loss_energy_C(&fluidvars, &fluidauxvars, &electrovars, &intvars, &xtype, &xC, &dt);
+        source_loss_momentum_C(&fluidvars, &fluidauxvars, &electrovars, &intvars, &xtype, &xC, &dt);
+        source_loss_mass_C(&fluidvars, &fluidauxvars, &electrovars, &intvars, &xtype, &xC, &dt);
+        clean_param_C(&three, &xtype, &xC, &fluidvars);
+        clean_param_C(&two, &xtype, &xC, &fluidvars);
+        clean_param_C(&one, &xtype, &xC, &fluidvars);
+      }
       if (gemhip_last_error()[0]) check(1, "step");
     }
     check(gemhip_get_interface_vels(vs1i.data(), vs2i.data(), vs3i.data()), "get_interface_vels");

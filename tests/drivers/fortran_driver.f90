@@ -7,6 +7,9 @@
 !>         (residency 0: each call copies to HBM, runs and copies back);
 !> mode 1: the state is pushed once, gemhip_transport_step runs each step on HBM, and the
 !>         state is pulled at the end.
+!> modes 2 and 3: the same for the whole transport block (src/libgemini.cpp:155-210);
+!> modes 4 and 5: the block followed by the sources substep (:223-232), call by call
+!>         or fused (gemhip_sources_block).
 program fortran_driver
   use, intrinsic :: iso_c_binding
   use gemhip_bind
@@ -21,6 +24,8 @@ program fortran_driver
   real(c_double), allocatable :: nn(:,:,:,:), Tn(:,:,:)
   real(c_double), allocatable, target :: fluidvars(:,:,:,:), fluidauxvars(:,:,:,:), electrovars(:,:,:,:)
   real(c_double), allocatable :: vs1i(:,:,:,:), vs2i(:,:,:,:), vs3i(:,:,:,:)
+  real(c_double), allocatable :: nn6(:,:,:,:), vn1(:,:,:), vn2(:,:,:), vn3(:,:,:)
+  real(c_double), allocatable :: Prion(:,:,:,:), Qeion(:,:,:), alt(:,:,:), g1(:,:,:)
   type(c_ptr) :: fvp, fap, evp, xC, intvars, cfgC
   character(len=4096) :: dir
   integer :: u, k, rc, w
@@ -42,6 +47,11 @@ program fortran_driver
   read(u) h1, h2, h3, h1x1i, h2x1i, h3x1i, h1x2i, h2x2i, h3x2i, h1x3i, h2x3i, h3x3i
   read(u) dx1, dx1i, dx2, dx2i, dx3, dx3i, nn, Tn, fluidvars, fluidauxvars, electrovars
   read(u) vs1i, vs2i, vs3i
+  if (mode >= 4) then
+    allocate(nn6(lx1,lx2,lx3,6), vn1(lx1,lx2,lx3), vn2(lx1,lx2,lx3), vn3(lx1,lx2,lx3))
+    allocate(Prion(lx1,lx2,lx3,6), Qeion(lx1,lx2,lx3), alt(-1:lx1+2,-1:lx2+2,-1:lx3+2), g1(lx1,lx2,lx3))
+    read(u) nn6, vn1, vn2, vn3, Prion, Qeion, alt, g1
+  end if
   close(u)
 
   rc = gemhip_init(0_c_int)
@@ -52,6 +62,11 @@ program fortran_driver
   rc = rc + gemhip_register_neutrals(nn, Tn)
   rc = rc + gemhip_register_config(solver, Teinf)
   rc = rc + gemhip_set_interface_vels(vs1i, vs2i, vs3i)
+  if (mode >= 4) then
+    rc = rc + gemhip_register_atmosphere(nn6, Tn, vn1, vn2, vn3)
+    rc = rc + gemhip_register_gravity(alt, g1)
+    rc = rc + gemhip_set_ionization_rates(Prion, Qeion)
+  end if
   if (rc /= 0) error stop 'set-up failed: '//gemhip_error_message()
 
   fvp = c_loc(fluidvars)
@@ -61,7 +76,7 @@ program fortran_driver
   intvars = c_null_ptr
   cfgC = c_null_ptr
   lsp = 7
-  if (mode == 1 .or. mode == 3) then
+  if (mode == 1 .or. mode == 3 .or. mode == 5) then
     rc = gemhip_set_residency(1_c_int)
     rc = rc + gemhip_state_push(fluidvars, fluidauxvars, electrovars)
     do k = 1, nsteps
@@ -70,10 +85,11 @@ program fortran_driver
       else
         rc = rc + gemhip_fluid_adv_block(dt)
       end if
+      if (mode == 5) rc = rc + gemhip_sources_block(dt)
     end do
     rc = rc + gemhip_state_pull(fluidvars, fluidauxvars)
     if (rc /= 0) error stop 'fused step failed: '//gemhip_error_message()
-  else if (mode == 2) then
+  else if (mode == 2 .or. mode == 4) then
     do k = 1, nsteps
       call v12rhov1_C(fvp, fap)
       call T2rhoe_C(fvp, fap)
@@ -96,6 +112,15 @@ program fortran_driver
       call energy_diffusion_C(cfgC, xtype, xC, fvp, evp, intvars, dt)
       call clean_param_C(3_c_int, xtype, xC, fvp)
       call T2rhoe_C(fvp, fap)
+      if (mode == 4) then
+        call set_global_boundaries_allspec_C(xtype, xC, fvp, fap, intvars, lsp)
+        call source_loss_energy_C(fvp, fap, evp, intvars, xtype, xC, dt)
+        call source_loss_momentum_C(fvp, fap, evp, intvars, xtype, xC, dt)
+        call source_loss_mass_C(fvp, fap, evp, intvars, xtype, xC, dt)
+        call clean_param_C(3_c_int, xtype, xC, fvp)
+        call clean_param_C(2_c_int, xtype, xC, fvp)
+        call clean_param_C(1_c_int, xtype, xC, fvp)
+      end if
       if (len(gemhip_error_message()) > 0) error stop 'step failed: '//gemhip_error_message()
     end do
   else
@@ -121,7 +146,7 @@ program fortran_driver
       if (len(gemhip_error_message()) > 0) error stop 'step failed: '//gemhip_error_message()
     end do
   end if
-  if (mode == 0 .or. mode == 2) rc = gemhip_get_interface_vels(vs1i, vs2i, vs3i)
+  if (mode == 0 .or. mode == 2 .or. mode == 4) rc = gemhip_get_interface_vels(vs1i, vs2i, vs3i)
   open(newunit=w, file=trim(dir)//'/out.bin', access='stream', form='unformatted', status='replace')
   write(w) fluidvars, fluidauxvars(:,:,:,1:14), vs1i, vs2i, vs3i
   close(w)

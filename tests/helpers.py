@@ -42,9 +42,11 @@ DRIVER_ARRAYS = ("h1", "h2", "h3", "h1x1i", "h2x1i", "h3x1i", "h1x2i", "h2x2i", 
                  "h1x3i", "h2x3i", "h3x3i", "dx1", "dx1i", "dx2", "dx2i", "dx3", "dx3i")
 
 
-def write_case(d, g, st, dt, nsteps, mode, solver=2, Teinf=1500.0):
+def write_case(d, g, st, dt, nsteps, mode, solver=2, Teinf=1500.0, atm=None):
     """Case files read by tests/drivers/*: case.txt (sizes, flags, step control) and
-    in.bin (every input array, Fortran order, little-endian float64, concatenated)."""
+    in.bin (every input array, Fortran order, little-endian float64, concatenated).  With an
+    atmosphere (modes 4 and 5) in.bin continues with nn(:,:,:,1:6), vn1, vn2, vn3,
+    Prionize, Qeionize, the grid's alt and g1."""
     import os
     isg = " ".join(str(int(bool(v))) for v in g.isglobal)
     xtype = 1 if g.kind == "cartesian" else 2
@@ -54,6 +56,8 @@ def write_case(d, g, st, dt, nsteps, mode, solver=2, Teinf=1500.0):
     parts = [np.asarray(getattr(g, n), dtype="<f8") for n in DRIVER_ARRAYS]
     parts += [st.nn[..., :3], st.Tn, st.fluidvars, st.fluidauxvars, st.electrovars,
               st.vs1i, st.vs2i, st.vs3i]
+    if atm is not None:
+        parts += [atm.nn, atm.vn1, atm.vn2, atm.vn3, atm.Prionize, atm.Qeionize, g.alt, g.g1]
     with open(os.path.join(d, "in.bin"), "wb") as f:
         for a in parts:
             f.write(np.asarray(a, dtype="<f8").tobytes(order="F"))
