@@ -88,10 +88,8 @@ int orc_transport_step(const orc_grid *g, double dt, double *fluidvars, double *
 /* The rest of the fluid_adv transport block (multifluid.f90:237-302, 553-688,
    sources_mpi.f90:55-112).  nullpts(lx1,lx2,lx3): nonzero marks a null cell (NULL: none).
    orc_clean_param returns -1 for a paramflag other than 1, 2, 3 (the reference's error
-   stop).  Q is (lx1,lx2,lx3,7).
-   Parity unpinned: the dipole border copy of clean_param 2 (multifluid.f90:623-667) is
-   restated line for line with no independent check; the rest is pinned in
-   tests/test_block.py. */
+   stop).  Q is (lx1,lx2,lx3,7).  Pinned in tests/test_block.py; the dipole border copy of
+   clean_param 2 (multifluid.f90:613-657) by every null mask of a 6-cell field line. */
 int orc_clean_param(const orc_grid *g, int iparm, double *param, const int *nullpts);
 void orc_VNRicht_artvisc(const orc_grid *g, const double *ns, const double *vs1, double *Q);
 void orc_RK2_global_boundary_allspec(const orc_grid *g, double *vs1, double *vs2, double *vs3);

@@ -202,3 +202,41 @@ def test_dt_select_rules():
     assert sel(glow=1, t=1.0, tglow=1.0 + free / 5) == (1.0 + free / 5) - 1.0
     assert sel(glow=0, t=1.0, tglow=1.0 + free / 5) == free
     assert sel(t=5.0, tout=5.0) == 1e-6
+
+
+@pytest.mark.parametrize("gridflag", [0, 1])
+def test_clean_param_velocity_border_rule_on_every_mask(gridflag):
+    """clean_param 2's dipole border copy (multifluid.f90:605-657) on every null mask of a
+    6-cell field line, against the rule stated through runs of the mask: null cells are
+    zeroed; on a closed dipole (gridflag 0) the first null cell from the bottom takes the
+    value above it, then the first non-null cell after that run of null cells takes the value
+    below it (each search stops at lx1, where nothing is copied); on an inverted open dipole
+    (gridflag 1) only the second copy is made, at the first null cell."""
+    import itertools
+    b = ga.cartesian_grid(6, 1, 1)
+    g = ga.Grid(b.lx1, b.lx2, b.lx3, b.x1, b.x2, b.x3, b.planes, gridflag=gridflag,
+                isglobal=b.isglobal, kind="cartesian")
+    lx1 = g.lx1
+    rng = np.random.default_rng(5)
+    for bits in itertools.product((0, 1), repeat=lx1):
+        mask = np.array(bits, dtype=bool)
+        st = ga.synthetic_state(g, seed=1)
+        st.fluidvars[..., 7:14] = rng.uniform(-100, 100, st.fluidvars[..., 7:14].shape)
+        col = st.fluidvars[:, 2, 2, 7:14]                      # padded: cell i1 at i1 + 1
+        want = col.copy()
+        want[2:-2][mask] = 0.0
+        nul = np.asfortranarray(mask.reshape(lx1, 1, 1).astype(np.int32))
+        oracle.op(g, st, "clean_param", which=2, nullpts=nul)
+        nulls = np.flatnonzero(mask[:lx1 - 1])                  # searches stop at lx1
+        first_null = int(nulls[0]) + 1 if len(nulls) else lx1  # 1-based
+        if gridflag == 0:
+            run = first_null
+            while run < lx1 and mask[run - 1]:
+                run += 1
+            if first_null != lx1:
+                want[first_null + 1] = want[first_null + 2]
+            if run != lx1:
+                want[run + 1] = want[run]
+        elif first_null != lx1:
+            want[first_null + 1] = want[first_null]             # from the cell below (a ghost at i1 = 1)
+        assert np.array_equal(col, want), bits
