@@ -19,6 +19,12 @@
  *   src/vendor/lapack95/gbsv.in.f90:58-97    (-> LAPACK dgbsv, Netlib dgbtrf/dgbtf2/
  *                                             dgbtrs/dtbsv/dger/dscal/idamax semantics)
  *   src/numerical/constants/phys_consts.in.f90 (constants)
+ *   src/multifluid/multifluid.f90:237-302,553-688 (artificial viscosity, compression,
+ *                                             clean_param), src/sources/sources_mpi.f90
+ *   src/temporal/temporal_mpi.f90:27-165     (dt_calc, dt_select)
+ *   src/sources/sources.f90, src/collisions/collisions.f90:11-159,
+ *   src/multifluid/multifluid.f90:130-175,472-549, calculus.in.f90:174-196,
+ *   gradient.f90:79-149                      (the source/loss substep)
  */
 #include "gemini_oracle.h"
 
