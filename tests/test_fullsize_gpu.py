@@ -27,8 +27,9 @@ class SubGrid:
         self.c = c
         self.lx1, self.lx2, self.lx3 = (n if i is None else 1 for n, i in zip(g.shape, c))
         self.gridflag, self.flagper, self.isglobal = g.gridflag, False, (True,) * 6
-        for n in ("h1", "h2", "h3"):
+        for n in ("h1", "h2", "h3", "alt"):
             setattr(self, n, np.asfortranarray(np.asarray(getattr(g, n))[self.padded_slices()]))
+        self.g1 = np.asfortranarray(np.asarray(g.g1)[self.interior_slices()])
         for k, axis_arrays in enumerate((("h1x1i", "h2x1i", "h3x1i"), ("h1x2i", "h2x2i", "h3x2i"),
                                          ("h1x3i", "h2x3i", "h3x3i"))):
             for n in axis_arrays:
@@ -61,6 +62,13 @@ def sub_state(sg, st):
     return State(f(st.fluidvars, P), f(st.fluidauxvars, P), f(st.electrovars, P),
                  f(st.vs1i, sg.iface_slices(0)), f(st.vs2i, sg.iface_slices(1)),
                  f(st.vs3i, sg.iface_slices(2)), f(st.nn, I), f(st.Tn, I))
+
+
+def sub_atmosphere(sg, atm):
+    I = sg.interior_slices()
+    f = lambda a: np.asfortranarray(a[I])
+    return ga.Atmosphere(f(atm.nn), f(atm.Tn), f(atm.vn1), f(atm.vn2), f(atm.vn3), f(atm.Prionize),
+                         f(atm.Qeionize))
 
 
 def host_state(g, dev):
@@ -141,6 +149,25 @@ def test_cut_lines_reproduce_the_whole_grid():
     for direction in (3, 1, 2):
         check_sweep(whole, g, st, direction, sample_lines(g, rng, direction, 5), dt)
     check_diffusion(whole, g, st, sample_lines(g, rng, 1, 5), dt)
+    # the sources on cut columns (their only non-local term, the x1 pressure gradient,
+    # stays inside the column)
+    atm = ga.synthetic_atmosphere(g, st, seed=3)
+    Q = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F")
+    oracle.op(g, st.copy(), "VNRicht_artvisc", Q=Q)
+    cols = sample_lines(g, rng, 1, 4)
+    for which in ("energy", "momentum", "mass"):
+        cut = []
+        for c in cols:
+            sg = SubGrid(g, c)
+            _, i2, i3 = c
+            cut.append((sg, sub_state(sg, st), sub_atmosphere(sg, atm),
+                        np.asfortranarray(Q[:, i2 - 1:i2, i3 - 1:i3, :])))
+        oracle.source_loss(g, st, atm, dt, which, Q=Q)
+        for sg, s, sa, q in cut:
+            oracle.source_loss(sg, s, sa, dt, which, Q=q)
+            for blk in ("fluidvars", "fluidauxvars"):
+                a = line_view(sg, getattr(st, blk), False)[..., :35]
+                assert np.array_equal(a, line_view(sg, getattr(s, blk), True)[..., :35]), (which, sg.c)
 
 
 @pytest.mark.gpu
@@ -170,3 +197,55 @@ def test_fullsize_sampled_parity():
         print(f"  x{direction} sweep", flush=True)
         check_sweep(tr, g, st, direction, sample_lines(g, rng, direction, 16), dt)
     print("  24 columns and 48 lines bit-identical", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(400)
+def test_fullsize_sampled_sources_parity():
+    """The source/loss entry points on the whole 512x256x192 grid (mode 0), 24 sampled
+    field lines checked against the oracle on their cut sub-grids, every updated field of
+    every species within 1e-12 of the oracle (relative to the field's largest magnitude on
+    the line; the rates call exp, erf and log, rounded independently on each side)."""
+    print("case dipole 512x256x192 sources: generating the state", flush=True)
+    from gemini_amd.synth_gpu import synthetic_atmosphere_device
+    g = ga.dipole_grid(*SHAPE)
+    dev = synthetic_state_device(g, seed=2, device="cuda:0")
+    dt = cfl_dt_device(g, dev)
+    L1, L2, L3 = g.padded
+    ne = dev["fluidvars"][6].view(L3, L2, L1)[2:-2, 2:-2, 2:-2]
+    a = synthetic_atmosphere_device(g, dev["nn"], dev["Tn"], ne, seed=2, device="cuda:0")
+    F = lambda t: np.asfortranarray(t.cpu().numpy().transpose(*range(t.dim() - 1, -1, -1)))
+    atm = ga.Atmosphere(F(a["nn"]), F(a["Tn"]), F(a["vn1"]), F(a["vn2"]), F(a["vn3"]),
+                        F(a["Prionize"]), F(a["Qeionize"]))
+    st = host_state(g, dev)
+    del dev, a
+    torch.cuda.empty_cache()
+    tr = Transport(g, st.nn, st.Tn, geometry_on_device=True)
+    tr.enable_sources(atm)
+    print("  conversions, ghost fill, artificial viscosity", flush=True)
+    for name in ("v12rhov1", "T2rhoe", "set_global_boundaries_allspec"):
+        getattr(tr, name)(st)
+    tr.VNRicht_artvisc(st)                       # the library keeps Q for the momentum sources
+    rng = np.random.default_rng(3)
+    cols = sample_lines(g, rng, 1, 24)
+    fields = {"energy": (("fluidvars", 28), ("fluidauxvars", 7)),
+              "momentum": (("fluidvars", 7), ("fluidauxvars", 0)), "mass": (("fluidvars", 0),)}
+    for which, outs in fields.items():
+        print(f"  source_loss_{which}", flush=True)
+        subs = []
+        for c in cols:
+            sg = SubGrid(g, c)
+            s = sub_state(sg, st)
+            Q = np.zeros((sg.lx1, 1, 1, 7), order="F")
+            oracle.op(sg, s.copy(), "VNRicht_artvisc", Q=Q)
+            subs.append((sg, s, sub_atmosphere(sg, atm), Q))
+        tr.source_loss(st, which, dt)
+        for sg, s, sa, Q in subs:
+            oracle.source_loss(sg, s, sa, dt, which, Q=Q)
+            for blk, k in outs:
+                a = line_view(sg, getattr(st, blk), False)[..., k:k + 7]
+                b = line_view(sg, getattr(s, blk), True)[..., k:k + 7]
+                scale = np.max(np.abs(b), axis=0)
+                rel = np.max(np.max(np.abs(a - b), axis=0) / np.where(scale > 0, scale, 1.0))
+                assert rel <= 1e-12, f"{which} column {sg.c} {blk}[{k}] {rel:.3e}"
+    print("  24 columns within 1e-12 for energy, momentum and mass", flush=True)
