@@ -568,6 +568,9 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle on host cores)")
         cpu = cpu_baseline(w, dt, args.cpu_seconds)
+        if whole is not None:
+            log("CPU baseline of the whole step (oracle transport block + sources)")
+            whole["cpu_baseline"] = cpu_baseline(WORKLOADS["step512"], dt, 10.0, block=5)
     adj = None
     if not args.no_adjoint:
         del tr
