@@ -826,8 +826,8 @@ void clear_ionization_arrays_C(void** intvars) {
   if (!ready() || !src_alloc()) return;
   Ctx& c = ctx();
   const long ni = (long)c.lx1 * c.lx2 * c.lx3;
-  hipok(hipMemsetAsync(const_cast<double*>(sources().Prionize), 0, 7 * (size_t)ni * sizeof(double),
-                       c.stream), "clear ionization arrays");
+  launch_zero(const_cast<double*>(sources().Prionize), 7 * ni, c.stream);   // Prionize, Qeionize
+  hipok(hipGetLastError(), "clear ionization arrays");
   if (c.residency == 0) hipok(hipStreamSynchronize(c.stream), "clear ionization arrays");
 }
 

@@ -146,6 +146,7 @@ void launch_source_loss_momentum(const DevGrid& g, const DevState& s, const DevS
                                  double dt, hipStream_t st);
 void launch_source_loss_mass(const DevGrid& g, const DevState& s, const DevSources& a,
                              double dt, hipStream_t st);
+void launch_zero(double* p, long n, hipStream_t st);
 
 // kernel-timing hooks implemented by the API layer
 void timing_begin(const char* name, hipStream_t st);

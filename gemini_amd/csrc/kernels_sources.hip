@@ -561,7 +561,18 @@ __global__ __launch_bounds__(TPB) void k_src_mass(DevGrid g, DevState s, DevSour
 
 int blocks_of(long n) { return (int)((n + TPB - 1) / TPB); }
 
+__global__ void k_zero(double* p, long n) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+    p[k] = 0.0;
+}
+
 }  // namespace
+
+// zero n doubles with a kernel (a memset node would be unsafe if a caller captures the call)
+void launch_zero(double* p, long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero, dim3((int)min(blocks_of(n), 16384L)), dim3(TPB), 0, st, p, n);
+}
 
 void launch_source_loss_energy(const DevGrid& g, const DevState& s, const DevSources& a,
                                double dt, hipStream_t st) {
