@@ -84,20 +84,32 @@ def test_each_source_loss_entry_point(kind):
         assert max_rel(a, b, f) <= RTOL, f"allparams {f}"
 
 
-@pytest.mark.parametrize("kind", ["cart", "dipole"])
-def test_sources_block_after_transport_block(kind):
+def null_cells(g, below):
+    alt = np.asarray(g.planes["alt"][0])[2:-2, 2:-2].T          # (lx1, lx2)
+    m = np.broadcast_to((alt < below)[:, :, None], (g.lx1, g.lx2, g.lx3))
+    return np.asfortranarray(m.astype(np.int32))
+
+
+@pytest.mark.parametrize("kind,nulls", [("cart", False), ("dipole", False), ("cart", True)])
+def test_sources_block_after_transport_block(kind, nulls):
     """One whole time step on the HBM state: gemhip_fluid_adv_block, then
     gemhip_sources_block, against the oracle's transport block and sources block, and the
-    reference's sources calls one by one in mode 0 against the fused block."""
-    print(f"case {kind}", flush=True)
+    reference's sources calls one by one in mode 0 against the fused block; with null
+    cells (altitude below 200 km) in one case."""
+    print(f"case {kind} nulls={nulls}", flush=True)
     g, st, atm = case(kind)
+    nul = null_cells(g, 200e3) if nulls else None
+    if nulls:
+        assert nul.sum() > 0
     dt = ga.cfl_dt(g, st)
     ref = st.copy()
     Q = np.zeros((g.lx1, g.lx2, g.lx3, 7), order="F")
     with oracle.pow_mode(1):
-        oracle.fluid_adv_block(g, ref, dt, Q=Q)
-    oracle.sources_block(g, ref, atm, dt, Q=Q)
+        oracle.fluid_adv_block(g, ref, dt, Q=Q, nullpts=nul)
+    oracle.sources_block(g, ref, atm, dt, Q=Q, nullpts=nul)
     tr = transport(g, st, atm)
+    if nulls:
+        tr.register_null_cells(nul)
     tr.set_residency(1)
     tr.push(st)
     tr.fluid_adv_block(dt)
@@ -111,6 +123,8 @@ def test_sources_block_after_transport_block(kind):
     tr.close()
     # the reference's calls of src/libgemini.cpp:155-232 one by one (mode 0)
     tr2 = transport(g, st, atm)
+    if nulls:
+        tr2.register_null_cells(nul)
     b = st.copy()
     tr2.fluid_adv_block_calls(b, dt)
     tr2.sources_block_calls(b, dt)
