@@ -63,6 +63,10 @@ BYTES_PER_CELL = {"sweep1": 408, "sweep2": 408, "sweep3": 408, "trbdf2": 224,
                   # writes rhovs1, vs1; mass reads ns (7), Ts (4), vs1-3 of O+, nn(1:6),
                   # Tn, vn1-3, Prionize (6) and writes ns
                   "source_energy": 520, "source_momentum": 568, "source_mass": 296}
+# plasma output packing (kernels_output.hip), per interior cell: flagoutput 1 reads ns, vs1,
+# Ts (7 each), vs2, vs3 (6 each), J1-J3 and writes 26 real32 values; flagoutput 2 reads 35
+# doubles and writes 9 real32; flagoutput 3 reads the electron density and writes it
+OUTPUT_BYTES_PER_CELL = {1: 392, 2: 316, 3: 12}
 PADDED_KERNELS = {"pre_convert", "post_convert", "rhov12v1", "rhoe2T"}
 KERNELS = ["pre_convert", "ghost_fill", "halo_periodic", "save_v1b", "sweep3", "sweep1",
            "sweep2", "post_convert", "rhov12v1", "rhoe2T", "trbdf2", "floor_ghosts",
@@ -178,6 +182,65 @@ def measure_whole_step(tr, g, dt, atm, steps=10, warmup=2):
             "sources_ms": round(sum(kern.get(k, 0.0) for k in
                                     ("source_energy", "source_momentum", "source_mass")), 4),
             "kernel_ms": kern, "state_finite_after_timing": state_is_finite(tr, g)}
+
+
+def measure_output(tr, g, electrovars, sample_shape, reps=3):
+    """plasma_output_nompi_C on the resident state of the headline run: the pack kernel
+    (HIP events on the library's stream) and the whole call into a file on local disk
+    (pack, one device-to-host copy per dataset overlapped with the HDF5 writes); the
+    oracle's packing of the same datasets on a host sample for comparison."""
+    import shutil
+    import tempfile
+    import numpy as np
+    import oracle
+    from gemini_amd.state import synthetic_state
+    ymd, ut = [2015, 4, 13], 12345.678910
+    outdir = tempfile.mkdtemp(prefix="gemhip_output_")
+    res = {"grid": f"{g.lx1}x{g.lx2}x{g.lx3}", "file_system": outdir.rsplit("/", 1)[0]}
+    try:
+        for flag, comp in ((2, 0), (1, 0), (3, 3)):
+            tr.register_output(outdir, flag, comp)
+            os.remove(tr.plasma_output(None, ymd, ut, 0, electrovars=electrovars))  # staging
+            walls = []
+            tr.kernel_timing_reset()
+            tr.kernel_timing(True)
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                path = tr.plasma_output(None, ymd, ut, 0, electrovars=electrovars)
+                walls.append(time.perf_counter() - t0)
+                size = os.path.getsize(path)
+                os.remove(path)
+            tr.kernel_timing(False)
+            tot, n = tr.kernel_time("output_pack")
+            pack_ms = tot / n
+            nb = OUTPUT_BYTES_PER_CELL[flag] * g.ncells
+            values = oracle.output_count(g.lx1, g.lx2, g.lx3, flag)
+            wall = sorted(walls)[len(walls) // 2]
+            res[f"flagoutput{flag}"] = {
+                "comp_lvl": comp, "pack_ms": round(pack_ms, 4),
+                "pack_algorithmic_bytes": nb,
+                "pack_GB/s": round(nb / (pack_ms / 1e3) / 1e9, 1),
+                "pack_frac": round(nb / (pack_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "real32_values": values, "file_bytes": size,
+                "call_s": round(wall, 3), "call_GB/s_of_real32": round(4 * values / wall / 1e9, 3)}
+            log(f"output flagoutput {flag} (comp {comp}): pack {pack_ms:.3f} ms "
+                f"({nb / (pack_ms / 1e3) / 1e9:.0f} GB/s), whole call {wall:.2f} s")
+    finally:
+        shutil.rmtree(outdir, ignore_errors=True)
+    gs = make_grid("dipole", sample_shape)
+    st = synthetic_state(gs, seed=1)
+    oracle.output_pack(st, gs.gridflag, 1)
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < 2.0:
+        oracle.output_pack(st, gs.gridflag, 1)
+        k += 1
+    el = time.perf_counter() - t0
+    res["cpu_pack"] = {"cells/s": round(gs.ncells * k / el), "cores": 1, "kind": "port",
+                       "sample": f"oracle output packing (flagoutput 1) of a {gs.lx1}x{gs.lx2}x"
+                                 f"{gs.lx3} dipole state, {k} repetitions in {el:.1f} s"}
+    res["gpu_pack_cells/s"] = round(g.ncells / (res["flagoutput1"]["pack_ms"] / 1e3))
+    return res
 
 
 def state_is_finite(tr, g):
@@ -411,6 +474,8 @@ def main():
                     help="central-difference directions checked by the adjoint measurement "
                          "(default 4 in the default run, 32 in the adjoint workload)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-output", action="store_true",
+                    help="skip the plasma-output figures of the default run")
     ap.add_argument("--no-whole-step", action="store_true",
                     help="skip the whole-step (transport block + sources) figure of the default run")
     ap.add_argument("--host-state", action="store_true",
@@ -488,6 +553,8 @@ def main():
     if w.get("sources"):
         tr.enable_sources(atm)
         atm = None
+    output = (args.workload == "dipole512" and not args.host_state and not args.no_output)
+    electrovars = st["electrovars"] if output else None
     del st
     torch.cuda.empty_cache()
     step = tr.transport_step
@@ -564,6 +631,11 @@ def main():
         whole = measure_whole_step(tr, g, dt, atm)
         del atm
         log(f"whole step {whole['ms_per_step']:.3f} ms, sources {whole['sources_ms']:.3f} ms")
+    out_res = None
+    if output:
+        log("plasma output (flagoutput 2, 1, 3) of the resident state")
+        out_res = measure_output(tr, g, electrovars, w["sample"])
+        del electrovars
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle on host cores)")
@@ -609,6 +681,7 @@ def main():
         "kernels": kern,
         "cpu_baseline": cpu,
         "whole_step": whole,
+        "output": out_res,
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": None if adj is None else dict(adj, workload=ADJOINT["desc"]),
     }

@@ -1,5 +1,8 @@
 """Build the HIP transport library (gemini_amd/libgemhip.so) for gfx950 with hipcc.
 
+The HDF5 file layer (csrc/h5io.cpp) is compiled against hdf5.h and loads libhdf5 when a
+file is first read or written, so the library has no load-time dependency on it.
+
 Flags that matter for results:
   -ffp-contract=off   no fused multiply-add: every product and sum is rounded on its
                       own, as in the reference's x86-64 build (bit-identical parity),
@@ -16,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgemhip.so")
 SOURCES = ["gemhip_api.hip", "kernels_advec.hip", "kernels_diffusion.hip", "kernels_adjoint.hip",
-           "kernels_block.hip", "kernels_sources.hip"]
+           "kernels_block.hip", "kernels_sources.hip", "kernels_output.hip", "h5io.cpp"]
 ARCH = os.environ.get("GEMHIP_ARCH", "gfx950")
 
 
@@ -27,6 +30,16 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def hdf5_include():
+    """Directory holding hdf5.h (the HDF5 C library's public header).  The library itself is
+    loaded at run time, so only its declarations are needed to build."""
+    for d in (os.environ.get("GEMHIP_HDF5_INCLUDE"), "/opt/conda/include", "/usr/include/hdf5/serial",
+              "/usr/include"):
+        if d and os.path.exists(os.path.join(d, "hdf5.h")):
+            return d
+    raise RuntimeError("hdf5.h not found; set GEMHIP_HDF5_INCLUDE to the HDF5 include directory")
+
+
 def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     """Compile every source into the shared library (always recompiles).
 
@@ -34,9 +47,10 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     depths) for tuning builds written to ``out``."""
     cmd = [hipcc(), "-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17",
            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-result", "-o", out + ".tmp"]
+           "-Wall", "-Wno-unused-result", "-idirafter", hdf5_include(), "-o", out + ".tmp"]
     cmd += [f"-D{d}" for d in defines]
     cmd += [os.path.join(CSRC, s) for s in SOURCES]
+    cmd += ["-ldl"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

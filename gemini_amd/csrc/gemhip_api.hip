@@ -6,6 +6,7 @@
 // reference-named entry points on top of the kernel launchers.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +17,7 @@
 
 #include "../../include/gemini3d_hip.h"
 #include "gemhip_internal.h"
+#include "gemhip_io.h"
 
 namespace gemhip {
 namespace {
@@ -54,6 +56,14 @@ struct Ctx {
   int flagglow = 0;             // cfg%flagglow
   int dt_calls = 0;             // the reference's iteration count as dt_select sees it
   int* d_null = nullptr;        // null-cell mask, lx1*lx2*lx3 (null when none registered)
+  std::string outdir;           // cfg%outdir, cfg%flagoutput and the file compression level
+  int flagoutput = 1;
+  int comp_lvl = 3;
+  bool output_ok = false;       // gemhip_register_output called
+  double* d_out = nullptr;      // output staging: J1 J2 J3 Phi, then the packed real32 datasets
+  size_t out_bytes = 0;
+  float* h_out = nullptr;       // pinned host copy of the packed datasets
+  size_t h_out_bytes = 0;
   double* d_kappa = nullptr;    // fitting extension (lx2*lx3 each)
   double* d_qtop = nullptr;
   double* d_adj = nullptr;      // adjoint work: a second field and the column scratch
@@ -143,6 +153,11 @@ void free_all() {
   }
   if (c.d_null) hipFree(c.d_null);
   c.d_null = nullptr;
+  if (c.d_out) hipFree(c.d_out);
+  if (c.h_out) hipHostFree(c.h_out);
+  c.d_out = nullptr;
+  c.h_out = nullptr;
+  c.out_bytes = c.h_out_bytes = 0;
   c.scratch_bytes = 0;
   c.adj_bytes = 0;
   c.s.kappa = c.s.qtop = nullptr;
@@ -1432,6 +1447,300 @@ int gemhip_eval_math(int op, const double* x, double* y, long n) {
   if (dx) hipFree(dx);
   if (dy) hipFree(dy);
   return ok ? 0 : 3;
+}
+
+}  // extern "C"
+
+// ---- plasma output and initial conditions (io_nompi.f90:183-345, timeutils.f90) ----
+namespace gemhip {
+namespace {
+
+// timeutils.f90:13-27: days of a month, leap years by the Gregorian rule
+int month_days(int year, int month) {
+  if (year < 1600 || year > 2500 || month < 1 || month > 12) return -1;
+  static const int len[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+  const bool leap = (year % 4 == 0 && year % 100 != 0) || year % 400 == 0;
+  return month == 2 && leap ? 29 : len[month - 1];
+}
+
+// timeutils.f90:110-130: carry days beyond the month into months, months beyond 12 into years
+bool carry_date(int& year, int& month, int& day) {
+  if (month < 1 || day < 1) return false;
+  while (true) {
+    year += (month - 1) / 12;
+    month = (month - 1) % 12 + 1;
+    const int dm = month_days(year, month);
+    if (dm < 0) return false;
+    if (day <= dm) return true;
+    day -= dm;
+    ++month;
+  }
+}
+
+// timeutils.f90:133-196: outdir/YYYYMMDD_SSSSS.ffffff for a double UTsec, rounded to the
+// nearest 10 ms (half away from zero, as nint); false where the reference stops.
+bool date_stem(const char* outdir, const int* ymd, double UTsec, std::string& out) {
+  int year = ymd[0], month = ymd[1], day = ymd[2];
+  const double hundredths = std::round(UTsec * 100.0);
+  if (!(std::fabs(hundredths) <= 2.0e8)) return false;
+  long ms = (long)hundredths * 10;
+  long sec = ms / 1000;
+  if (sec < 0 || sec > 86400) return false;
+  if (sec == 86400) {
+    ++day;
+    sec = ms = 0;
+    if (!carry_date(year, month, day)) return false;
+  }
+  const long usec = ms * 1000 - sec * 1000000;
+  if (year < 0 || year > 9999 || month < 0 || month > 99 || day < 0 || day > 99) return false;
+  char buf[64];
+  if (usec < 0)   // UTsec in (-1, 0): sec truncates to 0 and the I6.6 field overflows to '*'
+    snprintf(buf, sizeof buf, "%04d%02d%02d_%05ld.******", year, month, day, sec);
+  else
+    snprintf(buf, sizeof buf, "%04d%02d%02d_%05ld.%06ld", year, month, day, sec, usec);
+  out = std::string(outdir) + "/" + buf;
+  return true;
+}
+
+bool ensure_output_buffers(size_t dev_bytes, size_t host_bytes) {
+  Ctx& c = ctx();
+  if (c.out_bytes < dev_bytes) {
+    if (c.d_out) hipFree(c.d_out);
+    c.d_out = nullptr;
+    c.out_bytes = 0;
+    if (!hipok(hipMalloc((void**)&c.d_out, dev_bytes), "output staging")) return false;
+    c.out_bytes = dev_bytes;
+  }
+  if (c.h_out_bytes < host_bytes) {
+    if (c.h_out) hipHostFree(c.h_out);
+    c.h_out = nullptr;
+    c.h_out_bytes = 0;
+    if (!hipok(hipHostMalloc((void**)&c.h_out, host_bytes, hipHostMallocDefault),
+               "pinned output buffer"))
+      return false;
+    c.h_out_bytes = host_bytes;
+  }
+  return true;
+}
+
+}  // namespace
+}  // namespace gemhip
+
+extern "C" {
+
+int gemhip_register_output(const char* outdir, int flagoutput, int comp_lvl) {
+  Ctx& c = ctx();
+  if (!outdir) { fail("outdir is a null pointer"); return 1; }
+  if (comp_lvl < 0 || comp_lvl > 9) { fail("compression level %d not in 0..9", comp_lvl); return 2; }
+  c.outdir = outdir;
+  c.flagoutput = flagoutput;
+  c.comp_lvl = comp_lvl;
+  c.output_ok = true;
+  return 0;
+}
+
+int gemhip_output_filename(const char* outdir, const int* ymd, double UTsec, int identifier,
+                           char* out, long len) {
+  if (!outdir || !ymd || !out) { fail("null argument to gemhip_output_filename"); return 1; }
+  std::string p;
+  if (!date_stem(outdir, ymd, UTsec, p)) {
+    fail("timeutils.f90::utsec2filestem did NOT satisfy 0 <= seconds < 86400 (or a bad date)");
+    return 2;
+  }
+  p += "_" + std::to_string(identifier) + ".h5";
+  if ((long)p.size() + 1 > len) { fail("file name buffer too short (%ld)", len); return 3; }
+  memcpy(out, p.c_str(), p.size() + 1);
+  return 0;
+}
+
+// io_nompi.f90:183-268 (libgemini_c.f90:267-284 passes identifier and the limits, so the
+// name always carries the patch number and the file the limits and patchID)
+void plasma_output_nompi_C(void** cfgC, int* ymd, double* UTsec, double** fluidvars,
+                           double** electrovars, int* identifier, double* x1lims,
+                           double* x2lims, double* x3lims) {
+  (void)cfgC;
+  if (!ready()) return;
+  Ctx& c = ctx();
+  if (!c.output_ok) { fail("output directory not registered (gemhip_register_output)"); return; }
+  if (!ymd || !UTsec || !identifier || !x1lims || !x2lims || !x3lims) {
+    fail("null argument to plasma_output_nompi_C");
+    return;
+  }
+  if (!check_host(electrovars, "electrovars")) return;
+  if (c.residency == 0 && !check_host(fluidvars, "fluidvars")) return;
+  std::string path;
+  if (!date_stem(c.outdir.c_str(), ymd, *UTsec, path)) {
+    fail("timeutils.f90::utsec2filestem did NOT satisfy 0 <= seconds < 86400 (or a bad date)");
+    return;
+  }
+  path += "_" + std::to_string(*identifier) + ".h5";
+  std::string err;
+  if (!h5_available(err)) { fail("%s", err.c_str()); return; }
+
+  const DevGrid& g = c.g;
+  const int flag = c.flagoutput;
+  const long n = (long)c.lx1 * c.lx2 * c.lx3, m = (long)c.lx2 * c.lx3;
+  const long count = output_count(g, flag);
+  const size_t ebytes = 4 * (size_t)g.N * sizeof(double);
+  if (!ensure_output_buffers(ebytes + count * sizeof(float), count * sizeof(float))) return;
+  Io io{fluidvars, nullptr, electrovars};
+  if (!upload(io, {NS, VS1, VS2, VS3, TS})) return;
+  // J1 J2 J3 Phi are the contiguous electrovars fields 4-7
+  if (!hipok(hipMemcpyAsync(c.d_out, *electrovars + 3 * g.N, ebytes, hipMemcpyDefault, c.stream),
+             "electrovars upload"))
+    return;
+  float* packed = reinterpret_cast<float*>(reinterpret_cast<char*>(c.d_out) + ebytes);
+  launch_output_pack(g, c.d_fluid, c.d_out, flag, packed, c.stream);
+  if (!hipok(hipGetLastError(), "output pack launch")) return;
+
+  // field datasets in file order (name, species count, values)
+  struct Field { const char* name; int nsp; };
+  std::vector<Field> fields;
+  if (flag == 2)
+    fields = {{"/neall", 0}, {"/v1avgall", 0}, {"/Tavgall", 0}, {"/Teall", 0}, {"/J1all", 0},
+              {"/J2all", 0}, {"/J3all", 0}, {"/v2avgall", 0}, {"/v3avgall", 0}};
+  else if (flag == 3)
+    fields = {{"/neall", 0}};
+  else
+    fields = {{"/nsall", 7}, {"/vs1all", 7}, {"/Tsall", 7}, {"/J1all", 0}, {"/J2all", 0},
+              {"/J3all", 0}, {"/v2avgall", 0}, {"/v3avgall", 0}};
+  const int flagv = flag, id = *identifier;
+  const int ymdv[3] = {ymd[0], ymd[1], ymd[2]};
+  const float uthour = (float)(*UTsec / 3600.0);
+  const float lims[6] = {(float)x1lims[0], (float)x1lims[1], (float)x2lims[0],
+                         (float)x2lims[1], (float)x3lims[0], (float)x3lims[1]};
+  std::vector<H5Out> ds = {
+      {"/flagoutput", H5Out::I32, 0, {0}, &flagv, nullptr},
+      {"/time/ymd", H5Out::I32, 1, {3}, ymdv, nullptr},
+      {"/time/UThour", H5Out::F32, 0, {0}, &uthour, nullptr},
+      {"/x1lims", H5Out::F32, 1, {2}, lims, nullptr},
+      {"/x2lims", H5Out::F32, 1, {2}, lims + 2, nullptr},
+      {"/x3lims", H5Out::F32, 1, {2}, lims + 4, nullptr},
+      {"/patchID", H5Out::I32, 0, {0}, &id, nullptr}};
+  // one device-to-host copy per dataset, each followed by an event, so the file write of
+  // one dataset overlaps the copy of the next
+  std::vector<hipEvent_t> evs;
+  long off = 0;
+  bool ok = true;
+  auto add = [&](const char* name, int rank, std::initializer_list<long> dims, long len) {
+    hipEvent_t e = take_event();
+    evs.push_back(e);
+    ok = ok && hipok(hipMemcpyAsync(c.h_out + off, packed + off, len * sizeof(float),
+                                    hipMemcpyDeviceToHost, c.stream), "output download") &&
+         hipok(hipEventRecord(e, c.stream), "output event");
+    H5Out d{name, H5Out::F32, rank, {0, 0, 0, 0}, c.h_out + off, e};
+    int k = 0;
+    for (long v : dims) d.dims[k++] = v;
+    ds.push_back(d);
+    off += len;
+  };
+  for (const Field& f : fields) {
+    if (f.nsp) add(f.name, 4, {f.nsp, c.lx3, c.lx2, c.lx1}, f.nsp * n);
+    else add(f.name, 3, {c.lx3, c.lx2, c.lx1}, n);
+  }
+  add("/Phiall", 2, {c.lx3, c.lx2}, m);
+  if (ok && !h5_write(path.c_str(), ds.data(), (int)ds.size(), c.comp_lvl, err)) {
+    fail("plasma output %s: %s", path.c_str(), err.c_str());
+    ok = false;
+  }
+  hipStreamSynchronize(c.stream);
+  for (hipEvent_t e : evs) c.pool.push_back(e);
+}
+
+// io_nompi.f90:271-345 with the zeroing of plasma_input_hdf5.f90:78-86: sizes from
+// indatsize (/lx1 /lx2 /lx3, else /lxs, else /lx) must equal lx1, lx2, lx3; ns, vs1 and Ts
+// of fluidvars are zeroed and their interiors read from /nsall /vs1all /Tsall of indatfile;
+// the potential field of electrovars (if given) is zeroed and each interior x1 plane set to
+// /Phiall (0 if absent).  Host blocks only: with the HBM copy resident, push afterwards.
+int gemhip_read_initial_conditions(const char* indatsize, const char* indatfile, int lx1,
+                                   int lx2, int lx3, double* fluidvars, double* electrovars) {
+  if (!indatsize || !indatfile || !fluidvars) { fail("null argument to gemhip_read_initial_conditions"); return 1; }
+  if (lx1 < 1 || lx2 < 1 || lx3 < 1) { fail("grid size %d x %d x %d", lx1, lx2, lx3); return 1; }
+  std::string err;
+  int lin[3] = {0, 0, 0};
+  {
+    H5In f(indatsize, err);
+    if (!f.open()) { fail("%s", err.c_str()); return 2; }
+    long d[8];
+    auto scalar = [&](const char* name, int& v) {
+      const int r = f.ndims(name, d);
+      long cnt = 1;
+      for (int k = 0; k < r; ++k) cnt *= d[k];
+      int buf[8];
+      if (r < 0 || cnt < 1 || cnt > 8 || !f.read(name, true, buf)) return false;
+      v = buf[0];
+      return true;
+    };
+    bool got = false;
+    if (f.ndims("/lx1", d) >= 0) {
+      got = scalar("/lx1", lin[0]) && scalar("/lx2", lin[1]) && scalar("/lx3", lin[2]);
+    } else {
+      for (const char* name : {"/lxs", "/lx"}) {
+        const int r = f.ndims(name, d);
+        if (r < 0) continue;
+        int buf[8];
+        got = r == 1 && d[0] == 3 && f.read(name, true, buf);
+        if (got) for (int k = 0; k < 3; ++k) lin[k] = buf[k];
+        break;
+      }
+    }
+    if (!got) { fail("%s: no grid size (/lx1 /lx2 /lx3, /lxs or /lx)%s%s", indatsize,
+                     err.empty() ? "" : ": ", err.c_str()); return 3; }
+  }
+  if (lin[0] != lx1 || lin[1] != lx2 || lin[2] != lx3) {
+    fail("The input data must be the same size as the grid which you are running the simulation "
+         "on - use a script to interpolate up/down to the simulation grid (file %d x %d x %d, "
+         "grid %d x %d x %d)", lin[0], lin[1], lin[2], lx1, lx2, lx3);
+    return 4;
+  }
+  const long L1 = lx1 + 4, L2 = lx2 + 4, L3 = lx3 + 4, N = L1 * L2 * L3, B = 7 * N;
+  H5In f(indatfile, err);
+  if (!f.open()) { fail("%s", err.c_str()); return 5; }
+  const long mdims[4] = {7, L3, L2, L1}, start[4] = {0, 2, 2, 2};
+  const struct { const char* name; double* dst; } vars[3] = {
+      {"/nsall", fluidvars}, {"/vs1all", fluidvars + B}, {"/Tsall", fluidvars + 4 * B}};
+  for (const auto& v : vars) {
+    long d[8];
+    const int r = f.ndims(v.name, d);
+    if (r != 4 || d[0] != 7 || d[1] != lx3 || d[2] != lx2 || d[3] != lx1) {
+      fail("%s: %s must have the shape (lx1, lx2, lx3, 7) = (%d, %d, %d, 7)", indatfile, v.name,
+           lx1, lx2, lx3);
+      return 6;
+    }
+    memset(v.dst, 0, B * sizeof(double));
+    if (!f.read(v.name, false, v.dst, 4, mdims, start)) { fail("%s: %s", indatfile, err.c_str()); return 7; }
+  }
+  if (!electrovars) return 0;
+  std::vector<double> slab((size_t)lx2 * lx3, 0.0);
+  long d[8];
+  const int r = f.ndims("/Phiall", d);
+  if (r == 1) {
+    // a 1-D potential spans x3 when lx2 = 1 and x2 when lx3 = 1 (io_nompi.f90:318-333)
+    const long want = lx2 == 1 ? lx3 : lx2;
+    if (d[0] != want || (lx2 != 1 && lx3 != 1)) {
+      fail("%s: a 1-D /Phiall needs lx2 = 1 or lx3 = 1 and their other size (%ld values)",
+           indatfile, d[0]);
+      return 8;
+    }
+  } else if (r == 2) {
+    if (d[0] != lx3 || d[1] != lx2) {
+      fail("%s: /Phiall must have the shape (lx2, lx3) = (%d, %d)", indatfile, lx2, lx3);
+      return 8;
+    }
+  } else if (r >= 0) {
+    fail("%s: /Phiall has %d dimensions", indatfile, r);
+    return 8;
+  }
+  if (r >= 0 && !f.read("/Phiall", false, slab.data())) { fail("%s: %s", indatfile, err.c_str()); return 9; }
+  double* Phi = electrovars + 6 * N;
+  memset(Phi, 0, N * sizeof(double));
+  for (long i3 = 0; i3 < lx3; ++i3)
+    for (long i2 = 0; i2 < lx2; ++i2) {
+      const double v = slab[i2 + lx2 * i3];
+      double* row = Phi + 2 + L1 * ((i2 + 2) + L2 * (i3 + 2));
+      for (long i1 = 0; i1 < lx1; ++i1) row[i1] = v;
+    }
+  return 0;
 }
 
 }  // extern "C"

@@ -148,6 +148,15 @@ void launch_source_loss_mass(const DevGrid& g, const DevState& s, const DevSourc
                              double dt, hipStream_t st);
 void launch_zero(double* p, long n, hipStream_t st);
 
+// ---- plasma output (kernels_output.hip) ----
+// Number of real32 values of the field datasets plasma_output_nompi writes for flagoutput
+// (io_nompi.f90:230-266), and the kernel that forms them from the HBM state: fluid is the
+// 35-field fluidvars block, J the ghosted fields J1, J2, J3, Phi back to back; out receives
+// the datasets in file order (Phiall last).
+long output_count(const DevGrid& g, int flagoutput);
+void launch_output_pack(const DevGrid& g, const double* fluid, const double* J, int flagoutput,
+                        float* out, hipStream_t st);
+
 // kernel-timing hooks implemented by the API layer
 void timing_begin(const char* name, hipStream_t st);
 void timing_end(const char* name, hipStream_t st);

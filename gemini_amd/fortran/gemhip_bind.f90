@@ -21,7 +21,7 @@ module gemhip_bind
             halo_allparams_C, halo_fluidvars_C, clean_param_C, VNRicht_artvisc_C, &
             compression_C, RK2_prep_mpi_allspec_C, RK2_global_boundary_allspec_C, dt_select_C, &
             source_loss_allparams_C, source_loss_mass_C, source_loss_momentum_C, &
-            source_loss_energy_C, clear_ionization_arrays_C
+            source_loss_energy_C, clear_ionization_arrays_C, plasma_output_nompi_C
   public :: gemhip_init, gemhip_finalize, gemhip_set_error_mode, gemhip_clear_error, &
             gemhip_set_grid_size, gemhip_register_grid, gemhip_register_neutrals, &
             gemhip_register_config, gemhip_set_residency, gemhip_state_push, &
@@ -29,7 +29,7 @@ module gemhip_bind
             gemhip_transport_step, gemhip_synchronize, gemhip_error_message, &
             gemhip_register_null_cells, gemhip_fluid_adv_block, gemhip_register_cfl, &
             gemhip_register_atmosphere, gemhip_register_gravity, gemhip_set_ionization_rates, &
-            gemhip_sources_block
+            gemhip_sources_block, gemhip_register_output, gemhip_read_initial_conditions
 
   ! ------------------------------------------------ part 1: reference entry points
   interface
@@ -234,6 +234,13 @@ module gemhip_bind
       import :: c_ptr
       type(c_ptr) :: intvars
     end subroutine
+    subroutine plasma_output_nompi_C(cfgC, ymd, UTsec, fluidvars, electrovars, identifier, &
+                                     x1lims, x2lims, x3lims) bind(C, name="plasma_output_nompi_C")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: cfgC, fluidvars, electrovars
+      integer(c_int) :: ymd(3), identifier
+      real(c_double) :: UTsec, x1lims(2), x2lims(2), x3lims(2)
+    end subroutine
   end interface
 
   ! ------------------------------------------------ part 2: library set-up
@@ -338,6 +345,20 @@ module gemhip_bind
     end function
     integer(c_int) function gemhip_synchronize() bind(C, name="gemhip_synchronize")
       import :: c_int
+    end function
+    ! character arguments are C strings: trim(path) // c_null_char
+    integer(c_int) function gemhip_register_output(outdir, flagoutput, comp_lvl) &
+        bind(C, name="gemhip_register_output")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: outdir(*)
+      integer(c_int), value :: flagoutput, comp_lvl
+    end function
+    integer(c_int) function gemhip_read_initial_conditions(indatsize, indatfile, lx1, lx2, lx3, &
+        fluidvars, electrovars) bind(C, name="gemhip_read_initial_conditions")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: indatsize(*), indatfile(*)
+      integer(c_int), value :: lx1, lx2, lx3
+      real(c_double) :: fluidvars(*), electrovars(*)
     end function
     integer(c_long) function c_strlen(s) bind(C, name="strlen")
       import :: c_long, c_ptr

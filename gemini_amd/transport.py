@@ -75,8 +75,45 @@ def load_library(path: str = LIB_PATH):
     lib.gemhip_register_config.argtypes = [C.c_int, C.c_double]
     lib.gemhip_kernel_time.argtypes = [C.c_char_p, _dp, C.POINTER(C.c_long)]
     lib.gemhip_eval_math.argtypes = [C.c_int, _dp, _dp, C.c_long]
+    for name in ("gemhip_register_output", "gemhip_output_filename",
+                 "gemhip_read_initial_conditions"):
+        getattr(lib, name).restype = C.c_int
+    lib.gemhip_register_output.argtypes = [C.c_char_p, C.c_int, C.c_int]
+    lib.gemhip_output_filename.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.c_double, C.c_int,
+                                           C.c_char_p, C.c_long]
+    lib.gemhip_read_initial_conditions.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int,
+                                                   C.c_int, _dp, _dp]
     _lib = lib
     return lib
+
+
+def _host_call(name, *args):
+    """Call a host-only entry point (no device context needed) and raise on its error."""
+    lib = load_library()
+    lib.gemhip_set_error_mode(1)
+    rc = getattr(lib, name)(*args)
+    msg = lib.gemhip_last_error().decode()
+    if msg or rc:
+        lib.gemhip_clear_error()
+        raise GemhipError(msg or f"error code {rc}")
+    return rc
+
+
+def output_filename(outdir: str, ymd, UTsec: float, identifier: int) -> str:
+    """The file plasma_output_nompi_C writes: outdir/YYYYMMDD_SSSSS.ffffff_<id>.h5
+    (src/temporal/timeutils.f90:133-196, src/io/io_nompi.f90:215-221)."""
+    buf = C.create_string_buffer(4096)
+    _host_call("gemhip_output_filename", outdir.encode(), (C.c_int * 3)(*[int(v) for v in ymd]),
+               float(UTsec), int(identifier), buf, len(buf))
+    return buf.value.decode()
+
+
+def read_initial_conditions(simsize: str, icfile: str, st: State):
+    """Fill ns, vs1, Ts and the potential of st from the reference's input files
+    (simsize.h5, initial_conditions.h5: src/io/io_nompi.f90:271-345)."""
+    lx1, lx2, lx3 = (d - 4 for d in st.fluidvars.shape[:3])
+    _host_call("gemhip_read_initial_conditions", simsize.encode(), icfile.encode(), lx1, lx2, lx3,
+               _p(st.fluidvars), _p(st.electrovars))
 
 
 def _p(a):
@@ -89,6 +126,11 @@ def _p(a):
     if a.dtype.__repr__() != "torch.float64" or not a.is_contiguous():
         raise GemhipError("tensors must be contiguous float64")
     return C.cast(C.c_void_p(a.data_ptr()), _dp)
+
+
+def _addr(a) -> int:
+    """Address of a float64 numpy array (Fortran order) or contiguous float64 torch tensor."""
+    return C.cast(_p(a), C.c_void_p).value
 
 
 class _Ref:
@@ -142,6 +184,7 @@ class Transport:
         self._null = C.c_void_p(None)
         self.residency = 0
         self._call("gemhip_set_residency", 0)
+        self._outdir = None
 
     # ------------------------------------------------------------------ plumbing
     def _sel(self):
@@ -497,6 +540,30 @@ class Transport:
         replacing the Dirichlet top row; None restores it."""
         q = self._column_param(qtop)
         self._call("gemhip_register_top_heat_flux", None if q is None else _p(q))
+
+    def register_output(self, outdir: str, flagoutput: int = 1, comp_lvl: int = 3):
+        """cfg%outdir and cfg%flagoutput for plasma_output; comp_lvl is the deflate level of
+        the field datasets (the reference's phys_consts comp_lvl is 3; 0 stores them raw)."""
+        self._call("gemhip_register_output", outdir.encode(), int(flagoutput), int(comp_lvl))
+        self._outdir = outdir
+
+    def plasma_output(self, st, ymd, UTsec: float, identifier: int = 0, x1lims=(0.0, 0.0),
+                      x2lims=(0.0, 0.0), x3lims=(0.0, 0.0), electrovars=None) -> str:
+        """plasma_output_nompi_C: write the state to outdir/<date stem>_<identifier>.h5 and
+        return the path.  With the state resident in HBM the fluid fields come from there
+        (st may then be None); J1-J3 and the potential come from st.electrovars, or from
+        electrovars (a host array or device tensor holding the electrovars block)."""
+        ev = electrovars if electrovars is not None else st.electrovars
+        f = C.c_void_p(None if st is None else _addr(st.fluidvars))
+        e = C.c_void_p(_addr(ev))
+        y = (C.c_int * 3)(*[int(v) for v in ymd])
+        lims = [(C.c_double * 2)(*[float(v) for v in l]) for l in (x1lims, x2lims, x3lims)]
+        self._void("plasma_output_nompi_C", C.byref(self._null), y, C.byref(C.c_double(UTsec)),
+                   C.byref(f), C.byref(e), C.byref(C.c_int(identifier)), *lims)
+        if self._outdir is None:
+            raise GemhipError("the output directory was registered outside this Transport; "
+                              "call register_output")
+        return output_filename(self._outdir, ymd, UTsec, identifier)
 
     def checkpoint_size(self) -> int:
         """doubles in a restart checkpoint (fluidvars and the x1 boundary velocities)."""

@@ -125,6 +125,19 @@ void source_loss_energy_C(double** fluidvars, double** fluidauxvars, double** el
 /* gemini3d.h:90, libgemini.f90:1202-1207: Prionize and Qeionize to zero. */
 void clear_ionization_arrays_C(void** intvars);
 
+/* gemini3d.h:96, libgemini_c.f90:267-284 -> io_nompi.f90:183-268: write
+   <outdir>/<YYYYMMDD_SSSSS.ffffff>_<identifier>.h5 (timeutils.f90:133-196) holding
+   /flagoutput, /time/ymd, /time/UThour, /x1lims../x3lims, /patchID and the real32 field
+   datasets of cfg%flagoutput (outdir, flagoutput and the deflate level come from
+   gemhip_register_output): 2 -> neall v1avgall Tavgall Teall J1all J2all J3all v2avgall
+   v3avgall, 3 -> neall, otherwise nsall vs1all Tsall J1all J2all J3all v2avgall v3avgall;
+   then Phiall, the (lx2,lx3) potential at x1 = 1 (gridflag 1) or x1 = lx1.  A kernel forms
+   the datasets from the HBM state (from fluidvars in mode 0; in mode 1 fluidvars may be
+   null), J1-J3 and Phi come from electrovars (host or device address). */
+void plasma_output_nompi_C(void** cfgC, int* ymd, double* UTsec, double** fluidvars,
+                           double** electrovars, int* identifier, double* x1lims,
+                           double* x2lims, double* x3lims);
+
 /* ---------------- part 2: library set-up (no counterpart in the reference ABI) ---------------- */
 /* Array arguments of part 2 may be host or HBM addresses: copies use hipMemcpyDefault. */
 
@@ -223,6 +236,27 @@ int gemhip_synchronize(void);
 int gemhip_kernel_timing(int enable);
 int gemhip_kernel_time(const char* name, double* total_ms, long* launches);
 int gemhip_kernel_timing_reset(void);
+
+/* cfg%outdir and cfg%flagoutput (config.f90) for plasma_output_nompi_C, and the deflate
+   level of its field datasets (0 stores them contiguous; the reference's phys_consts
+   comp_lvl is 3).  The HDF5 C library is loaded when a file is first written or read:
+   $GEMHIP_HDF5_LIB, else libhdf5.so on the loader path, else /opt/conda/lib/libhdf5.so. */
+int gemhip_register_output(const char* outdir, int flagoutput, int comp_lvl);
+/* The path plasma_output_nompi_C writes for a date, time and patch (host-only; no
+   context needed).  Returns 0, or nonzero (message in gemhip_last_error) where
+   timeutils.f90:182 stops or out is shorter than the path. */
+int gemhip_output_filename(const char* outdir, const int* ymd, double UTsec, int identifier,
+                           char* out, long len);
+/* getICs_hdf5 (plasma_input_hdf5.f90:93-144; io_nompi.f90:271-345) into the caller's host
+   blocks, with the zeroing of plasma_input_hdf5.f90:78-86 (host-only; no context needed):
+   the sizes in indatsize (/lx1 /lx2 /lx3, else /lxs, else /lx; reader_hdf5.f90:64-92) must
+   be lx1, lx2, lx3; ns, vs1 and Ts of fluidvars are zeroed and their interiors read from
+   /nsall /vs1all /Tsall (shape (lx1,lx2,lx3,7), any real type) of indatfile; if
+   electrovars is given its potential field is zeroed and every interior x1 plane set to
+   /Phiall ((lx2,lx3), or 1-D when lx2 or lx3 is 1; zero when absent).  With the state
+   resident in HBM, gemhip_state_push afterwards. */
+int gemhip_read_initial_conditions(const char* indatsize, const char* indatfile, int lx1,
+                                   int lx2, int lx3, double* fluidvars, double* electrovars);
 
 /* Evaluate on the device the scalar routines the kernels use, for checks against the
    host libm: op 0 x**2.5 as the conductivity uses it, 1 sqrt, 2 reciprocal 1/x,

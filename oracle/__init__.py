@@ -474,3 +474,43 @@ def sources_block(g, st, atm, dt, Q=None, nullpts=None):
     if rc != 0:
         raise RuntimeError(f"oracle sources block failed ({rc})")
     return st
+
+
+# ---- plasma output (src/io/io_nompi.f90:183-268, src/temporal/timeutils.f90:151-196) ----
+def filestem(ymd, UTsec):
+    """The 21-character date stem of an output file, or None where the reference stops."""
+    buf = C.create_string_buffer(32)
+    rc = lib().orc_filestem((C.c_int * 3)(*[int(v) for v in ymd]), C.c_double(UTsec), buf)
+    return None if rc else buf.value.decode()
+
+
+def output_count(lx1, lx2, lx3, flagoutput):
+    lib().orc_output_count.restype = C.c_long
+    return int(lib().orc_output_count(lx1, lx2, lx3, int(flagoutput)))
+
+
+def output_pack(st, gridflag, flagoutput):
+    """The real32 field datasets plasma_output_nompi writes for state st, as a dict name ->
+    array in the file's C-order shape (Fortran shape reversed), in file order."""
+    lx1, lx2, lx3 = (d - 4 for d in st.fluidvars.shape[:3])
+    n, m = lx1 * lx2 * lx3, lx2 * lx3
+    out = np.empty(output_count(lx1, lx2, lx3, flagoutput), dtype=np.float32)
+    lib().orc_output_pack(lx1, lx2, lx3, int(gridflag), int(flagoutput), ptr(st.fluidvars),
+                          ptr(st.electrovars), out.ctypes.data_as(C.c_void_p))
+    if flagoutput == 2:
+        names = [("neall", 0), ("v1avgall", 0), ("Tavgall", 0), ("Teall", 0), ("J1all", 0),
+                 ("J2all", 0), ("J3all", 0), ("v2avgall", 0), ("v3avgall", 0)]
+    elif flagoutput == 3:
+        names = [("neall", 0)]
+    else:
+        names = [("nsall", 7), ("vs1all", 7), ("Tsall", 7), ("J1all", 0), ("J2all", 0),
+                 ("J3all", 0), ("v2avgall", 0), ("v3avgall", 0)]
+    res, off = {}, 0
+    for name, nsp in names:
+        k = (nsp or 1) * n
+        shape = (nsp, lx3, lx2, lx1) if nsp else (lx3, lx2, lx1)
+        res["/" + name] = out[off:off + k].reshape(shape)
+        off += k
+    res["/Phiall"] = out[off:off + m].reshape(lx3, lx2)
+    assert off + m == out.size
+    return res

@@ -29,6 +29,7 @@
 #include "gemini_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1625,3 +1626,130 @@ int orc_sources_block(const orc_grid *g, const orc_atmos *a, double dt, double *
   return 0;
 }
 #undef IIX
+
+/* ==== plasma output (io_nompi.f90:183-258, timeutils.f90:13-27,110-196) ==== */
+
+/* timeutils.f90:13-27 */
+static int days_in_month(int year, int month) {
+  static const int md[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+  if (year < 1600 || year > 2500 || month < 1 || month > 12) return -1;
+  int d = md[month - 1];
+  if (month == 2 && ((year % 4 == 0 && year % 100 != 0) || year % 400 == 0)) d = 29;
+  return d;
+}
+
+/* timeutils.f90:110-130, iteratively */
+static int day_wrap(int *year, int *month, int *day) {
+  if (*month < 1 || *day < 1) return -1;
+  for (;;) {
+    while (*month > 12) { *month -= 12; *year += 1; }
+    const int dm = days_in_month(*year, *month);
+    if (dm < 0) return -1;
+    if (*day <= dm) return 0;
+    *day -= dm;
+    *month += 1;
+  }
+}
+
+/* timeutils.f90:151-196, for a real(wp) UTsec: the 21-character file stem
+   YYYYMMDD_SSSSS.ffffff, time rounded to the nearest 10 ms (nint rounds half away from
+   zero).  Returns 0, or -1 where the reference stops with an error. */
+int orc_filestem(const int ymd[3], double UTsec, char out[22]) {
+  int year = ymd[0], month = ymd[1], day = ymd[2];
+  const double r = round(UTsec * 100);
+  if (!(r >= -2147483648.0 && r <= 2147483647.0) || fabs(r * 10) > 2147483647.0) return -1;
+  int millisec = (int)r * 10;
+  int seconds = millisec / 1000;
+  if (seconds < 0 || seconds > 86400) return -1;
+  if (seconds == 86400) {
+    day += 1;
+    seconds = 0;
+    millisec = 0;
+    if (day_wrap(&year, &month, &day)) return -1;
+  }
+  /* the reference forms this in default integers, whose wrap-around leaves the true value */
+  const long frac = (long)millisec * 1000 - (long)seconds * 1000000;
+  if (year < 0 || year > 9999 || month < 0 || month > 99 || day < 0 || day > 99) return -1;
+  char buf[64];
+  /* a negative count (UTsec just below zero: seconds truncates to 0) overflows the I6.6
+     field, which Fortran fills with asterisks */
+  if (frac < 0)
+    snprintf(buf, sizeof buf, "%04d%02d%02d_%05d.******", year, month, day, seconds);
+  else
+    snprintf(buf, sizeof buf, "%04d%02d%02d_%05d.%06ld", year, month, day, seconds, frac);
+  memcpy(out, buf, 21);
+  out[21] = 0;
+  return 0;
+}
+
+/* number of real32 values plasma_output_nompi writes for the field datasets, in order:
+   flagoutput 2: neall v1avgall Tavgall Teall J1all J2all J3all v2avgall v3avgall Phiall;
+   flagoutput 3: neall Phiall; otherwise: nsall vs1all Tsall J1all J2all J3all v2avgall
+   v3avgall Phiall (io_nompi.f90:230-258).  Phiall is the (lx2, lx3) topside slab. */
+long orc_output_count(int lx1, int lx2, int lx3, int flagoutput) {
+  const long n = (long)lx1 * lx2 * lx3, m = (long)lx2 * lx3;
+  if (flagoutput == 2) return 9 * n + m;
+  if (flagoutput == 3) return n + m;
+  return (3L * LSP + 5) * n + m;
+}
+
+/* io_nompi.f90:199-213 and the real() conversions of :230-258.  fluidvars is the ghosted
+   (lx1+4, lx2+4, lx3+4, 5*lsp) block (ns vs1 vs2 vs3 Ts), electrovars the ghosted
+   (..., 7) block (E1 E2 E3 J1 J2 J3 Phi).  Averages are sum over the lsp-1 ions of
+   ns*v, accumulated from zero in species order, divided by the electron density. */
+void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
+                     const double *fluidvars, const double *electrovars, float *out) {
+  const long L1 = lx1 + 4, L2 = lx2 + 4, L3 = lx3 + 4, Ng = L1 * L2 * L3;
+  const long n = (long)lx1 * lx2 * lx3, m = (long)lx2 * lx3, B = (long)LSP * Ng;
+  const double *ns = fluidvars, *vs1 = fluidvars + B, *vs2 = fluidvars + 2 * B,
+               *vs3 = fluidvars + 3 * B, *Ts = fluidvars + 4 * B;
+  const double *J1 = electrovars + 3 * Ng, *J2 = electrovars + 4 * Ng,
+               *J3 = electrovars + 5 * Ng, *Phi = electrovars + 6 * Ng;
+  float *o = out;
+  for (int i3 = 0; i3 < lx3; ++i3)
+    for (int i2 = 0; i2 < lx2; ++i2)
+      for (int i1 = 0; i1 < lx1; ++i1) {
+        const long c = (i1 + 2) + L1 * ((i2 + 2) + L2 * (i3 + 2));
+        const long k = i1 + (long)lx1 * (i2 + (long)lx2 * i3);
+        const long e = (long)(LSP - 1) * Ng + c;
+        double a2 = 0.0, a3 = 0.0, a1 = 0.0, aT = 0.0;
+        for (int isp = 0; isp < LSP - 1; ++isp) {
+          const long s = (long)isp * Ng + c;
+          a2 = a2 + ns[s] * vs2[s];
+          a3 = a3 + ns[s] * vs3[s];
+          a1 = a1 + ns[s] * vs1[s];
+          aT = aT + ns[s] * Ts[s];
+        }
+        const double v2avg = a2 / ns[e], v3avg = a3 / ns[e];
+        if (flagoutput == 3) {
+          o[k] = (float)ns[e];
+        } else if (flagoutput == 2) {
+          o[k] = (float)ns[e];
+          o[n + k] = (float)(a1 / ns[e]);
+          o[2 * n + k] = (float)(aT / ns[e]);
+          o[3 * n + k] = (float)Ts[e];
+          o[4 * n + k] = (float)J1[c];
+          o[5 * n + k] = (float)J2[c];
+          o[6 * n + k] = (float)J3[c];
+          o[7 * n + k] = (float)v2avg;
+          o[8 * n + k] = (float)v3avg;
+        } else {
+          for (int isp = 0; isp < LSP; ++isp) {
+            const long s = (long)isp * Ng + c;
+            o[isp * n + k] = (float)ns[s];
+            o[(LSP + isp) * n + k] = (float)vs1[s];
+            o[(2 * LSP + isp) * n + k] = (float)Ts[s];
+          }
+          o[3 * LSP * n + k] = (float)J1[c];
+          o[(3 * LSP + 1) * n + k] = (float)J2[c];
+          o[(3 * LSP + 2) * n + k] = (float)J3[c];
+          o[(3 * LSP + 3) * n + k] = (float)v2avg;
+          o[(3 * LSP + 4) * n + k] = (float)v3avg;
+        }
+      }
+  float *phi = out + orc_output_count(lx1, lx2, lx3, flagoutput) - m;
+  const int top = gridflag == 1 ? 0 : lx1 - 1;   /* Phi(1,:,:) inverted, else Phi(lx1,:,:) */
+  for (int i3 = 0; i3 < lx3; ++i3)
+    for (int i2 = 0; i2 < lx2; ++i2)
+      phi[i2 + (long)lx2 * i3] = (float)Phi[(top + 2) + L1 * ((i2 + 2) + L2 * (i3 + 2))];
+}

@@ -161,6 +161,13 @@ int orc_sources_block(const orc_grid *g, const orc_atmos *a, double dt, double *
                       double *fluidauxvars, const double *electrovars, const double *vs1i,
                       const double *Q, const int *nullpts);
 
+/* plasma output (io_nompi.f90:183-258, timeutils.f90:151-196): the file stem, the
+   number of real32 values of the field datasets and their values in dataset order */
+int orc_filestem(const int ymd[3], double UTsec, char out[22]);
+long orc_output_count(int lx1, int lx2, int lx3, int flagoutput);
+void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
+                     const double *fluidvars, const double *electrovars, float *out);
+
 /* 0: x**2.5 by libm pow, as the reference (default); 1: correctly rounded x**2.5 */
 void orc_set_pow_mode(int mode);
 void orc_eval_pow25(const double *x, double *y, long n, int mode);
