@@ -12,13 +12,22 @@
 #include <dlfcn.h>
 #include <hdf5.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <set>
 #include <string>
+#include <thread>
+#include <vector>
 
 namespace gemhip {
 namespace {
+
+// zlib's one-shot compressor (the call HDF5's deflate filter makes), found through the
+// HDF5 library's own dependency on zlib
+using compress2_fn = int (*)(unsigned char*, unsigned long*, const unsigned char*, unsigned long,
+                             int);
+using compress_bound_fn = unsigned long (*)(unsigned long);
 
 // Function table of the HDF5 calls used here, resolved by name.  The header declares the
 // functions; decltype keeps every pointer's type that of the declaration.
@@ -58,6 +67,10 @@ struct H5Api {
   const hid_t* native_double = nullptr;
   const hid_t* native_int = nullptr;
   const hid_t* dataset_create = nullptr;
+  // optional: pre-filtered chunk writes (HDF5 >= 1.10.3) and zlib, for parallel compression
+  decltype(&::H5Dwrite_chunk) H5Dwrite_chunk = nullptr;
+  compress2_fn z_compress2 = nullptr;
+  compress_bound_fn z_bound = nullptr;
 };
 
 const H5Api& api() {
@@ -114,6 +127,9 @@ const H5Api& api() {
     t.native_int = static_cast<const hid_t*>(sym("H5T_NATIVE_INT_g"));
     t.dataset_create = static_cast<const hid_t*>(sym("H5P_CLS_DATASET_CREATE_ID_g"));
     if (!all) return t;
+    t.H5Dwrite_chunk = reinterpret_cast<decltype(t.H5Dwrite_chunk)>(dlsym(h, "H5Dwrite_chunk"));
+    t.z_compress2 = reinterpret_cast<compress2_fn>(dlsym(h, "compress2"));
+    t.z_bound = reinterpret_cast<compress_bound_fn>(dlsym(h, "compressBound"));
     unsigned maj = 0, min = 0, rel = 0;
     if (t.H5open() < 0 || t.H5get_libversion(&maj, &min, &rel) < 0) {
       t.why = "HDF5 library initialisation failed";
@@ -137,8 +153,8 @@ bool have(std::string& err) {
   return a.ok;
 }
 
-// chunk shape for a compressed dataset: whole trailing (fastest) planes up to 1 MiB,
-// leading dimensions cut to 1 first
+// chunk shape for a compressed dataset: leading dimensions cut to 1 while the chunk is over
+// 1 MiB, then the second-fastest dimension cut to fit (whole rows of the fastest one)
 void chunk_dims(int rank, const long* dims, size_t elem, hsize_t* ch) {
   for (int i = 0; i < rank; ++i) ch[i] = (hsize_t)(dims[i] > 0 ? dims[i] : 1);
   const size_t cap = 1 << 20;
@@ -147,11 +163,102 @@ void chunk_dims(int rank, const long* dims, size_t elem, hsize_t* ch) {
     for (int i = 0; i < rank; ++i) b *= ch[i];
     return b;
   };
-  for (int i = 0; i < rank - 1 && bytes() > cap; ++i) ch[i] = 1;
+  for (int i = 0; i < rank - 2 && bytes() > cap; ++i) ch[i] = 1;
   if (bytes() > cap && rank >= 2) {
     const size_t row = elem * ch[rank - 1];
     ch[rank - 2] = (hsize_t)(cap / row > 0 ? cap / row : 1);
   }
+}
+
+int io_threads() {
+  for (const char* v : {"GEMHIP_IO_THREADS", "OMP_NUM_THREADS"}) {
+    const char* e = getenv(v);
+    if (e && atoi(e) > 0) return std::min(atoi(e), 64);
+  }
+  const unsigned h = std::thread::hardware_concurrency();
+  return h ? (int)std::min(h, 64u) : 1;
+}
+
+// Write a float dataset created with the shuffle and deflate filters by compressing its
+// chunks on a thread pool and handing HDF5 the filtered bytes (H5Dwrite_chunk, filter mask
+// 0: every filter of the pipeline applied).  Each chunk is gathered (zero-padded where it
+// passes the dataset's edge), byte-shuffled as the shuffle filter does (byte b of element
+// i to position b*n + i) and compressed with compress2 at the deflate level, which is what
+// HDF5's own filters would store for it.
+bool write_chunks_parallel(hid_t dset, int rank, const long* dims, const hsize_t* ch,
+                           const float* data, int level, std::string& err) {
+  const H5Api& a = api();
+  long nc[4], nchunks = 1, ce = 1;
+  for (int k = 0; k < rank; ++k) {
+    nc[k] = (dims[k] + (long)ch[k] - 1) / (long)ch[k];
+    nchunks *= nc[k];
+    ce *= (long)ch[k];
+  }
+  const size_t raw = (size_t)ce * sizeof(float);
+  std::vector<std::vector<unsigned char>> out((size_t)nchunks);
+  std::vector<int> bad((size_t)nchunks, 0);
+  const int nt = (int)std::min<long>(io_threads(), nchunks);
+  auto work = [&](int t) {
+    std::vector<float> buf((size_t)ce);
+    std::vector<unsigned char> shuf(raw);
+    const unsigned long cap = a.z_bound((unsigned long)raw);
+    for (long j = t; j < nchunks; j += nt) {
+      long idx[4], off[4];
+      long r = j;
+      for (int k = rank - 1; k >= 0; --k) { idx[k] = r % nc[k]; r /= nc[k]; off[k] = idx[k] * (long)ch[k]; }
+      // gather rows of the last dimension
+      const long rowlen = (long)ch[rank - 1];
+      const long valid = std::min(rowlen, dims[rank - 1] - off[rank - 1]);
+      const long nrows = ce / rowlen;
+      for (long q = 0; q < nrows; ++q) {
+        long rem = q, src = 0, stride = 1;
+        bool inside = true;
+        long pos[4];
+        for (int k = rank - 2; k >= 0; --k) { pos[k] = rem % (long)ch[k]; rem /= (long)ch[k]; }
+        for (int k = rank - 1; k >= 0; --k) {
+          const long c = k == rank - 1 ? off[k] : off[k] + pos[k];
+          if (c >= dims[k]) inside = false;
+          src += c * stride;
+          stride *= dims[k];
+        }
+        float* dst = buf.data() + q * rowlen;
+        if (inside) {
+          memcpy(dst, data + src, (size_t)valid * sizeof(float));
+          std::fill(dst + valid, dst + rowlen, 0.0f);
+        } else {
+          std::fill(dst, dst + rowlen, 0.0f);
+        }
+      }
+      const unsigned char* bytes = reinterpret_cast<const unsigned char*>(buf.data());
+      const unsigned char* src = bytes;
+      if (ce > 1) {
+        for (long i = 0; i < ce; ++i)
+          for (int b = 0; b < 4; ++b) shuf[(size_t)b * ce + i] = bytes[(size_t)i * 4 + b];
+        src = shuf.data();
+      }
+      std::vector<unsigned char>& o = out[(size_t)j];
+      o.resize(cap);
+      unsigned long len = cap;
+      if (a.z_compress2(o.data(), &len, src, (unsigned long)raw, level) != 0) { bad[(size_t)j] = 1; continue; }
+      o.resize(len);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (long j = 0; j < nchunks; ++j) {
+    if (bad[(size_t)j]) { err = "chunk compression failed"; return false; }
+    hsize_t off[4];
+    long r = j;
+    for (int k = rank - 1; k >= 0; --k) { off[k] = (hsize_t)((r % nc[k]) * (long)ch[k]); r /= nc[k]; }
+    if (a.H5Dwrite_chunk(dset, H5P_DEFAULT, 0, off, out[(size_t)j].size(), out[(size_t)j].data()) < 0) {
+      err = "could not write a compressed chunk";
+      return false;
+    }
+    std::vector<unsigned char>().swap(out[(size_t)j]);
+  }
+  return true;
 }
 
 // the access flags of H5Fpublic.h without the macros' library calls (H5check, H5open)
@@ -204,7 +311,17 @@ bool h5_write(const char* path, const H5Out* ds, int nds, int comp_lvl, std::str
       err = "waiting for the device copy of " + name + " failed";
       ok = false;
     }
-    if (ok && a.H5Dwrite(dset, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, d.data) < 0) {
+    const bool parallel = ok && dcpl != H5P_DEFAULT && d.type == H5Out::F32 && a.H5Dwrite_chunk &&
+                          a.z_compress2 && a.z_bound;
+    if (parallel) {
+      hsize_t ch[4];
+      chunk_dims(d.rank, d.dims, 4, ch);
+      if (!write_chunks_parallel(dset, d.rank, d.dims, ch, static_cast<const float*>(d.data),
+                                 comp_lvl, err)) {
+        err = name + ": " + err;
+        ok = false;
+      }
+    } else if (ok && a.H5Dwrite(dset, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, d.data) < 0) {
       err = "could not write dataset " + name;
       ok = false;
     }

@@ -150,3 +150,19 @@ def test_output_errors(tmp_path):
     path = tr.plasma_output(st, YMD, UTSEC, IDENT, *LIMS)   # the context still works
     check_file(path, st, g.gridflag, 1)
     tr.close()
+
+
+def test_compressed_output_in_many_chunks(tmp_path):
+    """x1-x2 planes over 1 MiB are stored in chunks of whole x1 rows, the last one partial;
+    the chunks are shuffled and deflated on the host's threads and written pre-filtered,
+    and any HDF5 reader decompresses them to the packed values."""
+    print("case cart 600x500x3 flagoutput 1 deflate 3", flush=True)
+    b = ga.cartesian_grid(600, 500, 3)
+    st = ga.synthetic_state(b, seed=9)
+    st.electrovars[..., 3:7] = np.random.default_rng(9).normal(size=st.electrovars[..., 3:7].shape)
+    tr = Transport(b, st.nn, st.Tn)
+    tr.register_output(str(tmp_path), 1, 3)
+    tr.set_residency(1)
+    tr.push(st)
+    path = tr.plasma_output(None, YMD, UTSEC, IDENT, *LIMS, electrovars=st.electrovars)
+    check_file(path, st, b.gridflag, 1)
