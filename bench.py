@@ -224,6 +224,7 @@ def measure_output(tr, g, electrovars, sample_shape, reps=3):
                 "pack_algorithmic_bytes": nb,
                 "pack_GB/s": round(nb / (pack_ms / 1e3) / 1e9, 1),
                 "pack_frac": round(nb / (pack_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "pack_traffic": load_traffic("dipole512", f"output_pack{flag}"),
                 "real32_values": values, "file_bytes": size,
                 "call_s": round(wall, 3), "call_GB/s_of_real32": round(4 * values / wall / 1e9, 3)}
             log(f"output flagoutput {flag} (deflate {comp}): pack {pack_ms:.3f} ms "

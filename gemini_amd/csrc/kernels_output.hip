@@ -25,13 +25,16 @@ template <int FLAG>
 __global__ __launch_bounds__(TPB) void k_output_pack(DevGrid g, const double* __restrict__ fluid,
                                                       const double* __restrict__ J,
                                                       float* __restrict__ out) {
+  // blocks in memory order: block b covers part b % nb of x1 line b / nb (nb blocks a
+  // line), so the index needs 32-bit arithmetic only and the grid streams through HBM
+  const int nb = (g.lx1 + TPB - 1) / TPB;
+  const int line = (int)(blockIdx.x / (unsigned)nb);
+  const int i1 = (int)(blockIdx.x % (unsigned)nb) * TPB + threadIdx.x;
+  if (i1 >= g.lx1) return;
+  const int i2 = line % g.lx2;
+  const int i3 = line / g.lx2;
   const long n = (long)g.lx1 * g.lx2 * g.lx3;
-  const long k = (long)blockIdx.x * TPB + threadIdx.x;
-  if (k >= n) return;
-  const int i1 = (int)(k % g.lx1);
-  const long r = k / g.lx1;
-  const int i2 = (int)(r % g.lx2);
-  const int i3 = (int)(r / g.lx2);
+  const long k = i1 + (long)g.lx1 * line;
   const long N = g.N, B = LSP * N;
   const long c = gix(g, i1 + 1, i2 + 1, i3 + 1);
   const double* ns = fluid;
@@ -104,8 +107,7 @@ long output_count(const DevGrid& g, int flagoutput) {
 
 void launch_output_pack(const DevGrid& g, const double* fluid, const double* J,
                         int flagoutput, float* out, hipStream_t st) {
-  const long n = (long)g.lx1 * g.lx2 * g.lx3;
-  const dim3 grid((unsigned)((n + TPB - 1) / TPB));
+  const dim3 grid((unsigned)(g.lx2 * g.lx3) * (unsigned)((g.lx1 + TPB - 1) / TPB));
   timing_begin("output_pack", st);
   if (flagoutput == 2)
     k_output_pack<2><<<grid, TPB, 0, st>>>(g, fluid, J, out);
