@@ -161,6 +161,12 @@ int main(int argc, char** argv) {
     }
     check(gemhip_get_interface_vels(vs1i.data(), vs2i.data(), vs3i.data()), "get_interface_vels");
   }
+  // the plasma output of the final state (gemini3d.h:96), flagoutput 1 + mode % 3, deflate 3
+  check(gemhip_register_output(dir.c_str(), 1 + mode % 3, 3), "register_output");
+  int ymd[3] = {2015, 4, 13}, ident = mode;
+  double utsec = 12345.67891, l1[2] = {0.0, 1.0}, l2[2] = {2.0, 3.0}, l3[2] = {4.0, 5.0};
+  plasma_output_nompi_C(&cfgC, ymd, &utsec, &fluidvars, &electrovars, &ident, l1, l2, l3);
+  if (gemhip_last_error()[0]) check(1, "plasma output");
   FILE* o = fopen((dir + "/out.bin").c_str(), "wb");
   fwrite(fluid.data(), sizeof(double), fluid.size(), o);
   fwrite(aux.data(), sizeof(double), 14 * L, o);

@@ -10,6 +10,8 @@
 !> modes 2 and 3: the same for the whole transport block (src/libgemini.cpp:155-210);
 !> modes 4 and 5: the block followed by the sources substep (:223-232), call by call
 !>         or fused (gemhip_sources_block).
+!> Every mode then writes the final state with plasma_output_nompi_C (flagoutput
+!> 1 + mod(mode, 3)) into the case directory.
 program fortran_driver
   use, intrinsic :: iso_c_binding
   use gemhip_bind
@@ -29,6 +31,8 @@ program fortran_driver
   type(c_ptr) :: fvp, fap, evp, xC, intvars, cfgC
   character(len=4096) :: dir
   integer :: u, k, rc, w
+  integer(c_int) :: ymd(3), ident
+  real(c_double) :: UTsec, x1lims(2), x2lims(2), x3lims(2)
 
   call get_command_argument(1, dir)
   open(newunit=u, file=trim(dir)//'/case.txt', status='old', action='read')
@@ -147,6 +151,16 @@ program fortran_driver
     end do
   end if
   if (mode == 0 .or. mode == 2 .or. mode == 4) rc = gemhip_get_interface_vels(vs1i, vs2i, vs3i)
+  ! the plasma output of the final state (gemini3d.h:96), flagoutput 1 + mod(mode, 3), deflate 3
+  rc = gemhip_register_output(trim(dir)//c_null_char, 1_c_int + mod(mode, 3_c_int), 3_c_int)
+  ymd = [2015, 4, 13]
+  UTsec = 12345.67891_c_double
+  ident = mode
+  x1lims = [0.0_c_double, 1.0_c_double]
+  x2lims = [2.0_c_double, 3.0_c_double]
+  x3lims = [4.0_c_double, 5.0_c_double]
+  call plasma_output_nompi_C(cfgC, ymd, UTsec, fvp, evp, ident, x1lims, x2lims, x3lims)
+  if (rc /= 0 .or. len(gemhip_error_message()) > 0) error stop 'output failed: '//gemhip_error_message()
   open(newunit=w, file=trim(dir)//'/out.bin', access='stream', form='unformatted', status='replace')
   write(w) fluidvars, fluidauxvars(:,:,:,1:14), vs1i, vs2i, vs3i
   close(w)

@@ -5,7 +5,9 @@ src/libgemini.cpp:155-208 (mode 0) or the fused step (mode 1), and the same for 
 transport block of src/libgemini.cpp:155-210 (modes 2 and 3).  Their results must be
 bit-identical to the oracle's (with the GPU's x**2.5).  Modes 4 and 5 follow the block with
 the sources substep of :223-232, call by call and fused; its collision rates call exp, erf
-and log, so those runs are held to 1e-12 (normwise, per species) instead."""
+and log, so those runs are held to 1e-12 (normwise, per species) instead.  Every run ends
+with the host's own plasma_output_nompi_C call, whose file must hold the driver's final
+state bit for bit."""
 import os
 import subprocess
 
@@ -15,6 +17,7 @@ import pytest
 import gemini_amd as ga
 import oracle
 from helpers import compare, write_case, read_case_output
+import h5lite
 
 pytestmark = pytest.mark.gpu
 
@@ -58,3 +61,12 @@ def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
     if mode in (0, 2):
         for a in ("vs1i", "vs2i", "vs3i"):
             assert np.array_equal(getattr(got, a), getattr(ref, a)), a
+    # the host's plasma_output_nompi_C call: the file holds the driver's final state
+    flag = 1 + mode % 3
+    path = os.path.join(str(tmp_path), f"{oracle.filestem([2015, 4, 13], 12345.67891)}_{mode}.h5")
+    want = oracle.output_pack(got, g.gridflag, flag)
+    with h5lite.File(path) as f:
+        assert f.read("/flagoutput") == flag and f.read("/patchID") == mode
+        for name, arr in want.items():
+            a = f.read(name)
+            assert a.shape == arr.shape and np.array_equal(a.view(np.uint32), arr.view(np.uint32)), name
