@@ -1502,6 +1502,42 @@ bool date_stem(const char* outdir, const int* ymd, double UTsec, std::string& ou
   return true;
 }
 
+// reader_hdf5.f90:64-92: /lx1 /lx2 /lx3, else /lxs, else /lx (three integers)
+bool read_simsize(const char* path, int lin[3]) {
+  std::string err;
+  H5In f(path, err);
+  if (!f.open()) { fail("%s", err.c_str()); return false; }
+  long d[8];
+  auto scalar = [&](const char* name, int& v) {
+    const int r = f.ndims(name, d);
+    long cnt = 1;
+    for (int k = 0; k < r; ++k) cnt *= d[k];
+    int buf[8];
+    if (r < 0 || cnt < 1 || cnt > 8 || !f.read(name, true, buf)) return false;
+    v = buf[0];
+    return true;
+  };
+  bool got = false;
+  if (f.ndims("/lx1", d) >= 0) {
+    got = scalar("/lx1", lin[0]) && scalar("/lx2", lin[1]) && scalar("/lx3", lin[2]);
+  } else {
+    for (const char* name : {"/lxs", "/lx"}) {
+      const int r = f.ndims(name, d);
+      if (r < 0) continue;
+      int buf[8];
+      got = r == 1 && d[0] == 3 && f.read(name, true, buf);
+      if (got) for (int k = 0; k < 3; ++k) lin[k] = buf[k];
+      break;
+    }
+  }
+  if (!got) {
+    fail("%s: no grid size (/lx1 /lx2 /lx3, /lxs or /lx)%s%s", path, err.empty() ? "" : ": ",
+         err.c_str());
+    return false;
+  }
+  return true;
+}
+
 bool ensure_output_buffers(size_t dev_bytes, size_t host_bytes) {
   Ctx& c = ctx();
   if (c.out_bytes < dev_bytes) {
@@ -1647,6 +1683,61 @@ void plasma_output_nompi_C(void** cfgC, int* ymd, double* UTsec, double** fluidv
   for (hipEvent_t e : evs) c.pool.push_back(e);
 }
 
+// reader_hdf5.f90:64-92 (get_simsize3): the full-grid sizes of a simsize file
+int gemhip_read_simsize(const char* path, int* lx1, int* lx2, int* lx3) {
+  if (!path || !lx1 || !lx2 || !lx3) { fail("null argument to gemhip_read_simsize"); return 1; }
+  int lin[3];
+  if (!read_simsize(path, lin)) return 2;
+  *lx1 = lin[0];
+  *lx2 = lin[1];
+  *lx3 = lin[2];
+  return 0;
+}
+
+// readgrid_hdf5.f90:11-39 (get_grid3_coords_hdf5): the ghosted coordinates x1(-1:lx1+2),
+// x2(-1:lx2+2), x3(-1:lx3+2) and the grid centre of path, or of path/simgrid.h5 when path
+// does not name a simgrid.h5 file
+int gemhip_read_grid_coords(const char* path, int lx1, int lx2, int lx3, double* x1, double* x2,
+                            double* x3, double* glonctr, double* glatctr) {
+  if (!path || !x1 || !x2 || !x3 || !glonctr || !glatctr) {
+    fail("null argument to gemhip_read_grid_coords");
+    return 1;
+  }
+  std::string fn = path;
+  if (fn.find("simgrid.h5") == std::string::npos) fn += "/simgrid.h5";
+  std::string err;
+  H5In f(fn.c_str(), err);
+  if (!f.open()) { fail("%s", err.c_str()); return 2; }
+  const struct { const char* name; double* dst; int n; } xs[3] = {
+      {"/x1", x1, lx1 + 4}, {"/x2", x2, lx2 + 4}, {"/x3", x3, lx3 + 4}};
+  for (const auto& x : xs) {
+    long d[8];
+    const int r = f.ndims(x.name, d);
+    if (r != 1 || d[0] != x.n) {
+      fail("%s: %s must hold %d values (the cells and two ghost cells at each end)", fn.c_str(),
+           x.name, x.n);
+      return 3;
+    }
+    if (!f.read(x.name, false, x.dst)) { fail("%s: %s", fn.c_str(), err.c_str()); return 4; }
+  }
+  for (const char* name : {"/glonctr", "/glatctr"}) {
+    long d[8];
+    const int r = f.ndims(name, d);
+    if (r < 0) {
+      fail("please specify /glonctr and /glatctr in %s", fn.c_str());
+      return 5;
+    }
+    long cnt = 1;
+    for (int k = 0; k < r; ++k) cnt *= d[k];
+    if (cnt != 1) { fail("%s: %s must be one value", fn.c_str(), name); return 5; }
+  }
+  if (!f.read("/glonctr", false, glonctr) || !f.read("/glatctr", false, glatctr)) {
+    fail("%s: %s", fn.c_str(), err.c_str());
+    return 6;
+  }
+  return 0;
+}
+
 // io_nompi.f90:271-345 with the zeroing of plasma_input_hdf5.f90:78-86: sizes from
 // indatsize (/lx1 /lx2 /lx3, else /lxs, else /lx) must equal lx1, lx2, lx3; ns, vs1 and Ts
 // of fluidvars are zeroed and their interiors read from /nsall /vs1all /Tsall of indatfile;
@@ -1658,35 +1749,7 @@ int gemhip_read_initial_conditions(const char* indatsize, const char* indatfile,
   if (lx1 < 1 || lx2 < 1 || lx3 < 1) { fail("grid size %d x %d x %d", lx1, lx2, lx3); return 1; }
   std::string err;
   int lin[3] = {0, 0, 0};
-  {
-    H5In f(indatsize, err);
-    if (!f.open()) { fail("%s", err.c_str()); return 2; }
-    long d[8];
-    auto scalar = [&](const char* name, int& v) {
-      const int r = f.ndims(name, d);
-      long cnt = 1;
-      for (int k = 0; k < r; ++k) cnt *= d[k];
-      int buf[8];
-      if (r < 0 || cnt < 1 || cnt > 8 || !f.read(name, true, buf)) return false;
-      v = buf[0];
-      return true;
-    };
-    bool got = false;
-    if (f.ndims("/lx1", d) >= 0) {
-      got = scalar("/lx1", lin[0]) && scalar("/lx2", lin[1]) && scalar("/lx3", lin[2]);
-    } else {
-      for (const char* name : {"/lxs", "/lx"}) {
-        const int r = f.ndims(name, d);
-        if (r < 0) continue;
-        int buf[8];
-        got = r == 1 && d[0] == 3 && f.read(name, true, buf);
-        if (got) for (int k = 0; k < 3; ++k) lin[k] = buf[k];
-        break;
-      }
-    }
-    if (!got) { fail("%s: no grid size (/lx1 /lx2 /lx3, /lxs or /lx)%s%s", indatsize,
-                     err.empty() ? "" : ": ", err.c_str()); return 3; }
-  }
+  if (!read_simsize(indatsize, lin)) return 3;
   if (lin[0] != lx1 || lin[1] != lx2 || lin[2] != lx3) {
     fail("The input data must be the same size as the grid which you are running the simulation "
          "on - use a script to interpolate up/down to the simulation grid (file %d x %d x %d, "

@@ -160,6 +160,14 @@ def cartesian_grid(lx1, lx2, lx3, dx2=5e3, dx3=5e3, zmin=80e3, zmax=1000e3,
     x1 = _tanh_stretched_alt(lx1, zmin, zmax)
     x2 = _ghosted_uniform(lx2, dx2)
     x3 = _ghosted_uniform(lx3, dx3)
+    return cartesian_mesh(x1, x2, x3, flagper)
+
+
+def cartesian_mesh(x1, x2, x3, flagper=False):
+    """The Cartesian mesh on ghosted coordinates x1 (altitude, m; lx1+4 values), x2 and x3
+    (m): metric factors 1, altitude x1, gravity -G Me / (Re + x1)^2 along x1."""
+    x1, x2, x3 = (np.asarray(x, dtype=np.float64) for x in (x1, x2, x3))
+    lx1, lx2, lx3 = x1.size - 4, x2.size - 4, x3.size - 4
     L1, L2, L3 = lx1 + 4, lx2 + 4, lx3 + 4
     ones = lambda n2, n1, n3: (np.ones((n2, n1)), n3)
     planes = {"h1": ones(L2, L1, L3), "h2": ones(L2, L1, L3), "h3": ones(L2, L1, L3),
@@ -223,6 +231,11 @@ def dipole_grid(lx1, lx2, lx3, Lmin=1.2, Lmax=5.0, altfoot=80e3, phispan=np.radi
     The q range is the one the innermost shell spans between its 80 km footpoints,
     so every cell lies above the footpoint altitude and no null points occur.
     """
+    x1, x2, x3 = _dipole_coords(lx1, lx2, lx3, Lmin, Lmax, altfoot, phispan)
+    return dipole_mesh(x1, x2, x3)
+
+
+def _dipole_coords(lx1, lx2, lx3, Lmin, Lmax, altfoot, phispan):
     uf = (RE + altfoot) / RE
     qmax = np.sqrt(1.0 - uf / Lmin) / uf ** 2
     # q centres: sinh stretch to refine near the footpoints, ghosts by edge spacing
@@ -237,6 +250,14 @@ def dipole_grid(lx1, lx2, lx3, Lmin=1.2, Lmax=5.0, altfoot=80e3, phispan=np.radi
     x2 = Lmin + dp * (np.arange(-1, lx2 + 3, dtype=np.float64) - 1)
     dphi = phispan / max(lx3, 1)
     x3 = dphi * (np.arange(-1, lx3 + 3, dtype=np.float64) - 1)
+    return x1, x2, x3
+
+
+def dipole_mesh(x1, x2, x3):
+    """The dipole mesh on ghosted coordinates x1 = q, x2 = p (L shell), x3 = phi: radius
+    and colatitude from (q, p), the metric factors, altitude and gravity along x1."""
+    x1, x2, x3 = (np.asarray(x, dtype=np.float64) for x in (x1, x2, x3))
+    lx1, lx2, lx3 = x1.size - 4, x2.size - 4, x3.size - 4
     L1, L2, L3 = lx1 + 4, lx2 + 4, lx3 + 4
     # cell centres (with ghosts); metric factors do not depend on phi: (x2, x1) planes
     r, th = _dipole_r_theta(x1[None, :], x2[:, None])

@@ -76,8 +76,12 @@ def load_library(path: str = LIB_PATH):
     lib.gemhip_kernel_time.argtypes = [C.c_char_p, _dp, C.POINTER(C.c_long)]
     lib.gemhip_eval_math.argtypes = [C.c_int, _dp, _dp, C.c_long]
     for name in ("gemhip_register_output", "gemhip_output_filename",
-                 "gemhip_read_initial_conditions"):
+                 "gemhip_read_initial_conditions", "gemhip_read_simsize",
+                 "gemhip_read_grid_coords"):
         getattr(lib, name).restype = C.c_int
+    lib.gemhip_read_simsize.argtypes = [C.c_char_p] + [C.POINTER(C.c_int)] * 3
+    lib.gemhip_read_grid_coords.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                                            _dp, _dp]
     lib.gemhip_register_output.argtypes = [C.c_char_p, C.c_int, C.c_int]
     lib.gemhip_output_filename.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.c_double, C.c_int,
                                            C.c_char_p, C.c_long]
@@ -106,6 +110,37 @@ def output_filename(outdir: str, ymd, UTsec: float, identifier: int) -> str:
     _host_call("gemhip_output_filename", outdir.encode(), (C.c_int * 3)(*[int(v) for v in ymd]),
                float(UTsec), int(identifier), buf, len(buf))
     return buf.value.decode()
+
+
+def read_simsize(path: str):
+    """(lx1, lx2, lx3) of a simsize file (src/io/reader_hdf5.f90:64-92)."""
+    n = [C.c_int(0) for _ in range(3)]
+    _host_call("gemhip_read_simsize", path.encode(), *[C.byref(v) for v in n])
+    return tuple(v.value for v in n)
+
+
+def read_grid_coords(path: str, sizes):
+    """Ghosted coordinates x1, x2, x3 and (glonctr, glatctr) of a simgrid file or of the
+    directory holding simgrid.h5 (src/numerical/grid/readgrid_hdf5.f90:11-39)."""
+    lx1, lx2, lx3 = sizes
+    xs = [np.empty(n + 4) for n in (lx1, lx2, lx3)]
+    ctr = [C.c_double(0.0), C.c_double(0.0)]
+    _host_call("gemhip_read_grid_coords", path.encode(), lx1, lx2, lx3,
+               *[x.ctypes.data_as(_dp) for x in xs], *[C.cast(C.byref(c), _dp) for c in ctr])
+    return xs[0], xs[1], xs[2], (ctr[0].value, ctr[1].value)
+
+
+def grid_from_files(simsize: str, simgrid: str, kind: str, flagper: bool = False) -> Grid:
+    """A Grid on the coordinates of the reference's input files.  The metric factors,
+    altitude and gravity come from this package's mesh formulas (grid.cartesian_mesh,
+    grid.dipole_mesh), not from the reference's mesh objects."""
+    from .grid import cartesian_mesh, dipole_mesh
+    x1, x2, x3, _ = read_grid_coords(simgrid, read_simsize(simsize))
+    if kind == "cartesian":
+        return cartesian_mesh(x1, x2, x3, flagper)
+    if kind == "dipole":
+        return dipole_mesh(x1, x2, x3)
+    raise GemhipError(f"unknown mesh kind {kind!r}")
 
 
 def read_initial_conditions(simsize: str, icfile: str, st: State):

@@ -247,6 +247,14 @@ int gemhip_register_output(const char* outdir, int flagoutput, int comp_lvl);
    timeutils.f90:182 stops or out is shorter than the path. */
 int gemhip_output_filename(const char* outdir, const int* ymd, double UTsec, int identifier,
                            char* out, long len);
+/* get_simsize3 (reader_hdf5.f90:64-92): the full-grid sizes in a simsize file (/lx1 /lx2
+   /lx3, else /lxs, else /lx).  Host-only; no context needed. */
+int gemhip_read_simsize(const char* path, int* lx1, int* lx2, int* lx3);
+/* get_grid3_coords_hdf5 (readgrid_hdf5.f90:11-39): the ghosted coordinates x1(lx1+4),
+   x2(lx2+4), x3(lx3+4) and the grid centre /glonctr /glatctr (required, as there) of path,
+   or of path/simgrid.h5 when path does not name a simgrid.h5 file.  Host-only. */
+int gemhip_read_grid_coords(const char* path, int lx1, int lx2, int lx3, double* x1, double* x2,
+                            double* x3, double* glonctr, double* glatctr);
 /* getICs_hdf5 (plasma_input_hdf5.f90:93-144; io_nompi.f90:271-345) into the caller's host
    blocks, with the zeroing of plasma_input_hdf5.f90:78-86 (host-only; no context needed):
    the sizes in indatsize (/lx1 /lx2 /lx3, else /lxs, else /lx; reader_hdf5.f90:64-92) must

@@ -12,6 +12,8 @@
   271-345) against files written here through the HDF5 C library (tests/h5lite.py):
   interiors land bit-exactly, ghost cells are zeroed, the potential is extended along x1,
   and malformed inputs are rejected.
+* The grid size and coordinate readers (reader_hdf5.f90:64-92, readgrid_hdf5.f90:11-39):
+  the coordinates written come back bit for bit, and the reference's errors are raised.
 """
 import numpy as np
 import pytest
@@ -316,3 +318,45 @@ def test_oracle_output_file_reads_back_as_initial_conditions(tmp_path):
     T.read_initial_conditions(simsize, f, back)
     data = {k: out[k].astype(np.float64) for k in ("/nsall", "/vs1all", "/Tsall")}
     _check_loaded(back, data, out["/Phiall"].astype(np.float64))
+
+
+# ---------------------------------------------------------------- grid size and coordinates
+@pytest.mark.parametrize("kind", ["cartesian", "dipole"])
+def test_grid_files_give_the_grid_back(tmp_path, kind):
+    """simsize.h5 and simgrid.h5 (src/io/reader_hdf5.f90:64-92, readgrid_hdf5.f90:11-39)
+    read back to the coordinates written, bit for bit; the mesh built on them is the
+    generator's own."""
+    g = ga.cartesian_grid(12, 5, 4) if kind == "cartesian" else ga.dipole_grid(16, 6, 3)
+    d = tmp_path / "inputs"
+    d.mkdir()
+    h5lite.write(str(d / "simsize.h5"), {"/lx1": np.int32(g.lx1), "/lx2": np.int32(g.lx2),
+                                         "/lx3": np.int32(g.lx3)})
+    h5lite.write(str(d / "simgrid.h5"), {"/x1": g.x1, "/x2": g.x2, "/x3": g.x3,
+                                         "/glonctr": np.float64(212.5), "/glatctr": np.float64(65.1)})
+    assert T.read_simsize(str(d / "simsize.h5")) == (g.lx1, g.lx2, g.lx3)
+    for path in (str(d), str(d / "simgrid.h5")):       # directory or file, as the reference
+        x1, x2, x3, ctr = T.read_grid_coords(path, (g.lx1, g.lx2, g.lx3))
+        assert np.array_equal(x1, g.x1) and np.array_equal(x2, g.x2) and np.array_equal(x3, g.x3)
+        assert ctr == (212.5, 65.1)
+    back = T.grid_from_files(str(d / "simsize.h5"), str(d), kind)
+    for name in T.Transport.GEOM + ("dx1", "dx1i", "dx2", "dx2i", "dx3", "dx3i", "alt", "g1"):
+        assert np.array_equal(getattr(back, name), getattr(g, name)), name
+    assert (back.gridflag, tuple(back.isglobal)) == (g.gridflag, tuple(g.isglobal))
+
+
+def test_grid_file_errors(tmp_path):
+    g = ga.cartesian_grid(8, 4, 3)
+    f = str(tmp_path / "simgrid.h5")
+    h5lite.write(f, {"/x1": g.x1, "/x2": g.x2, "/x3": g.x3})
+    with pytest.raises(T.GemhipError, match="please specify /glonctr and /glatctr"):
+        T.read_grid_coords(f, (8, 4, 3))
+    h5lite.write(f, {"/x1": g.x1[2:-2], "/x2": g.x2, "/x3": g.x3, "/glonctr": np.float64(0),
+                     "/glatctr": np.float64(0)})
+    with pytest.raises(T.GemhipError, match="/x1 must hold 12 values"):
+        T.read_grid_coords(f, (8, 4, 3))
+    h5lite.write(f, {"/x1": g.x1, "/x2": g.x2, "/x3": g.x3, "/glonctr": np.zeros(2),
+                     "/glatctr": np.float64(0)})
+    with pytest.raises(T.GemhipError, match="glonctr must be one value"):
+        T.read_grid_coords(f, (8, 4, 3))
+    with pytest.raises(T.GemhipError, match="could not open"):
+        T.read_grid_coords(str(tmp_path / "nowhere"), (8, 4, 3))
