@@ -249,3 +249,42 @@ def test_fullsize_sampled_sources_parity():
                 rel = np.max(np.max(np.abs(a - b), axis=0) / np.where(scale > 0, scale, 1.0))
                 assert rel <= 1e-12, f"{which} column {sg.c} {blk}[{k}] {rel:.3e}"
     print("  24 columns within 1e-12 for energy, momentum and mass", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(400)
+def test_fullsize_output_parity(tmp_path):
+    """plasma_output_nompi_C on the whole 512x256x192 state (HBM-resident): every value of
+    every dataset of flagoutput 1 and 2 equals the oracle's packing of the same host state
+    bit for bit (the packing is cell-local, so the oracle covers the whole grid here)."""
+    import h5lite
+    print("case dipole 512x256x192 output: generating the state", flush=True)
+    g = ga.dipole_grid(*SHAPE)
+    dev = synthetic_state_device(g, seed=4, device="cuda:0")
+    st = host_state(g, dev)
+    del dev
+    torch.cuda.empty_cache()
+    ev = st.electrovars
+    i = np.arange(ev[..., 0].size, dtype=np.float64).reshape(ev.shape[:3], order="F")
+    ev[..., 4] = 1e-7 * np.sin(1e-3 * i)
+    ev[..., 5] = -2e-7 * np.cos(3e-4 * i)
+    ev[..., 6] = 5e3 * np.sin(7e-6 * i)
+    del i
+    tr = Transport(g, st.nn, st.Tn, geometry_on_device=True)
+    tr.set_residency(1)
+    tr.push(st)
+    for flag in (1, 2):
+        print(f"  flagoutput {flag}", flush=True)
+        tr.register_output(str(tmp_path), flag, 0)
+        path = tr.plasma_output(None, [2015, 4, 13], 12345.67891, flag, electrovars=ev)
+        want = oracle.output_pack(st, g.gridflag, flag)
+        with h5lite.File(path) as f:
+            for name, arr in want.items():
+                a = f.read(name)
+                assert a.shape == arr.shape, name
+                assert np.array_equal(a.view(np.uint32), arr.view(np.uint32)), f"{name} differs"
+        del want
+        import os
+        os.remove(path)
+    tr.set_residency(0)
+    print("  every value of flagoutput 1 and 2 bit-identical", flush=True)
