@@ -76,7 +76,8 @@ def flang():
 def build_drivers(verbose: bool = False) -> list:
     """Compile the Fortran binding module and the two host drivers the test suite runs
     (tests/drivers/*): a Fortran program using gemini_amd/fortran/gemhip_bind.f90 and a
-    C++ program using include/gemini3d_hip.h, both linked against libgemhip.so."""
+    C++ program using include/gemini3d_hip.h, both linked against libgemhip.so; and the
+    HDF5 writer's test program (h5io_check)."""
     os.makedirs(DRIVER_BIN, exist_ok=True)
     rpath = "-Wl,-rpath,$ORIGIN/../../../gemini_amd"
     link = ["-L" + HERE, "-lgemhip", rpath]
@@ -84,6 +85,11 @@ def build_drivers(verbose: bool = False) -> list:
     cmds = [["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
              os.path.join(DRIVERS, "capi_driver.cpp"), "-o", os.path.join(DRIVER_BIN, "capi_driver")]
             + link]
+    # the HDF5 writer on its own, for the CPU tests of its chunked, threaded compression
+    cmds.append(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                 "-idirafter", hdf5_include(), os.path.join(DRIVERS, "h5io_check.cpp"),
+                 os.path.join(CSRC, "h5io.cpp"), "-o", os.path.join(DRIVER_BIN, "h5io_check"),
+                 "-L/opt/rocm/lib", "-lamdhip64", "-ldl", "-pthread", "-Wl,-rpath,/opt/rocm/lib"])
     fc = flang()
     if fc:
         mod = os.path.join(DRIVER_BIN, "gemhip_bind.o")

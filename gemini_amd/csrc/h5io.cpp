@@ -13,6 +13,7 @@
 #include <hdf5.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <set>
@@ -198,11 +199,13 @@ bool write_chunks_parallel(hid_t dset, int rank, const long* dims, const hsize_t
   std::vector<std::vector<unsigned char>> out((size_t)nchunks);
   std::vector<int> bad((size_t)nchunks, 0);
   const int nt = (int)std::min<long>(io_threads(), nchunks);
-  auto work = [&](int t) {
+  // chunks are taken from a shared counter, so whichever threads start finish them all
+  std::atomic<long> next(0);
+  auto work = [&]() {
     std::vector<float> buf((size_t)ce);
     std::vector<unsigned char> shuf(raw);
     const unsigned long cap = a.z_bound((unsigned long)raw);
-    for (long j = t; j < nchunks; j += nt) {
+    for (long j = next++; j < nchunks; j = next++) {
       long idx[4], off[4];
       long r = j;
       for (int k = rank - 1; k >= 0; --k) { idx[k] = r % nc[k]; r /= nc[k]; off[k] = idx[k] * (long)ch[k]; }
@@ -244,8 +247,14 @@ bool write_chunks_parallel(hid_t dset, int rank, const long* dims, const hsize_t
     }
   };
   std::vector<std::thread> pool;
-  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
-  work(0);
+  for (int t = 1; t < nt; ++t) {
+    try {
+      pool.emplace_back(work);
+    } catch (const std::exception&) {
+      break;   // fewer threads: the calling thread still drains the counter
+    }
+  }
+  work();
   for (auto& th : pool) th.join();
   for (long j = 0; j < nchunks; ++j) {
     if (bad[(size_t)j]) { err = "chunk compression failed"; return false; }

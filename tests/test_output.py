@@ -15,6 +15,8 @@
 * The grid size and coordinate readers (reader_hdf5.f90:64-92, readgrid_hdf5.f90:11-39):
   the coordinates written come back bit for bit, and the reference's errors are raised.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -360,3 +362,35 @@ def test_grid_file_errors(tmp_path):
         T.read_grid_coords(f, (8, 4, 3))
     with pytest.raises(T.GemhipError, match="could not open"):
         T.read_grid_coords(str(tmp_path / "nowhere"), (8, 4, 3))
+
+
+# ---------------------------------------------------------------- the writer on its own
+@pytest.mark.parametrize("level,threads", [(0, "1"), (3, "1"), (3, "3"), (6, "16")])
+def test_writer_chunks_and_threads(tmp_path, level, threads):
+    """The library's HDF5 writer (tests/drivers/h5io_check runs it on host arrays): datasets
+    whose 1 MiB chunks end partly outside them, compressed on 1 to 16 threads, read back
+    to the values written, with the shuffle+deflate pipeline declared in the file."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "drivers", "bin", "h5io_check")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (gemini_amd.build.build_drivers)")
+    f = str(tmp_path / "w.h5")
+    r = subprocess.run([exe, f, str(level)], env=dict(os.environ, GEMHIP_IO_THREADS=threads),
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(f + ".raw", dtype=np.float32)
+    a, b = raw[:3 * 700 * 600].reshape(3, 700, 600), raw[3 * 700 * 600:].reshape(5, 3, 450, 530)
+    with h5lite.File(f) as h:
+        assert h.names() == ["/a", "/c", "/g/b", "/i"]
+        assert np.array_equal(h.read("/a"), a) and np.array_equal(h.read("/g/b"), b)
+        assert h.read("/c").tolist() == [[3.5]] and h.read("/i") == 7
+    h5dump = "/opt/conda/bin/h5dump"        # the HDF5 tools' own view of the storage layout
+    dump = subprocess.run([h5dump, "-p", "-H", f], capture_output=True, text=True) \
+        if os.path.exists(h5dump) else None
+    if dump is not None and dump.returncode == 0:
+        text = dump.stdout
+        if level:
+            assert "CHUNKED ( 1, 436, 600 )" in text and "CHUNKED ( 1, 1, 450, 530 )" in text
+            assert "PREPROCESSING SHUFFLE" in text and f"DEFLATE {{ LEVEL {level} }}" in text
+        else:
+            assert "CONTIGUOUS" in text and "DEFLATE" not in text
