@@ -1589,70 +1589,73 @@ int gemhip_output_filename(const char* outdir, const int* ymd, double UTsec, int
   return 0;
 }
 
-// io_nompi.f90:183-268 (libgemini_c.f90:267-284 passes identifier and the limits, so the
-// name always carries the patch number and the file the limits and patchID)
-void plasma_output_nompi_C(void** cfgC, int* ymd, double* UTsec, double** fluidvars,
-                           double** electrovars, int* identifier, double* x1lims,
-                           double* x2lims, double* x3lims) {
-  (void)cfgC;
-  if (!ready()) return;
+}  // extern "C"
+
+namespace gemhip {
+namespace {
+// The output file of the state (both writers of the reference): the patch writer of
+// io_nompi.f90:183-268 (the name carries the patch number, the file the limits and
+// /patchID; libgemini_c.f90:267-284 always passes them) or, with root, the root writer of
+// plasma_output.f90:76-136 and plasma_output_hdf5.f90:10-65 (the plain date name, and TEall
+// in place of Teall).  Returns 0 or an error code.
+int write_plasma_output(const int* ymd, const double* UTsec, double** fluidvars,
+                        double** electrovars, bool root, int patch, const double* const* lims3) {
+  if (!ready()) return 1;
   Ctx& c = ctx();
-  if (!c.output_ok) { fail("output directory not registered (gemhip_register_output)"); return; }
-  if (!ymd || !UTsec || !identifier || !x1lims || !x2lims || !x3lims) {
-    fail("null argument to plasma_output_nompi_C");
-    return;
-  }
-  if (!check_host(electrovars, "electrovars")) return;
-  if (c.residency == 0 && !check_host(fluidvars, "fluidvars")) return;
+  if (!c.output_ok) { fail("output directory not registered (gemhip_register_output)"); return 2; }
+  if (!check_host(electrovars, "electrovars")) return 3;
+  if (c.residency == 0 && !check_host(fluidvars, "fluidvars")) return 3;
   std::string path;
   if (!date_stem(c.outdir.c_str(), ymd, *UTsec, path)) {
     fail("timeutils.f90::utsec2filestem did NOT satisfy 0 <= seconds < 86400 (or a bad date)");
-    return;
+    return 4;
   }
-  path += "_" + std::to_string(*identifier) + ".h5";
+  path += root ? std::string(".h5") : "_" + std::to_string(patch) + ".h5";
   std::string err;
-  if (!h5_available(err)) { fail("%s", err.c_str()); return; }
+  if (!h5_available(err)) { fail("%s", err.c_str()); return 5; }
 
   const DevGrid& g = c.g;
   const int flag = c.flagoutput;
   const long n = (long)c.lx1 * c.lx2 * c.lx3, m = (long)c.lx2 * c.lx3;
   const long count = output_count(g, flag);
   const size_t ebytes = 4 * (size_t)g.N * sizeof(double);
-  if (!ensure_output_buffers(ebytes + count * sizeof(float), count * sizeof(float))) return;
+  if (!ensure_output_buffers(ebytes + count * sizeof(float), count * sizeof(float))) return 6;
   Io io{fluidvars, nullptr, electrovars};
-  if (!upload(io, {NS, VS1, VS2, VS3, TS})) return;
+  if (!upload(io, {NS, VS1, VS2, VS3, TS})) return 6;
   // J1 J2 J3 Phi are the contiguous electrovars fields 4-7
   if (!hipok(hipMemcpyAsync(c.d_out, *electrovars + 3 * g.N, ebytes, hipMemcpyDefault, c.stream),
              "electrovars upload"))
-    return;
+    return 6;
   float* packed = reinterpret_cast<float*>(reinterpret_cast<char*>(c.d_out) + ebytes);
   launch_output_pack(g, c.d_fluid, c.d_out, flag, packed, c.stream);
-  if (!hipok(hipGetLastError(), "output pack launch")) return;
+  if (!hipok(hipGetLastError(), "output pack launch")) return 6;
 
   // field datasets in file order (name, species count, values)
   struct Field { const char* name; int nsp; };
   std::vector<Field> fields;
   if (flag == 2)
-    fields = {{"/neall", 0}, {"/v1avgall", 0}, {"/Tavgall", 0}, {"/Teall", 0}, {"/J1all", 0},
+    fields = {{"/neall", 0}, {"/v1avgall", 0}, {"/Tavgall", 0}, {root ? "/TEall" : "/Teall", 0}, {"/J1all", 0},
               {"/J2all", 0}, {"/J3all", 0}, {"/v2avgall", 0}, {"/v3avgall", 0}};
   else if (flag == 3)
     fields = {{"/neall", 0}};
   else
     fields = {{"/nsall", 7}, {"/vs1all", 7}, {"/Tsall", 7}, {"/J1all", 0}, {"/J2all", 0},
               {"/J3all", 0}, {"/v2avgall", 0}, {"/v3avgall", 0}};
-  const int flagv = flag, id = *identifier;
+  const int flagv = flag, id = patch;
   const int ymdv[3] = {ymd[0], ymd[1], ymd[2]};
   const float uthour = (float)(*UTsec / 3600.0);
-  const float lims[6] = {(float)x1lims[0], (float)x1lims[1], (float)x2lims[0],
-                         (float)x2lims[1], (float)x3lims[0], (float)x3lims[1]};
+  float lims[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; !root && k < 6; ++k) lims[k] = (float)lims3[k / 2][k % 2];
   std::vector<H5Out> ds = {
       {"/flagoutput", H5Out::I32, 0, {0}, &flagv, nullptr},
       {"/time/ymd", H5Out::I32, 1, {3}, ymdv, nullptr},
-      {"/time/UThour", H5Out::F32, 0, {0}, &uthour, nullptr},
-      {"/x1lims", H5Out::F32, 1, {2}, lims, nullptr},
-      {"/x2lims", H5Out::F32, 1, {2}, lims + 2, nullptr},
-      {"/x3lims", H5Out::F32, 1, {2}, lims + 4, nullptr},
-      {"/patchID", H5Out::I32, 0, {0}, &id, nullptr}};
+      {"/time/UThour", H5Out::F32, 0, {0}, &uthour, nullptr}};
+  if (!root) {
+    ds.push_back({"/x1lims", H5Out::F32, 1, {2}, lims, nullptr});
+    ds.push_back({"/x2lims", H5Out::F32, 1, {2}, lims + 2, nullptr});
+    ds.push_back({"/x3lims", H5Out::F32, 1, {2}, lims + 4, nullptr});
+    ds.push_back({"/patchID", H5Out::I32, 0, {0}, &id, nullptr});
+  }
   // one device-to-host copy per dataset, each followed by an event, so the file write of
   // one dataset overlaps the copy of the next
   std::vector<hipEvent_t> evs;
@@ -1681,7 +1684,49 @@ void plasma_output_nompi_C(void** cfgC, int* ymd, double* UTsec, double** fluidv
   }
   hipStreamSynchronize(c.stream);
   for (hipEvent_t e : evs) c.pool.push_back(e);
+  return ok ? 0 : 7;
 }
+}  // namespace
+}  // namespace gemhip
+
+extern "C" {
+
+// the name of the root writer's file: outdir/YYYYMMDD_SSSSS.ffffff.h5 (plasma_output_hdf5.f90:17)
+int gemhip_output_filename_root(const char* outdir, const int* ymd, double UTsec, char* out,
+                                long len) {
+  if (!outdir || !ymd || !out) { fail("null argument to gemhip_output_filename_root"); return 1; }
+  std::string p;
+  if (!date_stem(outdir, ymd, UTsec, p)) {
+    fail("timeutils.f90::utsec2filestem did NOT satisfy 0 <= seconds < 86400 (or a bad date)");
+    return 2;
+  }
+  p += ".h5";
+  if ((long)p.size() + 1 > len) { fail("file name buffer too short (%ld)", len); return 3; }
+  memcpy(out, p.c_str(), p.size() + 1);
+  return 0;
+}
+
+// io_nompi.f90:183-268 through libgemini_c.f90:267-284
+void plasma_output_nompi_C(void** cfgC, int* ymd, double* UTsec, double** fluidvars,
+                           double** electrovars, int* identifier, double* x1lims,
+                           double* x2lims, double* x3lims) {
+  (void)cfgC;
+  if (!ymd || !UTsec || !identifier || !x1lims || !x2lims || !x3lims) {
+    fail("null argument to plasma_output_nompi_C");
+    return;
+  }
+  const double* lims3[3] = {x1lims, x2lims, x3lims};
+  write_plasma_output(ymd, UTsec, fluidvars, electrovars, false, *identifier, lims3);
+}
+
+// output_plasma (plasma_output.f90:23-31, 76-136 -> plasma_output_hdf5.f90:10-65): the file
+// gemini.bin writes every output time; one process holds the whole grid, so the root's
+// gather is the identity
+int gemhip_output_plasma(const int* ymd, double UTsec, double** fluidvars, double** electrovars) {
+  if (!ymd) { fail("null argument to gemhip_output_plasma"); return 1; }
+  return write_plasma_output(ymd, &UTsec, fluidvars, electrovars, true, 0, nullptr);
+}
+
 
 // reader_hdf5.f90:64-92 (get_simsize3): the full-grid sizes of a simsize file
 int gemhip_read_simsize(const char* path, int* lx1, int* lx2, int* lx3) {

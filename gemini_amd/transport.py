@@ -75,6 +75,11 @@ def load_library(path: str = LIB_PATH):
     lib.gemhip_register_config.argtypes = [C.c_int, C.c_double]
     lib.gemhip_kernel_time.argtypes = [C.c_char_p, _dp, C.POINTER(C.c_long)]
     lib.gemhip_eval_math.argtypes = [C.c_int, _dp, _dp, C.c_long]
+    lib.gemhip_output_plasma.restype = C.c_int
+    lib.gemhip_output_plasma.argtypes = [C.POINTER(C.c_int), C.c_double, C.c_void_p, C.c_void_p]
+    lib.gemhip_output_filename_root.restype = C.c_int
+    lib.gemhip_output_filename_root.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.c_double,
+                                                C.c_char_p, C.c_long]
     for name in ("gemhip_register_output", "gemhip_output_filename",
                  "gemhip_read_initial_conditions", "gemhip_read_simsize",
                  "gemhip_read_grid_coords"):
@@ -109,6 +114,15 @@ def output_filename(outdir: str, ymd, UTsec: float, identifier: int) -> str:
     buf = C.create_string_buffer(4096)
     _host_call("gemhip_output_filename", outdir.encode(), (C.c_int * 3)(*[int(v) for v in ymd]),
                float(UTsec), int(identifier), buf, len(buf))
+    return buf.value.decode()
+
+
+def output_filename_root(outdir: str, ymd, UTsec: float) -> str:
+    """The file gemini.bin's output_plasma writes: outdir/YYYYMMDD_SSSSS.ffffff.h5
+    (src/io/plasma_output_hdf5.f90:17)."""
+    buf = C.create_string_buffer(4096)
+    _host_call("gemhip_output_filename_root", outdir.encode(),
+               (C.c_int * 3)(*[int(v) for v in ymd]), float(UTsec), buf, len(buf))
     return buf.value.decode()
 
 
@@ -599,6 +613,20 @@ class Transport:
             raise GemhipError("the output directory was registered outside this Transport; "
                               "call register_output")
         return output_filename(self._outdir, ymd, UTsec, identifier)
+
+    def output_plasma(self, st, ymd, UTsec: float, electrovars=None) -> str:
+        """gemini.bin's output file (src/io/plasma_output.f90:76-136 and
+        plasma_output_hdf5.f90:10-65): outdir/<date stem>.h5, the datasets of plasma_output
+        with TEall for flagoutput 2 and no patch datasets.  Arguments as plasma_output's."""
+        ev = electrovars if electrovars is not None else st.electrovars
+        f = C.c_void_p(None if st is None else _addr(st.fluidvars))
+        e = C.c_void_p(_addr(ev))
+        y = (C.c_int * 3)(*[int(v) for v in ymd])
+        self._call("gemhip_output_plasma", y, C.c_double(UTsec), C.byref(f), C.byref(e))
+        if self._outdir is None:
+            raise GemhipError("the output directory was registered outside this Transport; "
+                              "call register_output")
+        return output_filename_root(self._outdir, ymd, UTsec)
 
     def checkpoint_size(self) -> int:
         """doubles in a restart checkpoint (fluidvars and the x1 boundary velocities)."""

@@ -247,6 +247,15 @@ int gemhip_register_output(const char* outdir, int flagoutput, int comp_lvl);
    timeutils.f90:182 stops or out is shorter than the path. */
 int gemhip_output_filename(const char* outdir, const int* ymd, double UTsec, int identifier,
                            char* out, long len);
+/* output_plasma (plasma_output.f90:23-31, 76-136 -> plasma_output_hdf5.f90:10-65), the
+   file gemini.bin writes at every output time: <outdir>/<YYYYMMDD_SSSSS.ffffff>.h5 with
+   /flagoutput, /time/ymd, /time/UThour and the datasets of plasma_output_nompi_C, except
+   that flagoutput 2 names the electron temperature TEall.  One process holds the whole
+   grid, so the root's gather is the identity.  Arguments as plasma_output_nompi_C's. */
+int gemhip_output_plasma(const int* ymd, double UTsec, double** fluidvars, double** electrovars);
+/* The name gemhip_output_plasma writes (host-only). */
+int gemhip_output_filename_root(const char* outdir, const int* ymd, double UTsec, char* out,
+                                long len);
 /* get_simsize3 (reader_hdf5.f90:64-92): the full-grid sizes in a simsize file (/lx1 /lx2
    /lx3, else /lxs, else /lx).  Host-only; no context needed. */
 int gemhip_read_simsize(const char* path, int* lx1, int* lx2, int* lx3);

@@ -52,6 +52,8 @@ CARRIES = [([2016, 2, 28], 86400.0, "20160229_00000.000000"),
 def test_file_stem_known_answers(ymd, ut, stem):
     assert oracle.filestem(ymd, ut) == stem
     assert T.output_filename("/out/dir", ymd, ut, 17) == f"/out/dir/{stem}_17.h5"
+    assert T.output_filename("/o", ymd, ut, -5) == f"/o/{stem}_-5.h5"     # I0 of a negative
+    assert T.output_filename_root("/out/dir", ymd, ut) == f"/out/dir/{stem}.h5"
 
 
 @pytest.mark.parametrize("ut", [-1.0, -5.0, float("nan"), 1e12, 2.0e6])

@@ -55,11 +55,15 @@ def same(a, b):
     return np.array_equal(na, nb) and np.array_equal(a[~na].view(np.uint32), b[~nb].view(np.uint32))
 
 
-def check_file(path, st, gridflag, flag):
+def check_file(path, st, gridflag, flag, root=False):
+    """root: gemini.bin's file (plasma_output_hdf5.f90:10-65): no patch datasets, TEall."""
     want = oracle.output_pack(st, gridflag, flag)
-    header = {"/flagoutput": ("i", ()), "/time/ymd": ("i", (3,)), "/time/UThour": ("f", ()),
-              "/x1lims": ("f", (2,)), "/x2lims": ("f", (2,)), "/x3lims": ("f", (2,)),
-              "/patchID": ("i", ())}
+    header = {"/flagoutput": ("i", ()), "/time/ymd": ("i", (3,)), "/time/UThour": ("f", ())}
+    if root:
+        want = {("/TEall" if k == "/Teall" else k): v for k, v in want.items()}
+    else:
+        header.update({"/x1lims": ("f", (2,)), "/x2lims": ("f", (2,)), "/x3lims": ("f", (2,)),
+                       "/patchID": ("i", ())})
     with h5lite.File(path) as f:
         assert f.names() == sorted(list(header) + list(want))
         for name, (kind, shape) in header.items():
@@ -67,9 +71,10 @@ def check_file(path, st, gridflag, flag):
         assert f.read("/flagoutput") == flag
         assert list(f.read("/time/ymd")) == YMD
         assert f.read("/time/UThour").view(np.uint32) == np.float32(UTSEC / 3600.0).view(np.uint32)
-        for k, lim in enumerate(LIMS):
-            assert np.array_equal(f.read(f"/x{k + 1}lims"), np.array(lim, dtype=np.float32))
-        assert f.read("/patchID") == IDENT
+        if not root:
+            for k, lim in enumerate(LIMS):
+                assert np.array_equal(f.read(f"/x{k + 1}lims"), np.array(lim, dtype=np.float32))
+            assert f.read("/patchID") == IDENT
         for name, arr in want.items():
             assert f.info(name) == ("f", 4, True, arr.shape), name
             got = f.read(name)
@@ -91,6 +96,24 @@ def test_output_file_matches_oracle(tmp_path, kind, flag, mode):
     path = tr.plasma_output(st, YMD, UTSEC, IDENT, *LIMS)
     assert path == f"{tmp_path}/{oracle.filestem(YMD, UTSEC)}_{IDENT}.h5"
     check_file(path, st, g.gridflag, flag)
+
+
+@pytest.mark.parametrize("flag", [1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_root_output_file_matches_oracle(tmp_path, flag, mode):
+    """gemini.bin's own output file (output_plasma), as gemini3d.compare reads it."""
+    print(f"case dipole root writer flagoutput {flag} residency {mode}", flush=True)
+    g = grid("dipole")
+    st = state(g)
+    tr = Transport(g, st.nn, st.Tn)
+    tr.register_output(str(tmp_path), flag, 3)
+    tr.set_residency(mode)
+    if mode:
+        tr.push(st)
+    path = tr.output_plasma(st, YMD, UTSEC)
+    assert path == f"{tmp_path}/{oracle.filestem(YMD, UTSEC)}.h5"
+    check_file(path, st, g.gridflag, flag, root=True)
+    tr.set_residency(0)
 
 
 def test_output_reads_the_resident_state(tmp_path):
