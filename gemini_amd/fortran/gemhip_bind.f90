@@ -29,7 +29,8 @@ module gemhip_bind
             gemhip_transport_step, gemhip_synchronize, gemhip_error_message, &
             gemhip_register_null_cells, gemhip_fluid_adv_block, gemhip_register_cfl, &
             gemhip_register_atmosphere, gemhip_register_gravity, gemhip_set_ionization_rates, &
-            gemhip_sources_block, gemhip_register_output, gemhip_read_initial_conditions
+            gemhip_sources_block, gemhip_register_output, gemhip_read_initial_conditions, &
+            gemhip_output_plasma, gemhip_read_simsize, gemhip_read_grid_coords
 
   ! ------------------------------------------------ part 1: reference entry points
   interface
@@ -359,6 +360,25 @@ module gemhip_bind
       character(kind=c_char), intent(in) :: indatsize(*), indatfile(*)
       integer(c_int), value :: lx1, lx2, lx3
       real(c_double) :: fluidvars(*), electrovars(*)
+    end function
+    integer(c_int) function gemhip_output_plasma(ymd, UTsec, fluidvars, electrovars) &
+        bind(C, name="gemhip_output_plasma")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), intent(in) :: ymd(3)
+      real(c_double), value :: UTsec
+      type(c_ptr) :: fluidvars, electrovars
+    end function
+    integer(c_int) function gemhip_read_simsize(path, lx1, lx2, lx3) bind(C, name="gemhip_read_simsize")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: lx1, lx2, lx3
+    end function
+    integer(c_int) function gemhip_read_grid_coords(path, lx1, lx2, lx3, x1, x2, x3, glonctr, &
+        glatctr) bind(C, name="gemhip_read_grid_coords")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: lx1, lx2, lx3
+      real(c_double), intent(out) :: x1(*), x2(*), x3(*), glonctr, glatctr
     end function
     integer(c_long) function c_strlen(s) bind(C, name="strlen")
       import :: c_long, c_ptr

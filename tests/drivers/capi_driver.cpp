@@ -167,6 +167,7 @@ int main(int argc, char** argv) {
   double utsec = 12345.67891, l1[2] = {0.0, 1.0}, l2[2] = {2.0, 3.0}, l3[2] = {4.0, 5.0};
   plasma_output_nompi_C(&cfgC, ymd, &utsec, &fluidvars, &electrovars, &ident, l1, l2, l3);
   if (gemhip_last_error()[0]) check(1, "plasma output");
+  check(gemhip_output_plasma(ymd, utsec, &fluidvars, &electrovars), "output_plasma");
   FILE* o = fopen((dir + "/out.bin").c_str(), "wb");
   fwrite(fluid.data(), sizeof(double), fluid.size(), o);
   fwrite(aux.data(), sizeof(double), 14 * L, o);

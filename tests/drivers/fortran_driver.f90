@@ -10,8 +10,8 @@
 !> modes 2 and 3: the same for the whole transport block (src/libgemini.cpp:155-210);
 !> modes 4 and 5: the block followed by the sources substep (:223-232), call by call
 !>         or fused (gemhip_sources_block).
-!> Every mode then writes the final state with plasma_output_nompi_C (flagoutput
-!> 1 + mod(mode, 3)) into the case directory.
+!> Every mode then writes the final state with plasma_output_nompi_C and
+!> gemhip_output_plasma (flagoutput 1 + mod(mode, 3)) into the case directory.
 program fortran_driver
   use, intrinsic :: iso_c_binding
   use gemhip_bind
@@ -161,6 +161,8 @@ program fortran_driver
   x3lims = [4.0_c_double, 5.0_c_double]
   call plasma_output_nompi_C(cfgC, ymd, UTsec, fvp, evp, ident, x1lims, x2lims, x3lims)
   if (rc /= 0 .or. len(gemhip_error_message()) > 0) error stop 'output failed: '//gemhip_error_message()
+  rc = gemhip_output_plasma(ymd, UTsec, fvp, evp)          ! gemini.bin's own file
+  if (rc /= 0) error stop 'root output failed: '//gemhip_error_message()
   open(newunit=w, file=trim(dir)//'/out.bin', access='stream', form='unformatted', status='replace')
   write(w) fluidvars, fluidauxvars(:,:,:,1:14), vs1i, vs2i, vs3i
   close(w)

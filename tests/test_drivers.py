@@ -6,8 +6,8 @@ transport block of src/libgemini.cpp:155-210 (modes 2 and 3).  Their results mus
 bit-identical to the oracle's (with the GPU's x**2.5).  Modes 4 and 5 follow the block with
 the sources substep of :223-232, call by call and fused; its collision rates call exp, erf
 and log, so those runs are held to 1e-12 (normwise, per species) instead.  Every run ends
-with the host's own plasma_output_nompi_C call, whose file must hold the driver's final
-state bit for bit."""
+with the host's own plasma_output_nompi_C and gemhip_output_plasma calls, whose files must
+hold the driver's final state bit for bit."""
 import os
 import subprocess
 
@@ -70,3 +70,10 @@ def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
         for name, arr in want.items():
             a = f.read(name)
             assert a.shape == arr.shape and np.array_equal(a.view(np.uint32), arr.view(np.uint32)), name
+    # and gemini.bin's own file (gemhip_output_plasma): plain date name, TEall
+    root = os.path.join(str(tmp_path), f"{oracle.filestem([2015, 4, 13], 12345.67891)}.h5")
+    with h5lite.File(root) as f:
+        assert "/patchID" not in f.names()
+        for name, arr in want.items():
+            a = f.read("/TEall" if name == "/Teall" else name)
+            assert np.array_equal(a.view(np.uint32), arr.view(np.uint32)), name
