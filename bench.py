@@ -504,7 +504,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the transport "
-              f"path shards over x2/x3 across GPUs by design, but this build runs on one.",
+              f"path shards over x2/x3 across GPUs by design, but the library runs on one.",
               file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
