@@ -10,8 +10,11 @@ species.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload dipole512|cart256|cart128|ensemble64|adjoint]
 
-The default line also carries the differentiation form's adjoint/forward cost ratio,
-measured on its own configuration (ADJOINT below) with a few central-difference checks.
+The default line also carries the whole time step (transport block + source/loss
+substep) and the plasma output of the headline state (pack kernel and whole file write,
+skipped if the temporary directory lacks the space), and the differentiation form's
+adjoint/forward cost ratio, measured on its own configuration (ADJOINT below) with a few
+central-difference checks.
 """
 from __future__ import annotations
 
@@ -638,7 +641,20 @@ def main():
     out_res = None
     if output:
         log("plasma output (flagoutput 2, 1, 3) of the resident state")
-        out_res = measure_output(tr, g, electrovars, w["sample"])
+        import shutil
+        import tempfile
+        free = shutil.disk_usage(tempfile.gettempdir()).free
+        need = 4 * 26 * g.ncells * 2            # the largest file, twice over
+        if free < need:
+            out_res = {"skipped": f"{free / 1e9:.1f} GB free under {tempfile.gettempdir()}, "
+                                  f"{need / 1e9:.1f} GB wanted"}
+            log("output figures skipped: " + out_res["skipped"])
+        else:
+            try:
+                out_res = measure_output(tr, g, electrovars, w["sample"])
+            except Exception as e:              # the file system must not cost the step figure
+                out_res = {"error": f"{type(e).__name__}: {e}"}
+                log("output figures failed: " + out_res["error"])
         del electrovars
     cpu = None
     if not args.no_cpu_baseline:
