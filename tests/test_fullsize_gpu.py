@@ -257,7 +257,11 @@ def test_fullsize_output_parity(tmp_path):
     """plasma_output_nompi_C on the whole 512x256x192 state (HBM-resident): every value of
     every dataset of flagoutput 1 and 2 equals the oracle's packing of the same host state
     bit for bit (the packing is cell-local, so the oracle covers the whole grid here)."""
+    import shutil
     import h5lite
+    free = shutil.disk_usage(str(tmp_path)).free
+    if free < 8e9:                       # the flagoutput 1 file is 2.6 GB
+        pytest.skip(f"{free / 1e9:.1f} GB free under {tmp_path}: too little for the files")
     print("case dipole 512x256x192 output: generating the state", flush=True)
     g = ga.dipole_grid(*SHAPE)
     dev = synthetic_state_device(g, seed=4, device="cuda:0")
