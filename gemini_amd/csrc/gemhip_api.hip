@@ -1691,6 +1691,30 @@ int write_plasma_output(const int* ymd, const double* UTsec, double** fluidvars,
 
 extern "C" {
 
+// timeutils.f90:79-107 (dateinc, bound by libgemini_c.f90:1144-1150): advance the date and
+// UT seconds by dt, carrying whole days into the date
+void dateinc_C(double* dt, int* ymd, double* UTsec) {
+  if (!dt || !ymd || !UTsec) { fail("null argument to dateinc_C"); return; }
+  if (ymd[2] < 1) { fail("temporal:timeutils:dateinc(): days are positive integers"); return; }
+  if (*UTsec < 0) { fail("negative UTsec, simulation should go forward in time only!"); return; }
+  if (*dt < 0) { fail("negative dtsec, simulation should go forward in time only!"); return; }
+  if (*dt > 86400) {
+    fail("excessively large dtsec > 86400, simulation step should be small enough!");
+    return;
+  }
+  int year = ymd[0], month = ymd[1], day = ymd[2];
+  double ut = *UTsec + *dt;
+  while (ut >= 86400) {
+    ut = ut - 86400.0;
+    ++day;
+    if (!carry_date(year, month, day)) { fail("day_wrap: months and days must be positive"); return; }
+  }
+  ymd[0] = year;
+  ymd[1] = month;
+  ymd[2] = day;
+  *UTsec = ut;
+}
+
 // the name of the root writer's file: outdir/YYYYMMDD_SSSSS.ffffff.h5 (plasma_output_hdf5.f90:17)
 int gemhip_output_filename_root(const char* outdir, const int* ymd, double UTsec, char* out,
                                 long len) {

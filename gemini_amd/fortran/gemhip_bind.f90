@@ -21,7 +21,7 @@ module gemhip_bind
             halo_allparams_C, halo_fluidvars_C, clean_param_C, VNRicht_artvisc_C, &
             compression_C, RK2_prep_mpi_allspec_C, RK2_global_boundary_allspec_C, dt_select_C, &
             source_loss_allparams_C, source_loss_mass_C, source_loss_momentum_C, &
-            source_loss_energy_C, clear_ionization_arrays_C, plasma_output_nompi_C
+            source_loss_energy_C, clear_ionization_arrays_C, plasma_output_nompi_C, dateinc_C
   public :: gemhip_init, gemhip_finalize, gemhip_set_error_mode, gemhip_clear_error, &
             gemhip_set_grid_size, gemhip_register_grid, gemhip_register_neutrals, &
             gemhip_register_config, gemhip_set_residency, gemhip_state_push, &
@@ -234,6 +234,11 @@ module gemhip_bind
     subroutine clear_ionization_arrays_C(intvars) bind(C, name="clear_ionization_arrays_C")
       import :: c_ptr
       type(c_ptr) :: intvars
+    end subroutine
+    subroutine dateinc_C(dt, ymd, UTsec) bind(C, name="dateinc_C")
+      import :: c_int, c_double
+      real(c_double) :: dt, UTsec
+      integer(c_int) :: ymd(3)
     end subroutine
     subroutine plasma_output_nompi_C(cfgC, ymd, UTsec, fluidvars, electrovars, identifier, &
                                      x1lims, x2lims, x3lims) bind(C, name="plasma_output_nompi_C")

@@ -117,6 +117,20 @@ def output_filename(outdir: str, ymd, UTsec: float, identifier: int) -> str:
     return buf.value.decode()
 
 
+def dateinc(dt: float, ymd, UTsec: float):
+    """dateinc_C: (ymd, UTsec) advanced by dt (src/temporal/timeutils.f90:79-107)."""
+    y = (C.c_int * 3)(*[int(v) for v in ymd])
+    u = C.c_double(UTsec)
+    lib = load_library()
+    lib.gemhip_set_error_mode(1)
+    lib.dateinc_C(C.byref(C.c_double(dt)), y, C.byref(u))
+    msg = lib.gemhip_last_error().decode()
+    if msg:
+        lib.gemhip_clear_error()
+        raise GemhipError(msg)
+    return [y[0], y[1], y[2]], u.value
+
+
 def output_filename_root(outdir: str, ymd, UTsec: float) -> str:
     """The file gemini.bin's output_plasma writes: outdir/YYYYMMDD_SSSSS.ffffff.h5
     (src/io/plasma_output_hdf5.f90:17)."""

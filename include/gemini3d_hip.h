@@ -125,6 +125,10 @@ void source_loss_energy_C(double** fluidvars, double** fluidauxvars, double** el
 /* gemini3d.h:90, libgemini.f90:1202-1207: Prionize and Qeionize to zero. */
 void clear_ionization_arrays_C(void** intvars);
 
+/* gemini3d.h:95, libgemini_c.f90:1144-1150 -> timeutils.f90:79-107: advance ymd and UTsec
+   by dt (0 <= dt <= 86400), carrying whole days into the date by the Gregorian rules. */
+void dateinc_C(double* dt, int* ymd, double* UTsec);
+
 /* gemini3d.h:96, libgemini_c.f90:267-284 -> io_nompi.f90:183-268: write
    <outdir>/<YYYYMMDD_SSSSS.ffffff>_<identifier>.h5 (timeutils.f90:133-196) holding
    /flagoutput, /time/ymd, /time/UThour, /x1lims../x3lims, /patchID and the real32 field

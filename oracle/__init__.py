@@ -484,6 +484,16 @@ def filestem(ymd, UTsec):
     return None if rc else buf.value.decode()
 
 
+def dateinc(dtsec, ymd, UTsec):
+    """(ymd, UTsec) advanced by dtsec (timeutils.f90:79-107), or None where the reference
+    stops."""
+    y = (C.c_int * 3)(*[int(v) for v in ymd])
+    u = C.c_double(UTsec)
+    if lib().orc_dateinc(C.c_double(dtsec), y, C.byref(u)):
+        return None
+    return [y[0], y[1], y[2]], u.value
+
+
 def output_count(lx1, lx2, lx3, flagoutput):
     lib().orc_output_count.restype = C.c_long
     return int(lib().orc_output_count(lx1, lx2, lx3, int(flagoutput)))

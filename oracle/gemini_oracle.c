@@ -1682,6 +1682,20 @@ int orc_filestem(const int ymd[3], double UTsec, char out[22]) {
   return 0;
 }
 
+/* timeutils.f90:79-107: returns 0, or -1 where the reference stops */
+int orc_dateinc(double dtsec, int ymd[3], double *UTsec) {
+  int year = ymd[0], month = ymd[1], day = ymd[2];
+  if (day < 1 || *UTsec < 0 || dtsec < 0 || dtsec > 86400) return -1;
+  *UTsec = *UTsec + dtsec;
+  while (*UTsec >= 86400) {
+    *UTsec = *UTsec - 86400.0;
+    day = day + 1;
+    if (day_wrap(&year, &month, &day)) return -1;
+  }
+  ymd[0] = year; ymd[1] = month; ymd[2] = day;
+  return 0;
+}
+
 /* number of real32 values plasma_output_nompi writes for the field datasets, in order:
    flagoutput 2: neall v1avgall Tavgall Teall J1all J2all J3all v2avgall v3avgall Phiall;
    flagoutput 3: neall Phiall; otherwise: nsall vs1all Tsall J1all J2all J3all v2avgall

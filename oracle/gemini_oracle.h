@@ -164,6 +164,7 @@ int orc_sources_block(const orc_grid *g, const orc_atmos *a, double dt, double *
 /* plasma output (io_nompi.f90:183-258, timeutils.f90:151-196): the file stem, the
    number of real32 values of the field datasets and their values in dataset order */
 int orc_filestem(const int ymd[3], double UTsec, char out[22]);
+int orc_dateinc(double dtsec, int ymd[3], double *UTsec);    /* timeutils.f90:79-107 */
 long orc_output_count(int lx1, int lx2, int lx3, int flagoutput);
 void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
                      const double *fluidvars, const double *electrovars, float *out);

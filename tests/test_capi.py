@@ -31,7 +31,7 @@ def test_header_declares_reference_entry_points():
                  "VNRicht_artvisc_C", "compression_C", "RK2_prep_mpi_allspec_C",
                  "RK2_global_boundary_allspec_C", "dt_select_C", "source_loss_allparams_C",
                  "source_loss_mass_C", "source_loss_momentum_C", "source_loss_energy_C",
-                 "clear_ionization_arrays_C", "plasma_output_nompi_C"}
+                 "clear_ionization_arrays_C", "plasma_output_nompi_C", "dateinc_C"}
     reference |= {f"sweep{d}_{w}_C" for d in (1, 2, 3)
                   for w in ("allparams", "allspec_mass", "allspec_momentum", "allspec_energy")}
     assert reference <= names, sorted(reference - names)
@@ -86,5 +86,5 @@ def test_fortran_binding_declares_every_entry_point():
     names = set(re.findall(r"void\s+(\w+_C)\s*\(", part1))
     f90 = open(os.path.join(root, "gemini_amd", "fortran", "gemhip_bind.f90")).read()
     bound = set(re.findall(r'bind\(C,\s*name="(\w+)"\)', f90))
-    assert len(names) == 34
+    assert len(names) == 35
     assert names <= bound, sorted(names - bound)

@@ -396,3 +396,42 @@ def test_writer_chunks_and_threads(tmp_path, level, threads):
             assert "PREPROCESSING SHUFFLE" in text and f"DEFLATE {{ LEVEL {level} }}" in text
         else:
             assert "CONTIGUOUS" in text and "DEFLATE" not in text
+
+
+# ---------------------------------------------------------------- dateinc
+def test_dateinc_known_answers():
+    """test/temporal/test_formats.f90:35-49: a 10 ms step, then 60 one-second steps."""
+    for inc in (oracle.dateinc, T.dateinc):
+        ymd, ut = inc(0.01, [2013, 2, 20], 18000.0)
+        assert oracle.filestem(ymd, ut) == "20130220_18000.010000"
+        assert T.output_filename_root(".", ymd, ut) == "./20130220_18000.010000.h5"
+        for _ in range(60):
+            ymd, ut = inc(1.0, ymd, ut)
+        assert oracle.filestem(ymd, ut) == "20130220_18060.010000"
+
+
+@pytest.mark.parametrize("ymd,ut,dt,want", [
+    ([2016, 12, 31], 86399.5, 1.0, ([2017, 1, 1], 0.5)),
+    ([2016, 2, 28], 86000.0, 500.0, ([2016, 2, 29], 100.0)),
+    ([2015, 2, 28], 86000.0, 500.0, ([2015, 3, 1], 100.0)),
+    ([2015, 4, 30], 0.0, 86400.0, ([2015, 5, 1], 0.0)),
+    ([2015, 4, 13], 100.0, 0.0, ([2015, 4, 13], 100.0))])
+def test_dateinc_carries_days(ymd, ut, dt, want):
+    assert oracle.dateinc(dt, ymd, ut) == want
+    assert T.dateinc(dt, ymd, ut) == want
+
+
+@pytest.mark.parametrize("ymd,ut,dt", [([2015, 4, 0], 0.0, 1.0), ([2015, 4, 1], -1.0, 1.0),
+                                       ([2015, 4, 1], 0.0, -1.0), ([2015, 4, 1], 0.0, 86401.0)])
+def test_dateinc_rejects_what_the_reference_stops_on(ymd, ut, dt):
+    assert oracle.dateinc(dt, ymd, ut) is None
+    with pytest.raises(T.GemhipError):
+        T.dateinc(dt, ymd, ut)
+
+
+def test_dateinc_library_matches_oracle():
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        ymd = [int(rng.integers(1700, 2400)), int(rng.integers(1, 13)), int(rng.integers(1, 29))]
+        ut, dt = float(rng.uniform(0, 86400)), float(rng.uniform(0, 86400))
+        assert T.dateinc(dt, ymd, ut) == oracle.dateinc(dt, ymd, ut)
