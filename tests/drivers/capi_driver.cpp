@@ -165,6 +165,8 @@ int main(int argc, char** argv) {
   check(gemhip_register_output(dir.c_str(), 1 + mode % 3, 3), "register_output");
   int ymd[3] = {2015, 4, 13}, ident = mode;
   double utsec = 12345.67891, l1[2] = {0.0, 1.0}, l2[2] = {2.0, 3.0}, l3[2] = {4.0, 5.0};
+  dateinc_C(&dt, ymd, &utsec);                              // the clock after one step
+  if (gemhip_last_error()[0]) check(1, "dateinc");
   plasma_output_nompi_C(&cfgC, ymd, &utsec, &fluidvars, &electrovars, &ident, l1, l2, l3);
   if (gemhip_last_error()[0]) check(1, "plasma output");
   check(gemhip_output_plasma(ymd, utsec, &fluidvars, &electrovars), "output_plasma");

@@ -155,6 +155,7 @@ program fortran_driver
   rc = gemhip_register_output(trim(dir)//c_null_char, 1_c_int + mod(mode, 3_c_int), 3_c_int)
   ymd = [2015, 4, 13]
   UTsec = 12345.67891_c_double
+  call dateinc_C(dt, ymd, UTsec)                           ! the clock after one step
   ident = mode
   x1lims = [0.0_c_double, 1.0_c_double]
   x2lims = [2.0_c_double, 3.0_c_double]

@@ -6,8 +6,8 @@ transport block of src/libgemini.cpp:155-210 (modes 2 and 3).  Their results mus
 bit-identical to the oracle's (with the GPU's x**2.5).  Modes 4 and 5 follow the block with
 the sources substep of :223-232, call by call and fused; its collision rates call exp, erf
 and log, so those runs are held to 1e-12 (normwise, per species) instead.  Every run ends
-with the host's own plasma_output_nompi_C and gemhip_output_plasma calls, whose files must
-hold the driver's final state bit for bit."""
+with the host's own dateinc_C, plasma_output_nompi_C and gemhip_output_plasma calls: the
+files, named for the advanced clock, must hold the driver's final state bit for bit."""
 import os
 import subprocess
 
@@ -63,7 +63,9 @@ def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
             assert np.array_equal(getattr(got, a), getattr(ref, a)), a
     # the host's plasma_output_nompi_C call: the file holds the driver's final state
     flag = 1 + mode % 3
-    path = os.path.join(str(tmp_path), f"{oracle.filestem([2015, 4, 13], 12345.67891)}_{mode}.h5")
+    ymd, ut = oracle.dateinc(dt, [2015, 4, 13], 12345.67891)      # the host's dateinc_C
+    stem = oracle.filestem(ymd, ut)
+    path = os.path.join(str(tmp_path), f"{stem}_{mode}.h5")
     want = oracle.output_pack(got, g.gridflag, flag)
     with h5lite.File(path) as f:
         assert f.read("/flagoutput") == flag and f.read("/patchID") == mode
@@ -71,7 +73,7 @@ def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
             a = f.read(name)
             assert a.shape == arr.shape and np.array_equal(a.view(np.uint32), arr.view(np.uint32)), name
     # and gemini.bin's own file (gemhip_output_plasma): plain date name, TEall
-    root = os.path.join(str(tmp_path), f"{oracle.filestem([2015, 4, 13], 12345.67891)}.h5")
+    root = os.path.join(str(tmp_path), f"{stem}.h5")
     with h5lite.File(root) as f:
         assert "/patchID" not in f.names()
         for name, arr in want.items():
