@@ -21,6 +21,10 @@ LIMS = ((80e3, 1e6), (-2.5, 3.25), (0.0, 1.0e5))
 def grid(kind):
     if kind == "dipole":
         return ga.dipole_grid(32, 12, 6)
+    if kind == "cart_x2line":      # 2-D runs: a singleton x3 or x2
+        return ga.cartesian_grid(40, 9, 1)
+    if kind == "cart_x3line":
+        return ga.cartesian_grid(40, 1, 9)
     b = ga.cartesian_grid(24, 10, 7)
     if kind == "inverted":   # gridflag 1: the potential slab is the x1 = 1 plane
         return ga.Grid(b.lx1, b.lx2, b.lx3, b.x1, b.x2, b.x3, b.planes, gridflag=1,
@@ -81,7 +85,7 @@ def check_file(path, st, gridflag, flag, root=False):
             assert same(got, arr), f"{name}: differs from the oracle"
 
 
-@pytest.mark.parametrize("kind", ["cart", "dipole", "inverted"])
+@pytest.mark.parametrize("kind", ["cart", "dipole", "inverted", "cart_x2line", "cart_x3line"])
 @pytest.mark.parametrize("flag", [1, 2, 3])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_output_file_matches_oracle(tmp_path, kind, flag, mode):
@@ -89,7 +93,7 @@ def test_output_file_matches_oracle(tmp_path, kind, flag, mode):
     g = grid(kind)
     st = state(g)
     tr = Transport(g, st.nn, st.Tn)
-    tr.register_output(str(tmp_path), flag, 3 if kind == "dipole" else 0)
+    tr.register_output(str(tmp_path), flag, 0 if kind in ("cart", "inverted") else 3)
     tr.set_residency(mode)
     if mode:
         tr.push(st)
