@@ -38,6 +38,7 @@ struct H5Api {
 #define H5FN(name) decltype(&::name) name = nullptr;
   H5FN(H5open)
   H5FN(H5get_libversion)
+  H5FN(H5Eget_auto2)
   H5FN(H5Eset_auto2)
   H5FN(H5Fcreate)
   H5FN(H5Fopen)
@@ -98,6 +99,7 @@ const H5Api& api() {
 #define H5FN(name) t.name = reinterpret_cast<decltype(t.name)>(sym(#name));
     H5FN(H5open)
     H5FN(H5get_libversion)
+    H5FN(H5Eget_auto2)
     H5FN(H5Eset_auto2)
     H5FN(H5Fcreate)
     H5FN(H5Fopen)
@@ -141,7 +143,6 @@ const H5Api& api() {
               std::to_string(min) + "." + std::to_string(rel);
       return t;
     }
-    t.H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);   // errors are reported by return codes
     t.ok = true;
     return t;
   }();
@@ -153,6 +154,29 @@ bool have(std::string& err) {
   if (!a.ok) err = a.why;
   return a.ok;
 }
+
+// HDF5's automatic error printing is off while a call of this file runs (its failures are
+// reported through return codes) and restored afterwards: the setting is process-wide, and
+// the host may use the same HDF5 library for its own files.
+class QuietErrors {
+ public:
+  QuietErrors() {
+    const H5Api& a = api();
+    if (!a.ok) return;
+    on_ = a.H5Eget_auto2(H5E_DEFAULT, &func_, &data_) >= 0;
+    if (on_) a.H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+  }
+  ~QuietErrors() {
+    if (on_) api().H5Eset_auto2(H5E_DEFAULT, func_, data_);
+  }
+  QuietErrors(const QuietErrors&) = delete;
+  QuietErrors& operator=(const QuietErrors&) = delete;
+
+ private:
+  bool on_ = false;
+  H5E_auto2_t func_ = nullptr;
+  void* data_ = nullptr;
+};
 
 // chunk shape for a compressed dataset: leading dimensions cut to 1 while the chunk is over
 // 1 MiB, then the second-fastest dimension cut to fit (whole rows of the fastest one)
@@ -277,6 +301,7 @@ constexpr unsigned ACC_RDONLY = 0x0000u, ACC_TRUNC = 0x0002u;
 
 bool h5_write(const char* path, const H5Out* ds, int nds, int comp_lvl, std::string& err) {
   if (!have(err)) return false;
+  QuietErrors quiet;
   const H5Api& a = api();
   const hid_t f = a.H5Fcreate(path, ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
   if (f < 0) {
@@ -347,16 +372,19 @@ bool h5_write(const char* path, const H5Out* ds, int nds, int comp_lvl, std::str
 
 H5In::H5In(const char* path, std::string& err) : err_(err) {
   if (!have(err)) return;
+  QuietErrors quiet;
   fid_ = api().H5Fopen(path, ACC_RDONLY, H5P_DEFAULT);
   if (fid_ < 0) err = std::string("could not open ") + path;
 }
 
 H5In::~H5In() {
+  QuietErrors quiet;
   if (fid_ >= 0) api().H5Fclose(fid_);
 }
 
 int H5In::ndims(const char* name, long* dims) const {
   if (fid_ < 0) return -1;
+  QuietErrors quiet;
   const H5Api& a = api();
   if (a.H5Lexists(fid_, name, H5P_DEFAULT) <= 0) return -1;
   const hid_t d = a.H5Dopen2(fid_, name, H5P_DEFAULT);
@@ -377,6 +405,7 @@ int H5In::ndims(const char* name, long* dims) const {
 bool H5In::read(const char* name, bool ints, void* dst, int mrank, const long* mdims,
                 const long* start) const {
   if (fid_ < 0) return false;
+  QuietErrors quiet;
   const H5Api& a = api();
   long fd[8];
   const int fr = ndims(name, fd);

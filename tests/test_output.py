@@ -435,3 +435,29 @@ def test_dateinc_library_matches_oracle():
         ymd = [int(rng.integers(1700, 2400)), int(rng.integers(1, 13)), int(rng.integers(1, 29))]
         ut, dt = float(rng.uniform(0, 86400)), float(rng.uniform(0, 86400))
         assert T.dateinc(dt, ymd, ut) == oracle.dateinc(dt, ymd, ut)
+
+
+def test_library_leaves_the_hosts_hdf5_error_handler_alone():
+    """HDF5's automatic error printing is process-wide; the library turns it off only while
+    its own calls run, so a host's handler is the same before and after a failing call.
+    Run in a fresh process, so the library's first HDF5 call (its set-up) is inside it."""
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    code = (
+        "import ctypes as C, sys\n"
+        f"sys.path[:0] = [{os.path.dirname(here)!r}, {here!r}]\n"
+        "import h5lite\n"
+        "from gemini_amd import transport as T\n"
+        "L = h5lite.lib()\n"
+        "handler = C.cast(L.H5Eprint2, C.c_void_p)\n"
+        "L.H5Eset_auto2(h5lite.hid_t(0), handler, None)\n"
+        "try:\n"
+        "    T.read_simsize('/nonexistent/simsize.h5')\n"
+        "except T.GemhipError:\n"
+        "    pass\n"
+        "f, d = C.c_void_p(), C.c_void_p()\n"
+        "L.H5Eget_auto2(h5lite.hid_t(0), C.byref(f), C.byref(d))\n"
+        "sys.exit(0 if f.value == handler.value and d.value is None else 1)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
