@@ -250,6 +250,48 @@ def measure_output(tr, g, electrovars, sample_shape, reps=3):
     return res
 
 
+# dt_calc's grid rates (kernels_block.hip k_gridrate), per interior cell: Ts, vs1, vs2, vs3
+# of the 7 species and h1, h2, h3 read once
+GRIDRATE_BYTES_PER_CELL = 248
+
+
+def measure_dt_select(tr, g, sample_shape, reps=20):
+    """The device half of dt_select_C on the resident state of the headline run: the grid-rate
+    reduction (HIP events on the library's stream) and the whole gemhip_grid_rates call
+    (reduction, the 21 maxima copied back, synchronisation); the oracle's dt_calc on a host
+    sample beside it."""
+    import oracle
+    from gemini_amd.state import synthetic_state
+    tr.grid_rates()
+    tr.kernel_timing_reset()
+    tr.kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tr.grid_rates()
+    call_ms = 1e3 * (time.perf_counter() - t0) / reps
+    tr.kernel_timing(False)
+    tot, n = tr.kernel_time("gridrate")
+    k_ms = tot / n
+    nb = GRIDRATE_BYTES_PER_CELL * g.ncells
+    gs = make_grid("dipole", sample_shape)
+    st = synthetic_state(gs, seed=1)
+    oracle.dt_calc(gs, st, 0.9)
+    t0, k = time.perf_counter(), 0
+    while time.perf_counter() - t0 < 1.0:
+        oracle.dt_calc(gs, st, 0.9)
+        k += 1
+    el = time.perf_counter() - t0
+    return {"kernel_ms": round(k_ms, 4), "algorithmic_bytes": nb,
+            "GB/s": round(nb / (k_ms / 1e3) / 1e9, 1),
+            "frac": round(nb / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "call_ms": round(call_ms, 4), "calls": reps,
+            "cpu_dt_calc": {"cells/s": round(gs.ncells * k / el),
+                            "cores": 1,
+                            "kind": "port",
+                            "sample": f"oracle dt_calc on a {gs.lx1}x{gs.lx2}x{gs.lx3} dipole "
+                                      f"state, {k} calls in {el:.1f} s"}}
+
+
 def state_is_finite(tr, g):
     """Copy the state blocks back into HBM tensors (after timing) and check them."""
     import torch
@@ -638,6 +680,12 @@ def main():
         whole = measure_whole_step(tr, g, dt, atm)
         del atm
         log(f"whole step {whole['ms_per_step']:.3f} ms, sources {whole['sources_ms']:.3f} ms")
+    dtsel = None
+    if args.workload == "dipole512" and not args.host_state:
+        log("dt_select's grid-rate reduction on the resident state")
+        dtsel = measure_dt_select(tr, g, w["sample"])
+        log(f"grid rates {dtsel['kernel_ms']:.3f} ms ({dtsel['GB/s']:.0f} GB/s), "
+            f"call {dtsel['call_ms']:.3f} ms")
     out_res = None
     if output:
         log("plasma output (flagoutput 2, 1, 3) of the resident state")
@@ -701,6 +749,7 @@ def main():
         "kernels": kern,
         "cpu_baseline": cpu,
         "whole_step": whole,
+        "dt_select": dtsel,
         "output": out_res,
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": None if adj is None else dict(adj, workload=ADJOINT["desc"]),

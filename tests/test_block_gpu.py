@@ -137,3 +137,16 @@ def test_dt_select_matches_oracle(kind):
         dt_gpu = tr.dt_select(st, t, tout, tglow, dt_gpu)
         dt_ref = oracle.dt_select(g, st, 0.8, 1, t, tout, tglow, it, dt_ref)
         assert dt_gpu == dt_ref, (it, dt_gpu, dt_ref)
+
+
+def test_grid_rates_over_more_lines_than_blocks():
+    """More x1 lines than the reduction's blocks (2560 against 2048): blocks walk several
+    lines; the 21 maxima are still the oracle's, bit for bit."""
+    g = ga.cartesian_grid(8, 64, 40)
+    st = ga.synthetic_state(g, seed=6, perturb=True)
+    tr = Transport(g, st.nn, st.Tn)
+    tr.set_residency(1)
+    tr.push(st.copy())
+    _, ref_rates = oracle.dt_calc(g, st, 0.9)
+    assert np.array_equal(tr.grid_rates(), ref_rates)
+    tr.set_residency(0)

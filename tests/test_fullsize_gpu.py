@@ -292,3 +292,24 @@ def test_fullsize_output_parity(tmp_path):
         os.remove(path)
     tr.set_residency(0)
     print("  every value of flagoutput 1 and 2 bit-identical", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(400)
+def test_fullsize_grid_rates():
+    """dt_select's grid-rate reduction on the whole 512x256x192 state equals the oracle's
+    dt_calc over every cell (maxima are exact, so bit for bit)."""
+    print("case dipole 512x256x192 grid rates: generating the state", flush=True)
+    g = ga.dipole_grid(*SHAPE)
+    dev = synthetic_state_device(g, seed=5, device="cuda:0")
+    st = host_state(g, dev)
+    del dev
+    torch.cuda.empty_cache()
+    tr = Transport(g, st.nn, st.Tn, geometry_on_device=True)
+    tr.set_residency(1)
+    tr.push(st)
+    got = tr.grid_rates()
+    tr.set_residency(0)
+    _, ref = oracle.dt_calc(g, st, 0.9)
+    assert np.array_equal(got, ref), (got, ref)
+    print("  21 grid rates bit-identical", flush=True)
