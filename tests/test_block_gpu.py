@@ -139,10 +139,10 @@ def test_dt_select_matches_oracle(kind):
         assert dt_gpu == dt_ref, (it, dt_gpu, dt_ref)
 
 
-def test_grid_rates_over_more_lines_than_blocks():
-    """More x1 lines than the reduction's blocks (2560 against 2048): blocks walk several
-    lines; the 21 maxima are still the oracle's, bit for bit."""
-    g = ga.cartesian_grid(8, 64, 40)
+def test_grid_rates_when_threads_loop_over_cells():
+    """More interior cells (327 680) than the reduction's 1024 x 256 threads: each thread
+    takes several cells; the 21 maxima are still the oracle's, bit for bit."""
+    g = ga.cartesian_grid(64, 64, 80)
     st = ga.synthetic_state(g, seed=6, perturb=True)
     tr = Transport(g, st.nn, st.Tn)
     tr.set_residency(1)
