@@ -341,12 +341,23 @@ __global__ __launch_bounds__(TPB) void k_gridrate(DevGrid g, const double* Ts, c
   }
 }
 
-__global__ void k_gridrate_final(const double* part, int nblocks, double* out) {
-  const int k = threadIdx.x;
-  if (k >= NRATE) return;
+// one block per rate: the block's threads stride over the partial maxima, then a wave
+// shuffle tree and the block's four waves (a lone thread per rate walking the partials
+// took a third as long as the whole reduction kernel)
+__global__ __launch_bounds__(TPB) void k_gridrate_final(const double* part, int nblocks,
+                                                        double* out) {
+  const int k = blockIdx.x;
   double m = 0.0;
-  for (int b = 0; b < nblocks; ++b) m = fmax_f(part[(long)b * NRATE + k], m);
-  out[k] = m;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmax_f(part[(long)b * NRATE + k], m);
+  for (int off = 32; off >= 1; off >>= 1) m = fmax_f(__shfl_xor(m, off), m);
+  __shared__ double red[TPB / 64];
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < TPB / 64; ++w) r = fmax_f(red[w], r);
+    out[k] = r;
+  }
 }
 
 }  // namespace
@@ -361,7 +372,7 @@ void launch_gridrates(const DevGrid& g, const DevState& s, double* work, hipStre
   const int nb = gridrate_blocks(g);
   timing_begin("gridrate", st);
   hipLaunchKernelGGL(k_gridrate, dim3(nb), dim3(TPB), 0, st, g, s.Ts, s.vs1, s.vs2, s.vs3, work);
-  hipLaunchKernelGGL(k_gridrate_final, dim3(1), dim3(64), 0, st, work, nb,
+  hipLaunchKernelGGL(k_gridrate_final, dim3(NRATE), dim3(TPB), 0, st, work, nb,
                      work + (long)nb * NRATE);
   timing_end("gridrate", st);
 }
