@@ -21,16 +21,11 @@ namespace {
 
 constexpr int TPB = 256;
 
+// the datasets of one interior cell (i1, line), line = (i2 - 1) + lx2 (i3 - 1)
 template <int FLAG>
-__global__ __launch_bounds__(TPB) void k_output_pack(DevGrid g, const double* __restrict__ fluid,
-                                                      const double* __restrict__ J,
-                                                      float* __restrict__ out) {
-  // blocks in memory order: block b covers part b % nb of x1 line b / nb (nb blocks a
-  // line), so the index needs 32-bit arithmetic only and the grid streams through HBM
-  const int nb = (g.lx1 + TPB - 1) / TPB;
-  const int line = (int)(blockIdx.x / (unsigned)nb);
-  const int i1 = (int)(blockIdx.x % (unsigned)nb) * TPB + threadIdx.x;
-  if (i1 >= g.lx1) return;
+__device__ __forceinline__ void pack_cell(const DevGrid& g, const double* __restrict__ fluid,
+                                          const double* __restrict__ J, float* __restrict__ out,
+                                          int i1, int line) {
   const int i2 = line % g.lx2;
   const int i3 = line / g.lx2;
   const long n = (long)g.lx1 * g.lx2 * g.lx3;
@@ -96,6 +91,31 @@ __global__ __launch_bounds__(TPB) void k_output_pack(DevGrid g, const double* __
   }
 }
 
+#ifndef OUTPUT_CELLS_PER_THREAD
+// two cells a thread, a block apart: 1-2 % faster than one for the full and ISR-like
+// outputs and 18 % for the electron density alone at 512x256x192
+#define OUTPUT_CELLS_PER_THREAD 2
+#endif
+constexpr int CPT = OUTPUT_CELLS_PER_THREAD;
+
+// blocks in memory order: block b covers part b % nb of x1 line b / nb (nb blocks a line,
+// CPT cells a thread, a block's cells interleaved so each load instruction stays
+// coalesced), 32-bit index arithmetic only
+template <int FLAG>
+__global__ __launch_bounds__(TPB) void k_output_pack(DevGrid g, const double* __restrict__ fluid,
+                                                      const double* __restrict__ J,
+                                                      float* __restrict__ out) {
+  const int per = TPB * CPT;
+  const int nb = (g.lx1 + per - 1) / per;
+  const int line = (int)(blockIdx.x / (unsigned)nb);
+  const int i0 = (int)(blockIdx.x % (unsigned)nb) * per + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < CPT; ++u) {
+    const int i1 = i0 + u * TPB;
+    if (i1 < g.lx1) pack_cell<FLAG>(g, fluid, J, out, i1, line);
+  }
+}
+
 }  // namespace
 
 long output_count(const DevGrid& g, int flagoutput) {
@@ -107,7 +127,7 @@ long output_count(const DevGrid& g, int flagoutput) {
 
 void launch_output_pack(const DevGrid& g, const double* fluid, const double* J,
                         int flagoutput, float* out, hipStream_t st) {
-  const dim3 grid((unsigned)(g.lx2 * g.lx3) * (unsigned)((g.lx1 + TPB - 1) / TPB));
+  const dim3 grid((unsigned)(g.lx2 * g.lx3) * (unsigned)((g.lx1 + TPB * CPT - 1) / (TPB * CPT)));
   timing_begin("output_pack", st);
   if (flagoutput == 2)
     k_output_pack<2><<<grid, TPB, 0, st>>>(g, fluid, J, out);
