@@ -97,7 +97,10 @@ def empty_state(g: Grid) -> State:
 
 
 def _chapman(z, zpk, H):
-    zeta = (z - zpk) / H
+    # far below the peak exp(-zeta) overflows; there the layer is zero to double precision,
+    # and clamping zeta at -700 gives that zero without the overflow (values above the
+    # clamp are unchanged)
+    zeta = np.maximum((z - zpk) / H, -700.0)
     return np.exp(0.5 * (1.0 - zeta - np.exp(-zeta)))
 
 

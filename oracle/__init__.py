@@ -35,18 +35,28 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+def _load(path):
+    lib_ = C.CDLL(path)
+    lib_.orc_minmod.restype = C.c_double
+    lib_.orc_minmod.argtypes = [C.c_double, C.c_double]
+    for name in ("orc_gbsv", "orc_energy_diffusion", "orc_transport_step", "orc_num_threads",
+                 "orc_energy_diffusion_ext", "orc_transport_step_ext", "orc_clean_param",
+                 "orc_fluid_adv_block"):
+        getattr(lib_, name).restype = C.c_int
+    return lib_
+
+
 def lib():
+    """The oracle library: liboracle.so, or the build that $GEMINI_ORACLE_LIB names (the
+    mutation tests run the pins against copies with one planted mistake each)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        _lib = C.CDLL(LIB_PATH)
-        _lib.orc_minmod.restype = C.c_double
-        _lib.orc_minmod.argtypes = [C.c_double, C.c_double]
-        for name in ("orc_gbsv", "orc_energy_diffusion", "orc_transport_step", "orc_num_threads",
-                     "orc_energy_diffusion_ext", "orc_transport_step_ext", "orc_clean_param",
-                     "orc_fluid_adv_block"):
-            getattr(_lib, name).restype = C.c_int
+        path = os.environ.get("GEMINI_ORACLE_LIB")
+        if not path:
+            path = LIB_PATH
+            if not os.path.exists(path):
+                build()
+        _lib = _load(path)
     return _lib
 
 
