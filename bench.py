@@ -236,11 +236,11 @@ def measure_output(tr, g, electrovars, sample_shape, reps=3):
         shutil.rmtree(outdir, ignore_errors=True)
     gs = make_grid("dipole", sample_shape)
     st = synthetic_state(gs, seed=1)
-    oracle.output_pack(st, gs.gridflag, 1)
+    oracle.output_pack(st, gs.gridflag, 1, patch=True)
     t0 = time.perf_counter()
     k = 0
     while time.perf_counter() - t0 < 2.0:
-        oracle.output_pack(st, gs.gridflag, 1)
+        oracle.output_pack(st, gs.gridflag, 1, patch=True)
         k += 1
     el = time.perf_counter() - t0
     res["cpu_pack"] = {"cells/s": round(gs.ncells * k / el), "cores": 1, "kind": "port",

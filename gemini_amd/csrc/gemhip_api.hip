@@ -1627,7 +1627,7 @@ int write_plasma_output(const int* ymd, const double* UTsec, double** fluidvars,
              "electrovars upload"))
     return 6;
   float* packed = reinterpret_cast<float*>(reinterpret_cast<char*>(c.d_out) + ebytes);
-  launch_output_pack(g, c.d_fluid, c.d_out, flag, packed, c.stream);
+  launch_output_pack(g, c.d_fluid, c.d_out, flag, !root, packed, c.stream);
   if (!hipok(hipGetLastError(), "output pack launch")) return 6;
 
   // field datasets in file order (name, species count, values)

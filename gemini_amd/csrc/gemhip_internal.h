@@ -154,8 +154,9 @@ void launch_zero(double* p, long n, hipStream_t st);
 // 35-field fluidvars block, J the ghosted fields J1, J2, J3, Phi back to back; out receives
 // the datasets in file order (Phiall last).
 long output_count(const DevGrid& g, int flagoutput);
+// patch: J1-J3 as the patch writer reads them (ghost-origin corner), else the interior
 void launch_output_pack(const DevGrid& g, const double* fluid, const double* J, int flagoutput,
-                        float* out, hipStream_t st);
+                        bool patch, float* out, hipStream_t st);
 
 // kernel-timing hooks implemented by the API layer
 void timing_begin(const char* name, hipStream_t st);

@@ -8,6 +8,13 @@
 // ns*v accumulated from zero, then divided by the electron density; every value is then
 // rounded to the nearest real32 (Fortran's real()).
 //
+// The patch writer declares J1, J2, J3 as dimension(1:,1:,1:) (io_nompi.f90:192) and is
+// handed the ghosted electrovars fields (libgemini.f90:549-551), so its J(1:lx1,1:lx2,1:lx3)
+// is the ghosted block's ghost-origin corner: cell (i1, i2, i3) of its file holds J at
+// ghosted indices (i1-2, i2-2, i3-2).  The root writer's J (plasma_output.f90:46) keeps
+// the ghosted bounds, so its file holds the interior.  `jshift` is the offset between the
+// two (0 for the root writer).
+//
 // Per interior cell the full output (flagoutput 1) reads 36 doubles (ns, vs1, Ts of the 7
 // species, vs2 and vs3 of the 6 ions, J1-J3) and writes 26 floats: 392 bytes; flagoutput 2
 // reads 35 doubles and writes 9 floats (316 bytes); flagoutput 3 reads and writes the
@@ -25,7 +32,7 @@ constexpr int TPB = 256;
 template <int FLAG>
 __device__ __forceinline__ void pack_cell(const DevGrid& g, const double* __restrict__ fluid,
                                           const double* __restrict__ J, float* __restrict__ out,
-                                          int i1, int line) {
+                                          int i1, int line, long jshift) {
   const int i2 = line % g.lx2;
   const int i3 = line / g.lx2;
   const long n = (long)g.lx1 * g.lx2 * g.lx3;
@@ -48,9 +55,10 @@ __device__ __forceinline__ void pack_cell(const DevGrid& g, const double* __rest
       a2 = a2 + d * vs2[isp * N + c];
       a3 = a3 + d * vs3[isp * N + c];
     }
-    const float J1 = (float)J[c];
-    const float J2 = (float)J[N + c];
-    const float J3 = (float)J[2 * N + c];
+    const long cj = c - jshift;
+    const float J1 = (float)J[cj];
+    const float J2 = (float)J[N + cj];
+    const float J3 = (float)J[2 * N + cj];
     if (FLAG == 2) {
       double a1 = 0.0, aT = 0.0;
 #pragma unroll
@@ -104,7 +112,7 @@ constexpr int CPT = OUTPUT_CELLS_PER_THREAD;
 template <int FLAG>
 __global__ __launch_bounds__(TPB) void k_output_pack(DevGrid g, const double* __restrict__ fluid,
                                                       const double* __restrict__ J,
-                                                      float* __restrict__ out) {
+                                                      float* __restrict__ out, long jshift) {
   const int per = TPB * CPT;
   const int nb = (g.lx1 + per - 1) / per;
   const int line = (int)(blockIdx.x / (unsigned)nb);
@@ -112,7 +120,7 @@ __global__ __launch_bounds__(TPB) void k_output_pack(DevGrid g, const double* __
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
     const int i1 = i0 + u * TPB;
-    if (i1 < g.lx1) pack_cell<FLAG>(g, fluid, J, out, i1, line);
+    if (i1 < g.lx1) pack_cell<FLAG>(g, fluid, J, out, i1, line, jshift);
   }
 }
 
@@ -126,15 +134,16 @@ long output_count(const DevGrid& g, int flagoutput) {
 }
 
 void launch_output_pack(const DevGrid& g, const double* fluid, const double* J,
-                        int flagoutput, float* out, hipStream_t st) {
+                        int flagoutput, bool patch, float* out, hipStream_t st) {
+  const long jshift = patch ? 2 + 2L * g.L1 + 2L * g.L1 * g.L2 : 0;
   const dim3 grid((unsigned)(g.lx2 * g.lx3) * (unsigned)((g.lx1 + TPB * CPT - 1) / (TPB * CPT)));
   timing_begin("output_pack", st);
   if (flagoutput == 2)
-    k_output_pack<2><<<grid, TPB, 0, st>>>(g, fluid, J, out);
+    k_output_pack<2><<<grid, TPB, 0, st>>>(g, fluid, J, out, jshift);
   else if (flagoutput == 3)
-    k_output_pack<3><<<grid, TPB, 0, st>>>(g, fluid, J, out);
+    k_output_pack<3><<<grid, TPB, 0, st>>>(g, fluid, J, out, jshift);
   else
-    k_output_pack<1><<<grid, TPB, 0, st>>>(g, fluid, J, out);
+    k_output_pack<1><<<grid, TPB, 0, st>>>(g, fluid, J, out, jshift);
   timing_end("output_pack", st);
 }
 

@@ -509,13 +509,16 @@ def output_count(lx1, lx2, lx3, flagoutput):
     return int(lib().orc_output_count(lx1, lx2, lx3, int(flagoutput)))
 
 
-def output_pack(st, gridflag, flagoutput):
-    """The real32 field datasets plasma_output_nompi writes for state st, as a dict name ->
-    array in the file's C-order shape (Fortran shape reversed), in file order."""
+def output_pack(st, gridflag, flagoutput, patch):
+    """The real32 field datasets the output writers put in the file for state st, as a dict
+    name -> array in the file's C-order shape (Fortran shape reversed), in file order.
+    patch: the patch writer plasma_output_nompi, whose J1-J3 are the ghosted fields read
+    from their first element (io_nompi.f90:192); otherwise the root writer (interior J)."""
     lx1, lx2, lx3 = (d - 4 for d in st.fluidvars.shape[:3])
     n, m = lx1 * lx2 * lx3, lx2 * lx3
     out = np.empty(output_count(lx1, lx2, lx3, flagoutput), dtype=np.float32)
-    lib().orc_output_pack(lx1, lx2, lx3, int(gridflag), int(flagoutput), ptr(st.fluidvars),
+    lib().orc_output_pack(lx1, lx2, lx3, int(gridflag), int(flagoutput), int(bool(patch)),
+                          ptr(st.fluidvars),
                           ptr(st.electrovars), out.ctypes.data_as(C.c_void_p))
     if flagoutput == 2:
         names = [("neall", 0), ("v1avgall", 0), ("Tavgall", 0), ("Teall", 0), ("J1all", 0),

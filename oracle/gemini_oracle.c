@@ -1710,8 +1710,12 @@ long orc_output_count(int lx1, int lx2, int lx3, int flagoutput) {
 /* io_nompi.f90:199-213 and the real() conversions of :230-258.  fluidvars is the ghosted
    (lx1+4, lx2+4, lx3+4, 5*lsp) block (ns vs1 vs2 vs3 Ts), electrovars the ghosted
    (..., 7) block (E1 E2 E3 J1 J2 J3 Phi).  Averages are sum over the lsp-1 ions of
-   ns*v, accumulated from zero in species order, divided by the electron density. */
-void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
+   ns*v, accumulated from zero in species order, divided by the electron density.
+   With patch set, J1-J3 are read as the patch writer's dimension(1:,1:,1:) dummies see
+   the ghosted fields it is handed (io_nompi.f90:192, libgemini.f90:549-551): its
+   J(1:lx1,1:lx2,1:lx3) starts at the ghosted block's first element, ghosted index -1 in
+   each dimension.  The root writer's J keeps the ghosted bounds (plasma_output.f90:46). */
+void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput, int patch,
                      const double *fluidvars, const double *electrovars, float *out) {
   const long L1 = lx1 + 4, L2 = lx2 + 4, L3 = lx3 + 4, Ng = L1 * L2 * L3;
   const long n = (long)lx1 * lx2 * lx3, m = (long)lx2 * lx3, B = (long)LSP * Ng;
@@ -1726,6 +1730,7 @@ void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
         const long c = (i1 + 2) + L1 * ((i2 + 2) + L2 * (i3 + 2));
         const long k = i1 + (long)lx1 * (i2 + (long)lx2 * i3);
         const long e = (long)(LSP - 1) * Ng + c;
+        const long cj = patch ? i1 + L1 * (i2 + L2 * (long)i3) : c;
         double a2 = 0.0, a3 = 0.0, a1 = 0.0, aT = 0.0;
         for (int isp = 0; isp < LSP - 1; ++isp) {
           const long s = (long)isp * Ng + c;
@@ -1742,9 +1747,9 @@ void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
           o[n + k] = (float)(a1 / ns[e]);
           o[2 * n + k] = (float)(aT / ns[e]);
           o[3 * n + k] = (float)Ts[e];
-          o[4 * n + k] = (float)J1[c];
-          o[5 * n + k] = (float)J2[c];
-          o[6 * n + k] = (float)J3[c];
+          o[4 * n + k] = (float)J1[cj];
+          o[5 * n + k] = (float)J2[cj];
+          o[6 * n + k] = (float)J3[cj];
           o[7 * n + k] = (float)v2avg;
           o[8 * n + k] = (float)v3avg;
         } else {
@@ -1754,9 +1759,9 @@ void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
             o[(LSP + isp) * n + k] = (float)vs1[s];
             o[(2 * LSP + isp) * n + k] = (float)Ts[s];
           }
-          o[3 * LSP * n + k] = (float)J1[c];
-          o[(3 * LSP + 1) * n + k] = (float)J2[c];
-          o[(3 * LSP + 2) * n + k] = (float)J3[c];
+          o[3 * LSP * n + k] = (float)J1[cj];
+          o[(3 * LSP + 1) * n + k] = (float)J2[cj];
+          o[(3 * LSP + 2) * n + k] = (float)J3[cj];
           o[(3 * LSP + 3) * n + k] = (float)v2avg;
           o[(3 * LSP + 4) * n + k] = (float)v3avg;
         }

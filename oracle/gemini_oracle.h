@@ -166,7 +166,7 @@ int orc_sources_block(const orc_grid *g, const orc_atmos *a, double dt, double *
 int orc_filestem(const int ymd[3], double UTsec, char out[22]);
 int orc_dateinc(double dtsec, int ymd[3], double *UTsec);    /* timeutils.f90:79-107 */
 long orc_output_count(int lx1, int lx2, int lx3, int flagoutput);
-void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput,
+void orc_output_pack(int lx1, int lx2, int lx3, int gridflag, int flagoutput, int patch,
                      const double *fluidvars, const double *electrovars, float *out);
 
 /* 0: x**2.5 by libm pow, as the reference (default); 1: correctly rounded x**2.5 */

@@ -66,7 +66,7 @@ def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
     ymd, ut = oracle.dateinc(dt, [2015, 4, 13], 12345.67891)      # the host's dateinc_C
     stem = oracle.filestem(ymd, ut)
     path = os.path.join(str(tmp_path), f"{stem}_{mode}.h5")
-    want = oracle.output_pack(got, g.gridflag, flag)
+    want = oracle.output_pack(got, g.gridflag, flag, patch=True)
     with h5lite.File(path) as f:
         assert f.read("/flagoutput") == flag and f.read("/patchID") == mode
         for name, arr in want.items():
@@ -74,6 +74,7 @@ def test_host_driver_matches_oracle(tmp_path, driver, kind, shape, mode):
             assert a.shape == arr.shape and np.array_equal(a.view(np.uint32), arr.view(np.uint32)), name
     # and gemini.bin's own file (gemhip_output_plasma): plain date name, TEall
     root = os.path.join(str(tmp_path), f"{stem}.h5")
+    want = oracle.output_pack(got, g.gridflag, flag, patch=False)
     with h5lite.File(root) as f:
         assert "/patchID" not in f.names()
         for name, arr in want.items():

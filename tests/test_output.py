@@ -102,7 +102,7 @@ def test_each_value_is_the_nearest_real32():
     fv[2, 2, 3, 28] = np.nan
     fv[3, 2, 3, 28] = 1.0 + 2.0 ** -24          # halfway: ties to even
     fv[4, 2, 3, 28] = 1.0 + 3 * 2.0 ** -24
-    out = oracle.output_pack(st, g.gridflag, 1)
+    out = oracle.output_pack(st, g.gridflag, 1, patch=False)
     for name, k in (("/nsall", 0), ("/vs1all", 7), ("/Tsall", 28)):
         with np.errstate(over="ignore"):
             want = _interior(fv[..., k:k + 7]).astype(np.float32)
@@ -110,8 +110,8 @@ def test_each_value_is_the_nearest_real32():
     for j, name in ((3, "/J1all"), (4, "/J2all"), (5, "/J3all")):
         want = _interior(st.electrovars[..., j]).astype(np.float32)
         assert np.array_equal(_bits(out[name]), _bits(want)), name
-    out2 = oracle.output_pack(st, g.gridflag, 2)
-    out3 = oracle.output_pack(st, g.gridflag, 3)
+    out2 = oracle.output_pack(st, g.gridflag, 2, patch=True)
+    out3 = oracle.output_pack(st, g.gridflag, 3, patch=True)
     ne = _interior(fv[..., 6]).astype(np.float32)
     assert np.array_equal(_bits(out2["/neall"]), _bits(ne))
     assert np.array_equal(_bits(out3["/neall"]), _bits(ne))
@@ -145,8 +145,8 @@ def _exact_species_state(seed):
 
 def test_averages_reduce_to_the_common_ion_value():
     g, st, comps = _exact_species_state(1)
-    o1 = oracle.output_pack(st, g.gridflag, 1)
-    o2 = oracle.output_pack(st, g.gridflag, 2)
+    o1 = oracle.output_pack(st, g.gridflag, 1, patch=True)
+    o2 = oracle.output_pack(st, g.gridflag, 2, patch=True)
     for key, name in (("v2", "/v2avgall"), ("v3", "/v3avgall")):
         want = _interior(comps[key]).astype(np.float32)
         assert np.array_equal(_bits(o1[name]), _bits(want)), name
@@ -163,7 +163,7 @@ def test_average_of_a_single_ion_is_its_density_share(ion):
         fv[..., isp] = 0.0
     fv[..., ion] = 2.0 ** 36                 # half of ne
     fv[..., 14 + ion] = 3.0 * fv[..., 14 + ion] + 0.5
-    o = oracle.output_pack(st, g.gridflag, 2)
+    o = oracle.output_pack(st, g.gridflag, 2, patch=True)
     want = _interior((3.0 * comps["v2"] + 0.5) / 2).astype(np.float32)
     assert np.array_equal(_bits(o["/v2avgall"]), _bits(want))
 
@@ -171,7 +171,7 @@ def test_average_of_a_single_ion_is_its_density_share(ion):
 def test_negative_zero_products_sum_to_positive_zero():
     g, st, _ = _exact_species_state(3)
     st.fluidvars[..., 21:27] = -0.0
-    o = oracle.output_pack(st, g.gridflag, 1)
+    o = oracle.output_pack(st, g.gridflag, 1, patch=True)
     assert np.all(_bits(o["/v3avgall"]) == 0)
 
 
@@ -183,16 +183,40 @@ def test_potential_slab_is_the_top_x1_plane(gridflag):
     st.electrovars[..., 6] = 1000.0 * i1 + i2 + 0.125 * i3
     top = 1 if gridflag == 1 else g.lx1
     for flag in (1, 2, 3):
-        phi = oracle.output_pack(st, gridflag, flag)["/Phiall"]
+        phi = oracle.output_pack(st, gridflag, flag, patch=True)["/Phiall"]
         i2c, i3c = np.meshgrid(np.arange(1, g.lx2 + 1), np.arange(1, g.lx3 + 1), indexing="xy")
         assert np.array_equal(phi, (1000.0 * top + i2c + 0.125 * i3c).astype(np.float32))
 
 
+def test_patch_writer_reads_the_currents_from_the_ghost_origin():
+    """plasma_output_nompi declares J1-J3 dimension(1:,1:,1:) (io_nompi.f90:192) and is
+    handed the ghosted fields (libgemini.f90:549-551): its cell (i1, i2, i3) is the
+    ghosted field at (i1-2, i2-2, i3-2).  The root writer keeps ghosted bounds
+    (plasma_output.f90:46) and writes the interior.  Everything else is the same."""
+    g, st = _state(lx1=7, lx2=5, lx3=4)
+    L1, L2, L3 = st.electrovars.shape[:3]
+    i1, i2, i3 = np.meshgrid(np.arange(L1) - 1, np.arange(L2) - 1, np.arange(L3) - 1, indexing="ij")
+    for j in (3, 4, 5):
+        st.electrovars[..., j] = 1000.0 * i1 + 10.0 * i2 + 0.125 * i3 + j
+    c1, c2, c3 = np.meshgrid(np.arange(1, g.lx1 + 1), np.arange(1, g.lx2 + 1),
+                             np.arange(1, g.lx3 + 1), indexing="ij")
+    for flag in (1, 2):
+        patch = oracle.output_pack(st, g.gridflag, flag, patch=True)
+        root = oracle.output_pack(st, g.gridflag, flag, patch=False)
+        for j, name in ((3, "/J1all"), (4, "/J2all"), (5, "/J3all")):
+            for out, shift in ((patch, 2), (root, 0)):
+                want = 1000.0 * (c1 - shift) + 10.0 * (c2 - shift) + 0.125 * (c3 - shift) + j
+                assert np.array_equal(out[name], np.ascontiguousarray(want.T).astype(np.float32))
+        for k in patch:
+            if k not in ("/J1all", "/J2all", "/J3all"):
+                assert np.array_equal(_bits(patch[k]), _bits(root[k])), k
+
+
 def test_other_flags_write_the_full_set():
     g, st = _state()
-    full = oracle.output_pack(st, g.gridflag, 1)
+    full = oracle.output_pack(st, g.gridflag, 1, patch=True)
     for flag in (0, 4, -1):
-        o = oracle.output_pack(st, g.gridflag, flag)
+        o = oracle.output_pack(st, g.gridflag, flag, patch=True)
         assert list(o) == list(full)
         for k in full:
             assert np.array_equal(_bits(o[k]), _bits(full[k]))
@@ -313,7 +337,7 @@ def test_oracle_output_file_reads_back_as_initial_conditions(tmp_path):
     """An output file (real32 nsall/vs1all/Tsall and the potential slab) is a valid input:
     the interiors come back as the real32 values widened, the potential as its slab."""
     g, st = _state(lx1=8, lx2=5, lx3=3, seed=4)
-    out = oracle.output_pack(st, g.gridflag, 1)
+    out = oracle.output_pack(st, g.gridflag, 1, patch=True)
     f = str(tmp_path / "out.h5")
     h5lite.write(f, out)
     simsize = str(tmp_path / "s.h5")

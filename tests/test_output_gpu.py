@@ -61,7 +61,7 @@ def same(a, b):
 
 def check_file(path, st, gridflag, flag, root=False):
     """root: gemini.bin's file (plasma_output_hdf5.f90:10-65): no patch datasets, TEall."""
-    want = oracle.output_pack(st, gridflag, flag)
+    want = oracle.output_pack(st, gridflag, flag, patch=not root)
     header = {"/flagoutput": ("i", ()), "/time/ymd": ("i", (3,)), "/time/UThour": ("f", ())}
     if root:
         want = {("/TEall" if k == "/Teall" else k): v for k, v in want.items()}
