@@ -1318,8 +1318,16 @@ int gemhip_context_select(int handle) {
 
 int gemhip_context_current(void) { return g_current_handle; }
 
+bool context_in_graph(int handle);
+
 int gemhip_context_destroy(int handle) {
   if (handle <= 0 || handle > (int)contexts().size() || !contexts()[handle - 1]) return 1;
+  // a captured ensemble graph holds this context's HBM and stream: release it first
+  if (context_in_graph(handle)) {
+    fail("library context %d belongs to a captured ensemble graph (gemhip_ensemble_release)",
+         handle);
+    return 2;
+  }
   const int keep = g_current_handle == handle ? 0 : g_current_handle;
   gemhip_context_select(handle);
   gemhip_finalize();
@@ -1352,11 +1360,23 @@ struct EnsembleGraph {
   hipGraphExec_t exec = nullptr;
   hipStream_t origin = nullptr;
   std::vector<hipEvent_t> events;
+  std::vector<int> members;    // context handles whose memory and streams the graph uses
 };
 std::vector<EnsembleGraph*>& graphs() {
   static std::vector<EnsembleGraph*> v;
   return v;
 }
+}  // namespace
+
+bool context_in_graph(int handle) {
+  for (const EnsembleGraph* e : graphs())
+    if (e)
+      for (int h : e->members)
+        if (h == handle) return true;
+  return false;
+}
+
+namespace {
 
 void release_graph(EnsembleGraph* e) {
   if (e->exec) hipGraphExecDestroy(e->exec);
@@ -1381,11 +1401,18 @@ int gemhip_ensemble_capture(int n, const int* handles, const double* dt, int nst
       gemhip_context_select(keep);
       return -1;
     }
-    if (c.timing) { fail("disable kernel timing before capturing an ensemble"); return -1; }
-    if (!ensure_scratch()) { gemhip_context_select(keep); return -1; }
-    if (!hipok(hipStreamSynchronize(c.stream), "ensemble member sync")) return -1;
+    if (c.timing) {
+      fail("disable kernel timing before capturing an ensemble");
+      gemhip_context_select(keep);
+      return -1;
+    }
+    if (!ensure_scratch() || !hipok(hipStreamSynchronize(c.stream), "ensemble member sync")) {
+      gemhip_context_select(keep);
+      return -1;
+    }
   }
   auto* e = new EnsembleGraph();
+  e->members.assign(handles, handles + n);
   bool ok = hipok(hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking), "origin stream");
   e->events.resize(n + 1, nullptr);
   for (int m = 0; ok && m <= n; ++m)

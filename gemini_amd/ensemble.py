@@ -77,9 +77,16 @@ class Ensemble:
             tr.synchronize()
 
     def autotune(self, states, steps: int = 3):
-        """Choose each member's kernel variants by measurement, then re-push `states`."""
+        """Choose each member's kernel variants by measurement, then re-push `states`.
+        Graphs captured before hold the old variants, so they are released."""
         from .autotune import autotune
+        self._release_graphs()
         return autotune(self, states, steps)
+
+    def _release_graphs(self):
+        for gid in self._graphs.values():
+            self.lib.gemhip_ensemble_release(gid)
+        self._graphs.clear()
 
     def synchronize(self):
         for tr in self.trs:
@@ -92,9 +99,7 @@ class Ensemble:
         self.trs[i].pull(st)
 
     def close(self):
-        for gid in self._graphs.values():
-            self.lib.gemhip_ensemble_release(gid)
-        self._graphs.clear()
+        self._release_graphs()
         for tr in self.trs:
             tr.close()
         self.trs = []

@@ -257,7 +257,8 @@ class Transport:
     def close(self):
         """Release an own context (HBM buffers, stream)."""
         if self.handle:
-            self.lib.gemhip_context_destroy(int(self.handle))
+            if self.lib.gemhip_context_destroy(int(self.handle)) != 0:
+                raise GemhipError(self.lib.gemhip_last_error().decode())
             self.handle = None
 
     def set_tuning(self, knob: str, value: int):

@@ -291,6 +291,8 @@ int gemhip_eval_math(int op, const double* x, double* y, long n);
 int gemhip_context_create(void);
 int gemhip_context_select(int handle);
 int gemhip_context_current(void);
+/* Destroying a context that a captured ensemble graph uses is refused (returns 2) until
+   the graph is released. */
 int gemhip_context_destroy(int handle);
 
 /* Kernel variants of the selected context: "sweep_pf" (1 or 2, x2/x3 sweep pipeline

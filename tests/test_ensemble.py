@@ -97,3 +97,21 @@ def test_kernel_variants_do_not_change_results():
             compare(got, ref[i][1], exact=ALL)
     finally:
         ens.close()
+
+
+def test_members_of_a_live_graph_cannot_be_destroyed():
+    """A captured graph holds its members' HBM and streams: destroying a member is refused
+    until the graph is released; autotuning releases graphs captured before it."""
+    ms = members()[:2]
+    ens = Ensemble(ms)
+    try:
+        gid = ens.capture(1)
+        h = ens.trs[0].handle
+        assert ens.lib.gemhip_context_destroy(int(h)) == 2
+        assert b"ensemble graph" in ens.lib.gemhip_last_error()
+        ens.autotune([st for _, st, _ in ms], steps=1)
+        assert ens._graphs == {}
+        assert ens.lib.gemhip_ensemble_launch(gid, 1) != 0      # released with the old variants
+        ens.run(1)                                               # captured anew
+    finally:
+        ens.close()
