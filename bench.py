@@ -85,6 +85,7 @@ ENSEMBLE = dict(members=64, steps=20, seed=1, verify=3,
 
 ADJOINT = dict(shape=(256, 128, 96), steps=50, every=10, seed=1, fd_dirs=32, eps=1e-6,
                qmax=1e-5, fd_sweep_dirs=3, fd_sweep_eps=(1e-4, 1e-5, 1e-6, 1e-7, 1e-8),
+               fd_column_dirs=8,
                alts_km=(150, 200, 250, 300, 400, 500, 650, 800),
                desc="3-D Cartesian 256x128x96 (3.1M cells), 50 advection+TRBDF2 steps, "
                     "gradient of the squared Te mismatch at 8 altitudes against a seeded "
@@ -315,25 +316,10 @@ def load_traffic(workload, kernel):
         return None
 
 
-def ensemble_sizes(seed, n):
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    return [(int(rng.choice(np.arange(64, 513, 32))), int(rng.integers(16, 129)),
-             int(rng.integers(16, 97))) for _ in range(n)]
-
-
 def ensemble_member(shape, seed, own_context=True):
     """(grid, Transport with the seeded state pushed, dt) for one ensemble member."""
-    import gemini_amd as ga
-    from gemini_amd.synth_gpu import synthetic_state_device, cfl_dt_device
-    from gemini_amd.transport import Transport
-    g = ga.cartesian_grid(*shape)
-    st = synthetic_state_device(g, seed=seed, device="cuda:0")
-    dt = cfl_dt_device(g, st)
-    tr = Transport(g, st["nn"], st["Tn"], device=0, geometry_on_device=True,
-                   own_context=own_context)
-    tr.set_residency(1)
-    tr.push_blocks(st["fluidvars"], st["aux"], st["electrovars"])
+    from gemini_amd.ensemble import device_member
+    g, tr, dt, _ = device_member(shape, seed, own_context)
     return g, tr, dt
 
 
@@ -344,7 +330,8 @@ def bench_ensemble(args):
     from gemini_amd.ensemble import Ensemble
     from gemini_amd.autotune import best_variant
     E = ENSEMBLE
-    sizes = ensemble_sizes(E["seed"], E["members"])
+    from gemini_amd.ensemble import draw_sizes
+    sizes = draw_sizes(E["seed"], E["members"])
     seeds = [1000 * E["seed"] + m for m in range(len(sizes))]
     t0 = time.perf_counter()
     made = [ensemble_member(sz, sd) for sz, sd in zip(sizes, seeds)]
@@ -418,19 +405,30 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_adjoint(fd_dirs):
+def bench_adjoint(fd_dirs, column_dirs=None):
     """Differentiation form: forward run with checkpoints, then the hand-written adjoint
     (replay of each segment with the tape, adjoint kernels in reverse), through the
-    torch.autograd front end.  Returns the measurements as a dict."""
+    torch.autograd front end.  Returns the measurements as a dict.
+
+    The central-difference check runs the +eps and -eps forwards in lockstep with the
+    adjoint tape recorded, and counts the branches of the electron energy path that differ
+    between them (MC limiter choices per sweep, upwind sides, the 100 K floor): the loss is
+    only piecewise smooth, and the adjoint is the derivative of the map with its branches
+    frozen.  Besides the configured random directions it checks directions confined to one
+    (x2, x3) column."""
     import torch
     from gemini_amd.synth_gpu import synthetic_state_device, cfl_dt_device
-    from gemini_amd.adjoint import TransportRun, electron_temperature, interior
+    from gemini_amd.adjoint import TransportRun, electron_temperature, interior, branch_flips
     A = ADJOINT
+    column_dirs = A["fd_column_dirs"] if column_dirs is None else column_dirs
     t0 = time.perf_counter()
     g = make_grid("cart", A["shape"])
     st = synthetic_state_device(g, seed=A["seed"], device="cuda:0")
     dt = cfl_dt_device(g, st)
     run = TransportRun(g, st, dt, A["steps"], every=A["every"], geometry_on_device=True)
+    # a second run of the same setup: the -eps side of the lockstep difference check
+    run_m = TransportRun(g, st, dt, A["steps"], every=A["every"], geometry_on_device=True) \
+        if fd_dirs or column_dirs else None
     del st
     torch.cuda.empty_cache()
     dev = torch.device("cuda:0")
@@ -443,9 +441,11 @@ def bench_adjoint(fd_dirs):
     alts = torch.tensor([int(abs(z - 1e3 * a).argmin()) for a in A["alts_km"]], device=dev)
     target = interior(run, run.forward(kt, qt))[..., alts].clone()
 
+    def loss_of(Te):
+        return ((interior(run, Te)[..., alts] - target) ** 2).sum()
+
     def loss(k, q):
-        Te = interior(run, electron_temperature(run, k, q))
-        return ((Te[..., alts] - target) ** 2).sum()
+        return loss_of(electron_temperature(run, k, q))
 
     # warm-up through the same path (autograd's backward runs on its own device thread;
     # the first library calls there and the adjoint work buffers are set up here)
@@ -477,38 +477,80 @@ def bench_adjoint(fd_dirs):
         if n:
             kern[name] = {"ms": round(tot / n, 4), "launches": n}
     gk, gq = k.grad.detach().clone(), q.grad.detach().clone()
-    rels, sweep = [], {}
+
+    def central_difference(dk, dq, e):
+        """(fd, branch flips summed over the run) for direction (dk, dq) and step e."""
+        gp = run.recorded_steps(k0 + e * dk, q0 + e * dq)
+        gm = run_m.recorded_steps(k0 - e * dk, q0 - e * dq)
+        tot = {}
+        for _ in range(A["steps"]):
+            _, tp = next(gp)
+            _, tm = next(gm)
+            for key, v in branch_flips(run, tp, tm).items():
+                tot[key] = tot.get(key, 0) + v
+        for it in (gp, gm):
+            for _ in it:
+                pass
+        fd = (loss_of(run.last_Te) - loss_of(run_m.last_Te)).item() / (2 * e)
+        return fd, tot
+
+    dirs, sweep = [], {}
     with torch.no_grad():
-        for d in range(fd_dirs):
+        for d in range(fd_dirs + column_dirs):
             dk = torch.randn(sh, generator=gen, device=dev, dtype=torch.float64)
             dq = A["qmax"] * torch.randn(sh, generator=gen, device=dev, dtype=torch.float64)
+            kind = "random"
+            if d >= fd_dirs:                               # one (x2, x3) column only
+                kind = "column"
+                i3, i2 = (int(torch.randint(0, n, (1,), generator=gen, device=dev).item())
+                          for n in sh)
+                mask = torch.zeros(sh, dtype=torch.float64, device=dev)
+                mask[i3, i2] = 1.0
+                dk, dq = dk * mask, dq * mask
             ad = ((gk * dk).sum() + (gq * dq).sum()).item()
             # the configured eps on every direction; on the first few, smaller and larger
-            # steps too: the minmod limiter makes the loss piecewise smooth, and the number
-            # of limiter switches a +-eps perturbation crosses grows with eps
+            # steps too
             for e in ([A["eps"]] if d >= A["fd_sweep_dirs"] else A["fd_sweep_eps"]):
-                fd = (loss(k0 + e * dk, q0 + e * dq) - loss(k0 - e * dk, q0 - e * dq)).item() / (2 * e)
+                fd, fl = central_difference(dk, dq, e)
                 r = abs(fd - ad) / abs(fd)
+                nflip = fl.get("x1", 0) + fl.get("x2", 0) + fl.get("x3", 0) + fl.get("floor", 0)
                 if e == A["eps"]:
-                    rels.append(r)
+                    dirs.append({"kind": kind, "rel_err": float(f"{r:.3e}"), "flips": nflip,
+                                 "limiter_flips_x3_x1_x2": [fl.get("x3", 0), fl.get("x1", 0),
+                                                            fl.get("x2", 0)],
+                                 "floor_flips": fl.get("floor", 0),
+                                 "upwind_flips": sum(v for kk, v in fl.items()
+                                                     if kk.startswith("upwind"))})
                 if d < A["fd_sweep_dirs"]:
-                    sweep.setdefault(f"{e:.0e}", []).append(float(f"{r:.3e}"))
+                    sweep.setdefault(f"{e:.0e}", []).append(
+                        {"rel_err": float(f"{r:.3e}"), "flips": nflip})
     run.close()
-    rels.sort()
+    if run_m is not None:
+        run_m.close()
     res = {"forward_ms_per_step": round(1e3 * t_fwd / A["steps"], 4),
            "adjoint_ms_per_step": round(1e3 * t_adj / A["steps"], 4),
            "ratio": round(t_adj / t_fwd, 3), "loss": L.item(),
            "adjoint_ms_per_step_direct_call": round(1e3 * t_adj_direct / A["steps"], 4),
            "checkpoints": run.nseg, "tape_GB": round(run.tape_bytes() / 1e9, 2),
-           "kernels_ms": kern}
-    if rels:
-        res["fd_check"] = {"directions": len(rels), "eps": A["eps"],
-                           "max_rel_err": float(f"{rels[-1]:.3e}"),
-                           "median_rel_err": float(f"{rels[len(rels) // 2]:.3e}"),
-                           "eps_sweep_rel_err": sweep}
+           "kernels_ms": kern,
+           "qtop_range_W_m2": [0.0, A["qmax"]],
+           "qtop_note": "the configured 0-1e-3 W/m^2 drives the top electron temperatures "
+                        "past 1e6 K within a few steps (tests/test_adjoint.py::"
+                        "test_configured_top_flux_runs_away), so the fit draws 0-1e-5"}
+    if dirs:
+        rels = sorted(x["rel_err"] for x in dirs)
+        zero = [x["rel_err"] for x in dirs if x["flips"] == 0]
+        res["fd_check"] = {
+            "directions": len(dirs), "eps": A["eps"], "max_rel_err": rels[-1],
+            "median_rel_err": rels[len(rels) // 2],
+            "zero_flip_directions": len(zero),
+            "max_rel_err_zero_flip": max(zero) if zero else None,
+            "per_direction": dirs, "eps_sweep": sweep}
     log(f"adjoint: forward {res['forward_ms_per_step']} ms/step, adjoint "
         f"{res['adjoint_ms_per_step']} ms/step, ratio {res['ratio']}" +
-        (f", FD max rel err {rels[-1]:.2e} over {len(rels)} directions" if rels else ""))
+        (f", FD max rel err {res['fd_check']['max_rel_err']:.2e} over {len(dirs)} directions, "
+         f"{res['fd_check']['zero_flip_directions']} without branch flips (max "
+         f"{res['fd_check']['max_rel_err_zero_flip']})" if dirs else ""))
     return res, g
 
 
@@ -585,7 +627,9 @@ def main():
     log(f"workload {args.workload}: {w['desc']}")
     t0 = time.perf_counter()
     g = make_grid(w["kind"], w["shape"])
-    st = synthetic_state_device(g, seed=args.seed, device="cuda:0")
+    # the 3-D Cartesian configuration carries Gaussian density perturbations
+    st = synthetic_state_device(g, seed=args.seed, device="cuda:0",
+                                perturb=(w["kind"] == "cart" and g.lx3 > 1))
     dt = cfl_dt_device(g, st)
     torch.cuda.synchronize()
     log(f"inputs ready in {time.perf_counter() - t0:.1f} s, dt = {dt:.4g} s")
@@ -716,7 +760,7 @@ def main():
         del tr
         torch.cuda.empty_cache()
         log("adjoint measurement (" + ADJOINT["desc"] + ")")
-        adj, _ = bench_adjoint(4 if args.adjoint_fd is None else args.adjoint_fd)
+        adj, _ = bench_adjoint(4 if args.adjoint_fd is None else args.adjoint_fd, 4)
     out = {
         "metric": METRIC,
         "value": value,

@@ -119,6 +119,94 @@ class TransportRun:
         return gk, (gq if self.has_q else None)
 
 
+    # ------------------------------------------------------------------ branch records
+    def tape_fields(self, tape: torch.Tensor) -> dict:
+        """Views of one recorded step's tape: the electron energy density entering the
+        x3, x1 and x2 sweeps (ghosted, (L3, L2, L1)), the electron density at step start
+        and at the diffusion, the temperature entering the diffusion, and the electron
+        interface velocities of the three directions."""
+        L1, L2, L3 = self.g.padded
+        lx1, lx2, lx3 = self.g.shape
+        N = self.N
+        f = lambda k: tape[k * N:(k + 1) * N].view(L3, L2, L1)
+        o1 = 6 * N
+        o2 = o1 + (lx1 + 1) * lx2 * lx3
+        o3 = o2 + lx1 * (lx2 + 1) * lx3
+        return {"NE0": f(0), "ES3": f(1), "ES1": f(2), "ES2": f(3), "NE1": f(4), "TD": f(5),
+                "V1": tape[o1:o2].view(lx3, lx2, lx1 + 1),
+                "V2": tape[o2:o3].view(lx3, lx2 + 1, lx1),
+                "V3": tape[o3:o3 + lx1 * lx2 * (lx3 + 1)].view(lx3 + 1, lx2, lx1)}
+
+    def recorded_steps(self, kappa, qtop=None):
+        """Run the forward of ``kappa``, ``qtop`` step by step with the adjoint tape
+        recorded, yielding (step, tape) after each step; the final electron temperature
+        is in ``self.last_Te`` when the generator is exhausted.  The steps are the
+        forward's kernels, so the trajectory is the forward's bit for bit."""
+        self._params(kappa, qtop)
+        self.tr.checkpoint_load(self.init)
+        slot = self.tape[:self.tsize]
+        for k in range(self.nsteps):
+            self.tr.record_step(self.dt, slot.data_ptr())
+            self.tr.synchronize()
+            yield k, slot
+            torch.cuda.synchronize(self.dev)       # the tape slot is read before reuse
+        self.tr.checkpoint_save(self._final)
+        self.generation += 1
+        self.last_Te = self._final[TE_FIELD * self.N:(TE_FIELD + 1) * self.N].clone()
+
+
+def limiter_codes(f: torch.Tensor, dx: torch.Tensor, v: torch.Tensor, axis: int):
+    """The MC limiter's branch at every slope one sweep of the ghosted field f uses.
+
+    f is (L3, L2, L1), axis 0 (x3), 1 (x2) or 2 (x1) the swept one; dx the coordinate
+    differences dx(0:lx+2) of that axis; v the interface velocities (1:lx+1 along the
+    axis, interior across it).  advec.f90:396-401 forms slope(i) = minmod(c, minmod(2l, 2r))
+    for i = 0..lx+1; the code is 3*inner + outer, with inner 0 (2l 2r <= 0), 1 (|2l| <
+    |2r|: 2l) or 2 (2r), and outer 0 (c m <= 0), 1 (|c| < |m|: c) or 2 (m).  Slopes that
+    no flux reads (the upwind side of each interface, :416-421) are coded -1."""
+    g = f[2:-2, 2:-2, :] if axis == 2 else (f[2:-2, :, 2:-2] if axis == 1 else f[:, 2:-2, 2:-2])
+    g = g.movedim(axis, -1)
+    lx = g.shape[-1] - 4
+    fm, f0, fp = g[..., 0:lx + 2], g[..., 1:lx + 3], g[..., 2:lx + 4]
+    dxl, dxr = dx[0:lx + 2], dx[1:lx + 3]
+    l2 = 2.0 * ((f0 - fm) / dxl)
+    r2 = 2.0 * ((fp - f0) / dxr)
+    c = (fp - fm) / (dxl + dxr)
+    inner = torch.where(l2 * r2 <= 0, 0, torch.where(l2.abs() < r2.abs(), 1, 2))
+    m = torch.where(inner == 0, torch.zeros_like(l2), torch.where(inner == 1, l2, r2))
+    outer = torch.where(c * m <= 0, 0, torch.where(c.abs() < m.abs(), 1, 2))
+    code = (3 * inner + outer).to(torch.int8)
+    vv = v.movedim(axis, -1)                       # interfaces k = 1..lx+1 at positions 0..lx
+    used = torch.zeros_like(code, dtype=torch.bool)
+    used[..., 1:] |= vv < 0                        # slope(i), i = 1..lx+1, read for v(i) < 0
+    used[..., :lx + 1] |= vv >= 0                  # slope(i-1) read for v(i) >= 0
+    return torch.where(used, code, torch.full_like(code, -1))
+
+
+def branch_flips(run: TransportRun, tape_a: torch.Tensor, tape_b: torch.Tensor) -> dict:
+    """Branches of one step that differ between two recorded trajectories of the same run
+    setup: MC limiter choices of the electron energy sweeps (per direction), upwind sides
+    (the velocities do not depend on the fitting parameters, so these stay 0), and cells
+    where the 100 K floor of rhoe2T holds the temperature entering the diffusion in
+    exactly one of the two."""
+    a, b = run.tape_fields(tape_a), run.tape_fields(tape_b)
+    dev = tape_a.device
+    dxs = {k: torch.as_tensor(getattr(run.g, f"dx{k}"), device=dev) for k in (1, 2, 3)}
+    out = {}
+    for name, key, vkey, axis, dk in (("x3", "ES3", "V3", 0, 3), ("x1", "ES1", "V1", 2, 1),
+                                      ("x2", "ES2", "V2", 1, 2)):
+        if run.g.shape[dk - 1] == 1:
+            out[name] = 0
+            continue
+        ca = limiter_codes(a[key], dxs[dk], a[vkey], axis)
+        cb = limiter_codes(b[key], dxs[dk], b[vkey], axis)
+        out[name] = int((ca != cb).sum().item())
+        out["upwind_" + name] = int(((a[vkey] < 0) != (b[vkey] < 0)).sum().item())
+    fa, fb = a["TD"] == 100.0, b["TD"] == 100.0
+    out["floor"] = int((fa != fb).sum().item())
+    return out
+
+
 class _ElectronTemperature(torch.autograd.Function):
     @staticmethod
     def forward(ctx, kappa, qtop, run):

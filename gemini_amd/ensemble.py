@@ -15,6 +15,31 @@ import numpy as np
 from .transport import Transport, GemhipError, load_library
 
 
+def draw_sizes(seed: int, n: int):
+    """Seeded member sizes of the batched configuration: lx1 from {64, 96, ..., 512},
+    lx2 from 16-128, lx3 from 16-96."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice(np.arange(64, 513, 32))), int(rng.integers(16, 129)),
+             int(rng.integers(16, 97))) for _ in range(n)]
+
+
+def device_member(shape, seed: int, own_context: bool = True, device: int = 0):
+    """(grid, Transport with the seeded state generated and pushed in HBM, dt, state
+    blocks) for one member: a Cartesian grid of ``shape``, the CFL time step of its
+    state.  The interface velocities start at zero, as the reference allocates them."""
+    from .grid import cartesian_grid
+    from .synth_gpu import synthetic_state_device, cfl_dt_device
+    g = cartesian_grid(*shape)
+    st = synthetic_state_device(g, seed=seed, device=f"cuda:{device}")
+    dt = cfl_dt_device(g, st)
+    tr = Transport(g, st["nn"], st["Tn"], device=device, geometry_on_device=True,
+                   own_context=own_context)
+    tr.set_residency(1)
+    tr.push_blocks(st["fluidvars"], st["aux"], st["electrovars"])
+    tr.set_interface_vels_zero()
+    return g, tr, dt, st
+
+
 class Ensemble:
     def __init__(self, members, device: int = 0, diffsolvetype: int = 2, Teinf: float = 1500.0):
         """members: sequence of (grid, state, dt).  States are pushed to HBM (residency

@@ -52,10 +52,27 @@ def _smooth_field(gen, shape, out, scale, dev, nmodes=3, noise=0.05):
     return out
 
 
-def synthetic_state_device(g: Grid, seed: int = 0, device="cuda"):
+def _gaussian_perturbations(g: Grid, gen, FV, dev, nblobs: int = 6):
+    """Six 3-D Gaussian blobs (widths 20-50 km, centres in the interior) raising the ion
+    densities by up to 20 %."""
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    x1, x2, x3 = t(g.x1)[None, None, :], t(g.x2)[None, :, None], t(g.x3)[:, None, None]
+    pert = torch.zeros(FV.shape[1:], device=dev, dtype=torch.float64)
+    for _ in range(nblobs):
+        c = [float(_uniform(gen, x[2], x[n + 1], (1,), dev))
+             for x, n in ((g.x1, g.lx1), (g.x2, g.lx2), (g.x3, g.lx3))]
+        w = float(_uniform(gen, 20e3, 50e3, (1,), dev))
+        pert += torch.exp(-((x1 - c[0]) ** 2 + (x2 - c[1]) ** 2 + (x3 - c[2]) ** 2) / (2 * w * w))
+    fac = 1.0 + 0.2 * pert.clamp_max(1.0)
+    for isp in range(6):
+        FV[isp].mul_(fac)
+
+
+def synthetic_state_device(g: Grid, seed: int = 0, device="cuda", perturb: bool = False):
     """Dict of device tensors: fluidvars (35,L3,L2,L1), aux (14,L3,L2,L1) (rhovs1,
     rhoes left zero), electrovars (7,L3,L2,L1) with J1 in [3], nn (3,lx3,lx2,lx1),
-    Tn (lx3,lx2,lx1)."""
+    Tn (lx3,lx2,lx1).  ``perturb`` adds 3-D Gaussian density perturbations (amplitude
+    20 %, widths 20-50 km), as the 3-D Cartesian configuration asks."""
     dev = torch.device(device)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
@@ -78,6 +95,8 @@ def synthetic_state_device(g: Grid, seed: int = 0, device="cuda"):
         noise.mul_(0.05).exp_().mul_((prof[isp] + BACKGROUND)[None])
         torch.clamp_min(noise, MINDENS, out=FV[isp])
         del noise
+    if perturb:
+        _gaussian_perturbations(g, gen, FV, dev)
     torch.sum(FV[0:6], dim=0, out=FV[6])
     Tn2 = 1000.0 - 800.0 * torch.exp(-(z - 80e3).clamp_min(0.0) / 40e3)
     Ti = Tn2 + (1500.0 - Tn2) * _sig(z, 300e3, 100e3)

@@ -148,3 +148,62 @@ def test_extension_rejects_unsupported_modes():
         oracle.transport_step(g, st.copy(), 0.1, qtop=np.zeros((g.lx2, g.lx3)))
     with pytest.raises(RuntimeError):
         oracle.transport_step(g, st.copy(), 0.1, diffsolvetype=1, kappa=np.ones((g.lx2, g.lx3)))
+
+
+def test_configured_top_flux_runs_away():
+    """The differentiation workload names a top heat flux of 0-1e-3 W/m^2.  With the
+    conductivity lagged at each step's input temperature (as every coefficient of the
+    reference's step is), 1e-3 W/m^2 through the top face of the 256-cell altitude grid
+    drives the top electron temperatures past 1e6 K within eight steps, while the
+    physical topside range, up to 1e-5 W/m^2, stays below 4000 K.  The fit therefore
+    draws the flux from 0-1e-5 W/m^2."""
+    g = ga.cartesian_grid(256, 2, 2)
+    peaks = {}
+    for q in (1e-3, 1e-5):
+        st = ga.synthetic_state(g, seed=1)
+        dt = min(ga.cfl_dt(g, st), 0.5)
+        for _ in range(8):
+            oracle.transport_step(g, st, dt, kappa=np.ones((2, 2)), qtop=np.full((2, 2), q))
+        peaks[q] = float(np.nanmax(st.Ts[2:-2, 2:-2, 2:-2, 6]))
+    assert peaks[1e-3] > 1e6 and peaks[1e-5] < 4000.0, peaks
+
+
+def _minmod(a, b):
+    """advec.f90:438-447, and which argument it returned (0: zero, 1: a, 2: b)."""
+    if a * b <= 0.0:
+        return 0.0, 0
+    return (a, 1) if abs(a) < abs(b) else (b, 2)
+
+
+def test_limiter_codes_follow_the_reference_minmod():
+    """The branch codes the adjoint's difference check compares (limiter_codes) are the
+    choices advec.f90:396-401 makes, slope by slope, on a field with extrema, plateaus
+    and both velocity signs; unread slopes are coded -1."""
+    torch = pytest.importorskip("torch")
+    from gemini_amd.adjoint import limiter_codes
+    rng = np.random.default_rng(4)
+    L3, L2, L1 = 7, 6, 12
+    f = np.round(rng.uniform(0.0, 4.0, (L3, L2, L1)), 1)       # ties and flat runs
+    for axis in (0, 1, 2):
+        n = (L3, L2, L1)[axis]
+        lx = n - 4
+        dx = rng.uniform(0.5, 2.0, lx + 3)
+        vshape = [L3 - 4, L2 - 4, L1 - 4]
+        vshape[axis] = lx + 1
+        v = rng.uniform(-1.0, 1.0, vshape)
+        codes = limiter_codes(torch.tensor(f), torch.tensor(dx), torch.tensor(v), axis).numpy()
+        g = np.moveaxis(f[tuple(slice(None) if a == axis else slice(2, -2) for a in range(3))], axis, -1)
+        vv = np.moveaxis(v, axis, -1)
+        for idx in np.ndindex(g.shape[:-1]):
+            line, vl = g[idx], vv[idx]
+            for i in range(lx + 2):                               # slope(i), i = 0..lx+1
+                p = i + 1
+                l = (line[p] - line[p - 1]) / dx[i]
+                r = (line[p + 1] - line[p]) / dx[i + 1]
+                c = (line[p + 1] - line[p - 1]) / (dx[i] + dx[i + 1])
+                m, inner = _minmod(2 * l, 2 * r)
+                s, outer = _minmod(c, m)
+                assert s == oracle.minmod(c, oracle.minmod(2 * l, 2 * r))
+                used = (i >= 1 and vl[i - 1] < 0) or (i <= lx and vl[i] >= 0)
+                want = 3 * inner + outer if used else -1
+                assert codes[idx + (i,)] == want, (axis, idx, i)

@@ -16,73 +16,9 @@ from gemini_amd.synth_gpu import synthetic_state_device, cfl_dt_device
 from gemini_amd.state import State
 from gemini_amd.transport import Transport
 import oracle
+from cutgrid import SubGrid, sub_state, sub_atmosphere, host_state, line_view, sample_columns
 
 SHAPE = (512, 256, 192)
-
-
-class SubGrid:
-    """The cells of ``g`` around one line (axis given by the None entry of ``c``)."""
-
-    def __init__(self, g, c):
-        self.c = c
-        self.lx1, self.lx2, self.lx3 = (n if i is None else 1 for n, i in zip(g.shape, c))
-        self.gridflag, self.flagper, self.isglobal = g.gridflag, False, (True,) * 6
-        for n in ("h1", "h2", "h3", "alt"):
-            setattr(self, n, np.asfortranarray(np.asarray(getattr(g, n))[self.padded_slices()]))
-        self.g1 = np.asfortranarray(np.asarray(g.g1)[self.interior_slices()])
-        for k, axis_arrays in enumerate((("h1x1i", "h2x1i", "h3x1i"), ("h1x2i", "h2x2i", "h3x2i"),
-                                         ("h1x3i", "h2x3i", "h3x3i"))):
-            for n in axis_arrays:
-                setattr(self, n, np.asfortranarray(np.asarray(getattr(g, n))[self.iface_slices(k)]))
-        for k, i in enumerate(c):
-            dx, dxi = getattr(g, f"dx{k + 1}"), getattr(g, f"dx{k + 1}i")
-            if i is not None:
-                dx, dxi = dx[i - 1:i + 3], dxi[i - 1:i]
-            setattr(self, f"dx{k + 1}", np.ascontiguousarray(dx))
-            setattr(self, f"dx{k + 1}i", np.ascontiguousarray(dxi))
-
-    @property
-    def padded(self):
-        return (self.lx1 + 4, self.lx2 + 4, self.lx3 + 4)
-
-    def padded_slices(self):
-        return tuple(slice(None) if i is None else slice(i - 1, i + 4) for i in self.c)
-
-    def interior_slices(self):
-        return tuple(slice(None) if i is None else slice(i - 1, i) for i in self.c)
-
-    def iface_slices(self, axis):
-        return tuple(slice(None) if i is None else (slice(i - 1, i + 1) if k == axis else slice(i - 1, i))
-                     for k, i in enumerate(self.c))
-
-
-def sub_state(sg, st):
-    P, I = sg.padded_slices(), sg.interior_slices()
-    f = lambda a, s: np.asfortranarray(a[s])
-    return State(f(st.fluidvars, P), f(st.fluidauxvars, P), f(st.electrovars, P),
-                 f(st.vs1i, sg.iface_slices(0)), f(st.vs2i, sg.iface_slices(1)),
-                 f(st.vs3i, sg.iface_slices(2)), f(st.nn, I), f(st.Tn, I))
-
-
-def sub_atmosphere(sg, atm):
-    I = sg.interior_slices()
-    f = lambda a: np.asfortranarray(a[I])
-    return ga.Atmosphere(f(atm.nn), f(atm.Tn), f(atm.vn1), f(atm.vn2), f(atm.vn3), f(atm.Prionize),
-                         f(atm.Qeionize))
-
-
-def host_state(g, dev):
-    """The device-generated state as a host State in the reference's Fortran layout."""
-    L = g.padded
-    fv = np.asfortranarray(dev["fluidvars"].cpu().numpy().transpose(3, 2, 1, 0))
-    aux = np.zeros(L + (23,), order="F")
-    aux[..., :14] = dev["aux"].cpu().numpy().transpose(3, 2, 1, 0)
-    ev = np.asfortranarray(dev["electrovars"].cpu().numpy().transpose(3, 2, 1, 0))
-    z = lambda *s: np.zeros(s, order="F")
-    return State(fv, aux, ev, z(g.lx1 + 1, g.lx2, g.lx3, 7), z(g.lx1, g.lx2 + 1, g.lx3, 7),
-                 z(g.lx1, g.lx2, g.lx3 + 1, 7),
-                 np.asfortranarray(dev["nn"].cpu().numpy().transpose(3, 2, 1, 0)),
-                 np.asfortranarray(dev["Tn"].cpu().numpy().transpose(2, 1, 0)))
 
 
 def sample_lines(g, rng, direction, n):
@@ -93,11 +29,6 @@ def sample_lines(g, rng, direction, n):
         c[direction - 1] = None
         out.append(tuple(c))
     return out
-
-
-def line_view(sg, a, sub):
-    """The line's cells (interior along it) in a full (sub=False) or cut (sub=True) block."""
-    return a[tuple(slice(2, -2) if i is None else (2 if sub else i + 1) for i in sg.c)]
 
 
 def check_sweep(tr, g, st, direction, lines, dt):
@@ -168,6 +99,32 @@ def test_cut_lines_reproduce_the_whole_grid():
             for blk in ("fluidvars", "fluidauxvars"):
                 a = line_view(sg, getattr(st, blk), False)[..., :35]
                 assert np.array_equal(a, line_view(sg, getattr(s, blk), True)[..., :35]), (which, sg.c)
+
+
+@pytest.mark.parametrize("kind", ["cart", "dipole"])
+def test_cut_columns_reproduce_the_composite_step(kind):
+    """The premise of the sampled check of whole transport steps, on the CPU: the oracle's
+    composite step on cuts of half-width 2 around x1 columns (the state's ghost cells
+    around the cut taken as given, as from a neighbour) gives bit for bit, on the centre
+    column, what the whole-grid step gives, for every field; the cut's x1 ghost fill
+    reads the previous step's interface velocities like the whole grid's."""
+    g = ga.cartesian_grid(40, 14, 12) if kind == "cart" else ga.dipole_grid(40, 14, 12)
+    st = ga.synthetic_state(g, seed=3, perturb=kind == "cart")
+    dt = ga.cfl_dt(g, st)
+    oracle.transport_step(g, st, dt)                 # interface velocities of a real step
+    rng = np.random.default_rng(2)
+    cols = sample_columns(g, rng, 6, 2) + [(None, 1, 1), (None, g.lx2, g.lx3)]
+    subs = []
+    for c in cols:
+        sg = SubGrid(g, c, 2)
+        subs.append((sg, sub_state(sg, st)))
+    oracle.transport_step(g, st, dt)
+    for sg, s in subs:
+        oracle.transport_step(sg, s, dt)
+        for blk in ("fluidvars", "fluidauxvars"):
+            a = line_view(sg, getattr(st, blk), False)[..., :35]
+            b = line_view(sg, getattr(s, blk), True)[..., :35]
+            assert np.array_equal(a, b), (kind, sg.c, blk)
 
 
 @pytest.mark.gpu
