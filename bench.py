@@ -412,7 +412,8 @@ def bench_adjoint(fd_dirs, column_dirs=None):
 
     The central-difference check runs the +eps and -eps forwards in lockstep with the
     adjoint tape recorded, and counts the branches of the electron energy path that differ
-    between them (MC limiter choices per sweep, upwind sides, the 100 K floor): the loss is
+    between them (MC limiter choices per sweep, upwind sides, the 100 K floor, the band
+    solves' row interchanges): the loss is
     only piecewise smooth, and the adjoint is the derivative of the map with its branches
     frozen.  Besides the configured random directions it checks directions confined to one
     (x2, x3) column."""
@@ -443,6 +444,13 @@ def bench_adjoint(fd_dirs, column_dirs=None):
 
     def loss_of(Te):
         return ((interior(run, Te)[..., alts] - target) ** 2).sum()
+
+    def loss_change(Tp, Tm):
+        """loss(Tp) - loss(Tm) as sum (a - b)(a + b - 2 target): the difference is formed
+        element by element, not as the difference of two large sums (whose rounding would
+        swamp the change a direction confined to one column makes)."""
+        a, b = interior(run, Tp)[..., alts], interior(run, Tm)[..., alts]
+        return ((a - b) * ((a + b) - 2.0 * target)).sum()
 
     def loss(k, q):
         return loss_of(electron_temperature(run, k, q))
@@ -491,7 +499,7 @@ def bench_adjoint(fd_dirs, column_dirs=None):
         for it in (gp, gm):
             for _ in it:
                 pass
-        fd = (loss_of(run.last_Te) - loss_of(run_m.last_Te)).item() / (2 * e)
+        fd = loss_change(run.last_Te, run_m.last_Te).item() / (2 * e)
         return fd, tot
 
     dirs, sweep = [], {}
@@ -513,12 +521,14 @@ def bench_adjoint(fd_dirs, column_dirs=None):
             for e in ([A["eps"]] if d >= A["fd_sweep_dirs"] else A["fd_sweep_eps"]):
                 fd, fl = central_difference(dk, dq, e)
                 r = abs(fd - ad) / abs(fd)
-                nflip = fl.get("x1", 0) + fl.get("x2", 0) + fl.get("x3", 0) + fl.get("floor", 0)
+                nflip = (fl.get("x1", 0) + fl.get("x2", 0) + fl.get("x3", 0) + fl.get("floor", 0)
+                         + fl.get("pivot", 0))
                 if e == A["eps"]:
                     dirs.append({"kind": kind, "rel_err": float(f"{r:.3e}"), "flips": nflip,
                                  "limiter_flips_x3_x1_x2": [fl.get("x3", 0), fl.get("x1", 0),
                                                             fl.get("x2", 0)],
                                  "floor_flips": fl.get("floor", 0),
+                                 "pivot_flips": fl.get("pivot", 0),
                                  "upwind_flips": sum(v for kk, v in fl.items()
                                                      if kk.startswith("upwind"))})
                 if d < A["fd_sweep_dirs"]:

@@ -132,10 +132,14 @@ class TransportRun:
         o1 = 6 * N
         o2 = o1 + (lx1 + 1) * lx2 * lx3
         o3 = o2 + lx1 * (lx2 + 1) * lx3
+        o4 = o3 + lx1 * lx2 * (lx3 + 1)
+        npiv = lx2 * lx3 * 2 * ((lx1 + 31) // 32)
         return {"NE0": f(0), "ES3": f(1), "ES1": f(2), "ES2": f(3), "NE1": f(4), "TD": f(5),
                 "V1": tape[o1:o2].view(lx3, lx2, lx1 + 1),
                 "V2": tape[o2:o3].view(lx3, lx2 + 1, lx1),
-                "V3": tape[o3:o3 + lx1 * lx2 * (lx3 + 1)].view(lx3 + 1, lx2, lx1)}
+                "V3": tape[o3:o4].view(lx3 + 1, lx2, lx1),
+                # row interchanges of the electron columns' TR and BDF2 factorisations
+                "PIV": tape[o4:o4 + (npiv + 1) // 2].view(torch.int32)[:npiv]}
 
     def recorded_steps(self, kappa, qtop=None):
         """Run the forward of ``kappa``, ``qtop`` step by step with the adjoint tape
@@ -186,9 +190,9 @@ def limiter_codes(f: torch.Tensor, dx: torch.Tensor, v: torch.Tensor, axis: int)
 def branch_flips(run: TransportRun, tape_a: torch.Tensor, tape_b: torch.Tensor) -> dict:
     """Branches of one step that differ between two recorded trajectories of the same run
     setup: MC limiter choices of the electron energy sweeps (per direction), upwind sides
-    (the velocities do not depend on the fitting parameters, so these stay 0), and cells
+    (the velocities do not depend on the fitting parameters, so these stay 0), cells
     where the 100 K floor of rhoe2T holds the temperature entering the diffusion in
-    exactly one of the two."""
+    exactly one of the two, and row interchanges of the electron columns' band solves."""
     a, b = run.tape_fields(tape_a), run.tape_fields(tape_b)
     dev = tape_a.device
     dxs = {k: torch.as_tensor(getattr(run.g, f"dx{k}"), device=dev) for k in (1, 2, 3)}
@@ -204,6 +208,10 @@ def branch_flips(run: TransportRun, tape_a: torch.Tensor, tape_b: torch.Tensor) 
         out["upwind_" + name] = int(((a[vkey] < 0) != (b[vkey] < 0)).sum().item())
     fa, fb = a["TD"] == 100.0, b["TD"] == 100.0
     out["floor"] = int((fa != fb).sum().item())
+    # row interchanges of the band solves (LAPACK's partial pivoting) that differ
+    x = torch.bitwise_xor(a["PIV"], b["PIV"])
+    bits = torch.stack([(x >> k) & 1 for k in range(32)])
+    out["pivot"] = int(bits.sum().item())
     return out
 
 

@@ -608,13 +608,17 @@ namespace gemhip {
 namespace {
 // tape of one recorded step (doubles): NE0, ES3, ES1, ES2, NE1, TD (ghosted electron
 // fields), then the electron interface velocities V1, V2, V3
-long tape_doubles() { return 6L * ctx().g.N + nv1i() + nv2i() + nv3i(); }
+// the recorded fields, then the electron columns' pivot record (ints, padded to doubles)
+long tape_doubles() {
+  return 6L * ctx().g.N + nv1i() + nv2i() + nv3i() + (pivot_record_ints(ctx().g) + 1) / 2;
+}
 
 AdjTape tape_view(const double* t) {
   const long N = ctx().g.N;
   AdjTape v;
   v.NE0 = t; v.ES3 = t + N; v.ES1 = t + 2 * N; v.ES2 = t + 3 * N; v.NE1 = t + 4 * N;
   v.TD = t + 5 * N; v.V1 = t + 6 * N; v.V2 = v.V1 + nv1i(); v.V3 = v.V2 + nv2i();
+  v.PIV = reinterpret_cast<int*>(const_cast<double*>(v.V3 + nv3i()));
   return v;
 }
 
@@ -633,29 +637,47 @@ int run_step(double dt, double* tape) {
   auto keep = [&](long slot, const double* src) {
     hipMemcpyAsync(tape + slot * g.N, src + eo, fb, hipMemcpyDeviceToDevice, st);
   };
-  launch_v12rhov1_T2rhoe(g, s, true, true, st);          // libgemini.cpp:155-156
-  if (tape) keep(0, s.ns);
-  launch_set_global_boundaries(g, s, LSP, st);           // :168
-  halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :169
-  launch_save_v1_boundaries(g, s, st);                   // :170 (x1 planes kept for next step)
-  if (tape) {
-    keep(1, s.rhoes);
-    const AdjTape v = tape_view(tape);
-    launch_tape_vels(g, s, const_cast<double*>(v.V1), const_cast<double*>(v.V2),
-                     const_cast<double*>(v.V3), st);
+  // libgemini.cpp:155-156, 168-176: with all sides global and no periodic wrap, the
+  // conversions fold into the ghost fill and the x3 sweep (the adjoint's tape needs the
+  // converted energy entering the x3 sweep, so a recorded step converts as a pass)
+  const bool fuse = !tape && preconvert_fusable(g);
+  if (fuse) {
+    launch_set_global_boundaries_preconvert(g, s, st);
+    launch_save_v1_boundaries(g, s, st);
+    launch_sweep3_preconvert(g, s, dt, st, c.tune);
+  } else {
+    launch_v12rhov1_T2rhoe(g, s, true, true, st);        // libgemini.cpp:155-156
+    if (tape) keep(0, s.ns);
+    launch_set_global_boundaries(g, s, LSP, st);         // :168
+    halo_fields({NS, RHOVS1, RHOES, VS2, VS3});          // :169
+    launch_save_v1_boundaries(g, s, st);                 // :170 (x1 planes kept for next step)
+    if (tape) {
+      keep(1, s.rhoes);
+      const AdjTape v = tape_view(tape);
+      launch_tape_vels(g, s, const_cast<double*>(v.V1), const_cast<double*>(v.V2),
+                       const_cast<double*>(v.V3), st);
+    }
+    launch_sweep(g, s, 3, F_ALL, dt, true, st, c.tune);  // :174-176
   }
-  launch_sweep(g, s, 3, F_ALL, dt, true, st, c.tune);   // :174-176
   if (tape) keep(2, s.rhoes);
   launch_sweep(g, s, 1, F_ALL, dt, true, st, c.tune);   // :178-180
   halo_fields({NS, RHOVS1, RHOES});                      // :181
   if (tape) keep(3, s.rhoes);
-  launch_sweep(g, s, 2, F_ALL, dt, true, st, c.tune);   // :183-185
-  launch_post_convert(g, s, st);                         // :186, then multifluid.f90:179-189
+  if (!tape && postconvert_fusable(g)) {                 // :183-186 with the conversions
+    launch_sweep2_postconvert(g, s, dt, st, c.tune);     // folded into the x2 sweep
+  } else {
+    launch_sweep(g, s, 2, F_ALL, dt, true, st, c.tune);  // :183-185
+    launch_post_convert(g, s, st);                       // :186, then multifluid.f90:179-189
+  }
   if (tape) { keep(4, s.ns); keep(5, s.Ts); }
   if (!hipok(hipGetLastError(), "advection launch")) return 3;
   c.vsi_current = false;
-  if (!run_energy_diffusion(dt)) return 4;               // :208
-  return 0;
+  // a recorded step also keeps the electron columns' row interchanges (written by the
+  // fitting extension's kernel, the one a recorded run uses)
+  if (tape) c.s.pivrec = tape_view(tape).PIV;
+  const bool ok = run_energy_diffusion(dt);              // :208
+  c.s.pivrec = nullptr;
+  return ok ? 0 : 4;
 }
 // The transport block of src/libgemini.cpp:155-210: the advection part of run_step
 // (the energy density stays as the sweeps leave it), rhov12v1, clean_param 1 and 2,
@@ -668,11 +690,17 @@ int run_block(double dt) {
   const DevGrid& g = c.g;
   const DevState& s = c.s;
   hipStream_t st = c.stream;
-  launch_v12rhov1_T2rhoe(g, s, true, true, st);          // libgemini.cpp:155-156
-  launch_set_global_boundaries(g, s, LSP, st);           // :168
-  halo_fields({NS, RHOVS1, RHOES, VS2, VS3});            // :169
-  launch_save_v1_boundaries(g, s, st);                   // :170
-  launch_sweep(g, s, 3, F_ALL, dt, true, st, c.tune);   // :174-176
+  if (preconvert_fusable(g)) {                           // as in run_step
+    launch_set_global_boundaries_preconvert(g, s, st);
+    launch_save_v1_boundaries(g, s, st);
+    launch_sweep3_preconvert(g, s, dt, st, c.tune);
+  } else {
+    launch_v12rhov1_T2rhoe(g, s, true, true, st);        // libgemini.cpp:155-156
+    launch_set_global_boundaries(g, s, LSP, st);         // :168
+    halo_fields({NS, RHOVS1, RHOES, VS2, VS3});          // :169
+    launch_save_v1_boundaries(g, s, st);                 // :170
+    launch_sweep(g, s, 3, F_ALL, dt, true, st, c.tune);  // :174-176
+  }
   launch_sweep(g, s, 1, F_ALL, dt, true, st, c.tune);   // :178-180
   halo_fields({NS, RHOVS1, RHOES});                      // :181
   launch_sweep(g, s, 2, F_ALL, dt, true, st, c.tune);   // :183-185
@@ -1452,6 +1480,189 @@ int gemhip_ensemble_release(int id) {
   hipStreamSynchronize(graphs()[id - 1]->origin);
   release_graph(graphs()[id - 1]);
   graphs()[id - 1] = nullptr;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// An ensemble advanced by batched kernels (one persistent launch per kernel of the fused
+// step for all members), with its member tables in HBM and a stream of its own.
+struct Batch {
+  std::vector<int> handles;
+  std::vector<BatchMember> host;
+  BatchMember* d_m = nullptr;
+  long* d_pre = nullptr;
+  BatchTable t{};
+  hipStream_t stream = nullptr;
+  int device = 0, cus = 0;
+  int sweep_pf = 2, diff_chunk = 8, grid = 1;   // kernel variants and persistent grid size
+};
+std::vector<Batch*>& batches() {
+  static std::vector<Batch*> v;
+  return v;
+}
+Batch* batch_of(int id) {
+  if (id <= 0 || id > (int)batches().size() || !batches()[id - 1]) {
+    fail("no batch %d", id);
+    return nullptr;
+  }
+  return batches()[id - 1];
+}
+void free_batch(Batch* b) {
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->d_m) hipFree(b->d_m);
+  if (b->d_pre) hipFree(b->d_pre);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+// persistent grid sizes from the device's CU count: 8 blocks of 256 threads, one block of
+// seven sweep waves and eight one-wave TRBDF2 blocks resident per CU, times `grid`
+void batch_grids(Batch* b) {
+  b->t.blocks_thread = 8 * b->cus * b->grid;
+  b->t.blocks_sweep = b->cus * b->grid;
+  b->t.blocks_diff = 8 * b->cus * b->grid;
+}
+void batch_launch(Batch* b, hipStream_t st) {
+  launch_batch_advection(b->t, b->sweep_pf, st);
+  launch_batch_diffusion(b->t, b->diff_chunk, st);
+}
+}  // namespace
+
+extern "C" {
+
+// Members (contexts with their state resident, residency mode 1) advanced together by
+// batched kernels.  Every member must take the fused step's path (all sides global, x3
+// not periodic, lx2 and lx3 above 1), use TRBDF2 without the fitting extension, and live
+// on one device.  Returns a batch id >= 1.
+int gemhip_batch_create(int n, const int* handles) {
+  const int keep = g_current_handle;
+  if (n <= 0 || !handles) { fail("bad batch arguments"); return -1; }
+  auto* b = new Batch();
+  b->handles.assign(handles, handles + n);
+  for (int m = 0; m < n; ++m) {
+    if (gemhip_context_select(handles[m])) { delete b; return -1; }
+    Ctx& c = ctx();
+    const char* why = nullptr;
+    if (!ready(true) || c.residency != 1) why = "is not set up with its state in HBM";
+    else if (!preconvert_fusable(c.g) || !postconvert_fusable(c.g))
+      why = "needs global sides, no periodic x3 and lx2, lx3 > 1";
+    else if (c.diffsolvetype != 2 || c.s.kappa || c.s.qtop)
+      why = "must use TRBDF2 without the fitting extension";
+    else if (m > 0 && c.device != b->device) why = "is on another device";
+    if (why) {
+      fail("batch member %d %s", m, why);
+      gemhip_context_select(keep);
+      delete b;
+      return -1;
+    }
+    if (!ensure_scratch() || !hipok(hipStreamSynchronize(c.stream), "member sync")) {
+      gemhip_context_select(keep);
+      delete b;
+      return -1;
+    }
+    if (m == 0) b->device = c.device;
+    c.vsi_current = false;
+    b->host.push_back(BatchMember{c.g, c.s, 0.0, c.Teinf, c.d_scratch});
+  }
+  gemhip_context_select(keep);
+  std::vector<long> pre((size_t)BK_COUNT * (n + 1));
+  for (int k = 0; k < BK_COUNT; ++k) {
+    long acc = 0;
+    for (int m = 0; m < n; ++m) {
+      pre[(size_t)k * (n + 1) + m] = acc;
+      acc += batch_units(b->host[m].g, k);
+    }
+    pre[(size_t)k * (n + 1) + n] = acc;
+    b->t.total[k] = acc;
+  }
+  bool ok = hipok(hipSetDevice(b->device), "hipSetDevice") &&
+            hipok(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device),
+                  "CU count") &&
+            hipok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "batch stream") &&
+            hipok(hipMalloc((void**)&b->d_m, n * sizeof(BatchMember)), "batch members") &&
+            hipok(hipMalloc((void**)&b->d_pre, pre.size() * sizeof(long)), "batch units") &&
+            hipok(hipMemcpy(b->d_m, b->host.data(), n * sizeof(BatchMember), hipMemcpyHostToDevice),
+                  "batch members") &&
+            hipok(hipMemcpy(b->d_pre, pre.data(), pre.size() * sizeof(long), hipMemcpyHostToDevice),
+                  "batch units");
+  if (!ok) { free_batch(b); return -1; }
+  b->t.m = b->d_m;
+  b->t.pre = b->d_pre;
+  b->t.n = n;
+  batch_grids(b);
+  batches().push_back(b);
+  return (int)batches().size();
+}
+
+// The members' time steps (one per member), read by the batched kernels from HBM: a
+// captured batch graph picks up new values without being captured again.
+int gemhip_batch_set_dt(int id, const double* dt) {
+  Batch* b = batch_of(id);
+  if (!b) return 1;
+  if (!dt) { fail("null dt"); return 2; }
+  for (size_t m = 0; m < b->host.size(); ++m) b->host[m].dt = dt[m];
+  if (!hipok(hipStreamSynchronize(b->stream), "batch sync")) return 3;
+  return hipok(hipMemcpy(b->d_m, b->host.data(), b->host.size() * sizeof(BatchMember),
+                         hipMemcpyHostToDevice), "batch dt") ? 0 : 3;
+}
+
+// Kernel variants and the persistent grid: "sweep_pf" (1, 2), "diff_chunk" (4, 8) and
+// "grid" (1, 2: resident blocks per CU times this).  Results do not depend on them.
+int gemhip_batch_set_tuning(int id, const char* knob, int value) {
+  Batch* b = batch_of(id);
+  if (!b) return 1;
+  const std::string k = knob ? knob : "";
+  if (k == "sweep_pf" && (value == 1 || value == 2)) b->sweep_pf = value;
+  else if (k == "diff_chunk" && (value == 4 || value == 8)) b->diff_chunk = value;
+  else if (k == "grid" && (value == 1 || value == 2)) { b->grid = value; batch_grids(b); }
+  else { fail("unknown batch tuning knob or value: %s = %d", k.c_str(), value); return 2; }
+  return 0;
+}
+
+// One transport step of every member, enqueued on the batch's stream.
+int gemhip_batch_step(int id) {
+  Batch* b = batch_of(id);
+  if (!b) return 1;
+  batch_launch(b, b->stream);
+  return hipok(hipGetLastError(), "batch launch") ? 0 : 2;
+}
+
+int gemhip_batch_synchronize(int id) {
+  Batch* b = batch_of(id);
+  if (!b) return 1;
+  return hipok(hipStreamSynchronize(b->stream), "batch sync") ? 0 : 2;
+}
+
+// nsteps batched steps captured into one HIP graph (launch it with gemhip_ensemble_launch;
+// it holds the members like an ensemble graph does).  Returns a graph id >= 1.
+int gemhip_batch_capture(int id, int nsteps) {
+  Batch* b = batch_of(id);
+  if (!b) return -1;
+  if (nsteps <= 0) { fail("bad step count"); return -1; }
+  if (!hipok(hipStreamSynchronize(b->stream), "batch sync")) return -1;
+  auto* e = new EnsembleGraph();
+  e->members = b->handles;
+  bool ok = hipok(hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking), "origin stream") &&
+            hipok(hipStreamBeginCapture(e->origin, hipStreamCaptureModeGlobal), "begin capture");
+  for (int k = 0; ok && k < nsteps; ++k) {
+    batch_launch(b, e->origin);
+    ok = hipok(hipGetLastError(), "batch launch");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t endc = hipStreamEndCapture(e->origin, &graph);
+  e->graph = graph;
+  ok = ok && hipok(endc, "end capture");
+  ok = ok && hipok(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0), "instantiate");
+  if (!ok) { release_graph(e); return -1; }
+  graphs().push_back(e);
+  return (int)graphs().size();
+}
+
+int gemhip_batch_release(int id) {
+  if (id <= 0 || id > (int)batches().size() || !batches()[id - 1]) return 1;
+  free_batch(batches()[id - 1]);
+  batches()[id - 1] = nullptr;
   return 0;
 }
 

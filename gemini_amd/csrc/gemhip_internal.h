@@ -35,6 +35,10 @@ struct DevState {
   // parameter-fitting extension, null unless registered: electron conductivity scale and
   // downward top heat flux [W/m^2] per (x2,x3) column, lx2*lx3 each
   const double *kappa, *qtop;
+  // null unless an adjoint tape is being recorded: the row interchanges of the electron
+  // columns' two band factorisations (TR, BDF2), one bit per elimination step, 32 steps a
+  // word, [column][stage][word] (fitting extension kernels only)
+  int* pivrec;
 };
 
 __host__ __device__ inline long gix(const DevGrid& g, int i1, int i2, int i3) {
@@ -78,6 +82,16 @@ void launch_rhov12v1(const DevGrid& g, const DevState& s, hipStream_t st);
 void launch_rhoe2T(const DevGrid& g, const DevState& s, hipStream_t st);
 void launch_post_convert(const DevGrid& g, const DevState& s, hipStream_t st);  // both above
 void launch_set_global_boundaries(const DevGrid& g, const DevState& s, int lsp, hipStream_t st);
+// the fused step's form of v12rhov1 + T2rhoe + ghost fill + x3 sweep: the conversions are
+// folded into the ghost fill and the x3 sweep (all sides global, x3 not periodic)
+bool preconvert_fusable(const DevGrid& g);
+void launch_set_global_boundaries_preconvert(const DevGrid& g, const DevState& s, hipStream_t st);
+void launch_sweep3_preconvert(const DevGrid& g, const DevState& s, double dt, hipStream_t st,
+                              const Tuning& tune);
+// the fused step's x2 sweep with rhov12v1 + rhoe2T folded in (x2 not a single cell)
+bool postconvert_fusable(const DevGrid& g);
+void launch_sweep2_postconvert(const DevGrid& g, const DevState& s, double dt, hipStream_t st,
+                               const Tuning& tune);
 void launch_halo_periodic(const DevGrid& g, double* const* fields, int nfields, int lhalo,
                           int nspec, hipStream_t st);
 void launch_interface_vels(const DevGrid& g, const DevState& s, int lsp, hipStream_t st);
@@ -89,6 +103,58 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
 
 // ---- launchers (kernels_diffusion.hip) ----
 size_t diffusion_scratch_bytes(const DevGrid& g);
+
+// ---- batched ensemble steps (kernels_advec.hip, kernels_diffusion.hip) ----
+// One member of an ensemble advanced by batched kernels: every kernel of the fused
+// transport step runs once for all members, as a persistent grid whose blocks take the
+// members' work units in turn (the same per-cell arithmetic as the member's own step).
+struct BatchMember {
+  DevGrid g;
+  DevState s;
+  double dt;
+  double Teinf;
+  double* scratch;          // the member's TRBDF2 work space
+};
+// the batched kernels of a step, in order; units are threads or blocks as noted
+enum BatchKernel : int {
+  BK_BC1,    // x1 ghost fill: threads, L2 L3 lsp
+  BK_BC2,    // x2 ghost fill: threads, L1 L3 lsp 4
+  BK_BC3,    // x3 ghost fill: threads, L1 L2 lsp 4
+  BK_V1B,    // x1 boundary interface velocities: threads, lx2 lx3 lsp
+  BK_SW3,    // x3 sweep with the conversions: blocks (64 x 7), ceil(lx1/64) lx2
+  BK_SW1,    // x1 sweep of the ions: blocks (64 x 6), lx2 lx3
+  BK_SW1E,   // x1 sweep of the electron energy: blocks (64 x 1), lx2 lx3
+  BK_SW2,    // x2 sweep with rhov12v1 + rhoe2T: blocks (64 x 7), ceil(lx1/64) lx3
+  BK_POST,   // rhov12v1 + rhoe2T outside the interior box: threads, surface cells
+  BK_DIFF,   // TRBDF2: blocks (64), 8 ceil(ceil(lx2 lx3/64)/8) lsp (a multiple of 8)
+  BK_FLOOR,  // Ts ghost floor: threads, surface cells lsp
+  BK_COUNT
+};
+long batch_units(const DevGrid& g, int kernel);
+// the member whose units of a kernel include unit t: the last m with pre[m] <= t
+__device__ __forceinline__ int batch_member(const long* pre, int n, long t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+// members (HBM, n entries) and the prefix sums of each kernel's units (HBM,
+// [BK_COUNT][n + 1]); totals on the host; grids: blocks of the persistent launches
+struct BatchTable {
+  const BatchMember* m;
+  const long* pre;
+  int n;
+  long total[BK_COUNT];
+  int blocks_thread;        // persistent grid of the thread kernels (256-thread blocks)
+  int blocks_sweep;         // of the sweep kernels (one block per CU resident)
+  int blocks_diff;          // of the TRBDF2 kernel
+};
+void launch_batch_advection(const BatchTable& t, int sweep_pf, hipStream_t st);
+void launch_batch_diffusion(const BatchTable& t, int diff_chunk, hipStream_t st);
+void launch_batch_floor(const BatchTable& t, hipStream_t st);
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
                              double Teinf, double* scratch, hipStream_t st,
                              const Tuning& tune = Tuning{});
@@ -122,7 +188,10 @@ void launch_gridrates(const DevGrid& g, const DevState& s, double* work, hipStre
 struct AdjTape {
   const double *NE0, *ES3, *ES1, *ES2, *NE1, *TD;   // ghosted fields, N each
   const double *V1, *V2, *V3;                       // (lx1+1,lx2,lx3), (lx1,lx2+1,lx3), (lx1,lx2,lx3+1)
+  int* PIV;                                         // DevState::pivrec of the recorded step
 };
+// ints of DevState::pivrec for a grid
+long pivot_record_ints(const DevGrid& g);
 size_t adjoint_scratch_bytes(const DevGrid& g);
 void launch_tape_vels(const DevGrid& g, const DevState& s, double* v1, double* v2, double* v3,
                       hipStream_t st);

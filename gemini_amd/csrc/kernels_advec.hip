@@ -12,6 +12,8 @@
 //  * x2 and x3 sweeps map threads onto the contiguous x1 index and march each
 //    thread along its line, keeping the MC stencil in registers -- every load and
 //    store is a coalesced 64-lane access and no LDS or barrier is needed.
+#include <algorithm>
+
 #include "gemhip_internal.h"
 
 namespace gemhip {
@@ -73,11 +75,10 @@ __global__ void k_rhoe2T(DevGrid g, DevState s) {
   }
 }
 
-// rhov12v1 then rhoe2T of every padded cell in one pass: both are pointwise, so doing
-// them cell by cell gives what the two whole-array passes give, with ns read once.
-__global__ void k_post_convert(DevGrid g, DevState s) {
-  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < g.N;
-       k += (long)gridDim.x * blockDim.x) {
+// rhov12v1 then rhoe2T of one padded cell: both are pointwise, so doing them cell by cell
+// gives what the two whole-array passes give, with ns read once.
+__device__ __forceinline__ void post_convert_cell(const DevGrid& g, const DevState& s, long k) {
+  {
     double chrgflux = 0.0, sum = 0.0;
     double n[LSP];
 #pragma unroll
@@ -103,12 +104,64 @@ __global__ void k_post_convert(DevGrid g, DevState s) {
   }
 }
 
+__global__ void k_post_convert(DevGrid g, DevState s) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < g.N;
+       k += (long)gridDim.x * blockDim.x)
+    post_convert_cell(g, s, k);
+}
+
+// The padded cells outside the interior box, one thread each: first the four x1 ghosts of
+// every interior (x2, x3) column, then every cell of the columns with an x2 or x3 ghost
+// index.  Returns -1 past the end.
+__device__ __forceinline__ int ghost_index(int j, int lx) { return j < 2 ? j - 1 : lx + j - 1; }
+__device__ __forceinline__ long surface_cell(const DevGrid& g, long t) {
+  const long n1 = 4L * g.lx2 * g.lx3;
+  const long ncol = 4L * g.L2 + 4L * g.lx3;
+  if (t < n1) {
+    const long col = t / 4;
+    return gix(g, ghost_index((int)(t % 4), g.lx1), (int)(col % g.lx2) + 1, (int)(col / g.lx2) + 1);
+  }
+  if (t >= n1 + ncol * g.L1) return -1;
+  const long r = t - n1;
+  const int i1 = (int)(r % g.L1) - 1;
+  const long col = r / g.L1;
+  int i2, i3;
+  if (col < 4L * g.L2) {
+    i2 = (int)(col % g.L2) - 1;
+    i3 = ghost_index((int)(col / g.L2), g.lx3);
+  } else {
+    const long cc = col - 4L * g.L2;
+    i3 = (int)(cc % g.lx3) + 1;
+    i2 = ghost_index((int)(cc / g.lx3), g.lx2);
+  }
+  return gix(g, i1, i2, i3);
+}
+__host__ __device__ inline long surface_cells(const DevGrid& g) {
+  return 4L * g.lx2 * g.lx3 + (4L * g.L2 + 4L * g.lx3) * g.L1;
+}
+
+// the fused step's rhov12v1 + rhoe2T of the cells the x2 sweep does not convert
+__global__ void k_post_convert_surface(DevGrid g, DevState s) {
+  const long c = surface_cell(g, (long)blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= 0) post_convert_cell(g, s, c);
+}
+
 // ---------------------------------------------------------------- global ghost fill
-// advec.f90:42-79: x1 ends, one thread per padded (x2,x3) column and species
-__global__ void k_bc_x1(DevGrid g, DevState s) {
-  const int isp = blockIdx.y + 1;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.L2 * g.L3) return;
+// With CONV the fill also stands in for v12rhov1 and T2rhoe (multifluid.f90:193-214),
+// which the fused step no longer runs as a pass of their own: every rhovs1 or rhoes value
+// the fill reads that the conversions would have written, and that no earlier part of the
+// fill has overwritten, is formed on the fly as (n ms) v or ((n kB) T)/(gamma-1), the
+// conversions' own expressions, so every ghost value is the one the separate passes
+// give.  The interior cells are converted by the x3 sweep when it loads them.  This holds
+// with all six sides global and no periodic x3 wrap (the fused step checks).
+__device__ __forceinline__ double pre_momentum(double n, double v, int isp) { return (n * ms(isp)) * v; }
+__device__ __forceinline__ double pre_energy(double n, double T, int isp) {
+  return ((n * kB) * T) / (gammas(isp) - 1.0);
+}
+
+// advec.f90:42-79: x1 ends, one thread per padded (x2,x3) column t and species
+template <bool CONV>
+__device__ __forceinline__ void bc_x1_body(const DevGrid& g, const DevState& s, int isp, long t) {
   const int i2 = (int)(t % g.L2) - 1, i3 = (int)(t / g.L2) - 1;
   const bool inner = i2 >= 1 && i2 <= g.lx2 && i3 >= 1 && i3 <= g.lx3;
   const long o = (long)(isp - 1) * g.N + gix(g, 0, i2, i3) - 1;  // index of x1 = -1
@@ -116,77 +169,113 @@ __global__ void k_bc_x1(DevGrid g, DevState s) {
   double* r = s.rhovs1 + o + 1;
   double* e = s.rhoes + o + 1;
   const double* u = s.vs1 + o + 1;
+  const double* T = s.Ts + o + 1;
   const double msp = ms(isp);
   const int lx1 = g.lx1;
   const long vb = 2L * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) + 2L * g.lx2 * g.lx3 * (isp - 1);
   if (g.isglobal[0]) {
+    double r0;
     if (inner) {
       const double coeff = n[2] / n[3];
       n[0] = fmin_f(coeff * n[1], n[1]);
       n[-1] = fmin_f(coeff * n[0], n[0]);
-      r[0] = 2.0 * s.v1b[vb] - u[1];
+      r0 = 2.0 * s.v1b[vb] - u[1];
+    } else {
+      r0 = CONV ? pre_momentum(n[0], u[0], isp) : r[0];
     }
-    r[-1] = (r[0] + r[0]) - u[1];
-    r[-1] = (r[-1] * n[-1]) * msp;
-    r[0] = (r[0] * n[0]) * msp;
-    const double Tb = e[1] / n[1];
+    const double rm = (r0 + r0) - u[1];
+    r[-1] = (rm * n[-1]) * msp;
+    r[0] = (r0 * n[0]) * msp;
+    const double e1 = CONV ? pre_energy(n[1], T[1], isp) : e[1];
+    const double Tb = e1 / n[1];
     e[0] = Tb * n[0];
     e[-1] = Tb * n[-1];
   }
   if (g.isglobal[1]) {
+    double r1;
     if (inner) {
       const double coeff = n[lx1 - 1] / n[lx1 - 2];
       n[lx1 + 1] = fmin_f(coeff * n[lx1], n[lx1]);
       n[lx1 + 2] = fmin_f(coeff * n[lx1 + 1], n[lx1 + 1]);
-      r[lx1 + 1] = 2.0 * s.v1b[vb + 1] - u[lx1];
+      r1 = 2.0 * s.v1b[vb + 1] - u[lx1];
+    } else {
+      r1 = CONV ? pre_momentum(n[lx1 + 1], u[lx1 + 1], isp) : r[lx1 + 1];
     }
-    r[lx1 + 2] = (r[lx1 + 1] + r[lx1 + 1]) - u[lx1];
-    r[lx1 + 1] = (r[lx1 + 1] * n[lx1 + 1]) * msp;
-    r[lx1 + 2] = (r[lx1 + 2] * n[lx1 + 2]) * msp;
-    const double Tb = e[lx1] / n[lx1];
+    const double r2 = (r1 + r1) - u[lx1];
+    r[lx1 + 1] = (r1 * n[lx1 + 1]) * msp;
+    r[lx1 + 2] = (r2 * n[lx1 + 2]) * msp;
+    const double eN = CONV ? pre_energy(n[lx1], T[lx1], isp) : e[lx1];
+    const double Tb = eN / n[lx1];
     e[lx1 + 1] = Tb * n[lx1 + 1];
     e[lx1 + 2] = Tb * n[lx1 + 2];
   }
 }
-
-// advec.f90:83-104: x2 ends, every x1 and x3 position, fields vs2, ns, rhovs1, rhoes
-__global__ void k_bc_x2(DevGrid g, DevState s) {
-  const int isp = blockIdx.y + 1, q = blockIdx.z;
+template <bool CONV>
+__global__ void k_bc_x1(DevGrid g, DevState s) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.L1 * g.L3) return;
+  if (t < (long)g.L2 * g.L3) bc_x1_body<CONV>(g, s, blockIdx.y + 1, t);
+}
+
+// the value a copying fill reads from cell c: with CONV, momentum (q 2) and energy (q 3) of
+// a cell the x1 fill has not written are formed from ns, vs1, Ts
+template <bool CONV>
+__device__ __forceinline__ double fill_source(const DevState& s, int q, const double* f, long so,
+                                              long c, bool filled, int isp) {
+  if (!CONV || q < 2 || filled) return f[c];
+  const double n = s.ns[so + c];
+  return q == 2 ? pre_momentum(n, s.vs1[so + c], isp) : pre_energy(n, s.Ts[so + c], isp);
+}
+
+// advec.f90:83-104: x2 ends, every x1 and x3 position t, fields q = vs2, ns, rhovs1, rhoes
+template <bool CONV>
+__device__ __forceinline__ void bc_x2_body(const DevGrid& g, const DevState& s, int isp, int q,
+                                           long t) {
   const int i1 = (int)(t % g.L1) - 1, i3 = (int)(t / g.L1) - 1;
   double* base = q == 0 ? s.vs2 : q == 1 ? s.ns : q == 2 ? s.rhovs1 : s.rhoes;
-  double* f = base + (long)(isp - 1) * g.N;
+  const long so = (long)(isp - 1) * g.N;
+  double* f = base + so;
+  const bool x1ghost = i1 < 1 || i1 > g.lx1;          // written by the x1 fill
   if (g.isglobal[2]) {
-    const double v = f[gix(g, i1, 1, i3)];
+    const double v = fill_source<CONV>(s, q, f, so, gix(g, i1, 1, i3), x1ghost, isp);
     f[gix(g, i1, 0, i3)] = v;
     f[gix(g, i1, -1, i3)] = v;
   }
   if (g.isglobal[3]) {
-    const double v = f[gix(g, i1, g.lx2, i3)];
+    const double v = fill_source<CONV>(s, q, f, so, gix(g, i1, g.lx2, i3), x1ghost, isp);
     f[gix(g, i1, g.lx2 + 1, i3)] = v;
     f[gix(g, i1, g.lx2 + 2, i3)] = v;
   }
 }
-
-// advec.f90:107-132: x3 ends unless periodic, every x1 and x2 position
-__global__ void k_bc_x3(DevGrid g, DevState s) {
-  const int isp = blockIdx.y + 1, q = blockIdx.z;
+template <bool CONV>
+__global__ void k_bc_x2(DevGrid g, DevState s) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.L1 * g.L2) return;
+  if (t < (long)g.L1 * g.L3) bc_x2_body<CONV>(g, s, blockIdx.y + 1, blockIdx.z, t);
+}
+
+// advec.f90:107-132: x3 ends unless periodic, every x1 and x2 position t
+template <bool CONV>
+__device__ __forceinline__ void bc_x3_body(const DevGrid& g, const DevState& s, int isp, int q,
+                                           long t) {
   const int i1 = (int)(t % g.L1) - 1, i2 = (int)(t / g.L1) - 1;
   double* base = q == 0 ? s.vs3 : q == 1 ? s.ns : q == 2 ? s.rhovs1 : s.rhoes;
-  double* f = base + (long)(isp - 1) * g.N;
+  const long so = (long)(isp - 1) * g.N;
+  double* f = base + so;
+  const bool filled = i1 < 1 || i1 > g.lx1 || i2 < 1 || i2 > g.lx2;   // x1 or x2 fill wrote it
   if (g.isglobal[4]) {
-    const double v = f[gix(g, i1, i2, 1)];
+    const double v = fill_source<CONV>(s, q, f, so, gix(g, i1, i2, 1), filled, isp);
     f[gix(g, i1, i2, 0)] = v;
     f[gix(g, i1, i2, -1)] = v;
   }
   if (g.isglobal[5]) {
-    const double v = f[gix(g, i1, i2, g.lx3)];
+    const double v = fill_source<CONV>(s, q, f, so, gix(g, i1, i2, g.lx3), filled, isp);
     f[gix(g, i1, i2, g.lx3 + 1)] = v;
     f[gix(g, i1, i2, g.lx3 + 2)] = v;
   }
+}
+template <bool CONV>
+__global__ void k_bc_x3(DevGrid g, DevState s) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (long)g.L1 * g.L2) bc_x3_body<CONV>(g, s, blockIdx.y + 1, blockIdx.z, t);
 }
 
 // ---------------------------------------------------------------- periodic x3 wrap
@@ -250,15 +339,16 @@ __global__ void k_vs3i(DevGrid g, DevState s) {
 }
 
 // only the x1-boundary interface velocities, which the next step's ghost fill reads
-__global__ void k_save_v1b(DevGrid g, DevState s) {
-  const int isp = blockIdx.y + 1;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.lx2 * g.lx3) return;
+__device__ __forceinline__ void save_v1b_body(const DevGrid& g, const DevState& s, int isp, long t) {
   const int i2 = (int)(t % g.lx2) + 1, i3 = (int)(t / g.lx2) + 1;
   const double* u = s.vs1 + (long)(isp - 1) * g.N + gix(g, 0, i2, i3);
   const long vb = 2L * (t + (long)g.lx2 * g.lx3 * (isp - 1));
   s.v1b[vb] = v1i_at(g, u, 1);
   s.v1b[vb + 1] = v1i_at(g, u, g.lx1 + 1);
+}
+__global__ void k_save_v1b(DevGrid g, DevState s) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (long)g.lx2 * g.lx3) save_v1b_body(g, s, blockIdx.y + 1, t);
 }
 
 // ---------------------------------------------------------------- sweeps
@@ -392,11 +482,13 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
   k.dx = lane63(dx_p);
 }
 
+// line (i2, i3) = (bx + 1, by + 1)
 template <int NQ, bool OTF>
-__global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFields F, int isp0,
-                                                double dt) {
+__device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
+                                            const SweepFields& F, int isp0, double dt, int bx,
+                                            int by) {
   const int lx1 = g.lx1;
-  const int i2 = blockIdx.x + 1, i3 = blockIdx.y + 1;
+  const int i2 = bx + 1, i3 = by + 1;
   const int lane = threadIdx.x, isp = isp0 + threadIdx.y;
   const long c0 = gix(g, 0, i2, i3);                                       // cell p -> c0+p
   const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;  // iface p -> if0+p
@@ -426,6 +518,11 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFiel
     s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c + 1, lane, dt, vb1, vbN, B, A, k);
   }
 }
+template <int NQ, bool OTF>
+__global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFields F, int isp0,
+                                                double dt) {
+  sweep1_body<NQ, OTF>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y);
+}
 
 // ---------------------------------------------------------------- x2 / x3 sweeps
 // advec.f90:313-381 + :384-435.  Thread (i1, line, species): marches along its line
@@ -450,14 +547,38 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFiel
 #define SWEEP_PF1 4     // the same for single-field launches (fewer registers per step)
 #endif
 
-template <int DIR, int PF, int NQ, bool OTF>
-__global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevState s,
-                                                               SweepFields F, int isp0, double dt) {
+// CONV (x3, all fields, the fused step's first sweep): the cells the sweep reads inside the
+// line still hold what v12rhov1 and T2rhoe would have converted, so momentum and energy
+// are loaded as vs1 and Ts there and converted with the conversions' own expressions
+// from the cell's density (the ghost fill has written the ghost cells).  The electron
+// wave loads its density in field slot 0 and its momentum in slot 1 without advecting
+// them (the electrons' momentum is converted, as v12rhov1 does for every species, but
+// not swept), writing the loaded density and the converted momentum back, and advects
+// its energy in slot 2.
+//
+// POST (x2, all fields, the fused step's last sweep): rhov12v1 and rhoe2T
+// (multifluid.f90:179-189, 218-233) of every cell the sweep updates, from the values it
+// has just computed.  Each ion wave converts its own species (vs1, Ts) and passes its
+// density and current term (n qs) vs1 through LDS; after each group of PF steps the
+// electron wave sums them in species order for the electron density, velocity and
+// temperature of the previous group's cells.  The cells outside the interior box are
+// converted by k_post_convert_surface.
+// POST: species terms of the cells of two groups of PF steps, [group parity][step][ion][n,
+// current][lane], in LDS
+template <int PF>
+constexpr int sweep_xch_doubles() { return 2 * PF * (LSP - 1) * 2 * 64; }
+
+// block (bx, by): i1 = 64 bx + lane + 1 (lanes past lx1 redo column lx1), line by + 1
+template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = false>
+__device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s,
+                                             const SweepFields& F, int isp0, double dt, int bx,
+                                             int by, double* xch) {
+  static_assert(!(CONV || POST) || NQ == 3, "the conversions come with all three fields");
   const int lx = DIR == 2 ? g.lx2 : g.lx3;
   // lanes past lx1 redo column lx1's work: same inputs, same values written to the same cells
-  const int i1 = min((int)(blockIdx.x * blockDim.x + threadIdx.x), g.lx1 - 1) + 1;
+  const int i1 = min((int)(bx * 64 + threadIdx.x), g.lx1 - 1) + 1;
   const int isp = isp0 + threadIdx.y;
-  const int other = blockIdx.y + 1;                  // i3 for DIR 2, i2 for DIR 3
+  const int other = by + 1;                          // i3 for DIR 2, i2 for DIR 3
   const long cs_ = DIR == 2 ? (long)g.L1 : (long)g.L1 * g.L2;   // cell stride along the line
   const long c0 = DIR == 2 ? gix(g, i1, 0, other) : gix(g, i1, other, 0);  // cell k -> c0+k*cs_
   const long istr = DIR == 2 ? (long)g.lx1 : (long)g.lx1 * g.lx2;          // iface stride
@@ -474,19 +595,40 @@ __global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevSta
                      (long)(isp - 1) * (DIR == 2 ? (long)g.lx1 * (g.lx2 + 1) * g.lx3
                                                  : (long)g.lx1 * g.lx2 * (g.lx3 + 1));
   double* fp[NQ];
+  const double* lp[NQ];        // CONV: where a line-interior cell of the field is loaded from
   bool fr1[NQ];
   const bool e7 = F.elec_energy && isp == LSP;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     fp[q] = (e7 ? F.f[NQ - 1] : F.f[q]) + so + c0;
     fr1[q] = !e7 && F.frank[q] == 1;
+    lp[q] = fp[q];
   }
+  const double msp = ms(isp), gm1 = gammas(isp) - 1.0;
+  if (CONV) {
+    if (e7) {                                  // density and momentum: converted, not advected
+      fp[0] = s.ns + so + c0;
+      fp[1] = s.rhovs1 + so + c0;
+    }
+    lp[0] = fp[0];
+    lp[1] = s.vs1 + so + c0;
+    lp[2] = s.Ts + so + c0;
+  }
+  // CONV: turn the loaded slots of a line-interior cell into momentum and energy
+  auto convert = [&](double* v) {
+    if (CONV) {
+      const double n = v[0];
+      v[1] = (n * msp) * v[1];
+      v[2] = ((n * kB) * v[2]) / gm1;
+    }
+  };
   // pipeline registers of the steps in flight
   double pf[NQ][PF], pv[PF], pI[PF], pA[PF], pB[PF], p1[PF], p2[PF], p3[PF];
   auto issue = [&](int k, int u) {
     const int kf = min(k + 2, lx + 2);
+    const bool conv = CONV && kf <= lx;        // kf >= 2: inside the line up to lx
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) pf[q][u] = fp[q][(long)kf * cs_];
+    for (int q = 0; q < NQ; ++q) pf[q][u] = (conv ? lp[q] : fp[q])[(long)kf * cs_];
     const int ki = clampi(k, 1, lx + 1);
     const long ik = if0 + (long)ki * istr;
     pv[u] = OTF ? w[(long)ki * cs_] : vi[ik];
@@ -502,22 +644,55 @@ __global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevSta
   for (int q = 0; q < NQ; ++q) {
     fa[q] = fp[q][-cs_];       // f(-1)
     fb[q] = fp[q][0];          // f(0)
-    fc[q] = fp[q][cs_];        // f(1)
+    fc[q] = lp[q][cs_];        // f(1)
+  }
+  if (lx >= 1) convert(fc);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
     rsp[q] = (fb[q] - fa[q]) / dxcur;
     slp[q] = php[q] = 0.0;
   }
 #pragma unroll
   for (int u = 0; u < PF; ++u) issue(u, u);
+  // POST: per-wave constants, the group parity being written, and the electron wave's
+  // updated energies of the current and the previous group
+  const double gm1kB = (gammas(isp) - 1.0) / kB, qsp = qs(isp);
+  int xbuf = 0;
+  double eout[PF], eprev[PF];
+  const int lane = threadIdx.x;
+  auto xslot = [&](int b, int slot, int sp, int j) -> double& {
+    return xch[(((b * PF + slot) * (LSP - 1) + sp) * 2 + j) * 64 + lane];
+  };
+  // electron density, velocity and temperature of the cells k0-1 .. k0+cnt-2 of a group
+  // the ion waves have written into LDS parity b (electron energies in e)
+  auto finalize = [&](int b, int k0, int cnt, const double* e) {
+#pragma unroll
+    for (int slot = 0; slot < PF; ++slot) {
+      if (slot >= cnt) break;
+      double cf = 0.0, sum = 0.0;
+#pragma unroll
+      for (int sp = 0; sp < LSP - 1; ++sp) cf = cf + xslot(b, slot, sp, 1);
+#pragma unroll
+      for (int sp = 0; sp < LSP - 1; ++sp) sum = sum + xslot(b, slot, sp, 0);
+      const long c = c0 + (long)(k0 + slot - 1) * cs_ + so;
+      s.ns[c] = sum;
+      s.vs1[c] = -(((1.0 / fmax_f(sum, mindensdiv)) / qs(LSP)) * cf);
+      s.Ts[c] = fmax_f((gm1kB * e[slot]) / fmax_f(sum, mindensdiv), 100.0);
+    }
+  };
   // mode 0: step 0 (slopes only), 1: step 1 (flux, no store), 2: steps 2..lx+1
-  auto step = [&](int k, int u, int mode) {
+  // slot: the step's position in its group (POST)
+  auto step = [&](int k, int u, int mode, int slot = 0) {
     const double dxk = dxcur, dxk1 = dx[k + 1];
     dxcur = dxk1;
     const double wv = pv[u], hIv = pI[u], a = pA[u], b = pB[u], h1 = p1[u], h2 = p2[u], h3 = p3[u];
     double nxt[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) nxt[q] = pf[q][u];
+    if (CONV && min(k + 2, lx + 2) <= lx) convert(nxt);
     issue(k + PF, u);
     double v = 0.0, vq = 0.0, h2f0 = 0.0, h2f1 = 0.0, ha1f0 = 1.0, ha1f1 = 1.0, dxim = 1.0;
+    double newv[NQ];
     if (mode >= 1) {
       if (OTF) { v = 0.5 * (wprev + wv); wprev = wv; }
       else v = wv;
@@ -543,7 +718,10 @@ __global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevSta
           const double h2k = fr1[q] ? h2f1 : h2f0;
           const double h2m = fr1[q] ? h2p1 : h2p0;
           const double ha1 = fr1[q] ? ha1f1 : ha1f0;
-          fp[q][(long)(k - 1) * cs_] = fa[q] - ((dt * (h2k * ph - h2m * php[q])) / dxim) / ha1;
+          const double out = fa[q] - ((dt * (h2k * ph - h2m * php[q])) / dxim) / ha1;
+          // CONV: the electron wave's density and momentum go back unchanged
+          fp[q][(long)(k - 1) * cs_] = (CONV && q < 2 && e7) ? fa[q] : out;
+          if (POST) newv[q] = out;
         }
         php[q] = ph;
       }
@@ -554,60 +732,227 @@ __global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevSta
       fc[q] = nxt[q];
     }
     if (mode >= 1) { h2p0 = h2f0; h2p1 = h2f1; }
+    if (POST && mode == 2) {
+      if (e7) {
+        eout[slot] = newv[NQ - 1];
+      } else {                                  // ion: density, momentum, energy of cell k-1
+        const double n = newv[0];
+        const double vv = newv[1] / (msp * fmax_f(n, mindensdiv));
+        const long c = c0 + (long)(k - 1) * cs_ + so;
+        s.vs1[c] = vv;
+        s.Ts[c] = fmax_f((gm1kB * newv[2]) / fmax_f(n, mindensdiv), 100.0);
+        xslot(xbuf, slot, isp - 1, 0) = n;
+        xslot(xbuf, slot, isp - 1, 1) = (n * qsp) * vv;
+      }
+    }
+  };
+  // POST: LDS stores of one group are visible to the electron wave after this barrier; no
+  // fence on global memory, so loads issued ahead stay in flight
+  auto lds_barrier = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   };
   step(0, 0, 0);
   step(1, 1 % PF, 1);
   int kb = 2;
+  int groups = 0;
   for (; kb + PF - 1 <= lx + 1; kb += PF) {
 #pragma unroll
-    for (int u = 0; u < PF; ++u) step(kb + u, (2 + u) % PF, 2);
-    // keep the species waves of the block in step so the metric factors they all read
-    // are still cached when the slower waves reach them (no memory fence: nothing is
-    // exchanged between the waves)
-    __builtin_amdgcn_s_barrier();
+    for (int u = 0; u < PF; ++u) step(kb + u, (2 + u) % PF, 2, u);
+    if (POST) {
+      // the electron wave finishes the previous group (its LDS parity is not written in
+      // this group), then the ion waves may write that parity again after the barrier
+      if (e7 && groups > 0) finalize(xbuf ^ 1, kb - PF, PF, eprev);
+#pragma unroll
+      for (int u = 0; u < PF; ++u) eprev[u] = eout[u];
+      lds_barrier();
+      xbuf ^= 1;
+      ++groups;
+    } else {
+      // keep the species waves of the block in step so the metric factors they all read
+      // are still cached when the slower waves reach them (no memory fence: nothing is
+      // exchanged between the waves)
+      __builtin_amdgcn_s_barrier();
+    }
   }
+  int tail = 0;
 #pragma unroll
   for (int u = 0; u < PF - 1; ++u) {
     if (kb + u > lx + 1) break;
-    step(kb + u, (2 + u) % PF, 2);
+    step(kb + u, (2 + u) % PF, 2, u);
+    ++tail;
   }
+  if (POST) {
+    if (e7 && groups > 0) finalize(xbuf ^ 1, kb - PF, PF, eprev);
+    lds_barrier();
+    if (e7 && tail > 0) finalize(xbuf, kb, tail, eout);
+  }
+}
+
+template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = false>
+__global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevState s,
+                                                               SweepFields F, int isp0, double dt) {
+  __shared__ double xch[POST ? sweep_xch_doubles<PF>() : 1];
+  sweep23_body<DIR, PF, NQ, OTF, CONV, POST>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y, xch);
 }
 
 // Ts ghost cells: floor at 100 K (the reference leaves them undefined, see the
-// diffusion kernel).  One thread per ghost cell: first the four x1 ghosts of every
-// interior column, then every cell of the columns with an x2 or x3 ghost index.
-__device__ __forceinline__ int ghost_index(int j, int lx) { return j < 2 ? j - 1 : lx + j - 1; }
-
+// diffusion kernel).  One thread per ghost cell.
+__device__ __forceinline__ void floor_ghost_body(const DevGrid& g, double* Ts, int isp, long t) {
+  double* T = Ts + (long)(isp - 1) * g.N;
+  const long c = surface_cell(g, t);
+  if (c >= 0) T[c] = fmax_f(T[c], 100.0);
+}
 __global__ void k_floor_ghosts(DevGrid g, double* Ts) {
-  double* T = Ts + (long)blockIdx.y * g.N;
-  const long n1 = 4L * g.lx2 * g.lx3;
-  const long ncol = 4L * g.L2 + 4L * g.lx3;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long c;
-  if (t < n1) {
-    const long col = t / 4;
-    c = gix(g, ghost_index((int)(t % 4), g.lx1), (int)(col % g.lx2) + 1, (int)(col / g.lx2) + 1);
-  } else if (t < n1 + ncol * g.L1) {
-    const long r = t - n1;
-    const int i1 = (int)(r % g.L1) - 1;
-    const long col = r / g.L1;
-    int i2, i3;
-    if (col < 4L * g.L2) {
-      i2 = (int)(col % g.L2) - 1;
-      i3 = ghost_index((int)(col / g.L2), g.lx3);
-    } else {
-      const long cc = col - 4L * g.L2;
-      i3 = (int)(cc % g.lx3) + 1;
-      i2 = ghost_index((int)(cc / g.lx3), g.lx2);
+  floor_ghost_body(g, Ts, blockIdx.y + 1, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---------------------------------------------------------------- batched ensemble kernels
+// Persistent grids over the work units of every member (BatchTable): unit t of kernel k
+// belongs to the member m with pre[m] <= t < pre[m + 1] and is that member's unit
+// t - pre[m], decoded as the member's own launch numbers it.
+
+// the thread kernels: ghost fills, boundary interface velocities, surface conversions
+template <int K>
+__global__ __launch_bounds__(TPB) void kb_threads(const BatchMember* M, const long* pre, int n,
+                                                  long total) {
+  const long* p = pre + (long)K * (n + 1);
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    const int m = batch_member(p, n, t);
+    const DevGrid& g = M[m].g;
+    const DevState& s = M[m].s;
+    const long u = t - p[m];
+    if (K == BK_BC1) {
+      const long cols = (long)g.L2 * g.L3;
+      bc_x1_body<true>(g, s, (int)(u / cols) + 1, u % cols);
+    } else if (K == BK_BC2 || K == BK_BC3) {
+      const long cols = K == BK_BC2 ? (long)g.L1 * g.L3 : (long)g.L1 * g.L2;
+      const long per_q = cols * LSP;
+      const int q = (int)(u / per_q);
+      const long r = u % per_q;
+      if (K == BK_BC2) bc_x2_body<true>(g, s, (int)(r / cols) + 1, q, r % cols);
+      else bc_x3_body<true>(g, s, (int)(r / cols) + 1, q, r % cols);
+    } else if (K == BK_V1B) {
+      const long cols = (long)g.lx2 * g.lx3;
+      save_v1b_body(g, s, (int)(u / cols) + 1, u % cols);
+    } else if (K == BK_POST) {
+      post_convert_cell(g, s, surface_cell(g, u));
+    } else if (K == BK_FLOOR) {
+      const long cells = surface_cells(g);
+      floor_ghost_body(g, s.Ts, (int)(u / cells) + 1, u % cells);
     }
-    c = gix(g, i1, i2, i3);
-  } else {
-    return;
   }
-  T[c] = fmax_f(T[c], 100.0);
+}
+
+// the x2/x3 sweeps of all fields (x3 with the conversions folded in, x2 with rhov12v1 and
+// rhoe2T); blocks of 64 x 7
+template <int DIR, int PF>
+__global__ __launch_bounds__(448) void kb_sweep23(const BatchMember* M, const long* pre, int n,
+                                                  long total) {
+  __shared__ double xch[sweep_xch_doubles<PF>()];
+  constexpr int K = DIR == 3 ? BK_SW3 : BK_SW2;
+  const long* p = pre + (long)K * (n + 1);
+  for (long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int m = batch_member(p, n, t);
+    const DevGrid& g = M[m].g;
+    const DevState& s = M[m].s;
+    const long u = t - p[m];
+    const int nbx = (g.lx1 + 63) / 64;
+    SweepFields F{};
+    F.f[0] = s.ns; F.frank[0] = 0;
+    F.f[1] = s.rhovs1; F.frank[1] = 1;
+    F.f[2] = s.rhoes; F.frank[2] = 0;
+    F.elec_energy = 1;
+    sweep23_body<DIR, PF, 3, true, DIR == 3, DIR == 2>(g, s, F, 1, M[m].dt, (int)(u % nbx),
+                                                      (int)(u / nbx), xch);
+    __syncthreads();                               // the next unit reuses the LDS
+  }
+}
+
+// the x1 sweep: ions (blocks of 64 x 6, three fields) or the electron energy (64 x 1)
+template <bool ELEC>
+__global__ __launch_bounds__(448) void kb_sweep1(const BatchMember* M, const long* pre, int n,
+                                                 long total) {
+  constexpr int K = ELEC ? BK_SW1E : BK_SW1;
+  const long* p = pre + (long)K * (n + 1);
+  for (long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int m = batch_member(p, n, t);
+    const DevGrid& g = M[m].g;
+    const DevState& s = M[m].s;
+    const long u = t - p[m];
+    SweepFields F{};
+    if (ELEC) {
+      F.f[0] = s.rhoes; F.frank[0] = 0;
+      sweep1_body<1, true>(g, s, F, LSP, M[m].dt, (int)(u % g.lx2), (int)(u / g.lx2));
+    } else {
+      F.f[0] = s.ns; F.f[1] = s.rhovs1; F.f[2] = s.rhoes;   // x1: every field of rank 0
+      sweep1_body<3, true>(g, s, F, 1, M[m].dt, (int)(u % g.lx2), (int)(u / g.lx2));
+    }
+  }
 }
 
 }  // namespace
+
+long batch_units(const DevGrid& g, int kernel) {
+  switch (kernel) {
+    case BK_BC1: return (long)g.L2 * g.L3 * LSP;
+    case BK_BC2: return (long)g.L1 * g.L3 * LSP * 4;
+    case BK_BC3: return (long)g.L1 * g.L2 * LSP * 4;
+    case BK_V1B: return (long)g.lx2 * g.lx3 * LSP;
+    case BK_SW3: return (long)((g.lx1 + 63) / 64) * g.lx2;
+    case BK_SW1: case BK_SW1E: return (long)g.lx2 * g.lx3;
+    case BK_SW2: return (long)((g.lx1 + 63) / 64) * g.lx3;
+    case BK_POST: return surface_cells(g);
+    case BK_DIFF: return (((long)g.lx2 * g.lx3 + 63) / 64 + 7) / 8 * 8 * LSP;
+    case BK_FLOOR: return surface_cells(g) * LSP;
+  }
+  return 0;
+}
+
+// The advection part of a batched step for every member, in the fused step's order:
+// ghost fill (with v12rhov1, T2rhoe), x1 boundary interface velocities, x3 sweep
+// (converting), x1 sweep, x2 sweep (with rhov12v1, rhoe2T), the surface conversions.
+void launch_batch_advection(const BatchTable& t, int sweep_pf, hipStream_t st) {
+  const dim3 thr(TPB), s7(64, LSP), s6(64, LSP - 1), s1(64, 1);
+  auto tb = [&](long total) { return (unsigned)std::max(1L, std::min<long>(total, t.blocks_sweep)); };
+  auto th = [&](long total) {
+    return (unsigned)std::max(1L, std::min<long>((total + TPB - 1) / TPB, t.blocks_thread));
+  };
+  hipLaunchKernelGGL(kb_threads<BK_BC1>, dim3(th(t.total[BK_BC1])), thr, 0, st, t.m, t.pre, t.n,
+                     t.total[BK_BC1]);
+  hipLaunchKernelGGL(kb_threads<BK_BC2>, dim3(th(t.total[BK_BC2])), thr, 0, st, t.m, t.pre, t.n,
+                     t.total[BK_BC2]);
+  hipLaunchKernelGGL(kb_threads<BK_BC3>, dim3(th(t.total[BK_BC3])), thr, 0, st, t.m, t.pre, t.n,
+                     t.total[BK_BC3]);
+  hipLaunchKernelGGL(kb_threads<BK_V1B>, dim3(th(t.total[BK_V1B])), thr, 0, st, t.m, t.pre, t.n,
+                     t.total[BK_V1B]);
+  if (sweep_pf == 1)
+    hipLaunchKernelGGL((kb_sweep23<3, 1>), dim3(tb(t.total[BK_SW3])), s7, 0, st, t.m, t.pre, t.n,
+                       t.total[BK_SW3]);
+  else
+    hipLaunchKernelGGL((kb_sweep23<3, 2>), dim3(tb(t.total[BK_SW3])), s7, 0, st, t.m, t.pre, t.n,
+                       t.total[BK_SW3]);
+  hipLaunchKernelGGL(kb_sweep1<false>, dim3(tb(t.total[BK_SW1])), s6, 0, st, t.m, t.pre, t.n,
+                     t.total[BK_SW1]);
+  hipLaunchKernelGGL(kb_sweep1<true>, dim3(tb(t.total[BK_SW1E]) * 4), s1, 0, st, t.m, t.pre, t.n,
+                     t.total[BK_SW1E]);
+  if (sweep_pf == 1)
+    hipLaunchKernelGGL((kb_sweep23<2, 1>), dim3(tb(t.total[BK_SW2])), s7, 0, st, t.m, t.pre, t.n,
+                       t.total[BK_SW2]);
+  else
+    hipLaunchKernelGGL((kb_sweep23<2, 2>), dim3(tb(t.total[BK_SW2])), s7, 0, st, t.m, t.pre, t.n,
+                       t.total[BK_SW2]);
+  hipLaunchKernelGGL(kb_threads<BK_POST>, dim3(th(t.total[BK_POST])), thr, 0, st, t.m, t.pre, t.n,
+                     t.total[BK_POST]);
+}
+
+void launch_batch_floor(const BatchTable& t, hipStream_t st) {
+  const long total = t.total[BK_FLOOR];
+  const unsigned b = (unsigned)std::max(1L, std::min<long>((total + TPB - 1) / TPB, t.blocks_thread));
+  hipLaunchKernelGGL(kb_threads<BK_FLOOR>, dim3(b), dim3(TPB), 0, st, t.m, t.pre, t.n, total);
+}
 
 // ---------------------------------------------------------------- launchers
 void launch_v12rhov1_T2rhoe(const DevGrid& g, const DevState& s, bool mom, bool energy,
@@ -639,11 +984,32 @@ void launch_rhoe2T(const DevGrid& g, const DevState& s, hipStream_t st) {
 void launch_set_global_boundaries(const DevGrid& g, const DevState& s, int lsp,
                                   hipStream_t st) {
   timing_begin("ghost_fill", st);
-  hipLaunchKernelGGL(k_bc_x1, dim3(blocks_for((long)g.L2 * g.L3), lsp), dim3(TPB), 0, st, g, s);
-  hipLaunchKernelGGL(k_bc_x2, dim3(blocks_for((long)g.L1 * g.L3), lsp, 4), dim3(TPB), 0, st, g, s);
+  hipLaunchKernelGGL(k_bc_x1<false>, dim3(blocks_for((long)g.L2 * g.L3), lsp), dim3(TPB), 0, st,
+                     g, s);
+  hipLaunchKernelGGL(k_bc_x2<false>, dim3(blocks_for((long)g.L1 * g.L3), lsp, 4), dim3(TPB), 0,
+                     st, g, s);
   if (!g.flagper)
-    hipLaunchKernelGGL(k_bc_x3, dim3(blocks_for((long)g.L1 * g.L2), lsp, 4), dim3(TPB), 0, st, g,
-                       s);
+    hipLaunchKernelGGL(k_bc_x3<false>, dim3(blocks_for((long)g.L1 * g.L2), lsp, 4), dim3(TPB), 0,
+                       st, g, s);
+  timing_end("ghost_fill", st);
+}
+
+bool preconvert_fusable(const DevGrid& g) {
+  if (g.flagper || g.lx3 <= 1) return false;   // the x3 sweep converts the interior cells
+  for (int k = 0; k < 6; ++k)
+    if (!g.isglobal[k]) return false;
+  return true;
+}
+
+void launch_set_global_boundaries_preconvert(const DevGrid& g, const DevState& s,
+                                             hipStream_t st) {
+  timing_begin("ghost_fill", st);
+  hipLaunchKernelGGL(k_bc_x1<true>, dim3(blocks_for((long)g.L2 * g.L3), LSP), dim3(TPB), 0, st,
+                     g, s);
+  hipLaunchKernelGGL(k_bc_x2<true>, dim3(blocks_for((long)g.L1 * g.L3), LSP, 4), dim3(TPB), 0,
+                     st, g, s);
+  hipLaunchKernelGGL(k_bc_x3<true>, dim3(blocks_for((long)g.L1 * g.L2), LSP, 4), dim3(TPB), 0, st,
+                     g, s);
   timing_end("ghost_fill", st);
 }
 
@@ -730,9 +1096,51 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
   timing_end(name, st);
 }
 
+// The fused step's x3 sweep of all fields, with v12rhov1 and T2rhoe folded into its loads
+// (after launch_set_global_boundaries_preconvert; interface velocities on the fly).
+void launch_sweep3_preconvert(const DevGrid& g, const DevState& s, double dt, hipStream_t st,
+                              const Tuning& tune) {
+  SweepFields A{};
+  A.f[0] = s.ns; A.frank[0] = 0;
+  A.f[1] = s.rhovs1; A.frank[1] = 1;
+  A.f[2] = s.rhoes; A.frank[2] = 0;
+  A.elec_energy = 1;
+  const dim3 blk(64, LSP), grd((g.lx1 + 63) / 64, g.lx2);
+  timing_begin("sweep3", st);
+  if (tune.sweep_pf == 1)
+    hipLaunchKernelGGL((k_sweep23<3, 1, 3, true, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  else
+    hipLaunchKernelGGL((k_sweep23<3, 2, 3, true, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  timing_end("sweep3", st);
+}
+
+// The fused step's x2 sweep of all fields with rhov12v1 and rhoe2T of every cell it
+// updates, then the conversions of the cells outside the interior box.
+void launch_sweep2_postconvert(const DevGrid& g, const DevState& s, double dt, hipStream_t st,
+                               const Tuning& tune) {
+  SweepFields A{};
+  A.f[0] = s.ns; A.frank[0] = 0;
+  A.f[1] = s.rhovs1; A.frank[1] = 1;
+  A.f[2] = s.rhoes; A.frank[2] = 0;
+  A.elec_energy = 1;
+  const dim3 blk(64, LSP), grd((g.lx1 + 63) / 64, g.lx3);
+  timing_begin("sweep2", st);
+  if (tune.sweep_pf == 1)
+    hipLaunchKernelGGL((k_sweep23<2, 1, 3, true, false, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  else
+    hipLaunchKernelGGL((k_sweep23<2, 2, 3, true, false, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  timing_end("sweep2", st);
+  timing_begin("post_convert", st);
+  hipLaunchKernelGGL(k_post_convert_surface, dim3(blocks_for(surface_cells(g))), dim3(TPB), 0, st,
+                     g, s);
+  timing_end("post_convert", st);
+}
+
+bool postconvert_fusable(const DevGrid& g) { return g.lx2 > 1; }
+
 void launch_floor_ghosts(const DevGrid& g, double* Ts, hipStream_t st) {
-  const long n = 4L * g.lx2 * g.lx3 + (4L * g.L2 + 4L * g.lx3) * g.L1;
-  hipLaunchKernelGGL(k_floor_ghosts, dim3(blocks_for(n), LSP), dim3(TPB), 0, st, g, Ts);
+  hipLaunchKernelGGL(k_floor_ghosts, dim3(blocks_for(surface_cells(g)), LSP), dim3(TPB), 0, st, g,
+                     Ts);
 }
 
 }  // namespace gemhip

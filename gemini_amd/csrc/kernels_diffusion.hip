@@ -48,11 +48,11 @@ struct Elim {
 //     and runs the sequential pivoted elimination of the first solve (TR, or backward
 //     Euler) with its right-hand side.  For TRBDF2 the same pass also factors the BDF2
 //     matrix, which does not depend on the TR solution: its pivot rows, multipliers and
-//     row interchanges are stored, with the part of its right-hand side that depends on
-//     T alone;
+//     row interchanges are stored;
 //   2 (bottom-up): back substitution of the first solve (into Ts for backward Euler);
-//   3 (top-down, TRBDF2): the BDF2 right-hand side from the TR solution, taken through
-//     the stored interchanges and multipliers (dgbtrs's forward solve);
+//   3 (top-down, TRBDF2): the BDF2 right-hand side from the input temperatures and the
+//     TR solution, taken through the stored interchanges and multipliers (dgbtrs's
+//     forward solve);
 //   4 (bottom-up, TRBDF2): back substitution of the BDF2 solve into Ts.
 // Every value is formed with the operations, and in the order, LAPACK's dgbtf2, dgbtrs
 // and dtbsv would use on the reference's band matrix: factorising a matrix before its
@@ -85,9 +85,8 @@ enum : int {
                                        // pass 3 reuses SY for the BDF2 rhs
   STR = 4,                             // TR solution
   SV0 = 5, SV1 = 6, SV2 = 7, SL = 8,   // BDF2 pivot rows and multipliers
-  SAT = 9,                             // -((1/3 T)/(dt/3)): the T part of the BDF2 rhs
-  NSLOT = 10
-};
+  NSLOT = 9                            // (the T part of the BDF2 rhs is formed in pass 3
+};                                     // from Ts, which pass 4 is the first to overwrite)
 
 struct Group {
   int N;
@@ -109,7 +108,7 @@ __device__ __forceinline__ void column_of(const DevGrid& g, int gcol, int& i2, i
 // Pivoted elimination step j (dgbtf2 with dgbtrs's forward solve) bringing in row j+1 =
 // (a, b, cc | d), applied to the state only when `real`.  The row at pivot position j is
 // stored at row sj (a step that is not real stores where the real step sj stores later).
-__device__ __forceinline__ void elim_rhs(const Group& q, Elim& e, int j, int sj, bool real,
+__device__ __forceinline__ bool elim_rhs(const Group& q, Elim& e, int j, int sj, bool real,
                                          double a, double b, double cc, double d) {
   const bool swap = fabs(a) > fabs(e.r0);
   double p0, p1, p2, pd, o0, o1, o2, od;
@@ -130,6 +129,7 @@ __device__ __forceinline__ void elim_rhs(const Group& q, Elim& e, int j, int sj,
     e.singular = e.singular || sing;
     e.r0 = o1; e.r1 = o2; e.rb = od;
   }
+  return swap;
 }
 
 // The same step on the matrix alone (the BDF2 factorisation): stores the pivot row, the
@@ -349,25 +349,41 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   if (PH == 2) {
     f.singular = *mysing != 0;
   } else {
+  // the record of the electron columns' interchanges (fitting extension, adjoint tapes)
+  const int nw = (N + 31) / 32;
+  int* prec = (EXT && ELEC && s.pivrec) ? s.pivrec + (long)col * 2 * nw : nullptr;
+  int tbits = 0;
+  auto record_tr = [&](int j, bool swap) {   // TR step j (1..N-1)
+    tbits |= (swap ? 1 : 0) << (j & 31);
+    if ((j & 31) == 31 || j == N - 1) {
+      prec[j >> 5] = tbits;
+      tbits = 0;
+    }
+  };
   // pass 1
   rows([&](int ro, bool real, double t1f, double t2f, double t3f, double Tm, double Tc,
            double Tp) {
     const int j = ro - 1, sj = clampi(j, 1, N - 1);
     double a, b, c, d;
     first_row(t1f, t2f, t3f, Tm, Tc, Tp, a, b, c, d);
-    elim_rhs(q, e, j, sj, real, a, b, c, d);
+    const bool sw = elim_rhs(q, e, j, sj, real, a, b, c, d);
+    if (EXT && ELEC && prec && real) record_tr(j, sw);
     if (!BE) {                                       // PDEparabolic.f90:146-216 matrix
       elim_mat(q, f, j, sj, real, -t1f + t2f, (bdf.K + t3f) + t1f, -t3f - t2f);
-      S2(q, clampi(ro, 1, N), SAT) = -((bdf.c13 * Tc) / bdf.dt3);
     }
   });
-  elim_rhs(q, e, N - 1, N - 1, true, aN, bN, 0.0, Tmax);         // boundary row N
+  {
+    const bool sw = elim_rhs(q, e, N - 1, N - 1, true, aN, bN, 0.0, Tmax);   // boundary row N
+    if (EXT && ELEC && prec) record_tr(N - 1, sw);
+  }
   if (e.r0 == 0.0) e.singular = true;
   S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SU2) = 0.0; S2(q, N, SY) = e.rb;
   if (!BE) {
     elim_mat(q, f, N - 1, N - 1, true, aN, bN, 0.0);
     if (f.r0 == 0.0) f.singular = true;
     S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0; S2(q, N, SV2) = 0.0;
+    if (EXT && ELEC && prec)                         // the BDF2 interchanges, as stored
+      for (int w = 0; w < nw; ++w) prec[nw + w] = q.swb[w * 64];
   }
   // a singular factorisation leaves dgbsv's right-hand side as the solution: for the
   // lanes concerned, store the first solve's right-hand side in the y slot (a slow pass
@@ -400,14 +416,17 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   }  // passes 1 and 2
 
   // pass 3: d(r) = AT(r) + ((4/3) TR(r))/(dt/3) for rows 2..N-1 (Tmin, Tmax at the ends),
-  // through the interchanges and multipliers of pass 1; singular lanes keep d itself
+  // with AT(r) = -((1/3) T(r))/(dt/3) from the input temperature (its column of Ts, which
+  // only pass 4 overwrites), through the interchanges and multipliers of pass 1; singular
+  // lanes keep d itself
   {
     constexpr int PF = R;
     double pat[PF], ptr[PF], pl[PF];
     int pb[PF];
+    const double* Tcol = s.Ts + so + gix(g, 0, col % g.lx2 + 1, col / g.lx2 + 1);
     auto fetch = [&](int r, int u) {                 // inputs of step j = r-1
       const int rc = clampi(r, 1, N), jc = clampi(r - 1, 1, N - 1);
-      pat[u] = S2(q, rc, SAT); ptr[u] = S2(q, rc, STR); pl[u] = S2(q, jc, SL);
+      pat[u] = Tcol[rc]; ptr[u] = S2(q, rc, STR); pl[u] = S2(q, jc, SL);
       pb[u] = q.swb[(jc >> 5) * 64];
     };
 #pragma unroll
@@ -419,7 +438,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int j = j0 + u, r = j + 1;
-        const double at = pat[u], trr = ptr[u], l = pl[u];
+        const double at = -((bdf.c13 * pat[u]) / bdf.dt3), trr = ptr[u], l = pl[u];
         const int bits = pb[u];
         fetch(r + PF, u);
         const bool live = j <= N - 1;
@@ -466,18 +485,58 @@ __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double*
 void launch_floor_ghosts(const DevGrid& g, double* Ts, hipStream_t st);
 
 namespace {
-long diffusion_groups(const DevGrid& g) { return ((long)g.lx2 * g.lx3 + 63) / 64 * LSP; }
-size_t factor_bytes(const DevGrid& g) {
+// layout of a grid's diffusion work space: the pivot-row slots of every column group, the
+// BDF2 interchange words, one singular flag per lane
+__host__ __device__ inline long diffusion_groups(const DevGrid& g) {
+  return ((long)g.lx2 * g.lx3 + 63) / 64 * LSP;
+}
+__host__ __device__ inline size_t factor_bytes(const DevGrid& g) {
   return (size_t)diffusion_groups(g) * (size_t)g.lx1 * NSLOT * 64 * sizeof(double);
 }
-size_t swap_bytes(const DevGrid& g);
-}  // namespace
-
-namespace {
-size_t swap_bytes(const DevGrid& g) {
+__host__ __device__ inline size_t swap_bytes(const DevGrid& g) {
   return (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
 }
+
+// The TRBDF2 solve of every member of a batched step: a persistent grid of one-wave blocks
+// taking the members' column groups in turn, numbered as each member's own launch numbers
+// them (every member's count is a multiple of 8, so a group keeps its XCD).  The
+// coefficients are formed from each member's dt as launch_energy_diffusion forms them.
+template <int R>
+__global__ __launch_bounds__(64) void kb_diffusion(const BatchMember* M, const long* pre, int n,
+                                                   long total) {
+  __shared__ double sT[R][LP], sC[R][LP], sB[R][LP], sD[R][LP];
+  const long* p = pre + (long)BK_DIFF * (n + 1);
+  for (long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int m = batch_member(p, n, t);
+    const DevGrid& g = M[m].g;
+    const DevState& s = M[m].s;
+    const long v = t - p[m];
+    const long ncol = (long)g.lx2 * g.lx3;
+    const long gps = (ncol + 63) / 64;
+    const long slot = v / 8;
+    const long cg = (slot / LSP) * 8 + v % 8;
+    if (cg >= gps) continue;
+    const int isp = (int)(slot % LSP) + 1;
+    const double dt = M[m].dt;
+    Coef tr{}, bdf{};
+    tr.dt2 = dt / 2.0; tr.K = 1.0 / tr.dt2; tr.dt = dt;
+    bdf.dt3 = dt / 3.0; bdf.K = 1.0 / bdf.dt3; bdf.dt = dt;
+    bdf.c13 = 1.0 / 3.0; bdf.c43 = 4.0 / 3.0;
+    double* scratch = M[m].scratch;
+    int* swbits = (int*)((char*)scratch + factor_bytes(g));
+    int* lst = (int*)((char*)swbits + swap_bytes(g));
+    if (isp == LSP)
+      diffusion_group<R, false, true, 0, false>(g, s, scratch, swbits, lst, tr, bdf, M[m].Teinf,
+                                                isp, cg, gps, sT, sC, sB, sD);
+    else
+      diffusion_group<R, false, false, 0, false>(g, s, scratch, swbits, lst, tr, bdf, M[m].Teinf,
+                                                 isp, cg, gps, sT, sC, sB, sD);
+  }
+}
 }  // namespace
+
+
+long pivot_record_ints(const DevGrid& g) { return (long)g.lx2 * g.lx3 * 2 * ((g.lx1 + 31) / 32); }
 
 // pivot rows, right-hand sides and solutions, the BDF2 interchange bits, one flag per lane
 size_t diffusion_scratch_bytes(const DevGrid& g) {
@@ -512,6 +571,17 @@ void diffusion_launch(const DevGrid& g, const DevState& s, int diffsolvetype, do
   }
 }
 }  // namespace
+
+void launch_batch_diffusion(const BatchTable& t, int diff_chunk, hipStream_t st) {
+  const long total = t.total[BK_DIFF];
+  const unsigned b = (unsigned)(total < t.blocks_diff ? total : t.blocks_diff);
+  if (diff_chunk == 4)
+    hipLaunchKernelGGL(kb_diffusion<4>, dim3(b ? b : 1), dim3(64), 0, st, t.m, t.pre, t.n, total);
+  else
+    hipLaunchKernelGGL(kb_diffusion<DIFF_CHUNK>, dim3(b ? b : 1), dim3(64), 0, st, t.m, t.pre, t.n,
+                       total);
+  launch_batch_floor(t, st);
+}
 
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
                              double Teinf, double* scratch, hipStream_t st, const Tuning& tune) {

@@ -333,7 +333,11 @@ int gemhip_checkpoint_load(const double* src);
    one ghosted field) = dL/dTe after that step, overwrites it with dL/dTe before the step,
    and adds dL/dkappa and dL/dq per column into gkappa and gq (HBM, lx2*lx3 each; either
    may be NULL).  Limiter, upwind and floor branches are those of the recorded step.
-   Needs diffsolvetype 2, and lx3 >= 2 on periodic grids. */
+   Needs diffsolvetype 2, and lx3 >= 2 on periodic grids.  With a conductivity scale or
+   heat flux registered, the tape ends with the row interchanges of the electron
+   columns' TR and BDF2 band factorisations (one bit per elimination step, 32 to an
+   int, [column][stage][word]), so that two recorded runs can be compared branch by
+   branch. */
 long gemhip_adjoint_tape_size(void);
 int gemhip_adjoint_record_step(double dt, double* tape);
 int gemhip_adjoint_step(double dt, const double* tape, double* lam, double* gkappa, double* gq);

@@ -109,6 +109,7 @@ def test_members_of_a_live_graph_cannot_be_destroyed():
         h = ens.trs[0].handle
         assert ens.lib.gemhip_context_destroy(int(h)) == 2
         assert b"ensemble graph" in ens.lib.gemhip_last_error()
+        ens.lib.gemhip_clear_error()
         ens.autotune([st for _, st, _ in ms], steps=1)
         assert ens._graphs == {}
         assert ens.lib.gemhip_ensemble_launch(gid, 1) != 0      # released with the old variants
