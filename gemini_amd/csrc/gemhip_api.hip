@@ -19,8 +19,22 @@
 #include "gemhip_internal.h"
 #include "gemhip_io.h"
 
+// Work space of gemhip_energy_diffusion_noalloc (TRBDF2 pivot rows, right-hand sides,
+// solutions and row interchanges): a static device array of NOALLOC_SCRATCH_BYTES, one copy
+// on every GPU the library runs on, held until the process ends.  Host code never uses
+// this name as an address: gemhip_init looks the array's device address up once per
+// device (noalloc_scratch below) and the kernels receive that address.
+#define NOALLOC_SCRATCH_BYTES (256ul << 20)
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace gemhip {
 namespace {
+
+// device address of `scratch` on each device (index: device ordinal), set by gemhip_init
+constexpr int MAX_DEVICES = 64;
+void* noalloc_scratch[MAX_DEVICES] = {};
+hipError_t noalloc_lookup[MAX_DEVICES] = {};   // result of the address lookup, per device
+bool noalloc_looked_up[MAX_DEVICES] = {};
 
 struct TimedLaunch {
   std::string name;
@@ -383,6 +397,14 @@ int gemhip_init(int device) {
   if (device < 0 || device >= n) { fail("device %d out of range (%d devices)", device, n); return 3; }
   if (!hipok(hipSetDevice(device), "hipSetDevice")) return 4;
   if (!hipok(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) return 5;
+  // the allocation-free entry point's work space on this device: its address, looked up
+  // once per device with the device current (the result is checked where it is used)
+  if (device < MAX_DEVICES && !noalloc_looked_up[device]) {
+    void* p = nullptr;
+    noalloc_lookup[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    noalloc_scratch[device] = noalloc_lookup[device] == hipSuccess ? p : nullptr;
+    noalloc_looked_up[device] = true;
+  }
   c.device = device;
   c.inited = true;
   return 0;
@@ -1282,6 +1304,51 @@ void energy_diffusion_C(void** cfgC, int* xtype, void** xC, double** fluidvars,
   if (!upload(io, {NS, TS, J1})) return;
   if (!run_energy_diffusion(*dt)) return;
   download(io, {TS});
+}
+
+// energy_diffusion_C's kernels on device fields the caller owns, with the work space in the
+// static `scratch` array instead of library allocations: Ts and ns are the 7 ghosted species
+// fields of fluidvars(:,:,:,29:35) and (:,:,:,1:7), J1 one ghosted field, all in HBM.  The
+// selected context supplies the grid, neutrals and configuration.  Enqueued on the
+// context's stream (gemhip_synchronize waits for it).  Every pivot row, right-hand side,
+// solution and interchange word is written before it is read within a call, so the work
+// space needs no clearing; calls that overlap on one GPU share it and race.
+int gemhip_energy_diffusion_noalloc(double* Ts, const double* ns, const double* J1, double dt) {
+  if (!ready(true)) return 1;
+  Ctx& c = ctx();
+  if (!Ts || !ns || !J1) { fail("null field pointer"); return 2; }
+  if (c.timing) {
+    fail("kernel timing creates events on demand: disable it before allocation-free calls");
+    return 3;
+  }
+  const size_t need = diffusion_scratch_bytes(c.g);
+  if (need > (size_t)NOALLOC_SCRATCH_BYTES) {
+    fail("grid %dx%dx%d needs %zu bytes of diffusion work space; the allocation-free entry "
+         "point has %zu", c.lx1, c.lx2, c.lx3, need, (size_t)NOALLOC_SCRATCH_BYTES);
+    return 4;
+  }
+  if (c.device >= MAX_DEVICES || !noalloc_scratch[c.device]) {
+    fail("no work space address on device %d: %s", c.device,
+         c.device < MAX_DEVICES ? hipGetErrorString(noalloc_lookup[c.device]) : "device ordinal");
+    return 5;
+  }
+  if (c.diffsolvetype != 1 && c.diffsolvetype != 2) {
+    fail("Unsupported diffusion solver type/mode:  %d.  Should be either 1 or 2.",
+         c.diffsolvetype);
+    return 6;
+  }
+  if (c.lx1 < 4) { fail("energy diffusion needs lx1 >= 4 (got %d)", c.lx1); return 6; }
+  if ((c.s.kappa || c.s.qtop) && c.diffsolvetype != 2) {
+    fail("the conductivity scale and top heat flux need the TRBDF2 solver (diffsolvetype 2)");
+    return 6;
+  }
+  DevState s = c.s;
+  s.Ts = Ts;
+  s.ns = const_cast<double*>(ns);
+  s.J1 = J1;
+  launch_energy_diffusion(c.g, s, dt, c.diffsolvetype, c.Teinf,
+                          static_cast<double*>(noalloc_scratch[c.device]), c.stream, c.tune);
+  return hipok(hipGetLastError(), "energy diffusion launch") ? 0 : 7;
 }
 
 void halo_allparams_C(int* xtype, void** xC, double** fluidvars, double** fluidauxvars) {
