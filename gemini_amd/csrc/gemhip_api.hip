@@ -1414,6 +1414,7 @@ int gemhip_context_select(int handle) {
 int gemhip_context_current(void) { return g_current_handle; }
 
 bool context_in_graph(int handle);
+bool context_in_batch(int handle);
 
 int gemhip_context_destroy(int handle) {
   if (handle <= 0 || handle > (int)contexts().size() || !contexts()[handle - 1]) return 1;
@@ -1421,6 +1422,10 @@ int gemhip_context_destroy(int handle) {
   if (context_in_graph(handle)) {
     fail("library context %d belongs to a captured ensemble graph (gemhip_ensemble_release)",
          handle);
+    return 2;
+  }
+  if (context_in_batch(handle)) {
+    fail("library context %d belongs to a batch (gemhip_batch_release)", handle);
     return 2;
   }
   const int keep = g_current_handle == handle ? 0 : g_current_handle;
@@ -1456,6 +1461,7 @@ struct EnsembleGraph {
   hipStream_t origin = nullptr;
   std::vector<hipEvent_t> events;
   std::vector<int> members;    // context handles whose memory and streams the graph uses
+  bool owns_origin = true;     // false: the origin is a batch's stream
 };
 std::vector<EnsembleGraph*>& graphs() {
   static std::vector<EnsembleGraph*> v;
@@ -1477,7 +1483,7 @@ void release_graph(EnsembleGraph* e) {
   if (e->exec) hipGraphExecDestroy(e->exec);
   if (e->graph) hipGraphDestroy(e->graph);
   for (auto ev : e->events) hipEventDestroy(ev);
-  if (e->origin) hipStreamDestroy(e->origin);
+  if (e->origin && e->owns_origin) hipStreamDestroy(e->origin);
   delete e;
 }
 }  // namespace
@@ -1564,11 +1570,23 @@ struct Batch {
   hipStream_t stream = nullptr;
   int device = 0, cus = 0;
   int sweep_pf = 2, diff_chunk = 8, grid = 1;   // kernel variants and persistent grid size
+  std::vector<int> graph_ids;  // graphs captured from this batch (launched on its stream)
 };
 std::vector<Batch*>& batches() {
   static std::vector<Batch*> v;
   return v;
 }
+}  // namespace
+
+bool context_in_batch(int handle) {
+  for (const Batch* b : batches())
+    if (b)
+      for (int h : b->handles)
+        if (h == handle) return true;
+  return false;
+}
+
+namespace {
 Batch* batch_of(int id) {
   if (id <= 0 || id > (int)batches().size() || !batches()[id - 1]) {
     fail("no batch %d", id);
@@ -1578,6 +1596,11 @@ Batch* batch_of(int id) {
 }
 void free_batch(Batch* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
+  for (int id : b->graph_ids)
+    if (id <= (int)graphs().size() && graphs()[id - 1]) {
+      release_graph(graphs()[id - 1]);
+      graphs()[id - 1] = nullptr;
+    }
   if (b->d_m) hipFree(b->d_m);
   if (b->d_pre) hipFree(b->d_pre);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -1617,14 +1640,16 @@ int gemhip_batch_create(int n, const int* handles) {
     else if (c.diffsolvetype != 2 || c.s.kappa || c.s.qtop)
       why = "must use TRBDF2 without the fitting extension";
     else if (m > 0 && c.device != b->device) why = "is on another device";
-    if (why) {
-      fail("batch member %d %s", m, why);
+    if (why) {                                 // reported in the caller's context
       gemhip_context_select(keep);
+      fail("batch member %d %s", m, why);
       delete b;
       return -1;
     }
     if (!ensure_scratch() || !hipok(hipStreamSynchronize(c.stream), "member sync")) {
+      const std::string msg = c.last_error;
       gemhip_context_select(keep);
+      fail("batch member %d: %s", m, msg.c_str());
       delete b;
       return -1;
     }
@@ -1701,8 +1726,10 @@ int gemhip_batch_synchronize(int id) {
   return hipok(hipStreamSynchronize(b->stream), "batch sync") ? 0 : 2;
 }
 
-// nsteps batched steps captured into one HIP graph (launch it with gemhip_ensemble_launch;
-// it holds the members like an ensemble graph does).  Returns a graph id >= 1.
+// nsteps batched steps captured into one HIP graph, launched with gemhip_ensemble_launch on
+// the batch's stream (so it is ordered with gemhip_batch_step); it holds the members like
+// an ensemble graph does and is released with the batch at the latest.  Returns a graph
+// id >= 1.
 int gemhip_batch_capture(int id, int nsteps) {
   Batch* b = batch_of(id);
   if (!b) return -1;
@@ -1710,8 +1737,9 @@ int gemhip_batch_capture(int id, int nsteps) {
   if (!hipok(hipStreamSynchronize(b->stream), "batch sync")) return -1;
   auto* e = new EnsembleGraph();
   e->members = b->handles;
-  bool ok = hipok(hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking), "origin stream") &&
-            hipok(hipStreamBeginCapture(e->origin, hipStreamCaptureModeGlobal), "begin capture");
+  e->origin = b->stream;
+  e->owns_origin = false;
+  bool ok = hipok(hipStreamBeginCapture(e->origin, hipStreamCaptureModeGlobal), "begin capture");
   for (int k = 0; ok && k < nsteps; ++k) {
     batch_launch(b, e->origin);
     ok = hipok(hipGetLastError(), "batch launch");
@@ -1723,6 +1751,7 @@ int gemhip_batch_capture(int id, int nsteps) {
   ok = ok && hipok(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0), "instantiate");
   if (!ok) { release_graph(e); return -1; }
   graphs().push_back(e);
+  b->graph_ids.push_back((int)graphs().size());
   return (int)graphs().size();
 }
 

@@ -118,8 +118,8 @@ struct BatchMember {
 // the batched kernels of a step, in order; units are threads or blocks as noted
 enum BatchKernel : int {
   BK_BC1,    // x1 ghost fill: threads, L2 L3 lsp
-  BK_BC2,    // x2 ghost fill: threads, L1 L3 lsp 4
-  BK_BC3,    // x3 ghost fill: threads, L1 L2 lsp 4
+  BK_BC2,    // x2 ghost fill: threads, L1 L3 lsp (times 4 fields in blockIdx.y)
+  BK_BC3,    // x3 ghost fill: threads, L1 L2 lsp (times 4 fields in blockIdx.y)
   BK_V1B,    // x1 boundary interface velocities: threads, lx2 lx3 lsp
   BK_SW3,    // x3 sweep with the conversions: blocks (64 x 7), ceil(lx1/64) lx2
   BK_SW1,    // x1 sweep of the ions: blocks (64 x 6), lx2 lx3
@@ -131,6 +131,21 @@ enum BatchKernel : int {
   BK_COUNT
 };
 long batch_units(const DevGrid& g, int kernel);
+__host__ __device__ inline long surface_cells(const DevGrid& g) {
+  return 4L * g.lx2 * g.lx3 + (4L * g.L2 + 4L * g.lx3) * g.L1;
+}
+// threads of a member's own launch of a thread kernel (the x2/x3 fills: per field)
+__host__ __device__ inline long batch_thread_units(const DevGrid& g, int kernel) {
+  switch (kernel) {
+    case BK_BC1: return (long)g.L2 * g.L3 * LSP;
+    case BK_BC2: return (long)g.L1 * g.L3 * LSP;
+    case BK_BC3: return (long)g.L1 * g.L2 * LSP;
+    case BK_V1B: return (long)g.lx2 * g.lx3 * LSP;
+    case BK_POST: return surface_cells(g);
+    case BK_FLOOR: return surface_cells(g) * LSP;
+  }
+  return 0;
+}
 // the member whose units of a kernel include unit t: the last m with pre[m] <= t
 __device__ __forceinline__ int batch_member(const long* pre, int n, long t) {
   int lo = 0, hi = n - 1;

@@ -136,9 +136,6 @@ __device__ __forceinline__ long surface_cell(const DevGrid& g, long t) {
   }
   return gix(g, i1, i2, i3);
 }
-__host__ __device__ inline long surface_cells(const DevGrid& g) {
-  return 4L * g.lx2 * g.lx3 + (4L * g.L2 + 4L * g.lx3) * g.L1;
-}
 
 // the fused step's rhov12v1 + rhoe2T of the cells the x2 sweep does not convert
 __global__ void k_post_convert_surface(DevGrid g, DevState s) {
@@ -813,32 +810,41 @@ __global__ void k_floor_ghosts(DevGrid g, double* Ts) {
 // belongs to the member m with pre[m] <= t < pre[m + 1] and is that member's unit
 // t - pre[m], decoded as the member's own launch numbers it.
 
-// the thread kernels: ghost fills, boundary interface velocities, surface conversions
+// Batched kernels copy the member's grid and state by value, as a member's own launch
+// receives them: hipcc (ROCm 7.2) has been seen to drop the last arm of the x2/x3 fills'
+// four-way choice of field (q == 0 ? .. : q == 1 ? .. : q == 2 ? .. : rhoes) when the
+// struct is read through a reference to memory, so that q = 3 wrote density.
+
+// the thread kernels: ghost fills, boundary interface velocities, surface conversions.
+// Every member's unit count is padded to whole waves, so the member is the same for all
+// lanes of a wave and is read into scalar registers (readfirstlane): the bodies then see
+// uniform grid and state, as in the members' own launches.  The x2 and x3 fills take the
+// field from blockIdx.y, also uniform, as their own launches do.
 template <int K>
 __global__ __launch_bounds__(TPB) void kb_threads(const BatchMember* M, const long* pre, int n,
                                                   long total) {
   const long* p = pre + (long)K * (n + 1);
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long)gridDim.x * blockDim.x) {
-    const int m = batch_member(p, n, t);
-    const DevGrid& g = M[m].g;
-    const DevState& s = M[m].s;
+    const int m = __builtin_amdgcn_readfirstlane(batch_member(p, n, t));
+    const DevGrid g = M[m].g;                        // by value, as a member's own launch
+    const DevState s = M[m].s;                       // receives them
     const long u = t - p[m];
+    if (u >= batch_thread_units(g, K)) continue;     // padding of the member's last wave
     if (K == BK_BC1) {
       const long cols = (long)g.L2 * g.L3;
       bc_x1_body<true>(g, s, (int)(u / cols) + 1, u % cols);
     } else if (K == BK_BC2 || K == BK_BC3) {
       const long cols = K == BK_BC2 ? (long)g.L1 * g.L3 : (long)g.L1 * g.L2;
-      const long per_q = cols * LSP;
-      const int q = (int)(u / per_q);
-      const long r = u % per_q;
-      if (K == BK_BC2) bc_x2_body<true>(g, s, (int)(r / cols) + 1, q, r % cols);
-      else bc_x3_body<true>(g, s, (int)(r / cols) + 1, q, r % cols);
+      const int q = blockIdx.y;
+      if (K == BK_BC2) bc_x2_body<true>(g, s, (int)(u / cols) + 1, q, u % cols);
+      else bc_x3_body<true>(g, s, (int)(u / cols) + 1, q, u % cols);
     } else if (K == BK_V1B) {
       const long cols = (long)g.lx2 * g.lx3;
       save_v1b_body(g, s, (int)(u / cols) + 1, u % cols);
     } else if (K == BK_POST) {
-      post_convert_cell(g, s, surface_cell(g, u));
+      const long c = surface_cell(g, u);
+      if (c >= 0) post_convert_cell(g, s, c);
     } else if (K == BK_FLOOR) {
       const long cells = surface_cells(g);
       floor_ghost_body(g, s.Ts, (int)(u / cells) + 1, u % cells);
@@ -856,8 +862,8 @@ __global__ __launch_bounds__(448) void kb_sweep23(const BatchMember* M, const lo
   const long* p = pre + (long)K * (n + 1);
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     const int m = batch_member(p, n, t);
-    const DevGrid& g = M[m].g;
-    const DevState& s = M[m].s;
+    const DevGrid g = M[m].g;                        // by value, as a member's own launch
+    const DevState s = M[m].s;                       // receives them
     const long u = t - p[m];
     const int nbx = (g.lx1 + 63) / 64;
     SweepFields F{};
@@ -879,8 +885,8 @@ __global__ __launch_bounds__(448) void kb_sweep1(const BatchMember* M, const lon
   const long* p = pre + (long)K * (n + 1);
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     const int m = batch_member(p, n, t);
-    const DevGrid& g = M[m].g;
-    const DevState& s = M[m].s;
+    const DevGrid g = M[m].g;                        // by value, as a member's own launch
+    const DevState s = M[m].s;                       // receives them
     const long u = t - p[m];
     SweepFields F{};
     if (ELEC) {
@@ -895,18 +901,16 @@ __global__ __launch_bounds__(448) void kb_sweep1(const BatchMember* M, const lon
 
 }  // namespace
 
+// a member's units of a kernel: threads (padded to whole waves; the x2 and x3 fills per
+// field, the field being blockIdx.y) or blocks
 long batch_units(const DevGrid& g, int kernel) {
   switch (kernel) {
-    case BK_BC1: return (long)g.L2 * g.L3 * LSP;
-    case BK_BC2: return (long)g.L1 * g.L3 * LSP * 4;
-    case BK_BC3: return (long)g.L1 * g.L2 * LSP * 4;
-    case BK_V1B: return (long)g.lx2 * g.lx3 * LSP;
+    case BK_BC1: case BK_BC2: case BK_BC3: case BK_V1B: case BK_POST: case BK_FLOOR:
+      return (batch_thread_units(g, kernel) + 63) / 64 * 64;
     case BK_SW3: return (long)((g.lx1 + 63) / 64) * g.lx2;
     case BK_SW1: case BK_SW1E: return (long)g.lx2 * g.lx3;
     case BK_SW2: return (long)((g.lx1 + 63) / 64) * g.lx3;
-    case BK_POST: return surface_cells(g);
     case BK_DIFF: return (((long)g.lx2 * g.lx3 + 63) / 64 + 7) / 8 * 8 * LSP;
-    case BK_FLOOR: return surface_cells(g) * LSP;
   }
   return 0;
 }
@@ -922,10 +926,10 @@ void launch_batch_advection(const BatchTable& t, int sweep_pf, hipStream_t st) {
   };
   hipLaunchKernelGGL(kb_threads<BK_BC1>, dim3(th(t.total[BK_BC1])), thr, 0, st, t.m, t.pre, t.n,
                      t.total[BK_BC1]);
-  hipLaunchKernelGGL(kb_threads<BK_BC2>, dim3(th(t.total[BK_BC2])), thr, 0, st, t.m, t.pre, t.n,
-                     t.total[BK_BC2]);
-  hipLaunchKernelGGL(kb_threads<BK_BC3>, dim3(th(t.total[BK_BC3])), thr, 0, st, t.m, t.pre, t.n,
-                     t.total[BK_BC3]);
+  hipLaunchKernelGGL(kb_threads<BK_BC2>, dim3(th(t.total[BK_BC2]), 4), thr, 0, st, t.m, t.pre,
+                     t.n, t.total[BK_BC2]);
+  hipLaunchKernelGGL(kb_threads<BK_BC3>, dim3(th(t.total[BK_BC3]), 4), thr, 0, st, t.m, t.pre,
+                     t.n, t.total[BK_BC3]);
   hipLaunchKernelGGL(kb_threads<BK_V1B>, dim3(th(t.total[BK_V1B])), thr, 0, st, t.m, t.pre, t.n,
                      t.total[BK_V1B]);
   if (sweep_pf == 1)

@@ -36,6 +36,7 @@ struct Coef {
 struct Elim {
   double r0, r1, rb;
   bool singular;
+  int word, bits;              // interchange bits of steps 32*word .. 32*word+31
 };
 
 // ---------------------------------------------------------------- column-group kernel
@@ -87,11 +88,17 @@ enum : int {
   SV0 = 5, SV1 = 6, SV2 = 7, SL = 8,   // BDF2 pivot rows and multipliers
   NSLOT = 9                            // (the T part of the BDF2 rhs is formed in pass 3
 };                                     // from Ts, which pass 4 is the first to overwrite)
+// The third entry of a pivot row (U2, V2) is the fill-in of an interchange and zero
+// otherwise: it is stored only for steps that interchanged, and the back substitutions
+// read it only for rows whose interchange bit is set (the other lanes of the load read
+// one shared address, so a row without interchanges costs no memory traffic for it).
 
 struct Group {
   int N;
   double* scr;                 // this lane's scratch: slot k of row i at scr[((i-1)*NSLOT+k)*64]
   int* swb;                    // BDF2 row interchanges, bit (j&31) of word j>>5 at swb[(j>>5)*64]
+  int* swt;                    // the first solve's row interchanges, laid out as swb
+  const double* shared;        // the address the lanes without a fill-in entry load
 };
 
 __device__ __forceinline__ double& S2(const Group& q, int i, int k) {
@@ -124,11 +131,17 @@ __device__ __forceinline__ bool elim_rhs(const Group& q, Elim& e, int j, int sj,
   } else {
     sing = true;
   }
-  S2(q, sj, SU0) = p0; S2(q, sj, SU1) = p1; S2(q, sj, SU2) = p2; S2(q, sj, SY) = pd;
+  S2(q, sj, SU0) = p0; S2(q, sj, SU1) = p1; S2(q, sj, SY) = pd;
+  if (swap) S2(q, sj, SU2) = p2;       // a step that is not real may leave one here: its
+                                       // row's bit (set by the real step) decides the read
   if (real) {
     e.singular = e.singular || sing;
     e.r0 = o1; e.r1 = o2; e.rb = od;
+    const int w = j >> 5;
+    e.bits = (w == e.word ? e.bits : 0) | ((swap ? 1 : 0) << (j & 31));
+    e.word = w;
   }
+  q.swt[e.word * 64] = e.bits;
   return swap;
 }
 
@@ -156,7 +169,8 @@ __device__ __forceinline__ void elim_mat(const Group& q, Fact& f, int j, int sj,
   } else {
     sing = true;
   }
-  S2(q, sj, SV0) = p0; S2(q, sj, SV1) = p1; S2(q, sj, SV2) = p2; S2(q, sj, SL) = l;
+  S2(q, sj, SV0) = p0; S2(q, sj, SV1) = p1; S2(q, sj, SL) = l;
+  if (swap) S2(q, sj, SV2) = p2;
   if (real) {
     f.singular = f.singular || sing;
     f.r0 = o1; f.r1 = o2;
@@ -184,7 +198,10 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   Group q;
   q.N = N;
   q.scr = scratch + grp * (long)N * NSLOT * 64 + lane;
-  q.swb = swbits + grp * (long)((N + 31) / 32) * 64 + lane;
+  const long nwl = (long)((N + 31) / 32) * 64;       // interchange words of a group
+  q.swb = swbits + grp * nwl + lane;
+  q.swt = swbits + (LSP * gps + grp) * nwl + lane;
+  q.shared = scratch + grp * (long)N * NSLOT * 64;   // lane 0's first slot
   const long so = (long)(isp - 1) * g.N;
   const long ni = (long)g.lx1 * g.lx2 * g.lx3;
   const long kb = (long)g.lx1 * col;
@@ -286,16 +303,20 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     }
   };
 
-  // dtbsv over stored pivot rows (slots s0 s1 s2, rhs sy), column-oriented: row i takes
-  // x(i+2)*U2 off first, then x(i+1)*U1, and divides by U0; a zero x(j) contributes
-  // nothing.  Rows are taken in the aligned chunks [kR+1, kR+R] from the top; each
-  // chunk's results go to `put`, then `flush(k)` runs.
-  auto backsub = [&](int s0, int s1, int s2, int sy, bool singular, auto put, auto flush) {
+  // dtbsv over stored pivot rows (slots s0 s1 s2, rhs sy; interchange bits sw),
+  // column-oriented: row i takes x(i+2)*U2 off first, then x(i+1)*U1, and divides by U0;
+  // a zero x(j) contributes nothing.  Rows are taken in the aligned chunks [kR+1, kR+R]
+  // from the top; each chunk's results go to `put`, then `flush(k)` runs.
+  auto backsub = [&](int s0, int s1, int s2, int sy, const int* sw, bool singular, auto put,
+                     auto flush) {
     double pu0[R], pu1[R], pu2[R], py[R];
     const int kt = (N - 1) / R;                      // top chunk
     auto fetch = [&](int i, int u) {
       const int ic = clampi(i, 1, N);
-      pu0[u] = S2(q, ic, s0); pu1[u] = S2(q, ic, s1); pu2[u] = S2(q, ic, s2); py[u] = S2(q, ic, sy);
+      const int jb = min(ic, N - 1);                 // row N has no interchange step
+      const bool fill = ic < N && ((sw[(jb >> 5) * 64] >> (jb & 31)) & 1);
+      const double v2 = *(fill ? &S2(q, ic, s2) : q.shared);
+      pu0[u] = S2(q, ic, s0); pu1[u] = S2(q, ic, s1); pu2[u] = fill ? v2 : 0.0; py[u] = S2(q, ic, sy);
     };
 #pragma unroll
     for (int u = 0; u < R; ++u) fetch(kt * R + 1 + u, u);     // row i lives in slot (i-1) % R
@@ -344,7 +365,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 
   // the BDF2 factorisation's singular flag, kept between the launches of a split solve
   int* mysing = lst + (((long)(isp - 1) * gps + cg) * 64 + lane);
-  Elim e{1.0, 0.0, Tmin, false};
+  Elim e{1.0, 0.0, Tmin, false, 0, 0};
   Fact f{1.0, 0.0, false, 0, 0};
   if (PH == 2) {
     f.singular = *mysing != 0;
@@ -377,11 +398,11 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     if (EXT && ELEC && prec) record_tr(N - 1, sw);
   }
   if (e.r0 == 0.0) e.singular = true;
-  S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SU2) = 0.0; S2(q, N, SY) = e.rb;
+  S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SY) = e.rb;
   if (!BE) {
     elim_mat(q, f, N - 1, N - 1, true, aN, bN, 0.0);
     if (f.r0 == 0.0) f.singular = true;
-    S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0; S2(q, N, SV2) = 0.0;
+    S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0;
     if (EXT && ELEC && prec)                         // the BDF2 interchanges, as stored
       for (int w = 0; w < nw; ++w) prec[nw + w] = q.swb[w * 64];
   }
@@ -403,11 +424,11 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   // pass 2
   __builtin_amdgcn_wave_barrier();
   if (BE) {
-    backsub(SU0, SU1, SU2, SY, e.singular, put_out, flush_out);
+    backsub(SU0, SU1, SU2, SY, q.swt, e.singular, put_out, flush_out);
     return;
   }
-  backsub(SU0, SU1, SU2, SY, e.singular, [&](int i, int u, double x) { S2(q, i, STR) = x; },
-          [&](int) {});
+  backsub(SU0, SU1, SU2, SY, q.swt, e.singular,
+          [&](int i, int u, double x) { S2(q, i, STR) = x; }, [&](int) {});
 
   if (PH == 1) {
     *mysing = f.singular ? 1 : 0;
@@ -456,7 +477,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   }
 
   // pass 4
-  backsub(SV0, SV1, SV2, SY, f.singular, put_out, flush_out);
+  backsub(SV0, SV1, SV2, SY, q.swb, f.singular, put_out, flush_out);
 }
 
 // PH 0: the whole solve; for a solve split in two launches, PH 1 runs passes 1-2 and
@@ -493,8 +514,8 @@ __host__ __device__ inline long diffusion_groups(const DevGrid& g) {
 __host__ __device__ inline size_t factor_bytes(const DevGrid& g) {
   return (size_t)diffusion_groups(g) * (size_t)g.lx1 * NSLOT * 64 * sizeof(double);
 }
-__host__ __device__ inline size_t swap_bytes(const DevGrid& g) {
-  return (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
+__host__ __device__ inline size_t swap_bytes(const DevGrid& g) {   // BDF2 words, then TR
+  return 2 * (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
 }
 
 // The TRBDF2 solve of every member of a batched step: a persistent grid of one-wave blocks
@@ -502,7 +523,7 @@ __host__ __device__ inline size_t swap_bytes(const DevGrid& g) {
 // them (every member's count is a multiple of 8, so a group keeps its XCD).  The
 // coefficients are formed from each member's dt as launch_energy_diffusion forms them.
 template <int R>
-__global__ __launch_bounds__(64) void kb_diffusion(const BatchMember* M, const long* pre, int n,
+__global__ __launch_bounds__(64, 2) void kb_diffusion(const BatchMember* M, const long* pre, int n,
                                                    long total) {
   __shared__ double sT[R][LP], sC[R][LP], sB[R][LP], sD[R][LP];
   const long* p = pre + (long)BK_DIFF * (n + 1);

@@ -123,8 +123,118 @@ class Ensemble:
     def pull(self, i: int, st):
         self.trs[i].pull(st)
 
+    def batched(self) -> "Batch":
+        """The members advanced by batched kernels (see :class:`Batch`)."""
+        return Batch(self)
+
     def close(self):
         self._release_graphs()
         for tr in self.trs:
             tr.close()
         self.trs = []
+
+
+class Batch:
+    """An ensemble's members advanced by batched kernels: each kernel of the fused
+    transport step runs once for all members, a persistent grid walking a table of every
+    member's blocks, instead of once per member.  Members must take the fused step's path
+    (global sides, x3 not periodic, lx2 and lx3 above 1) with TRBDF2 diffusion and no
+    fitting extension.  Results are bit-identical to the members' plain steps.
+
+    ``step(n)`` issues n steps on the batch's stream; ``run(n)`` launches n steps captured
+    in one HIP graph (cached per n); the knobs ``sweep_pf``, ``diff_chunk`` (kernel
+    variants) and ``grid`` (persistent blocks per CU) change speed only."""
+
+    KNOBS = {"sweep_pf": (1, 2), "diff_chunk": (4, 8), "grid": (1, 2)}
+
+    def __init__(self, ensemble: Ensemble):
+        self.ens = ensemble
+        self.lib = ensemble.lib
+        ensemble.trs[0]._sel()
+        self.id = self.lib.gemhip_batch_create(len(ensemble.trs), ensemble._handles())
+        if self.id <= 0:
+            raise GemhipError("batch creation failed: " + self.lib.gemhip_last_error().decode())
+        self._graphs = {}
+        self.tuning = {"sweep_pf": 2, "diff_chunk": 8, "grid": 1}
+        self.set_dt(ensemble.dts)
+
+    def _sel(self):
+        self.ens.trs[0]._sel()          # errors are reported in the first member's context
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise GemhipError(f"{what} failed: " + self.lib.gemhip_last_error().decode())
+
+    def set_dt(self, dts):
+        """Per-member time steps; captured graphs read them from HBM at every launch."""
+        d = np.ascontiguousarray(np.asarray(dts, dtype=np.float64))
+        if d.shape != (len(self.ens),):
+            raise ValueError("one time step per member")
+        self._sel()
+        self._check(self.lib.gemhip_batch_set_dt(self.id, d.ctypes.data_as(C.POINTER(C.c_double))),
+                    "batch dt")
+        self.ens.dts = d.copy()
+
+    def set_tuning(self, knob: str, value: int):
+        self._sel()
+        self._check(self.lib.gemhip_batch_set_tuning(self.id, knob.encode(), int(value)),
+                    "batch tuning")
+        if self.tuning.get(knob) != value:
+            self._release_graphs()           # graphs hold the variants they were captured with
+        self.tuning[knob] = value
+
+    def step(self, nsteps: int = 1, wait: bool = True):
+        self._sel()
+        for _ in range(nsteps):
+            self._check(self.lib.gemhip_batch_step(self.id), "batch step")
+        if wait:
+            self.synchronize()
+
+    def capture(self, nsteps: int) -> int:
+        if nsteps not in self._graphs:
+            self._sel()
+            gid = self.lib.gemhip_batch_capture(self.id, int(nsteps))
+            if gid <= 0:
+                raise GemhipError("batch capture failed: " + self.lib.gemhip_last_error().decode())
+            self._graphs[nsteps] = gid
+        return self._graphs[nsteps]
+
+    def run(self, nsteps: int, wait: bool = True):
+        gid = self.capture(nsteps)
+        self._sel()
+        self._check(self.lib.gemhip_ensemble_launch(gid, int(wait)), "batch graph launch")
+
+    def synchronize(self):
+        self._sel()
+        self._check(self.lib.gemhip_batch_synchronize(self.id), "batch synchronize")
+
+    def autotune(self, restore, steps: int = 2):
+        """Time every combination of KNOBS over `steps` batched steps, keep the fastest,
+        then call ``restore()`` to put the members' states back (timing advances them)."""
+        import itertools
+        import time
+        times = {}
+        for combo in itertools.product(*self.KNOBS.values()):
+            for k, v in zip(self.KNOBS, combo):
+                self.set_tuning(k, v)
+            self.step(1)                                   # warm-up of this variant
+            t0 = time.perf_counter()
+            self.step(steps)
+            times[combo] = (time.perf_counter() - t0) / steps
+        best = min(times, key=times.get)
+        for k, v in zip(self.KNOBS, best):
+            self.set_tuning(k, v)
+        restore()
+        return dict(zip(self.KNOBS, best)), times
+
+    def _release_graphs(self):
+        for gid in self._graphs.values():
+            self.lib.gemhip_ensemble_release(gid)
+        self._graphs.clear()
+
+    def close(self):
+        if self.id > 0:
+            self._sel()
+            self._release_graphs()
+            self.lib.gemhip_batch_release(self.id)
+            self.id = 0

@@ -67,6 +67,11 @@ def load_library(path: str = LIB_PATH):
     lib.gemhip_adjoint_record_step.argtypes = [C.c_double, C.c_void_p]
     lib.gemhip_adjoint_step.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
+    for name in ("gemhip_batch_create", "gemhip_batch_set_dt", "gemhip_batch_set_tuning",
+                 "gemhip_batch_step", "gemhip_batch_synchronize", "gemhip_batch_capture",
+                 "gemhip_batch_release"):
+        getattr(lib, name).restype = C.c_int
+    lib.gemhip_batch_set_tuning.argtypes = [C.c_int, C.c_char_p, C.c_int]
     lib.gemhip_set_tuning.argtypes = [C.c_char_p, C.c_int]
     lib.gemhip_get_tuning.argtypes = [C.c_char_p]
     lib.gemhip_last_error.restype = C.c_char_p

@@ -349,11 +349,33 @@ int gemhip_adjoint_step(double dt, const double* tape, double* lam, double* gkap
    fluidvars(:,:,:,29:35) and (:,:,:,1:7), J1 one ghosted field -- with the grid,
    neutrals and configuration of the selected context.  The TRBDF2 work space is a static
    device array of 256 MiB (one per GPU, held until the process ends); grids that need
-   more (7 * ceil(lx2*lx3/64) * (4608*lx1 + 256*ceil(lx1/32) + 256) bytes, about 1000
+   more (7 * ceil(lx2*lx3/64) * (4608*lx1 + 512*ceil(lx1/32) + 256) bytes, about 1000
    columns at lx1 = 512) are rejected with an error.  Calls no allocator; kernel timing
    must be off.  Enqueued on the context's stream (gemhip_synchronize waits).  Calls that
    overlap on one GPU share the work space and race.  Returns 0 or an error code. */
 int gemhip_energy_diffusion_noalloc(double* Ts, const double* ns, const double* J1, double dt);
+
+/* ---------------- part 6: batched ensembles (extension, no counterpart in the reference) ---------------- */
+/* The transport step of many members (contexts in residency mode 1, state pushed) run by
+   batched kernels: each kernel of the step is one persistent launch walking a table of
+   every member's blocks.  Members must have global sides, a non-periodic x3, lx2 and
+   lx3 above 1, TRBDF2 diffusion without the fitting extension, and one device.  Results
+   are bit-identical to gemhip_transport_step on each member.  Steps run on the batch's
+   stream: synchronize it (or launch graphs with wait = 1) before touching a member
+   through its context, and do not reallocate a member's state (residency, grid) while
+   the batch exists; members of a batch cannot be destroyed.  The time steps live in HBM,
+   so gemhip_batch_set_dt also changes what captured graphs do.  Tuning knobs:
+   "sweep_pf" (1, 2), "diff_chunk" (4, 8), "grid" (1, 2: persistent blocks per CU times
+   this); they change speed only.  gemhip_batch_capture returns a graph id for
+   gemhip_ensemble_launch (launched on the batch's stream), released with the batch at
+   the latest. */
+int gemhip_batch_create(int n, const int* handles);      /* batch id >= 1, or < 0 */
+int gemhip_batch_set_dt(int batch, const double* dt);    /* one per member */
+int gemhip_batch_set_tuning(int batch, const char* knob, int value);
+int gemhip_batch_step(int batch);
+int gemhip_batch_synchronize(int batch);
+int gemhip_batch_capture(int batch, int nsteps);         /* graph id >= 1, or < 0 */
+int gemhip_batch_release(int batch);
 
 #ifdef __cplusplus
 }

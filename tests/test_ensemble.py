@@ -9,7 +9,7 @@ import pytest
 
 import gemini_amd as ga
 from gemini_amd.ensemble import Ensemble
-from gemini_amd.transport import Transport
+from gemini_amd.transport import Transport, GemhipError
 import oracle
 from helpers import compare
 
@@ -115,4 +115,101 @@ def test_members_of_a_live_graph_cannot_be_destroyed():
         assert ens.lib.gemhip_ensemble_launch(gid, 1) != 0      # released with the old variants
         ens.run(1)                                               # captured anew
     finally:
+        ens.close()
+
+
+BATCH_SHAPES = [(16, 6, 4), (24, 5, 3), (40, 4, 4), (16, 9, 2), (32, 6, 5), (64, 3, 7),
+                (96, 17, 9)]
+
+
+def batch_members(seed0=60):
+    out = []
+    for k, shape in enumerate(BATCH_SHAPES):
+        g = ga.cartesian_grid(*shape)
+        st = ga.synthetic_state(g, seed=seed0 + k, perturb=True)
+        out.append((g, st, ga.cfl_dt(g, st)))
+    return out
+
+
+def eager_with_dts(ms, factors):
+    """Each member stepped alone through the implicit context, step k with dt * factors[k]."""
+    hist = []
+    for g, st, dt in ms:
+        tr = Transport(g, st.nn, st.Tn)
+        tr.set_residency(1)
+        tr.push(st)
+        per = []
+        for f in factors:
+            tr.transport_step(dt * f)
+            out = st.copy()
+            tr.pull(out)
+            per.append(out)
+        tr.set_residency(0)
+        hist.append(per)
+    return hist
+
+
+def _check_members(ens, ms, ref, step, tag):
+    for i, (g, st, dt) in enumerate(ms):
+        got = st.copy()
+        ens.pull(i, got)
+        print(f"  {tag}: member {i} {g.shape}", flush=True)
+        compare(got, ref[i][step], exact=ALL)
+
+
+def test_batched_kernels_match_plain_steps():
+    """Batched kernels (one launch per kernel for all members) against each member's plain
+    steps, bit for bit: an eager batched step, then graph replays under other kernel
+    variants and persistent grid sizes, and a replay after the time steps changed in HBM."""
+    ms = batch_members()
+    print(f"batch of {len(ms)} members", flush=True)
+    ref = eager_with_dts(ms, [1.0, 1.0, 1.0, 0.5])
+    ens = Ensemble(ms)
+    b = ens.batched()
+    try:
+        b.step(1)
+        _check_members(ens, ms, ref, 0, "eager batched step")
+        b.set_tuning("sweep_pf", 1)
+        b.set_tuning("diff_chunk", 4)
+        b.set_tuning("grid", 2)
+        b.run(1)
+        _check_members(ens, ms, ref, 1, "graph, sweep_pf 1, diff_chunk 4, grid 2")
+        b.run(1)                                       # the same graph again
+        _check_members(ens, ms, ref, 2, "graph replay")
+        b.set_dt(np.array([dt * 0.5 for _, _, dt in ms]))
+        b.run(1)                                       # captured before: reads dt from HBM
+        _check_members(ens, ms, ref, 3, "graph replay at half the time step")
+    finally:
+        b.close()
+        ens.close()
+
+
+def test_batch_refuses_members_off_the_fused_path_and_holds_its_members():
+    ms = batch_members(seed0=80)[:2]
+    g = ga.cartesian_grid(24, 7, 1)                   # lx3 = 1: not on the fused path
+    st = ga.synthetic_state(g, seed=3)
+    ens = Ensemble(ms + [(g, st, ga.cfl_dt(g, st))])
+    try:
+        with pytest.raises(GemhipError, match="lx2, lx3 > 1"):
+            ens.batched()
+        ens.lib.gemhip_clear_error()
+    finally:
+        ens.close()
+    ens = Ensemble(ms)
+    b = ens.batched()
+    try:
+        b.capture(2)
+        assert ens.lib.gemhip_context_destroy(int(ens.trs[1].handle)) == 2
+        assert b"graph" in ens.lib.gemhip_last_error()     # the graph holds the member
+        ens.lib.gemhip_clear_error()
+        b._release_graphs()
+        assert ens.lib.gemhip_context_destroy(int(ens.trs[1].handle)) == 2
+        assert b"batch" in ens.lib.gemhip_last_error()     # and so does the batch
+        ens.lib.gemhip_clear_error()
+        gid = b.capture(1)
+        b.close()                                          # releases its graphs too
+        assert ens.lib.gemhip_ensemble_launch(gid, 1) != 0
+        ens.lib.gemhip_clear_error()
+    finally:
+        b.close()
         ens.close()
