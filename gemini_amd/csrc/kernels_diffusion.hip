@@ -36,7 +36,6 @@ struct Coef {
 struct Elim {
   double r0, r1, rb;
   bool singular;
-  int word, bits;              // interchange bits of steps 32*word .. 32*word+31
 };
 
 // ---------------------------------------------------------------- column-group kernel
@@ -88,17 +87,11 @@ enum : int {
   SV0 = 5, SV1 = 6, SV2 = 7, SL = 8,   // BDF2 pivot rows and multipliers
   NSLOT = 9                            // (the T part of the BDF2 rhs is formed in pass 3
 };                                     // from Ts, which pass 4 is the first to overwrite)
-// The third entry of a pivot row (U2, V2) is the fill-in of an interchange and zero
-// otherwise: it is stored only for steps that interchanged, and the back substitutions
-// read it only for rows whose interchange bit is set (the other lanes of the load read
-// one shared address, so a row without interchanges costs no memory traffic for it).
 
 struct Group {
   int N;
   double* scr;                 // this lane's scratch: slot k of row i at scr[((i-1)*NSLOT+k)*64]
   int* swb;                    // BDF2 row interchanges, bit (j&31) of word j>>5 at swb[(j>>5)*64]
-  int* swt;                    // the first solve's row interchanges, laid out as swb
-  const double* shared;        // the address the lanes without a fill-in entry load
 };
 
 __device__ __forceinline__ double& S2(const Group& q, int i, int k) {
@@ -131,17 +124,11 @@ __device__ __forceinline__ bool elim_rhs(const Group& q, Elim& e, int j, int sj,
   } else {
     sing = true;
   }
-  S2(q, sj, SU0) = p0; S2(q, sj, SU1) = p1; S2(q, sj, SY) = pd;
-  if (swap) S2(q, sj, SU2) = p2;       // a step that is not real may leave one here: its
-                                       // row's bit (set by the real step) decides the read
+  S2(q, sj, SU0) = p0; S2(q, sj, SU1) = p1; S2(q, sj, SU2) = p2; S2(q, sj, SY) = pd;
   if (real) {
     e.singular = e.singular || sing;
     e.r0 = o1; e.r1 = o2; e.rb = od;
-    const int w = j >> 5;
-    e.bits = (w == e.word ? e.bits : 0) | ((swap ? 1 : 0) << (j & 31));
-    e.word = w;
   }
-  q.swt[e.word * 64] = e.bits;
   return swap;
 }
 
@@ -169,8 +156,7 @@ __device__ __forceinline__ void elim_mat(const Group& q, Fact& f, int j, int sj,
   } else {
     sing = true;
   }
-  S2(q, sj, SV0) = p0; S2(q, sj, SV1) = p1; S2(q, sj, SL) = l;
-  if (swap) S2(q, sj, SV2) = p2;
+  S2(q, sj, SV0) = p0; S2(q, sj, SV1) = p1; S2(q, sj, SV2) = p2; S2(q, sj, SL) = l;
   if (real) {
     f.singular = f.singular || sing;
     f.r0 = o1; f.r1 = o2;
@@ -198,10 +184,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   Group q;
   q.N = N;
   q.scr = scratch + grp * (long)N * NSLOT * 64 + lane;
-  const long nwl = (long)((N + 31) / 32) * 64;       // interchange words of a group
-  q.swb = swbits + grp * nwl + lane;
-  q.swt = swbits + (LSP * gps + grp) * nwl + lane;
-  q.shared = scratch + grp * (long)N * NSLOT * 64;   // lane 0's first slot
+  q.swb = swbits + grp * (long)((N + 31) / 32) * 64 + lane;
   const long so = (long)(isp - 1) * g.N;
   const long ni = (long)g.lx1 * g.lx2 * g.lx3;
   const long kb = (long)g.lx1 * col;
@@ -303,20 +286,16 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     }
   };
 
-  // dtbsv over stored pivot rows (slots s0 s1 s2, rhs sy; interchange bits sw),
-  // column-oriented: row i takes x(i+2)*U2 off first, then x(i+1)*U1, and divides by U0;
-  // a zero x(j) contributes nothing.  Rows are taken in the aligned chunks [kR+1, kR+R]
-  // from the top; each chunk's results go to `put`, then `flush(k)` runs.
-  auto backsub = [&](int s0, int s1, int s2, int sy, const int* sw, bool singular, auto put,
-                     auto flush) {
+  // dtbsv over stored pivot rows (slots s0 s1 s2, rhs sy), column-oriented: row i takes
+  // x(i+2)*U2 off first, then x(i+1)*U1, and divides by U0; a zero x(j) contributes
+  // nothing.  Rows are taken in the aligned chunks [kR+1, kR+R] from the top; each
+  // chunk's results go to `put`, then `flush(k)` runs.
+  auto backsub = [&](int s0, int s1, int s2, int sy, bool singular, auto put, auto flush) {
     double pu0[R], pu1[R], pu2[R], py[R];
     const int kt = (N - 1) / R;                      // top chunk
     auto fetch = [&](int i, int u) {
       const int ic = clampi(i, 1, N);
-      const int jb = min(ic, N - 1);                 // row N has no interchange step
-      const bool fill = ic < N && ((sw[(jb >> 5) * 64] >> (jb & 31)) & 1);
-      const double v2 = *(fill ? &S2(q, ic, s2) : q.shared);
-      pu0[u] = S2(q, ic, s0); pu1[u] = S2(q, ic, s1); pu2[u] = fill ? v2 : 0.0; py[u] = S2(q, ic, sy);
+      pu0[u] = S2(q, ic, s0); pu1[u] = S2(q, ic, s1); pu2[u] = S2(q, ic, s2); py[u] = S2(q, ic, sy);
     };
 #pragma unroll
     for (int u = 0; u < R; ++u) fetch(kt * R + 1 + u, u);     // row i lives in slot (i-1) % R
@@ -365,7 +344,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 
   // the BDF2 factorisation's singular flag, kept between the launches of a split solve
   int* mysing = lst + (((long)(isp - 1) * gps + cg) * 64 + lane);
-  Elim e{1.0, 0.0, Tmin, false, 0, 0};
+  Elim e{1.0, 0.0, Tmin, false};
   Fact f{1.0, 0.0, false, 0, 0};
   if (PH == 2) {
     f.singular = *mysing != 0;
@@ -398,11 +377,11 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     if (EXT && ELEC && prec) record_tr(N - 1, sw);
   }
   if (e.r0 == 0.0) e.singular = true;
-  S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SY) = e.rb;
+  S2(q, N, SU0) = e.r0; S2(q, N, SU1) = 0.0; S2(q, N, SU2) = 0.0; S2(q, N, SY) = e.rb;
   if (!BE) {
     elim_mat(q, f, N - 1, N - 1, true, aN, bN, 0.0);
     if (f.r0 == 0.0) f.singular = true;
-    S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0;
+    S2(q, N, SV0) = f.r0; S2(q, N, SV1) = 0.0; S2(q, N, SV2) = 0.0;
     if (EXT && ELEC && prec)                         // the BDF2 interchanges, as stored
       for (int w = 0; w < nw; ++w) prec[nw + w] = q.swb[w * 64];
   }
@@ -424,11 +403,11 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   // pass 2
   __builtin_amdgcn_wave_barrier();
   if (BE) {
-    backsub(SU0, SU1, SU2, SY, q.swt, e.singular, put_out, flush_out);
+    backsub(SU0, SU1, SU2, SY, e.singular, put_out, flush_out);
     return;
   }
-  backsub(SU0, SU1, SU2, SY, q.swt, e.singular,
-          [&](int i, int u, double x) { S2(q, i, STR) = x; }, [&](int) {});
+  backsub(SU0, SU1, SU2, SY, e.singular, [&](int i, int u, double x) { S2(q, i, STR) = x; },
+          [&](int) {});
 
   if (PH == 1) {
     *mysing = f.singular ? 1 : 0;
@@ -439,15 +418,31 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   // pass 3: d(r) = AT(r) + ((4/3) TR(r))/(dt/3) for rows 2..N-1 (Tmin, Tmax at the ends),
   // with AT(r) = -((1/3) T(r))/(dt/3) from the input temperature (its column of Ts, which
   // only pass 4 overwrites), through the interchanges and multipliers of pass 1; singular
-  // lanes keep d itself
+  // lanes keep d itself.  The input temperatures come in R-row chunks loaded by the wave
+  // cooperatively (as pass 1 loads its inputs) into two alternating LDS buffers, the next
+  // chunk's loads in flight while a chunk is used.
   {
     constexpr int PF = R;
-    double pat[PF], ptr[PF], pl[PF];
+    double ptr[PF], pl[PF];
     int pb[PF];
-    const double* Tcol = s.Ts + so + gix(g, 0, col % g.lx2 + 1, col / g.lx2 + 1);
+    double tv[R];
+    auto issue_T = [&](int r0) {                     // rows r0..r0+R-1 (past N: row N)
+      const int rr = min(r0 + lr, N);
+#pragma unroll
+      for (int kk = 0; kk < R; ++kk) tv[kk] = s.Ts[so + cbk[kk] + rr];
+    };
+    auto store_T = [&](double (*buf)[LP]) {
+#pragma unroll
+      for (int kk = 0; kk < R; ++kk) buf[lr][kk * CPI + cs] = tv[kk];
+    };
+    __builtin_amdgcn_wave_barrier();
+    issue_T(2);
+    store_T(sT);
+    __builtin_amdgcn_wave_barrier();
+    issue_T(2 + R);
     auto fetch = [&](int r, int u) {                 // inputs of step j = r-1
       const int rc = clampi(r, 1, N), jc = clampi(r - 1, 1, N - 1);
-      pat[u] = Tcol[rc]; ptr[u] = S2(q, rc, STR); pl[u] = S2(q, jc, SL);
+      ptr[u] = S2(q, rc, STR); pl[u] = S2(q, jc, SL);
       pb[u] = q.swb[(jc >> 5) * 64];
     };
 #pragma unroll
@@ -455,11 +450,13 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     const bool fs = f.singular;
     S2(q, 1, SY) = Tmin;
     double rb = Tmin;
-    for (int j0 = 1; j0 <= N - 1; j0 += PF) {
+    int ch = 0;                                      // chunk of rows 2 + ch R ..
+    for (int j0 = 1; j0 <= N - 1; j0 += PF, ++ch) {
+      double (*tb)[LP] = (ch & 1) ? sC : sT;
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int j = j0 + u, r = j + 1;
-        const double at = -((bdf.c13 * pat[u]) / bdf.dt3), trr = ptr[u], l = pl[u];
+        const double at = -((bdf.c13 * tb[u][lane]) / bdf.dt3), trr = ptr[u], l = pl[u];
         const int bits = pb[u];
         fetch(r + PF, u);
         const bool live = j <= N - 1;
@@ -472,12 +469,16 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
         S2(q, live ? (fs ? r : j) : N, SY) = fs ? d : pd;
         rb = live ? od : rb;
       }
+      __builtin_amdgcn_wave_barrier();
+      store_T((ch & 1) ? sT : sC);                   // chunk ch + 1
+      __builtin_amdgcn_wave_barrier();
+      issue_T(2 + (ch + 2) * R);
     }
     S2(q, N, SY) = fs ? Tmax : rb;
   }
 
   // pass 4
-  backsub(SV0, SV1, SV2, SY, q.swb, f.singular, put_out, flush_out);
+  backsub(SV0, SV1, SV2, SY, f.singular, put_out, flush_out);
 }
 
 // PH 0: the whole solve; for a solve split in two launches, PH 1 runs passes 1-2 and
@@ -514,8 +515,8 @@ __host__ __device__ inline long diffusion_groups(const DevGrid& g) {
 __host__ __device__ inline size_t factor_bytes(const DevGrid& g) {
   return (size_t)diffusion_groups(g) * (size_t)g.lx1 * NSLOT * 64 * sizeof(double);
 }
-__host__ __device__ inline size_t swap_bytes(const DevGrid& g) {   // BDF2 words, then TR
-  return 2 * (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
+__host__ __device__ inline size_t swap_bytes(const DevGrid& g) {
+  return (size_t)diffusion_groups(g) * ((g.lx1 + 31) / 32) * 64 * sizeof(int);
 }
 
 // The TRBDF2 solve of every member of a batched step: a persistent grid of one-wave blocks

@@ -349,7 +349,7 @@ int gemhip_adjoint_step(double dt, const double* tape, double* lam, double* gkap
    fluidvars(:,:,:,29:35) and (:,:,:,1:7), J1 one ghosted field -- with the grid,
    neutrals and configuration of the selected context.  The TRBDF2 work space is a static
    device array of 256 MiB (one per GPU, held until the process ends); grids that need
-   more (7 * ceil(lx2*lx3/64) * (4608*lx1 + 512*ceil(lx1/32) + 256) bytes, about 1000
+   more (7 * ceil(lx2*lx3/64) * (4608*lx1 + 256*ceil(lx1/32) + 256) bytes, about 1000
    columns at lx1 = 512) are rejected with an error.  Calls no allocator; kernel timing
    must be off.  Enqueued on the context's stream (gemhip_synchronize waits).  Calls that
    overlap on one GPU share the work space and race.  Returns 0 or an error code. */
