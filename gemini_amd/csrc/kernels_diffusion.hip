@@ -97,6 +97,16 @@ struct Group {
 __device__ __forceinline__ double& S2(const Group& q, int i, int k) {
   return q.scr[((long)(i - 1) * NSLOT + k) * 64];
 }
+// Scratch traffic is non-temporal (the nt bit: streamed through L2 at low priority).  A
+// slot is written once and read once, a pass later; as ordinary accesses the ~256 waves
+// of an XCD streaming their pivot rows evicted the input lines that the cooperative loads
+// of the next row chunk still needed (those loads read 64-byte halves of 128-byte lines).
+__device__ __forceinline__ void put2(const Group& q, int i, int k, double v) {
+  __builtin_nontemporal_store(v, &S2(q, i, k));
+}
+__device__ __forceinline__ double get2(const Group& q, int i, int k) {
+  return __builtin_nontemporal_load(&S2(q, i, k));
+}
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -124,7 +134,7 @@ __device__ __forceinline__ bool elim_rhs(const Group& q, Elim& e, int j, int sj,
   } else {
     sing = true;
   }
-  S2(q, sj, SU0) = p0; S2(q, sj, SU1) = p1; S2(q, sj, SU2) = p2; S2(q, sj, SY) = pd;
+  put2(q, sj, SU0, p0); put2(q, sj, SU1, p1); put2(q, sj, SU2, p2); put2(q, sj, SY, pd);
   if (real) {
     e.singular = e.singular || sing;
     e.r0 = o1; e.r1 = o2; e.rb = od;
@@ -156,7 +166,7 @@ __device__ __forceinline__ void elim_mat(const Group& q, Fact& f, int j, int sj,
   } else {
     sing = true;
   }
-  S2(q, sj, SV0) = p0; S2(q, sj, SV1) = p1; S2(q, sj, SV2) = p2; S2(q, sj, SL) = l;
+  put2(q, sj, SV0, p0); put2(q, sj, SV1, p1); put2(q, sj, SV2, p2); put2(q, sj, SL, l);
   if (real) {
     f.singular = f.singular || sing;
     f.r0 = o1; f.r1 = o2;
@@ -295,7 +305,8 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     const int kt = (N - 1) / R;                      // top chunk
     auto fetch = [&](int i, int u) {
       const int ic = clampi(i, 1, N);
-      pu0[u] = S2(q, ic, s0); pu1[u] = S2(q, ic, s1); pu2[u] = S2(q, ic, s2); py[u] = S2(q, ic, sy);
+      pu0[u] = get2(q, ic, s0); pu1[u] = get2(q, ic, s1); pu2[u] = get2(q, ic, s2);
+      py[u] = get2(q, ic, sy);
     };
 #pragma unroll
     for (int u = 0; u < R; ++u) fetch(kt * R + 1 + u, u);     // row i lives in slot (i-1) % R
@@ -406,7 +417,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     backsub(SU0, SU1, SU2, SY, e.singular, put_out, flush_out);
     return;
   }
-  backsub(SU0, SU1, SU2, SY, e.singular, [&](int i, int u, double x) { S2(q, i, STR) = x; },
+  backsub(SU0, SU1, SU2, SY, e.singular, [&](int i, int u, double x) { put2(q, i, STR, x); },
           [&](int) {});
 
   if (PH == 1) {
@@ -442,7 +453,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     issue_T(2 + R);
     auto fetch = [&](int r, int u) {                 // inputs of step j = r-1
       const int rc = clampi(r, 1, N), jc = clampi(r - 1, 1, N - 1);
-      ptr[u] = S2(q, rc, STR); pl[u] = S2(q, jc, SL);
+      ptr[u] = get2(q, rc, STR); pl[u] = get2(q, jc, SL);
       pb[u] = q.swb[(jc >> 5) * 64];
     };
 #pragma unroll
@@ -466,7 +477,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
         double od = swap ? rb : d;
         if (!fs && pd != 0.0) od = od + l * (-pd);
         // steps past N-1 store on row N, which the store after the loop overwrites
-        S2(q, live ? (fs ? r : j) : N, SY) = fs ? d : pd;
+        put2(q, live ? (fs ? r : j) : N, SY, fs ? d : pd);
         rb = live ? od : rb;
       }
       __builtin_amdgcn_wave_barrier();
