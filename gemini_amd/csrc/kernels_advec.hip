@@ -566,11 +566,22 @@ template <int PF>
 constexpr int sweep_xch_doubles() { return 2 * PF * (LSP - 1) * 2 * 64; }
 
 // block (bx, by): i1 = 64 bx + lane + 1 (lanes past lx1 redo column lx1), line by + 1
+// The fused step's sweeps (CONV or POST: blocks of the 7 species waves) share the metric
+// factors of each step through LDS: waves 0-5 each load one of hI, hA, hB (interface k)
+// and h1, h2, h3 (cell k-1) two groups of steps ahead and store them one group ahead into
+// met[group parity][step][factor][lane]; every wave reads them there.
+#ifndef SWEEP_SHARE_METRICS
+#define SWEEP_SHARE_METRICS 1
+#endif
+template <int PF>
+constexpr int sweep_met_doubles() { return 2 * PF * 6 * 64; }
+
 template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = false>
 __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s,
                                              const SweepFields& F, int isp0, double dt, int bx,
-                                             int by, double* xch) {
+                                             int by, double* xch, double* met) {
   static_assert(!(CONV || POST) || NQ == 3, "the conversions come with all three fields");
+  constexpr bool SHM = SWEEP_SHARE_METRICS && (CONV || POST);
   const int lx = DIR == 2 ? g.lx2 : g.lx3;
   // lanes past lx1 redo column lx1's work: same inputs, same values written to the same cells
   const int i1 = min((int)(bx * 64 + threadIdx.x), g.lx1 - 1) + 1;
@@ -629,9 +640,38 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
     const int ki = clampi(k, 1, lx + 1);
     const long ik = if0 + (long)ki * istr;
     pv[u] = OTF ? w[(long)ki * cs_] : vi[ik];
-    pI[u] = hI[ik]; pA[u] = hA[ik]; pB[u] = hB[ik];
+    if (!SHM) {
+      pI[u] = hI[ik]; pA[u] = hA[ik]; pB[u] = hB[ik];
+      const long cm = c0 + (long)clampi(k - 1, 1, lx) * cs_;
+      p1[u] = g.h1[cm]; p2[u] = g.h2[cm]; p3[u] = g.h3[cm];
+    }
+  };
+  // SHM: this wave's metric factor (wave wy < 6 loads factor wy) at step k, and a group's
+  // PF steps into registers or from registers into LDS
+  const int wy = threadIdx.y;
+  auto met_at = [&](int k) -> double {
+    const int ki = clampi(k, 1, lx + 1);
+    const long ik = if0 + (long)ki * istr;
     const long cm = c0 + (long)clampi(k - 1, 1, lx) * cs_;
-    p1[u] = g.h1[cm]; p2[u] = g.h2[cm]; p3[u] = g.h3[cm];
+    if (wy == 0) return hI[ik];
+    if (wy == 1) return hA[ik];
+    if (wy == 2) return hB[ik];
+    if (wy == 3) return g.h1[cm];
+    if (wy == 4) return g.h2[cm];
+    return g.h3[cm];
+  };
+  auto met_load = [&](int G, double* r) {            // group G: steps 2 + G PF ..
+#pragma unroll
+    for (int u = 0; u < PF; ++u) r[u] = wy < 6 ? met_at(2 + G * PF + u) : 0.0;
+  };
+  auto met_store = [&](int G, const double* r) {
+    if (wy < 6) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) met[(((G & 1) * PF + u) * 6 + wy) * 64 + threadIdx.x] = r[u];
+    }
+  };
+  auto met_get = [&](int G, int u, int m) -> double {
+    return met[(((G & 1) * PF + u) * 6 + m) * 64 + threadIdx.x];
   };
   double fa[NQ], fb[NQ], fc[NQ], rsp[NQ], slp[NQ], php[NQ];
   double dxcur = dx[0];
@@ -678,11 +718,22 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
     }
   };
   // mode 0: step 0 (slopes only), 1: step 1 (flux, no store), 2: steps 2..lx+1
-  // slot: the step's position in its group (POST)
-  auto step = [&](int k, int u, int mode, int slot = 0) {
+  // slot: the step's position in its group; G: the group (SHM reads its metric factors)
+  auto step = [&](int k, int u, int mode, int slot = 0, int G = 0) {
     const double dxk = dxcur, dxk1 = dx[k + 1];
     dxcur = dxk1;
-    const double wv = pv[u], hIv = pI[u], a = pA[u], b = pB[u], h1 = p1[u], h2 = p2[u], h3 = p3[u];
+    const double wv = pv[u];
+    double hIv, a, b, h1, h2, h3;
+    if (!SHM) {
+      hIv = pI[u]; a = pA[u]; b = pB[u]; h1 = p1[u]; h2 = p2[u]; h3 = p3[u];
+    } else if (mode == 2) {
+      hIv = met_get(G, slot, 0); a = met_get(G, slot, 1); b = met_get(G, slot, 2);
+      h1 = met_get(G, slot, 3); h2 = met_get(G, slot, 4); h3 = met_get(G, slot, 5);
+    } else {                                   // step 1 (step 0 uses none)
+      const long ik = if0 + (long)clampi(k, 1, lx + 1) * istr;
+      hIv = hI[ik]; a = hA[ik]; b = hB[ik];
+      h1 = h2 = h3 = 1.0;
+    }
     double nxt[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) nxt[q] = pf[q][u];
@@ -750,13 +801,24 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   };
+  double rmA[PF], rmB[PF];                    // SHM: factors of two groups ahead
+  if (SHM) {
+    met_load(0, rmA);
+    met_store(0, rmA);
+    met_load(1, rmB);
+    lds_barrier();
+  }
   step(0, 0, 0);
   step(1, 1 % PF, 1);
   int kb = 2;
   int groups = 0;
-  for (; kb + PF - 1 <= lx + 1; kb += PF) {
+  // group G (steps kb..kb+PF-1): SHM loads the factors of group G+2 into rl and stores
+  // those of group G+1 from rw before the barrier
+  auto group = [&](double* rl, double* rw) {
+    if (SHM) met_load(groups + 2, rl);
 #pragma unroll
-    for (int u = 0; u < PF; ++u) step(kb + u, (2 + u) % PF, 2, u);
+    for (int u = 0; u < PF; ++u) step(kb + u, (2 + u) % PF, 2, u, groups);
+    if (SHM) met_store(groups + 1, rw);
     if (POST) {
       // the electron wave finishes the previous group (its LDS parity is not written in
       // this group), then the ion waves may write that parity again after the barrier
@@ -765,19 +827,27 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
       for (int u = 0; u < PF; ++u) eprev[u] = eout[u];
       lds_barrier();
       xbuf ^= 1;
-      ++groups;
+    } else if (SHM) {
+      lds_barrier();
     } else {
       // keep the species waves of the block in step so the metric factors they all read
       // are still cached when the slower waves reach them (no memory fence: nothing is
       // exchanged between the waves)
       __builtin_amdgcn_s_barrier();
     }
+    ++groups;
+    kb += PF;
+  };
+  while (kb + PF - 1 <= lx + 1) {              // two groups a turn: rmA and rmB alternate
+    group(rmA, rmB);
+    if (kb + PF - 1 > lx + 1) break;
+    group(rmB, rmA);
   }
   int tail = 0;
 #pragma unroll
   for (int u = 0; u < PF - 1; ++u) {
     if (kb + u > lx + 1) break;
-    step(kb + u, (2 + u) % PF, 2, u);
+    step(kb + u, (2 + u) % PF, 2, u, groups);
     ++tail;
   }
   if (POST) {
@@ -791,7 +861,9 @@ template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = fals
 __global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevState s,
                                                                SweepFields F, int isp0, double dt) {
   __shared__ double xch[POST ? sweep_xch_doubles<PF>() : 1];
-  sweep23_body<DIR, PF, NQ, OTF, CONV, POST>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y, xch);
+  __shared__ double met[(CONV || POST) ? sweep_met_doubles<PF>() : 1];
+  sweep23_body<DIR, PF, NQ, OTF, CONV, POST>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y, xch,
+                                             met);
 }
 
 // Ts ghost cells: floor at 100 K (the reference leaves them undefined, see the
@@ -858,6 +930,7 @@ template <int DIR, int PF>
 __global__ __launch_bounds__(448) void kb_sweep23(const BatchMember* M, const long* pre, int n,
                                                   long total) {
   __shared__ double xch[sweep_xch_doubles<PF>()];
+  __shared__ double met[sweep_met_doubles<PF>()];
   constexpr int K = DIR == 3 ? BK_SW3 : BK_SW2;
   const long* p = pre + (long)K * (n + 1);
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
@@ -872,7 +945,7 @@ __global__ __launch_bounds__(448) void kb_sweep23(const BatchMember* M, const lo
     F.f[2] = s.rhoes; F.frank[2] = 0;
     F.elec_energy = 1;
     sweep23_body<DIR, PF, 3, true, DIR == 3, DIR == 2>(g, s, F, 1, M[m].dt, (int)(u % nbx),
-                                                      (int)(u / nbx), xch);
+                                                      (int)(u / nbx), xch, met);
     __syncthreads();                               // the next unit reuses the LDS
   }
 }
