@@ -818,7 +818,8 @@ def main():
         del tr
         torch.cuda.empty_cache()
         log("adjoint measurement (" + ADJOINT["desc"] + ")")
-        adj, _ = bench_adjoint(4 if args.adjoint_fd is None else args.adjoint_fd, 4)
+        adj, _ = bench_adjoint(ADJOINT["fd_dirs"] if args.adjoint_fd is None else args.adjoint_fd,
+                               ADJOINT["fd_column_dirs"])
     out = {
         "metric": METRIC,
         "value": value,
