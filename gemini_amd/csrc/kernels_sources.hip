@@ -8,21 +8,21 @@
 // every cell (ghosts included, as the reference's whole-array statements do) then converts
 // back to temperature / velocity / electron density.  The rates never touch HBM.
 //
-// Arithmetic follows the reference's expressions (-ffp-contract=off), with two liberties
-// that cost at most an ulp per operation: a divide by a compile-time constant (a mass sum,
-// 3) is a multiply by its correctly rounded reciprocal, and within a Coulomb pair 1/Teff**1.5
-// and 1/W are formed once.  The transcendental functions (exp, log, log10, erf, tanh, sin,
-// and x**p for the fitted exponents) are the device library's, which agree with glibc to
-// about an ulp; x**0.5 is the correctly rounded sqrt and x**1.5 is rounded correctly in
-// double-double (pow15).  These passes therefore match the reference to rounding (the tests
-// hold them to 1e-12 of each field's magnitude) rather than bit for bit.
+// Arithmetic follows the reference's expressions (-ffp-contract=off), with liberties that
+// cost a few ulp per operation: a divide by a compile-time constant (a mass sum, 3) is a
+// multiply by its correctly rounded reciprocal; within a Coulomb pair 1/Teff is formed once
+// and gives Teff**-1.5 and W**2, and exp(-W**2) below W = 0.1 is a Taylor series.  The
+// transcendental functions (exp, log, log10, erf, tanh, sin, and x**p for the fitted
+// exponents) are the device library's, which agree with glibc to about an ulp; x**0.5 is
+// the correctly rounded sqrt.  These passes therefore match the reference to rounding (the
+// tests hold them to 1e-12 of each field's magnitude) rather than bit for bit.
 //
 // Ion-ion (Coulomb) terms: Teff, Wsj, Phisj and Psisj of coulomb_colln are symmetric in
 // the species pair, so each unordered pair is evaluated once and feeds both species.  The
 // pairs run in lexicographic order, which adds each species' terms in the reference's
 // isp2 order; self-collision terms are exact zeros and are skipped.  The pair loop is
-// rolled (one copy of exp, erf and pow15 instead of 21, per-species values selected by a
-// wave-uniform index), which keeps the energy kernel within 256 registers.
+// rolled in the energy kernel (one copy of exp and erf instead of 21, per-species values
+// selected by a wave-uniform index), which keeps it within 256 registers.
 #include "gemhip_internal.h"
 
 namespace gemhip {
@@ -135,28 +135,6 @@ __constant__ double c_inv_msum[21] = {
     1.0 / ((1.0 * amu) + (32.0 * amu)), 1.0 / ((5.485799090e-4 * amu) + (32.0 * amu)),
     1.0 / ((1.0 * amu) + (14.0 * amu)), 1.0 / ((5.485799090e-4 * amu) + (14.0 * amu)),
     1.0 / ((5.485799090e-4 * amu) + (1.0 * amu))};
-__constant__ double c_2kB_mred[21] = {
-    (2.0 * kB) / (((16.0 * amu) * (30.0 * amu)) / ((16.0 * amu) + (30.0 * amu))),
-    (2.0 * kB) / (((16.0 * amu) * (28.0 * amu)) / ((16.0 * amu) + (28.0 * amu))),
-    (2.0 * kB) / (((16.0 * amu) * (32.0 * amu)) / ((16.0 * amu) + (32.0 * amu))),
-    (2.0 * kB) / (((16.0 * amu) * (14.0 * amu)) / ((16.0 * amu) + (14.0 * amu))),
-    (2.0 * kB) / (((16.0 * amu) * (1.0 * amu)) / ((16.0 * amu) + (1.0 * amu))),
-    (2.0 * kB) / (((16.0 * amu) * (5.485799090e-4 * amu)) / ((16.0 * amu) + (5.485799090e-4 * amu))),
-    (2.0 * kB) / (((30.0 * amu) * (28.0 * amu)) / ((30.0 * amu) + (28.0 * amu))),
-    (2.0 * kB) / (((30.0 * amu) * (32.0 * amu)) / ((30.0 * amu) + (32.0 * amu))),
-    (2.0 * kB) / (((30.0 * amu) * (14.0 * amu)) / ((30.0 * amu) + (14.0 * amu))),
-    (2.0 * kB) / (((30.0 * amu) * (1.0 * amu)) / ((30.0 * amu) + (1.0 * amu))),
-    (2.0 * kB) / (((30.0 * amu) * (5.485799090e-4 * amu)) / ((30.0 * amu) + (5.485799090e-4 * amu))),
-    (2.0 * kB) / (((28.0 * amu) * (32.0 * amu)) / ((28.0 * amu) + (32.0 * amu))),
-    (2.0 * kB) / (((28.0 * amu) * (14.0 * amu)) / ((28.0 * amu) + (14.0 * amu))),
-    (2.0 * kB) / (((28.0 * amu) * (1.0 * amu)) / ((28.0 * amu) + (1.0 * amu))),
-    (2.0 * kB) / (((28.0 * amu) * (5.485799090e-4 * amu)) / ((28.0 * amu) + (5.485799090e-4 * amu))),
-    (2.0 * kB) / (((32.0 * amu) * (14.0 * amu)) / ((32.0 * amu) + (14.0 * amu))),
-    (2.0 * kB) / (((32.0 * amu) * (1.0 * amu)) / ((32.0 * amu) + (1.0 * amu))),
-    (2.0 * kB) / (((32.0 * amu) * (5.485799090e-4 * amu)) / ((32.0 * amu) + (5.485799090e-4 * amu))),
-    (2.0 * kB) / (((14.0 * amu) * (1.0 * amu)) / ((14.0 * amu) + (1.0 * amu))),
-    (2.0 * kB) / (((14.0 * amu) * (5.485799090e-4 * amu)) / ((14.0 * amu) + (5.485799090e-4 * amu))),
-    (2.0 * kB) / (((1.0 * amu) * (5.485799090e-4 * amu)) / ((1.0 * amu) + (5.485799090e-4 * amu)))};
 __constant__ double c_csj_1e6[7][7] = {
     {(double)0.22f * 1e-6, (double)0.26f * 1e-6, (double)0.25f * 1e-6, (double)0.26f * 1e-6, (double)0.22f * 1e-6, (double)0.077f * 1e-6, (double)1.87e-3f * 1e-6},
     {(double)0.14f * 1e-6, (double)0.16f * 1e-6, (double)0.16f * 1e-6, (double)0.17f * 1e-6, (double)0.13f * 1e-6, (double)0.042f * 1e-6, (double)9.97e-4f * 1e-6},
@@ -166,20 +144,51 @@ __constant__ double c_csj_1e6[7][7] = {
     {(double)1.23f * 1e-6, (double)1.25f * 1e-6, (double)1.25f * 1e-6, (double)1.25f * 1e-6, (double)1.23f * 1e-6, (double)0.90f * 1e-6, (double)29.7e-3f * 1e-6},
     {(double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)54.5f * 1e-6, (double)38.537f * 1e-6}};
 
+// mred/(2 kB) of each pair, so that W**2 = (dv**2) mred/(2 kB Teff)
+__constant__ double c_mred_2kB[21] = {
+#define GEMHIP_MRED_2KB(ma, mb) ((((ma) * amu) * ((mb) * amu)) / (((ma) * amu) + ((mb) * amu))) / (2.0 * kB)
+    GEMHIP_MRED_2KB(16.0, 30.0), GEMHIP_MRED_2KB(16.0, 28.0), GEMHIP_MRED_2KB(16.0, 32.0),
+    GEMHIP_MRED_2KB(16.0, 14.0), GEMHIP_MRED_2KB(16.0, 1.0), GEMHIP_MRED_2KB(16.0, 5.485799090e-4),
+    GEMHIP_MRED_2KB(30.0, 28.0), GEMHIP_MRED_2KB(30.0, 32.0), GEMHIP_MRED_2KB(30.0, 14.0),
+    GEMHIP_MRED_2KB(30.0, 1.0), GEMHIP_MRED_2KB(30.0, 5.485799090e-4), GEMHIP_MRED_2KB(28.0, 32.0),
+    GEMHIP_MRED_2KB(28.0, 14.0), GEMHIP_MRED_2KB(28.0, 1.0), GEMHIP_MRED_2KB(28.0, 5.485799090e-4),
+    GEMHIP_MRED_2KB(32.0, 14.0), GEMHIP_MRED_2KB(32.0, 1.0), GEMHIP_MRED_2KB(32.0, 5.485799090e-4),
+    GEMHIP_MRED_2KB(14.0, 1.0), GEMHIP_MRED_2KB(14.0, 5.485799090e-4),
+    GEMHIP_MRED_2KB(1.0, 5.485799090e-4)};
+#undef GEMHIP_MRED_2KB
+
+// exp(-x) for 0 <= x < 0.01: Taylor series to x**6 (the remainder is below x**7/5040, 2e-18)
+__device__ __forceinline__ double exp_neg_small(double x) {
+  double r = 1.0 / 720.0;
+  r = r * x - 1.0 / 120.0;
+  r = r * x + 1.0 / 24.0;
+  r = r * x - 1.0 / 6.0;
+  r = r * x + 0.5;
+  r = r * x - 1.0;
+  return r * x + 1.0;
+}
+
 // the pair-symmetric part of coulomb_colln (collisions.f90:138-157) for pair pr = (a, b),
-// a < b: 1/Teff**1.5, Phisj, Psisj
+// a < b: 1/Teff**1.5, Phisj, Psisj.  1/Teff is formed once: Teff**-1.5 = (1/Teff)
+// sqrt(1/Teff) and W**2 = dv**2 (mred/(2 kB)) (1/Teff), each a few ulp from the
+// reference's forms.  Below W = 0.1 (W**2 = 0.01), where Phisj = 1, W itself is not needed
+// and Psisj = exp(-W**2) comes from its Taylor series.
 struct Pair { double iT15, Phi, Psi; };
 __device__ __forceinline__ Pair coulomb_pair(int pr, double Ta, double Tb, double va, double vb) {
   const int a = c_pair_a[pr], b = c_pair_b[pr];
   const double ma = c_ms[a - 1], mb = c_ms[b - 1];
   const double Teff = (mb * Ta + ma * Tb) * c_inv_msum[pr];
-  const double W = fabs(va - vb) / sqrt(Teff * c_2kB_mred[pr]);
+  const double iTeff = 1.0 / Teff;
+  const double dv = va - vb;
+  const double W2 = ((dv * dv) * c_mred_2kB[pr]) * iTeff;
   Pair p;
-  p.iT15 = 1.0 / pow15(Teff);
-  p.Psi = exp(-(W * W));
-  if (W < 0.1) {
+  p.iT15 = iTeff * sqrt(iTeff);
+  if (W2 < 0.01) {
     p.Phi = 1.0;
+    p.Psi = exp_neg_small(W2);
   } else {
+    const double W = sqrt(W2);
+    p.Psi = exp(-W2);
     const double iW = 1.0 / W, iW2 = iW * iW;
     p.Phi = (((3.0 / 4.0) * sqrt(PI)) * erf(W)) * (iW2 * iW) - ((3.0 / 2.0) * iW2) * p.Psi;
   }
