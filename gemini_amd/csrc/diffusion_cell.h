@@ -19,18 +19,25 @@ struct CellIn {
   double kap;
 };
 
+// (the metric factors come from DevGrid::hcol at index hc, see hcol_index; gcol: the
+// column, lx2 (i3-1) + i2-1, for the conductivity scale)
 template <bool ELEC, bool EXT = false>
 __device__ __forceinline__ void cell_load(const DevGrid& g, const DevState& s, long so, long cb,
-                                          long kb, long ni, int r, CellIn<ELEC>& c,
+                                          long kb, long ni, int r, CellIn<ELEC>& c, long hc,
                                           int gcol = 0) {
   c.T = s.Ts[so + cb + r];
   c.n = s.ns[so + cb + r];
-  c.h1 = g.h1[cb + r]; c.h2 = g.h2[cb + r]; c.h3 = g.h3[cb + r];
+  c.h1 = g.hcol[hc]; c.h2 = g.hcol[g.hstride + hc]; c.h3 = g.hcol[2 * g.hstride + hc];
   if (ELEC) {
     c.n1 = s.nn[kb + r - 1]; c.n2 = s.nn[ni + kb + r - 1]; c.n3 = s.nn[2 * ni + kb + r - 1];
     c.J = s.J1[cb + r];
     if (EXT) c.kap = s.kappa ? s.kappa[gcol] : 1.0;
   }
+}
+
+// index in DevGrid::hcol of row r (1..lx1) of column gcol
+__device__ __forceinline__ long hcol_index(const DevGrid& g, int gcol, int r) {
+  return ((long)(gcol >> 6) * g.lx1 + (r - 1)) * 64 + (gcol & 63);
 }
 
 // conductivity (collisions.f90:162-199), times the column scale under the extension

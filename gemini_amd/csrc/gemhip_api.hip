@@ -493,9 +493,12 @@ int gemhip_register_grid(int xtype, int gridflag, int flagper, const int* isglob
                     nv3i(), nv3i(), nv3i(), c.lx1 + 3, c.lx1, c.lx2 + 3, c.lx2, c.lx3 + 3, c.lx3};
   long total = 0;
   for (long k : n) total += k;
+  g.hstride = diffusion_hcol_doubles(g) / 3;
   if (c.d_geom) hipFree(c.d_geom);
   c.d_geom = nullptr;
-  if (!hipok(hipMalloc((void**)&c.d_geom, total * sizeof(double)), "geometry")) return 5;
+  if (!hipok(hipMalloc((void**)&c.d_geom, (total + 32 + 3 * g.hstride) * sizeof(double)),
+             "geometry"))
+    return 5;
   const double** dst[] = {&g.h1, &g.h2, &g.h3, &g.h1x1i, &g.h2x1i, &g.h3x1i, &g.h1x2i, &g.h2x2i,
                           &g.h3x2i, &g.h1x3i, &g.h2x3i, &g.h3x3i, &g.dx1, &g.dx1i, &g.dx2, &g.dx2i,
                           &g.dx3, &g.dx3i};
@@ -507,6 +510,10 @@ int gemhip_register_grid(int xtype, int gridflag, int flagper, const int* isglob
       return 6;
     off += n[k];
   }
+  off = (off + 31) & ~31L;                     // 256-byte aligned rows of 64 columns
+  g.hcol = c.d_geom + off;                     // the diffusion's copy of h1, h2, h3
+  launch_fill_hcol(g, c.d_geom + off, c.stream);
+  if (!hipok(hipStreamSynchronize(c.stream), "geometry reorder")) return 6;
   g.gridflag = gridflag;
   g.flagper = flagper ? 1 : 0;
   for (int k = 0; k < 6; ++k) g.isglobal[k] = isglobal6[k] ? 1 : 0;

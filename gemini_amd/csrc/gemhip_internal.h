@@ -21,6 +21,10 @@ struct DevGrid {
   const double *h1x2i, *h2x2i, *h3x2i;
   const double *h1x3i, *h2x3i, *h3x3i;
   const double *dx1, *dx1i, *dx2, *dx2i, *dx3, *dx3i;
+  // h1, h2, h3 of the interior rows in the energy diffusion's column-group order: factor m
+  // of row r (1..lx1) of column gcol at hcol[m hstride + ((gcol/64) lx1 + r-1) 64 + gcol%64]
+  const double* hcol;
+  long hstride;
 };
 
 // State pointers in HBM.  ns..Ts point into one fluidvars block of 35 fields,
@@ -103,6 +107,8 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
 
 // ---- launchers (kernels_diffusion.hip) ----
 size_t diffusion_scratch_bytes(const DevGrid& g);
+long diffusion_hcol_doubles(const DevGrid& g);              // size of DevGrid::hcol
+void launch_fill_hcol(const DevGrid& g, double* hcol, hipStream_t st);
 
 // ---- batched ensemble steps (kernels_advec.hip, kernels_diffusion.hip) ----
 // One member of an ensemble advanced by batched kernels: every kernel of the fused

@@ -476,7 +476,8 @@ __global__ __launch_bounds__(64) void k_adj_trbdf2(DevGrid g, DevState s, AdjArg
   // both solutions and both adjoint solutions
   struct Row { double T, C, B, D, lam_e, h1, h23, x1, x2, w1, w2; CellIn<true> in; };
   auto fetch_in = [&](int r, CellIn<true>& c) {
-    cell_load<true, true>(g, se, 0, cb, kb, ni, clampi(r, 1, N), c, (int)col);
+    cell_load<true, true>(g, se, 0, cb, kb, ni, clampi(r, 1, N), c,
+                          hcol_index(g, (int)col, clampi(r, 1, N)), (int)col);
   };
   auto make_row = [&](const CellIn<true>& c, Row& w) {
     w.in = c;

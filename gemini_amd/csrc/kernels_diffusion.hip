@@ -214,7 +214,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   if (EXT && ELEC && s.qtop) {
     CellIn<ELEC> top;
     cell_load<ELEC, EXT>(g, s, so, gix(g, 0, col % g.lx2 + 1, col / g.lx2 + 1), kb, ni, N, top,
-                         col);
+                         hcol_index(g, col, N), col);
     aN = -1.0 / dx1[N];
     bN = 1.0 / dx1[N];
     Tmax = (s.qtop[col] * top.h1) / cell_lambda<ELEC, EXT>(top, isp);
@@ -241,12 +241,16 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   // load and convert the rows r0..r0+R-1 of the 64 columns into LDS (rows past N repeat N)
   auto load_chunk = [&](int r0) {
     const int r = min(r0 + lr, N);
+    // hcol slot of this lane's first column at row r: the group's 64 columns are contiguous
+    // (slots past the last column repeat it, as these lanes do)
+    const long hrow = ((long)cg * N + (r - 1)) * 64 + cs;
 #pragma unroll
     for (int k0 = 0; k0 < R; k0 += NB) {
       CellIn<ELEC> in[NB];
 #pragma unroll
       for (int k = 0; k < NB; ++k)
-        cell_load<ELEC, EXT>(g, s, so, cbk[k0 + k], kbk[k0 + k], ni, r, in[k], gck[k0 + k]);
+        cell_load<ELEC, EXT>(g, s, so, cbk[k0 + k], kbk[k0 + k], ni, r, in[k],
+                             hrow + (k0 + k) * CPI, gck[k0 + k]);
 #pragma unroll
       for (int k = 0; k < NB; ++k) {
         double C_, B_, D_;
@@ -569,7 +573,32 @@ __global__ __launch_bounds__(64, 2) void kb_diffusion(const BatchMember* M, cons
 }  // namespace
 
 
+// the metric factors in column-group order (DevGrid::hcol), one thread per entry; lanes
+// past the last column repeat it
+__global__ void k_fill_hcol(DevGrid g, double* hcol) {
+  const long ncol = (long)g.lx2 * g.lx3;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.hstride;
+       t += (long)gridDim.x * blockDim.x) {
+    const int lane = (int)(t & 63);
+    const long q = t >> 6;
+    const int r = (int)(q % g.lx1) + 1;
+    const long gcol = min((q / g.lx1) * 64 + lane, ncol - 1);
+    const long cb = gix(g, 0, (int)(gcol % g.lx2) + 1, (int)(gcol / g.lx2) + 1);
+    hcol[t] = g.h1[cb + r];
+    hcol[g.hstride + t] = g.h2[cb + r];
+    hcol[2 * g.hstride + t] = g.h3[cb + r];
+  }
+}
+
 long pivot_record_ints(const DevGrid& g) { return (long)g.lx2 * g.lx3 * 2 * ((g.lx1 + 31) / 32); }
+
+long diffusion_hcol_doubles(const DevGrid& g) {
+  return 3 * (((long)g.lx2 * g.lx3 + 63) / 64) * g.lx1 * 64;
+}
+
+void launch_fill_hcol(const DevGrid& g, double* hcol, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_hcol, dim3(4096), dim3(256), 0, st, g, hcol);
+}
 
 // pivot rows, right-hand sides and solutions, the BDF2 interchange bits, one flag per lane
 size_t diffusion_scratch_bytes(const DevGrid& g) {
