@@ -496,7 +496,8 @@ int gemhip_register_grid(int xtype, int gridflag, int flagper, const int* isglob
   g.hstride = diffusion_hcol_doubles(g) / 3;
   if (c.d_geom) hipFree(c.d_geom);
   c.d_geom = nullptr;
-  if (!hipok(hipMalloc((void**)&c.d_geom, (total + 32 + 3 * g.hstride) * sizeof(double)),
+  // every array starts 256-byte aligned (the sweeps' DMA loads read 16-byte pairs)
+  if (!hipok(hipMalloc((void**)&c.d_geom, (total + 19 * 32 + 3 * g.hstride) * sizeof(double)),
              "geometry"))
     return 5;
   const double** dst[] = {&g.h1, &g.h2, &g.h3, &g.h1x1i, &g.h2x1i, &g.h3x1i, &g.h1x2i, &g.h2x2i,
@@ -508,7 +509,7 @@ int gemhip_register_grid(int xtype, int gridflag, int flagper, const int* isglob
     if (!hipok(hipMemcpy(c.d_geom + off, src[k], n[k] * sizeof(double), hipMemcpyDefault),
                "geometry upload"))
       return 6;
-    off += n[k];
+    off = (off + n[k] + 31) & ~31L;
   }
   off = (off + 31) & ~31L;                     // 256-byte aligned rows of 64 columns
   g.hcol = c.d_geom + off;                     // the diffusion's copy of h1, h2, h3

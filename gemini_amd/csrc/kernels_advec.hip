@@ -13,6 +13,8 @@
 //    thread along its line, keeping the MC stencil in registers -- every load and
 //    store is a coalesced 64-lane access and no LDS or barrier is needed.
 #include <algorithm>
+#include <cstdint>
+#include <initializer_list>
 
 #include "gemhip_internal.h"
 
@@ -573,15 +575,109 @@ constexpr int sweep_xch_doubles() { return 2 * PF * (LSP - 1) * 2 * 64; }
 #ifndef SWEEP_SHARE_METRICS
 #define SWEEP_SHARE_METRICS 1
 #endif
-template <int PF>
-constexpr int sweep_met_doubles() { return 2 * PF * 6 * 64; }
 
-template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = false>
+// The fused step's sweeps load each step's three fields and cell velocity by LDS DMA
+// (global_load_lds_dwordx4, which holds no registers while in flight) SWEEP_DMA steps
+// ahead, into a ring of SWEEP_DMA + 1 slots per wave; a slot is four 64-double rows
+// [field 0, field 1, field 2, velocity] and one instruction fills two of them (lanes 0-31
+// the 16-byte pairs of the first row, lanes 32-63 of the second).  The compiler does not
+// order LDS reads after these loads, so each step waits on the vector memory counter
+// itself: that counter retires loads and stores in issue order, and after step k's loads
+// (issued at the end of step k - D, or in the prologue) at least sweep_dma_after(...)
+// vector memory operations were issued, so waiting until that many remain outstanding
+// means step k's rows have landed.  Slot k % (D + 1) is refilled at the end of step
+// k + 1, after its rows were read in step k.
+#ifndef SWEEP_DMA
+#define SWEEP_DMA 6
+#endif
+template <int D>
+constexpr int sweep_ring_doubles() { return LSP * (D + 1) * 4 * 64; }
+// metric-factor slots: two groups in registers-then-LDS, or four DMA slots
+template <bool DMAV, int PF>
+constexpr int sweep_met_slots() { return DMAV && SWEEP_DMA > 0 ? 4 : 2; }
+template <int PF, bool DMAV = false>
+constexpr int sweep_met_doubles() { return sweep_met_slots<DMAV, PF>() * PF * 6 * 64; }
+// vector memory operations issued after the loads of step k: two per step at least; from
+// the first step whose D - 1 predecessors all store (k > D), also the step's stores
+template <int D>
+constexpr int sweep_dma_after(bool steady, int stores) { return (D - 1) * (2 + (steady ? stores : 0)); }
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt holds six bits");
+  // gfx9 encoding: vmcnt[3:0], expcnt[6:4] (7: no wait), lgkmcnt[11:8] (15: no wait),
+  // vmcnt[5:4] at [15:14]
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+  __asm__ volatile("" ::: "memory");         // no LDS read moves above the wait
+}
+__device__ __forceinline__ void dma16(const double* src, double* lds) {
+  __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)src,
+                                   (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
+}
+// the four rows of a ring slot at this lane's offset.  Read in assembly (which waits for
+// its own reads): an LDS read the compiler sees after a DMA load makes it wait for every
+// vector memory operation in flight.
+__device__ __forceinline__ void ring_read4(const double* p, double& a, double& b, double& c,
+                                           double& d) {
+  const unsigned ad = (unsigned)(unsigned long)(__attribute__((address_space(3))) const double*)p;
+  __asm__ volatile(
+      "ds_read_b64 %0, %4\n\t"
+      "ds_read_b64 %1, %4 offset:512\n\t"
+      "ds_read_b64 %2, %4 offset:1024\n\t"
+      "ds_read_b64 %3, %4 offset:1536\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
+      : "v"(ad)
+      : "memory");
+}
+// six rows 1 KiB apart (the metric factors of one step in their DMA slot), likewise
+__device__ __forceinline__ void lds_read6(const double* p, double* v) {
+  const unsigned ad = (unsigned)(unsigned long)(__attribute__((address_space(3))) const double*)p;
+  __asm__ volatile(
+      "ds_read_b64 %0, %6\n\t"
+      "ds_read_b64 %1, %6 offset:1024\n\t"
+      "ds_read_b64 %2, %6 offset:2048\n\t"
+      "ds_read_b64 %3, %6 offset:3072\n\t"
+      "ds_read_b64 %4, %6 offset:4096\n\t"
+      "ds_read_b64 %5, %6 offset:5120\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5])
+      : "v"(ad)
+      : "memory");
+}
+// a read-only grid array (dx, dxi) at a wave-uniform index, by a scalar load: a vector load
+// would be the newest vector memory operation and waiting for it would wait for the ring
+__device__ __forceinline__ double ld_uniform(const double* p) {
+  return *(const __attribute__((address_space(4))) double*)p;
+}
+// The sweeps' blocks go to the XCDs so that the x1 blocks of a line share one L2:
+// workgroups are dispatched to the 8 XCDs round robin, so block b runs on XCD b % 8, and
+// XCD x takes lines x, x + 8, ..., each line's x1 blocks one after the other.  The 64-cell
+// row segments of adjacent x1 blocks share a 128-byte line at each end (a row starts two
+// ghost cells into its line), which x1 blocks on different XCDs each fetched from HBM.
+__host__ __device__ inline long sweep_blocks(int nbx, int nlines) {
+  return (long)((nlines + 7) / 8) * 8 * nbx;
+}
+__device__ __forceinline__ bool sweep_block(int nbx, int nlines, int& bx, int& by) {
+  const int b = blockIdx.x, t = b >> 3;
+  bx = t % nbx;
+  by = (t / nbx) * 8 + (b & 7);
+  return by < nlines;
+}
+
+template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = false,
+          bool DMAV = false>
 __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s,
                                              const SweepFields& F, int isp0, double dt, int bx,
-                                             int by, double* xch, double* met) {
+                                             int by, double* xch, double* met, double* ring) {
   static_assert(!(CONV || POST) || NQ == 3, "the conversions come with all three fields");
   constexpr bool SHM = SWEEP_SHARE_METRICS && (CONV || POST);
+  static_assert(!DMAV || (OTF && NQ == 3 && (CONV || POST)), "DMA: the fused sweeps only");
+  constexpr bool DMA = DMAV && SWEEP_DMA > 0;
+  constexpr int D = DMA ? SWEEP_DMA : 1;
+  constexpr bool MDMA = DMA && SHM;           // metric factors by DMA as well
+  constexpr int MS = sweep_met_slots<DMAV, PF>();
+  static_assert(!MDMA || PF == 2, "one DMA instruction loads a factor's two steps");
+  static_assert(!MDMA || 3 * PF >= D, "the factors are loaded at least as far ahead as the rows");
   const int lx = DIR == 2 ? g.lx2 : g.lx3;
   // lanes past lx1 redo column lx1's work: same inputs, same values written to the same cells
   const int i1 = min((int)(bx * 64 + threadIdx.x), g.lx1 - 1) + 1;
@@ -646,6 +742,26 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
       p1[u] = g.h1[cm]; p2[u] = g.h2[cm]; p3[u] = g.h3[cm];
     }
   };
+  // DMA: this lane loads x1 cells pp, pp+1 (0-based in the block's 64) of a row; past the
+  // line it loads the block's first pair again (never read).  The row's cell of lane i is
+  // at offset i of the slot row; lanes past lx1 read column lx1's (rel).
+  const bool hi = threadIdx.x >= 32;
+  const int rel = i1 - 1 - bx * 64;
+  const int pp0 = bx * 64 + 2 * (threadIdx.x & 31);
+  const long dc = (long)(pp0 <= g.lx1 - 1 ? pp0 : bx * 64) - (i1 - 1);
+  auto ring_slot = [&](int k) -> double* {
+    return ring + (threadIdx.y * (D + 1) + k % (D + 1)) * 256;
+  };
+  auto dma_issue = [&](int k) {                // the rows of step k
+    const int kf = min(k + 2, lx + 2);
+    const bool conv = CONV && kf <= lx;
+    const int ki = clampi(k, 1, lx + 1);
+    const double* a = hi ? (conv ? lp[1 % NQ] : fp[1 % NQ]) : (conv ? lp[0] : fp[0]);
+    const double* b = hi ? w + (long)(ki - kf) * cs_ : (conv ? lp[2 % NQ] : fp[2 % NQ]);
+    double* sl = ring_slot(k);
+    dma16(a + dc + (long)kf * cs_, sl);
+    dma16(b + dc + (long)kf * cs_, sl + 128);
+  };
   // SHM: this wave's metric factor (wave wy < 6 loads factor wy) at step k, and a group's
   // PF steps into registers or from registers into LDS
   const int wy = threadIdx.y;
@@ -673,6 +789,37 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
   auto met_get = [&](int G, int u, int m) -> double {
     return met[(((G & 1) * PF + u) * 6 + m) * 64 + threadIdx.x];
   };
+  // MDMA: the factors by DMA into a ring of MS group slots [slot][factor][step][64], three
+  // groups ahead; wave wy < 6 loads factor wy of a group's two steps with one instruction
+  // (lanes 0-31 the first step's row, lanes 32-63 the second's)
+  auto met_ptr = [&](int k) -> const double* {
+    const int ki = clampi(k, 1, lx + 1);
+    const long ik = if0 + (long)ki * istr;
+    const long cm = c0 + (long)clampi(k - 1, 1, lx) * cs_;
+    if (wy == 0) return hI + ik;
+    if (wy == 1) return hA + ik;
+    if (wy == 2) return hB + ik;
+    if (wy == 3) return g.h1 + cm;
+    if (wy == 4) return g.h2 + cm;
+    return g.h3 + cm;
+  };
+  auto met_dma = [&](int G) {
+    if (wy < 6) dma16(met_ptr(2 + G * PF + (hi ? 1 : 0)) + dc, met + ((G % MS) * 6 + wy) * PF * 64);
+  };
+  // before the barrier that ends group G, this wave's load of group G+1's factors has
+  // landed: it was followed by the loads and stores of the steps of groups G-2 .. G (or,
+  // for G < 2, by the ring's prologue loads and steps 0 and 1)
+  auto met_wait = [&](int G) {
+    if (wy >= 6) return;
+    if (G < 2) wait_vmcnt<2 * D + 4>();
+    else if (POST && !e7) wait_vmcnt<3 * PF * (2 + NQ + 2)>();
+    else wait_vmcnt<3 * PF * (2 + NQ)>();
+  };
+  if (MDMA) {
+    met_dma(0);
+    met_dma(1);
+    met_dma(2);
+  }
   double fa[NQ], fb[NQ], fc[NQ], rsp[NQ], slp[NQ], php[NQ];
   double dxcur = dx[0];
   double wprev = w[0];         // vs(0) for the on-the-fly interface velocity
@@ -689,8 +836,13 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
     rsp[q] = (fb[q] - fa[q]) / dxcur;
     slp[q] = php[q] = 0.0;
   }
+  if (DMA) {
 #pragma unroll
-  for (int u = 0; u < PF; ++u) issue(u, u);
+    for (int k = 0; k < D; ++k) dma_issue(k);
+  } else {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) issue(u, u);
+  }
   // POST: per-wave constants, the group parity being written, and the electron wave's
   // updated energies of the current and the previous group
   const double gm1kB = (gammas(isp) - 1.0) / kB, qsp = qs(isp);
@@ -720,12 +872,26 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
   // mode 0: step 0 (slopes only), 1: step 1 (flux, no store), 2: steps 2..lx+1
   // slot: the step's position in its group; G: the group (SHM reads its metric factors)
   auto step = [&](int k, int u, int mode, int slot = 0, int G = 0) {
-    const double dxk = dxcur, dxk1 = dx[k + 1];
+    const double dxk = dxcur, dxk1 = ld_uniform(dx + k + 1);
     dxcur = dxk1;
-    const double wv = pv[u];
+    double wv, nxt[NQ];
+    if (DMA) {                                 // step k's rows have landed in its slot
+      if (k <= D) wait_vmcnt<sweep_dma_after<D>(false, 0)>();
+      else if (POST && !e7) wait_vmcnt<sweep_dma_after<D>(true, NQ + 2)>();
+      else wait_vmcnt<sweep_dma_after<D>(true, NQ)>();
+      ring_read4(ring_slot(k) + rel, nxt[0], nxt[1 % NQ], nxt[2 % NQ], wv);
+    } else {
+      wv = pv[u];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) nxt[q] = pf[q][u];
+    }
     double hIv, a, b, h1, h2, h3;
     if (!SHM) {
       hIv = pI[u]; a = pA[u]; b = pB[u]; h1 = p1[u]; h2 = p2[u]; h3 = p3[u];
+    } else if (mode == 2 && MDMA) {
+      double m6[6];
+      lds_read6(met + ((G % MS) * 6 * PF + slot) * 64 + rel, m6);
+      hIv = m6[0]; a = m6[1]; b = m6[2]; h1 = m6[3]; h2 = m6[4]; h3 = m6[5];
     } else if (mode == 2) {
       hIv = met_get(G, slot, 0); a = met_get(G, slot, 1); b = met_get(G, slot, 2);
       h1 = met_get(G, slot, 3); h2 = met_get(G, slot, 4); h3 = met_get(G, slot, 5);
@@ -734,11 +900,8 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
       hIv = hI[ik]; a = hA[ik]; b = hB[ik];
       h1 = h2 = h3 = 1.0;
     }
-    double nxt[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) nxt[q] = pf[q][u];
+    if (!DMA) issue(k + PF, u);                // (after this step's slot u was read)
     if (CONV && min(k + 2, lx + 2) <= lx) convert(nxt);
-    issue(k + PF, u);
     double v = 0.0, vq = 0.0, h2f0 = 0.0, h2f1 = 0.0, ha1f0 = 1.0, ha1f1 = 1.0, dxim = 1.0;
     double newv[NQ];
     if (mode >= 1) {
@@ -751,7 +914,7 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
     if (mode == 2) {
       ha1f0 = (h1 * h2) * h3;
       ha1f1 = ((h1 * h1) * h2) * h3;
-      dxim = dxi[k - 2];
+      dxim = ld_uniform(dxi + k - 2);
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -793,16 +956,25 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
         xslot(xbuf, slot, isp - 1, 1) = (n * qsp) * vv;
       }
     }
+    if (DMA) dma_issue(k + D);                // into the slot step k - 1 read
   };
   // POST: LDS stores of one group are visible to the electron wave after this barrier; no
   // fence on global memory, so loads issued ahead stay in flight
   auto lds_barrier = [&]() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (DMA) {
+      // the fences would also wait for the ring's loads: this wave's LDS accesses are
+      // complete (lgkmcnt) before the barrier, and nothing moves across it
+      __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    } else {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
   };
   double rmA[PF], rmB[PF];                    // SHM: factors of two groups ahead
-  if (SHM) {
+  if (MDMA) {
+    // (the ring's prologue loads are issued after these: see met_wait)
+  } else if (SHM) {
     met_load(0, rmA);
     met_store(0, rmA);
     met_load(1, rmB);
@@ -810,15 +982,22 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
   }
   step(0, 0, 0);
   step(1, 1 % PF, 1);
+  if (MDMA) {                                 // group 0's factors published
+    wait_vmcnt<2 * D + 4>();
+    lds_barrier();
+  }
   int kb = 2;
   int groups = 0;
   // group G (steps kb..kb+PF-1): SHM loads the factors of group G+2 into rl and stores
-  // those of group G+1 from rw before the barrier
+  // those of group G+1 from rw before the barrier (MDMA: loads those of group G+3 into
+  // their slot and waits for its load of group G+1's before the barrier)
   auto group = [&](double* rl, double* rw) {
-    if (SHM) met_load(groups + 2, rl);
+    if (MDMA) met_dma(groups + 3);
+    else if (SHM) met_load(groups + 2, rl);
 #pragma unroll
     for (int u = 0; u < PF; ++u) step(kb + u, (2 + u) % PF, 2, u, groups);
-    if (SHM) met_store(groups + 1, rw);
+    if (MDMA) met_wait(groups);
+    else if (SHM) met_store(groups + 1, rw);
     if (POST) {
       // the electron wave finishes the previous group (its LDS parity is not written in
       // this group), then the ion waves may write that parity again after the barrier
@@ -855,15 +1034,20 @@ __device__ __forceinline__ void sweep23_body(const DevGrid& g, const DevState& s
     lds_barrier();
     if (e7 && tail > 0) finalize(xbuf, kb, tail, eout);
   }
+  if (DMA) wait_vmcnt<0>();                   // no load still writing the ring on exit
 }
 
-template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = false>
+// (a 1-D grid of sweep_blocks(nbx, lines) blocks, see sweep_block)
+template <int DIR, int PF, int NQ, bool OTF, bool CONV = false, bool POST = false,
+          bool DMAV = false>
 __global__ __launch_bounds__(448) SWEEP23_WAVES void k_sweep23(DevGrid g, DevState s,
                                                                SweepFields F, int isp0, double dt) {
   __shared__ double xch[POST ? sweep_xch_doubles<PF>() : 1];
-  __shared__ double met[(CONV || POST) ? sweep_met_doubles<PF>() : 1];
-  sweep23_body<DIR, PF, NQ, OTF, CONV, POST>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y, xch,
-                                             met);
+  __shared__ double met[(CONV || POST) ? sweep_met_doubles<PF, DMAV>() : 1];
+  __shared__ double ring[DMAV && SWEEP_DMA > 0 ? sweep_ring_doubles<SWEEP_DMA>() : 1];
+  int bx, by;
+  if (!sweep_block((g.lx1 + 63) / 64, DIR == 2 ? g.lx3 : g.lx2, bx, by)) return;
+  sweep23_body<DIR, PF, NQ, OTF, CONV, POST, DMAV>(g, s, F, isp0, dt, bx, by, xch, met, ring);
 }
 
 // Ts ghost cells: floor at 100 K (the reference leaves them undefined, see the
@@ -931,6 +1115,7 @@ __global__ __launch_bounds__(448) void kb_sweep23(const BatchMember* M, const lo
                                                   long total) {
   __shared__ double xch[sweep_xch_doubles<PF>()];
   __shared__ double met[sweep_met_doubles<PF>()];
+  __shared__ double ring[1];                       // (register prefetch)
   constexpr int K = DIR == 3 ? BK_SW3 : BK_SW2;
   const long* p = pre + (long)K * (n + 1);
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
@@ -945,7 +1130,7 @@ __global__ __launch_bounds__(448) void kb_sweep23(const BatchMember* M, const lo
     F.f[2] = s.rhoes; F.frank[2] = 0;
     F.elec_energy = 1;
     sweep23_body<DIR, PF, 3, true, DIR == 3, DIR == 2>(g, s, F, 1, M[m].dt, (int)(u % nbx),
-                                                      (int)(u / nbx), xch, met);
+                                                      (int)(u / nbx), xch, met, ring);
     __syncthreads();                               // the next unit reuses the LDS
   }
 }
@@ -1119,12 +1304,23 @@ void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t 
   timing_end("save_v1b", st);
 }
 
+// The DMA ring loads 16-byte pairs of cells: every pair must be 16-byte aligned, so rows of
+// an even length (lx1 + 4) and fields at 16-byte aligned addresses; otherwise the sweeps
+// prefetch through registers.
+static bool sweep_dma_ok(const DevGrid& g, std::initializer_list<const double*> fields) {
+  if (SWEEP_DMA <= 0 || g.L1 % 2 != 0) return false;
+  for (const double* f : fields)
+    if (reinterpret_cast<uintptr_t>(f) % 16 != 0) return false;
+  return true;
+}
+
 namespace {
 template <int NQ, int PFM = SWEEP_PF>
 void sweep_launch(const DevGrid& g, const DevState& s, int dir, const SweepFields& F, int isp0,
                   int nsp, double dt, bool otf, hipStream_t st) {
   const dim3 blk(64, nsp);
-  const dim3 grd(dir == 1 ? g.lx2 : (g.lx1 + 63) / 64, dir == 1 ? g.lx3 : dir == 2 ? g.lx3 : g.lx2);
+  const dim3 grd = dir == 1 ? dim3(g.lx2, g.lx3)
+                            : dim3(sweep_blocks((g.lx1 + 63) / 64, dir == 2 ? g.lx3 : g.lx2));
   if (dir == 1) {
     if (otf) hipLaunchKernelGGL((k_sweep1<NQ, true>), grd, blk, 0, st, g, s, F, isp0, dt);
     else hipLaunchKernelGGL((k_sweep1<NQ, false>), grd, blk, 0, st, g, s, F, isp0, dt);
@@ -1182,12 +1378,16 @@ void launch_sweep3_preconvert(const DevGrid& g, const DevState& s, double dt, hi
   A.f[1] = s.rhovs1; A.frank[1] = 1;
   A.f[2] = s.rhoes; A.frank[2] = 0;
   A.elec_energy = 1;
-  const dim3 blk(64, LSP), grd((g.lx1 + 63) / 64, g.lx2);
+  const dim3 blk(64, LSP), grd(sweep_blocks((g.lx1 + 63) / 64, g.lx2));
+  const bool dma = sweep_dma_ok(g, {s.ns, s.rhovs1, s.rhoes, s.vs1, s.Ts, s.vs3, g.h1, g.h2,
+                                    g.h3, g.h1x3i, g.h2x3i, g.h3x3i});
   timing_begin("sweep3", st);
-  if (tune.sweep_pf == 1)
+  if (tune.sweep_pf == 1) {
     hipLaunchKernelGGL((k_sweep23<3, 1, 3, true, true>), grd, blk, 0, st, g, s, A, 1, dt);
-  else
-    hipLaunchKernelGGL((k_sweep23<3, 2, 3, true, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  } else {
+    if (dma) hipLaunchKernelGGL((k_sweep23<3, 2, 3, true, true, false, true>), grd, blk, 0, st, g, s, A, 1, dt);
+    else hipLaunchKernelGGL((k_sweep23<3, 2, 3, true, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  }
   timing_end("sweep3", st);
 }
 
@@ -1200,12 +1400,16 @@ void launch_sweep2_postconvert(const DevGrid& g, const DevState& s, double dt, h
   A.f[1] = s.rhovs1; A.frank[1] = 1;
   A.f[2] = s.rhoes; A.frank[2] = 0;
   A.elec_energy = 1;
-  const dim3 blk(64, LSP), grd((g.lx1 + 63) / 64, g.lx3);
+  const dim3 blk(64, LSP), grd(sweep_blocks((g.lx1 + 63) / 64, g.lx3));
+  const bool dma = sweep_dma_ok(g, {s.ns, s.rhovs1, s.rhoes, s.vs1, s.Ts, s.vs2, g.h1, g.h2,
+                                    g.h3, g.h1x2i, g.h2x2i, g.h3x2i});
   timing_begin("sweep2", st);
-  if (tune.sweep_pf == 1)
+  if (tune.sweep_pf == 1) {
     hipLaunchKernelGGL((k_sweep23<2, 1, 3, true, false, true>), grd, blk, 0, st, g, s, A, 1, dt);
-  else
-    hipLaunchKernelGGL((k_sweep23<2, 2, 3, true, false, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  } else {
+    if (dma) hipLaunchKernelGGL((k_sweep23<2, 2, 3, true, false, true, true>), grd, blk, 0, st, g, s, A, 1, dt);
+    else hipLaunchKernelGGL((k_sweep23<2, 2, 3, true, false, true>), grd, blk, 0, st, g, s, A, 1, dt);
+  }
   timing_end("sweep2", st);
   timing_begin("post_convert", st);
   hipLaunchKernelGGL(k_post_convert_surface, dim3(blocks_for(surface_cells(g))), dim3(TPB), 0, st,

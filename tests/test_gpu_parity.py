@@ -36,6 +36,8 @@ def transport_for(g, st, **kw):
 
 CASES = [("cart", (128, 64, 1), 0.5),        # the 2-D Cartesian configuration
          ("cart", (24, 10, 7), None),
+         ("cart", (70, 6, 5), None),           # x1 blocks of 64 and 6 cells
+         ("cart", (23, 10, 7), None),          # odd rows: x2/x3 loads through registers
          ("dipole", (32, 12, 6), None)]
 
 
