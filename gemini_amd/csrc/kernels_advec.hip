@@ -381,8 +381,7 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // neighbours move one lane up by a DPP wave shift (lane 0 takes lane 63's of the previous
 // chunk), and the lane writes the update of cell p-1.  Two register buffers alternate:
 // the loads of chunk c+1 are issued before chunk c is computed, and chunk c writes only
-// cells chunk c+1 no longer reads.  The chunk count is rounded up to even; the extra
-// chunk only puts the last ghost value back.
+// cells chunk c+1 no longer reads.  An odd last chunk runs after the pairs.
 __device__ __forceinline__ double lane_up1(double v, double carry) {
   // lane i receives lane i-1's value, lane 0 the carry (DPP wave_shr:1)
   const long long vi = __double_as_longlong(v), ci = __double_as_longlong(carry);
@@ -509,18 +508,27 @@ __device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
   for (int q = 0; q < NQ; ++q) k.rs[q] = k.sl[q] = k.ph[q] = k.f[q] = 0.0;
   k.u = k.h2 = 0.0;
   k.ha = k.dx = 1.0;
-  const int nchunk = ((lx1 + 3 + 63) / 64 + 1) & ~1;                       // p = -1 .. lx1+1
+  const int nchunk = (lx1 + 3 + 63) / 64;                                 // p = -1 .. lx1+1
   S1Chunk<NQ> A, B;
   s1_load<NQ, OTF>(g, fp, u, v1i, c0, if0, lane - 1, A);
-  for (int c = 0; c < nchunk; c += 2) {
+  int c = 0;
+  for (; c + 1 < nchunk; c += 2) {
     s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k);
     s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c + 1, lane, dt, vb1, vbN, B, A, k);
   }
+  if (c < nchunk) s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k);
 }
 template <int NQ, bool OTF>
 __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFields F, int isp0,
                                                 double dt) {
-  sweep1_body<NQ, OTF>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y);
+  if (NQ == 3 && F.elec_energy && isp0 + (int)threadIdx.y == LSP) {
+    // the electron wave of an all-species block sweeps its energy only
+    SweepFields E{};
+    E.f[0] = F.f[2];
+    sweep1_body<1, OTF>(g, s, E, isp0, dt, blockIdx.x, blockIdx.y);
+  } else {
+    sweep1_body<NQ, OTF>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y);
+  }
 }
 
 // ---------------------------------------------------------------- x2 / x3 sweeps
@@ -535,6 +543,9 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFiel
 // Steps 0 and 1 (no store) are peeled off so every step of the main loop stores.
 #ifndef SWEEP_PF
 #define SWEEP_PF 2      // x2/x3 march pipeline depth (steps of loads in flight) with 2-3 fields
+#endif
+#ifndef SWEEP1_E7_IN_BLOCK
+#define SWEEP1_E7_IN_BLOCK 1  // x1: electrons as a seventh wave of the species block
 #endif
 #ifndef SWEEP_E7_IN_BLOCK
 #define SWEEP_E7_IN_BLOCK 1   // x2/x3: electrons as a seventh wave of the species block
@@ -1350,12 +1361,16 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
   timing_begin(name, st);
   if (na == 1 && fields == F_ENERGY) {       // energy alone: all seven species in one launch
     sweep_launch<1>(g, s, dir, A, 1, LSP, dt, onthefly, st);
+  } else if (na == 3 && dir == 1 && SWEEP1_E7_IN_BLOCK) {
+    // x1, all fields: the electron wave joins the block and sweeps its energy only
+    A.elec_energy = 1;
+    sweep_launch<3>(g, s, dir, A, 1, LSP, dt, onthefly, st);
   } else if (na == 3 && dir != 1 && SWEEP_E7_IN_BLOCK) {
     // x2/x3, all fields: the electron wave joins the block and does energy only
     A.elec_energy = 1;
     if (tune.sweep_pf == 1) sweep_launch<3, 1>(g, s, dir, A, 1, LSP, dt, onthefly, st);
     else sweep_launch<3, 2>(g, s, dir, A, 1, LSP, dt, onthefly, st);
-  } else {                                   // (x1: six-wave blocks pack the CUs better)
+  } else {                                   // fewer fields: ions, then the electrons
     if (na == 1) sweep_launch<1>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);
     else if (na == 2) sweep_launch<2>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);
     else sweep_launch<3>(g, s, dir, A, 1, LSP - 1, dt, onthefly, st);
