@@ -123,6 +123,35 @@ def test_each_entry_point(kind, shape):
     tr.configure(2, 1500.0)
 
 
+@pytest.mark.parametrize("solver", [2, 1])
+def test_singular_band_solve_keeps_the_right_hand_side(solver):
+    """dgbsv's INFO > 0 path on the GPU: zero temperature on rows 5-9 of two columns of
+    species 2 and of the electrons, no current, and an infinite time step make those rows
+    of the band matrix zero (no conduction and no time term), so the elimination meets a
+    zero pivot; LAPACK then leaves the right-hand side as the solution.  The other columns
+    of the same waves solve normally.  Everything must match the oracle bit for bit."""
+    print(f"case singular band solve, solver {solver}", flush=True)
+    g, st = make("cart", (24, 10, 7), seed=3)
+    for isp in (2, 7):
+        for i2, i3 in ((3, 2), (7, 5)):
+            st.fluidvars[5 + 1:9 + 2, i2 + 1, i3 + 1, 28 + isp - 1] = 0.0
+    st.electrovars[..., 3] = 0.0
+    dt = float("inf")
+    tr = transport_for(g, st)
+    tr.configure(solver, 1500.0)
+    ref, got = st.copy(), st.copy()
+    with oracle.pow_mode(1):
+        oracle.op(g, ref, "energy_diffusion", dt=dt, diffsolvetype=solver)
+    tr.energy_diffusion(got, dt)
+    tr.configure(2, 1500.0)
+    compare(got, ref, exact=ADV + ("Ts",))
+    for isp in (2, 7):
+        for i2, i3 in ((3, 2), (7, 5)):
+            col = ref.fluidvars[2 + 1:g.lx1 + 1, i2 + 1, i3 + 1, 28 + isp - 1]   # rows 2..N-1
+            assert np.all(col == 100.0), (isp, i2, i3)     # the right-hand side, floored
+    print("  singular columns keep the right-hand side; bit-identical to the oracle", flush=True)
+
+
 def test_allparams_equals_three_calls():
     g, st = make("cart", (12, 6, 5), seed=11)
     dt = ga.cfl_dt(g, st)
