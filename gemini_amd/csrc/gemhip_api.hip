@@ -805,10 +805,11 @@ int run_sources_block(double dt) {
   hipStream_t st = c.stream;
   const int* nul = c.d_null;
   launch_set_global_boundaries(g, s, LSP, st);           // :223
-  launch_sources(7, dt);                                 // :225-227
-  launch_clean_param(g, s, 3, nul, st);                  // :229-231
+  launch_sources(3, dt);                                 // :225-226
+  // :227 with clean_param 3 and 1 of :229-231 in the same pass (they touch Ts and ns;
+  // clean_param 2 between them touches vs1 only)
+  launch_source_loss_mass(g, s, sources(), dt, st, nul, true);
   launch_clean_param(g, s, 2, nul, st);
-  launch_clean_param(g, s, 1, nul, st);
   return hipok(hipGetLastError(), "sources block launch") ? 0 : 3;
 }
 

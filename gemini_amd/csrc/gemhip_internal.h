@@ -234,8 +234,10 @@ void launch_source_loss_energy(const DevGrid& g, const DevState& s, const DevSou
                                double dt, hipStream_t st);
 void launch_source_loss_momentum(const DevGrid& g, const DevState& s, const DevSources& a,
                                  double dt, hipStream_t st);
+// clean: also clean_param 3 and 1 of the sources block (null cells from clean_nul)
 void launch_source_loss_mass(const DevGrid& g, const DevState& s, const DevSources& a,
-                             double dt, hipStream_t st);
+                             double dt, hipStream_t st, const int* clean_nul = nullptr,
+                             bool clean = false);
 void launch_zero(double* p, long n, hipStream_t st);
 
 // ---- plasma output (kernels_output.hip) ----

@@ -26,6 +26,7 @@ constexpr double kB = 1.38064852e-23;
 constexpr double elchrg = 1.60217662e-19;
 constexpr double amu = 1.660539040e-27;
 constexpr double mindensdiv = 1.0e-5;
+constexpr double mindens = 1.0e-100, mindensnull = 1.0e-20;
 
 // ms = [16,30,28,32,14,1,5.485799090e-4]*amu (phys_consts.in.f90:43); the product is
 // a single IEEE multiply, identical whether folded at compile time or not.

@@ -14,7 +14,6 @@ namespace gemhip {
 namespace {
 
 constexpr int TPB = 256;
-constexpr double mindens = 1.0e-100, mindensnull = 1.0e-20;   // phys_consts.in.f90:82-83
 constexpr double xicon = 3.0;                                 // multifluid.f90:43
 
 inline unsigned grid_exact(long n) { return (unsigned)((n + TPB - 1) / TPB); }

@@ -443,10 +443,28 @@ __device__ __forceinline__ double poly4(double t, double c0, double c1, double c
 }
 
 // ---- source_loss_mass: srcsContinuity (sources.f90:18-225), Prionize, ETD, ne ----
-__global__ __launch_bounds__(TPB) void k_src_mass(DevGrid g, DevState s, DevSources a, double dt) {
+// CLEAN (the sources block, libgemini.cpp:225-231): clean_param 3 and 1 follow the three
+// source passes, with clean_param 2 (velocities only) between them, so this last pass
+// also applies them to its cell: temperatures floored at 100 K (ghost and null cells at
+// 100 K), ion densities floored at mindens with the electrons their sum (ghost cells at
+// mindensdiv, null cells at mindensnull).  Its own reads of Ts come first.
+template <bool CLEAN>
+__global__ __launch_bounds__(TPB) void k_src_mass(DevGrid g, DevState s, DevSources a, double dt,
+                                                  const int* nul) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   const Cell q = decode(g, c);
+  if (CLEAN) {
+    const bool null = q.interior && nul && nul[q.k] != 0;
+    if (!q.interior || null) {
+#pragma unroll
+      for (int i = 0; i < LSP; ++i) {
+        s.Ts[i * g.N + c] = 100.0;
+        s.ns[i * g.N + c] = !q.interior ? mindensdiv : (i == 0 ? mindensnull * 1e-2 : mindensnull);
+      }
+      return;
+    }
+  }
   double n[LSP];
 #pragma unroll
   for (int i = 0; i < LSP; ++i) n[i] = s.ns[i * g.N + c];
@@ -559,7 +577,12 @@ __global__ __launch_bounds__(TPB) void k_src_mass(DevGrid g, DevState s, DevSour
 #pragma unroll
     for (int i = 0; i < LSP - 1; ++i) {
       n[i] = etd(n[i], P[i] + a.Prionize[i * ni + k], L[i], dt);
+      if (CLEAN) n[i] = fmax_f(n[i], mindens);
       s.ns[i * g.N + c] = n[i];
+    }
+    if (CLEAN) {
+#pragma unroll
+      for (int i = 0; i < LSP; ++i) s.Ts[i * g.N + c] = fmax_f(s.Ts[i * g.N + c], 100.0);
     }
   }
   double sum = 0.0;
@@ -598,9 +621,14 @@ void launch_source_loss_momentum(const DevGrid& g, const DevState& s, const DevS
 }
 
 void launch_source_loss_mass(const DevGrid& g, const DevState& s, const DevSources& a,
-                             double dt, hipStream_t st) {
+                             double dt, hipStream_t st, const int* clean_nul, bool clean) {
   timing_begin("source_mass", st);
-  hipLaunchKernelGGL(k_src_mass, dim3(blocks_of(g.N)), dim3(TPB), 0, st, g, s, a, dt);
+  if (clean)
+    hipLaunchKernelGGL(k_src_mass<true>, dim3(blocks_of(g.N)), dim3(TPB), 0, st, g, s, a, dt,
+                       clean_nul);
+  else
+    hipLaunchKernelGGL(k_src_mass<false>, dim3(blocks_of(g.N)), dim3(TPB), 0, st, g, s, a, dt,
+                       nullptr);
   timing_end("source_mass", st);
 }
 
