@@ -127,10 +127,9 @@ enum BatchKernel : int {
   BK_BC2,    // x2 ghost fill: threads, L1 L3 lsp (times 4 fields in blockIdx.y)
   BK_BC3,    // x3 ghost fill: threads, L1 L2 lsp (times 4 fields in blockIdx.y)
   BK_V1B,    // x1 boundary interface velocities: threads, lx2 lx3 lsp
-  BK_SW3,    // x3 sweep with the conversions: blocks (64 x 7), ceil(lx1/64) lx2
-  BK_SW1,    // x1 sweep of the ions: blocks (64 x 6), lx2 lx3
-  BK_SW1E,   // x1 sweep of the electron energy: blocks (64 x 1), lx2 lx3
-  BK_SW2,    // x2 sweep with rhov12v1 + rhoe2T: blocks (64 x 7), ceil(lx1/64) lx3
+  BK_SW3,    // x3 sweep with the conversions: blocks (64 x 7), sweep_blocks(ceil(lx1/64), lx2)
+  BK_SW1,    // x1 sweep (the electron wave: energy only): blocks (64 x 7), lx2 lx3
+  BK_SW2,    // x2 sweep with rhov12v1 + rhoe2T: blocks (64 x 7), sweep_blocks(.., lx3)
   BK_POST,   // rhov12v1 + rhoe2T outside the interior box: threads, surface cells
   BK_DIFF,   // TRBDF2: blocks (64), 8 ceil(ceil(lx2 lx3/64)/8) lsp (a multiple of 8)
   BK_FLOOR,  // Ts ghost floor: threads, surface cells lsp

@@ -1133,34 +1133,37 @@ __global__ __launch_bounds__(448) void kb_sweep23(const BatchMember* M, const lo
     const int m = batch_member(p, n, t);
     const DevGrid g = M[m].g;                        // by value, as a member's own launch
     const DevState s = M[m].s;                       // receives them
+    // the member's units in the XCD-aware order of its own launch (sweep_block): the
+    // member's first unit and the grid are multiples of 8, so unit u runs on XCD u % 8
     const long u = t - p[m];
-    const int nbx = (g.lx1 + 63) / 64;
+    const int nbx = (g.lx1 + 63) / 64, nl = DIR == 2 ? g.lx3 : g.lx2;
+    const int bx = (int)((u >> 3) % nbx), by = (int)((u >> 3) / nbx) * 8 + (int)(u & 7);
+    if (by >= nl) continue;                        // (the whole block)
     SweepFields F{};
     F.f[0] = s.ns; F.frank[0] = 0;
     F.f[1] = s.rhovs1; F.frank[1] = 1;
     F.f[2] = s.rhoes; F.frank[2] = 0;
     F.elec_energy = 1;
-    sweep23_body<DIR, PF, 3, true, DIR == 3, DIR == 2>(g, s, F, 1, M[m].dt, (int)(u % nbx),
-                                                      (int)(u / nbx), xch, met, ring);
+    sweep23_body<DIR, PF, 3, true, DIR == 3, DIR == 2>(g, s, F, 1, M[m].dt, bx, by, xch, met,
+                                                      ring);
     __syncthreads();                               // the next unit reuses the LDS
   }
 }
 
-// the x1 sweep: ions (blocks of 64 x 6, three fields) or the electron energy (64 x 1)
-template <bool ELEC>
+// the x1 sweep: blocks of 64 x 7, the electron wave sweeping its energy only (the one-field
+// body), as a member's own launch
 __global__ __launch_bounds__(448) void kb_sweep1(const BatchMember* M, const long* pre, int n,
                                                  long total) {
-  constexpr int K = ELEC ? BK_SW1E : BK_SW1;
-  const long* p = pre + (long)K * (n + 1);
+  const long* p = pre + (long)BK_SW1 * (n + 1);
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     const int m = batch_member(p, n, t);
     const DevGrid g = M[m].g;                        // by value, as a member's own launch
     const DevState s = M[m].s;                       // receives them
     const long u = t - p[m];
     SweepFields F{};
-    if (ELEC) {
+    if (threadIdx.y == LSP - 1) {
       F.f[0] = s.rhoes; F.frank[0] = 0;
-      sweep1_body<1, true>(g, s, F, LSP, M[m].dt, (int)(u % g.lx2), (int)(u / g.lx2));
+      sweep1_body<1, true>(g, s, F, 1, M[m].dt, (int)(u % g.lx2), (int)(u / g.lx2));
     } else {
       F.f[0] = s.ns; F.f[1] = s.rhovs1; F.f[2] = s.rhoes;   // x1: every field of rank 0
       sweep1_body<3, true>(g, s, F, 1, M[m].dt, (int)(u % g.lx2), (int)(u / g.lx2));
@@ -1176,9 +1179,9 @@ long batch_units(const DevGrid& g, int kernel) {
   switch (kernel) {
     case BK_BC1: case BK_BC2: case BK_BC3: case BK_V1B: case BK_POST: case BK_FLOOR:
       return (batch_thread_units(g, kernel) + 63) / 64 * 64;
-    case BK_SW3: return (long)((g.lx1 + 63) / 64) * g.lx2;
-    case BK_SW1: case BK_SW1E: return (long)g.lx2 * g.lx3;
-    case BK_SW2: return (long)((g.lx1 + 63) / 64) * g.lx3;
+    case BK_SW3: return sweep_blocks((g.lx1 + 63) / 64, g.lx2);
+    case BK_SW1: return (long)g.lx2 * g.lx3;
+    case BK_SW2: return sweep_blocks((g.lx1 + 63) / 64, g.lx3);
     case BK_DIFF: return (((long)g.lx2 * g.lx3 + 63) / 64 + 7) / 8 * 8 * LSP;
   }
   return 0;
@@ -1188,7 +1191,7 @@ long batch_units(const DevGrid& g, int kernel) {
 // ghost fill (with v12rhov1, T2rhoe), x1 boundary interface velocities, x3 sweep
 // (converting), x1 sweep, x2 sweep (with rhov12v1, rhoe2T), the surface conversions.
 void launch_batch_advection(const BatchTable& t, int sweep_pf, hipStream_t st) {
-  const dim3 thr(TPB), s7(64, LSP), s6(64, LSP - 1), s1(64, 1);
+  const dim3 thr(TPB), s7(64, LSP);
   auto tb = [&](long total) { return (unsigned)std::max(1L, std::min<long>(total, t.blocks_sweep)); };
   auto th = [&](long total) {
     return (unsigned)std::max(1L, std::min<long>((total + TPB - 1) / TPB, t.blocks_thread));
@@ -1207,10 +1210,8 @@ void launch_batch_advection(const BatchTable& t, int sweep_pf, hipStream_t st) {
   else
     hipLaunchKernelGGL((kb_sweep23<3, 2>), dim3(tb(t.total[BK_SW3])), s7, 0, st, t.m, t.pre, t.n,
                        t.total[BK_SW3]);
-  hipLaunchKernelGGL(kb_sweep1<false>, dim3(tb(t.total[BK_SW1])), s6, 0, st, t.m, t.pre, t.n,
+  hipLaunchKernelGGL(kb_sweep1, dim3(tb(t.total[BK_SW1])), s7, 0, st, t.m, t.pre, t.n,
                      t.total[BK_SW1]);
-  hipLaunchKernelGGL(kb_sweep1<true>, dim3(tb(t.total[BK_SW1E]) * 4), s1, 0, st, t.m, t.pre, t.n,
-                     t.total[BK_SW1E]);
   if (sweep_pf == 1)
     hipLaunchKernelGGL((kb_sweep23<2, 1>), dim3(tb(t.total[BK_SW2])), s7, 0, st, t.m, t.pre, t.n,
                        t.total[BK_SW2]);
