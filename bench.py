@@ -553,12 +553,12 @@ def bench_adjoint(fd_dirs, column_dirs=None):
                 dk, dq = dk * mask, dq * mask
             ad = ((gk * dk).sum() + (gq * dq).sum()).item()
             # the configured eps on every direction; on the first few, smaller and larger
-            # steps too; on column directions eps/10 and 10 eps beside it (rounding in the
-            # difference grows as 1/eps, truncation as eps^2)
+            # steps too; on column directions eps/10, 10 eps and 100 eps beside it (rounding
+            # in the difference grows as 1/eps, truncation as eps^2)
             if d < A["fd_sweep_dirs"]:
                 eps_list = A["fd_sweep_eps"]
             elif kind == "column":
-                eps_list = [10 * A["eps"], A["eps"], A["eps"] / 10]
+                eps_list = [100 * A["eps"], 10 * A["eps"], A["eps"], A["eps"] / 10]
             else:
                 eps_list = [A["eps"]]
             by_eps = {}
@@ -577,11 +577,11 @@ def bench_adjoint(fd_dirs, column_dirs=None):
                                  "pivot_flips": fl.get("pivot", 0),
                                  "upwind_flips": sum(v for kk, v in fl.items()
                                                      if kk.startswith("upwind"))})
-            if kind == "column":
+            if len(eps_list) > 1:
                 dirs[-1]["by_eps"] = by_eps
-                if d < A["fd_sweep_dirs"]:
-                    sweep.setdefault(f"{e:.0e}", []).append(
-                        {"rel_err": float(f"{r:.3e}"), "flips": nflip})
+            if d < A["fd_sweep_dirs"]:
+                for ek, v in by_eps.items():
+                    sweep.setdefault(ek, []).append(v)
     run.close()
     if run_m is not None:
         run_m.close()
@@ -598,11 +598,15 @@ def bench_adjoint(fd_dirs, column_dirs=None):
     if dirs:
         rels = sorted(x["rel_err"] for x in dirs)
         zero = [x["rel_err"] for x in dirs if x["flips"] == 0]
+        # the same directions at the best of their swept steps that crosses no branch
+        best = [min(v["rel_err"] for v in x["by_eps"].values() if v["flips"] == 0)
+                for x in dirs if x["flips"] == 0 and "by_eps" in x]
         res["fd_check"] = {
             "directions": len(dirs), "eps": A["eps"], "max_rel_err": rels[-1],
             "median_rel_err": rels[len(rels) // 2],
             "zero_flip_directions": len(zero),
             "max_rel_err_zero_flip": max(zero) if zero else None,
+            "max_rel_err_zero_flip_best_swept_eps": max(best) if best else None,
             "per_direction": dirs, "eps_sweep": sweep}
     log(f"adjoint: forward {res['forward_ms_per_step']} ms/step, adjoint "
         f"{res['adjoint_ms_per_step']} ms/step, ratio {res['ratio']}" +
