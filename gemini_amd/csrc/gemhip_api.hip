@@ -320,7 +320,7 @@ bool ensure_scratch() {
   return true;
 }
 
-bool run_energy_diffusion(double dt) {
+bool run_energy_diffusion(double dt, bool floor_ghosts = true) {
   Ctx& c = ctx();
   if (c.diffsolvetype != 1 && c.diffsolvetype != 2) {
     fail("Unsupported diffusion solver type/mode:  %d.  Should be either 1 or 2.",
@@ -337,7 +337,8 @@ bool run_energy_diffusion(double dt) {
     return false;
   }
   if (!ensure_scratch()) return false;
-  launch_energy_diffusion(c.g, c.s, dt, c.diffsolvetype, c.Teinf, c.d_scratch, c.stream, c.tune);
+  launch_energy_diffusion(c.g, c.s, dt, c.diffsolvetype, c.Teinf, c.d_scratch, c.stream, c.tune,
+                          floor_ghosts);
   return hipok(hipGetLastError(), "energy diffusion launch");
 }
 
@@ -705,7 +706,9 @@ int run_step(double dt, double* tape) {
   // a recorded step also keeps the electron columns' row interchanges (written by the
   // fitting extension's kernel, the one a recorded run uses)
   if (tape) c.s.pivrec = tape_view(tape).PIV;
-  const bool ok = run_energy_diffusion(dt);              // :208
+  // :208; the conversions after the x2 sweep floored every ghost temperature at 100 K
+  // already (multifluid.f90:340's floor is idempotent there)
+  const bool ok = run_energy_diffusion(dt, false);
   c.s.pivrec = nullptr;
   return ok ? 0 : 4;
 }

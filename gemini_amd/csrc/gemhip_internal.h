@@ -175,9 +175,11 @@ struct BatchTable {
 void launch_batch_advection(const BatchTable& t, int sweep_pf, hipStream_t st);
 void launch_batch_diffusion(const BatchTable& t, int diff_chunk, hipStream_t st);
 void launch_batch_floor(const BatchTable& t, hipStream_t st);
+// floor_ghosts: then Ts of the ghost cells floored at 100 K (a caller whose ghost
+// temperatures are floored already skips it: the solve writes interior rows only)
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
                              double Teinf, double* scratch, hipStream_t st,
-                             const Tuning& tune = Tuning{});
+                             const Tuning& tune = Tuning{}, bool floor_ghosts = true);
 
 // ---- the rest of fluid_adv's transport block (kernels_block.hip) ----
 // iparm 1 density, 2 velocity (vs1), 3 temperature; nul: null-cell mask (lx1*lx2*lx3,

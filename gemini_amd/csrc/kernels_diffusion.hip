@@ -642,11 +642,13 @@ void launch_batch_diffusion(const BatchTable& t, int diff_chunk, hipStream_t st)
   else
     hipLaunchKernelGGL(kb_diffusion<DIFF_CHUNK>, dim3(b ? b : 1), dim3(64), 0, st, t.m, t.pre, t.n,
                        total);
-  launch_batch_floor(t, st);
+  // (no ghost floor: the batched step's surface conversions floored the ghost temperatures
+  // at 100 K, and the solve writes interior rows only)
 }
 
 void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int diffsolvetype,
-                             double Teinf, double* scratch, hipStream_t st, const Tuning& tune) {
+                             double Teinf, double* scratch, hipStream_t st, const Tuning& tune,
+                             bool floor_ghosts) {
   Coef tr{}, bdf{}, be{};
   tr.dt2 = dt / 2.0; tr.K = 1.0 / tr.dt2; tr.dt = dt;
   bdf.dt3 = dt / 3.0; bdf.K = 1.0 / bdf.dt3; bdf.dt = dt;
@@ -656,6 +658,7 @@ void launch_energy_diffusion(const DevGrid& g, const DevState& s, double dt, int
   if (tune.diff_chunk == 4) diffusion_launch<4>(g, s, diffsolvetype, Teinf, scratch, tr, bdf, be, st);
   else diffusion_launch<DIFF_CHUNK>(g, s, diffsolvetype, Teinf, scratch, tr, bdf, be, st);
   timing_end("trbdf2", st);
+  if (!floor_ghosts) return;
   timing_begin("floor_ghosts", st);
   launch_floor_ghosts(g, s.Ts, st);
   timing_end("floor_ghosts", st);
