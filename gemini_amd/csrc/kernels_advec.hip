@@ -406,7 +406,7 @@ struct S1Chunk {
   double dxp1, dxim;        // dx1(p+1), dx1i(p-2)
 };
 
-template <int NQ, bool OTF>
+template <int NQ, bool OTF, bool LM = false>
 __device__ __forceinline__ void s1_load(const DevGrid& g, double* const* fp, const double* u,
                                         const double* v1i, long c0, long if0, int p,
                                         S1Chunk<NQ>& d) {
@@ -420,6 +420,7 @@ __device__ __forceinline__ void s1_load(const DevGrid& g, double* const* fp, con
     d.fn[q] = fp[q][pn];
   }
   d.u = OTF ? u[pc] : v1i[if0 + pi];
+  if (LM) return;                            // (the line's factors are in LDS)
   d.i1 = g.h1x1i[if0 + pi]; d.i2 = g.h2x1i[if0 + pi]; d.i3 = g.h3x1i[if0 + pi];
   d.c1 = g.h1[c0 + pcell]; d.c2 = g.h2[c0 + pcell]; d.c3 = g.h3[c0 + pcell];
   d.dxp1 = g.dx1[clampi(p + 1, 0, lx1 + 2)];
@@ -432,26 +433,39 @@ struct S1Carry {
   double u, h2, ha, dx;
 };
 
-template <int NQ, bool OTF>
+template <int NQ, bool OTF, bool LM = false>
 __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, const double* u,
                                          const double* v1i, long c0, long if0, int c, int lane,
                                          double dt, double vb1, double vbN,
                                          const S1Chunk<NQ>& cur, S1Chunk<NQ>& nxt,
-                                         S1Carry<NQ>& k) {
+                                         S1Carry<NQ>& k, const double* lm) {
   const int lx1 = g.lx1;
   const int p = 64 * c + lane - 1;
-  s1_load<NQ, OTF>(g, fp, u, v1i, c0, if0, p + 64, nxt);                  // next chunk
+  s1_load<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, p + 64, nxt);              // next chunk
+  // h1x1i, h2x1i h3x1i (interface p), (h1 h2) h3 (cell p), dx1(p+1), dx1i(p-2)
+  double i1v, h2p, hap, dx_p, dxim;
+  if (LM) {
+    const int L = lx1 + 3, pi = clampi(p, 1, lx1 + 1);
+    i1v = lm[pi];
+    h2p = lm[L + pi];
+    hap = lm[2 * L + clampi(p, 1, lx1)];
+    dx_p = lm[3 * L + clampi(p + 1, 0, lx1 + 2)];
+    dxim = lm[4 * L + clampi(p - 2, 0, lx1 - 1)];
+  } else {
+    i1v = cur.i1;
+    h2p = cur.i2 * cur.i3;
+    hap = (cur.c1 * cur.c2) * cur.c3;
+    dx_p = cur.dxp1;
+    dxim = cur.dxim;
+  }
   const bool upd = p >= 2 && p <= lx1 + 1;
   const double um = lane_up1(cur.u, k.u);                                  // vs1(p-1)
   double v;
   if (OTF) v = p == 1 ? vb1 : (p == lx1 + 1 ? vbN : 0.5 * (um + cur.u));
   else v = cur.u;
-  const double vq = v / cur.i1;
-  const double h2p = cur.i2 * cur.i3;
+  const double vq = v / i1v;
   const double h2m = lane_up1(h2p, k.h2);
-  const double hap = (cur.c1 * cur.c2) * cur.c3;                           // ha1 of cell p
   const double ha1m = lane_up1(hap, k.ha);                                 // ha1 of cell p-1
-  const double dx_p = cur.dxp1;                                            // dx(p+1)
   const double dx_0 = lane_up1(dx_p, k.dx);                                // dx(p)
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -466,7 +480,7 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
     const double phv = fm1 * v + ((0.5 * v) * (dx_0 - vq * dt)) * slm;
     const double ph = v < 0.0 ? phn : phv;                                 // phi(p), p >= 1
     const double phm = lane_up1(ph, k.ph[q]);                              // phi(p-1)
-    const double out = fm1 - ((dt * (h2p * ph - h2m * phm)) / cur.dxim) / ha1m;
+    const double out = fm1 - ((dt * (h2p * ph - h2m * phm)) / dxim) / ha1m;
     // cells 1..lx1 take the update; the other lanes put the ghost value they cover back
     fp[q][clampi(p - 1, -1, lx1 + 2)] = upd ? out : (p < 0 ? f0 : fm1);
     k.rs[q] = lane63(rs);
@@ -481,10 +495,10 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
 }
 
 // line (i2, i3) = (bx + 1, by + 1)
-template <int NQ, bool OTF>
+template <int NQ, bool OTF, bool LM = false>
 __device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
                                             const SweepFields& F, int isp0, double dt, int bx,
-                                            int by) {
+                                            int by, const double* lm = nullptr) {
   const int lx1 = g.lx1;
   const int i2 = bx + 1, i3 = by + 1;
   const int lane = threadIdx.x, isp = isp0 + threadIdx.y;
@@ -510,24 +524,43 @@ __device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
   k.ha = k.dx = 1.0;
   const int nchunk = (lx1 + 3 + 63) / 64;                                 // p = -1 .. lx1+1
   S1Chunk<NQ> A, B;
-  s1_load<NQ, OTF>(g, fp, u, v1i, c0, if0, lane - 1, A);
+  s1_load<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, lane - 1, A);
   int c = 0;
   for (; c + 1 < nchunk; c += 2) {
-    s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k);
-    s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c + 1, lane, dt, vb1, vbN, B, A, k);
+    s1_chunk<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k, lm);
+    s1_chunk<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, c + 1, lane, dt, vb1, vbN, B, A, k, lm);
   }
-  if (c < nchunk) s1_chunk<NQ, OTF>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k);
+  if (c < nchunk) s1_chunk<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k, lm);
 }
-template <int NQ, bool OTF>
+// LM: the line's metric factors in LDS (dynamic, sweep1_lm_bytes): the block loads h1x1i,
+// the products h2x1i h3x1i and (h1 h2) h3, dx1 and dx1i of its line once, in the order the
+// walk forms them, instead of every wave loading them for every chunk
+__host__ __device__ inline int sweep1_lm_bytes(int lx1) { return 5 * (lx1 + 3) * 8; }
+template <int NQ, bool OTF, bool LM = false>
 __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFields F, int isp0,
                                                 double dt) {
+  extern __shared__ double lm[];
+  if (LM) {
+    const int lx1 = g.lx1, L = lx1 + 3, i2 = blockIdx.x + 1, i3 = blockIdx.y + 1;
+    const long c0 = gix(g, 0, i2, i3);
+    const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;
+    for (int t = threadIdx.y * blockDim.x + threadIdx.x; t < L; t += blockDim.x * blockDim.y) {
+      const int pi = clampi(t, 1, lx1 + 1), pc = clampi(t, 1, lx1);
+      lm[t] = g.h1x1i[if0 + pi];
+      lm[L + t] = g.h2x1i[if0 + pi] * g.h3x1i[if0 + pi];
+      lm[2 * L + t] = (g.h1[c0 + pc] * g.h2[c0 + pc]) * g.h3[c0 + pc];
+      lm[3 * L + t] = g.dx1[min(t, lx1 + 2)];
+      lm[4 * L + t] = g.dx1i[min(t, lx1 - 1)];
+    }
+    __syncthreads();
+  }
   if (NQ == 3 && F.elec_energy && isp0 + (int)threadIdx.y == LSP) {
     // the electron wave of an all-species block sweeps its energy only
     SweepFields E{};
     E.f[0] = F.f[2];
-    sweep1_body<1, OTF>(g, s, E, isp0, dt, blockIdx.x, blockIdx.y);
+    sweep1_body<1, OTF, LM>(g, s, E, isp0, dt, blockIdx.x, blockIdx.y, lm);
   } else {
-    sweep1_body<NQ, OTF>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y);
+    sweep1_body<NQ, OTF, LM>(g, s, F, isp0, dt, blockIdx.x, blockIdx.y, lm);
   }
 }
 
@@ -543,6 +576,9 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFiel
 // Steps 0 and 1 (no store) are peeled off so every step of the main loop stores.
 #ifndef SWEEP_PF
 #define SWEEP_PF 2      // x2/x3 march pipeline depth (steps of loads in flight) with 2-3 fields
+#endif
+#ifndef SWEEP1_LM_MAX
+#define SWEEP1_LM_MAX 49152   // x1 sweep: largest LDS copy of a line's metric factors (bytes)
 #endif
 #ifndef SWEEP1_E7_IN_BLOCK
 #define SWEEP1_E7_IN_BLOCK 1  // x1: electrons as a seventh wave of the species block
@@ -1365,7 +1401,12 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
   } else if (na == 3 && dir == 1 && SWEEP1_E7_IN_BLOCK) {
     // x1, all fields: the electron wave joins the block and sweeps its energy only
     A.elec_energy = 1;
-    sweep_launch<3>(g, s, dir, A, 1, LSP, dt, onthefly, st);
+    if (onthefly && sweep1_lm_bytes(g.lx1) <= SWEEP1_LM_MAX) {
+      hipLaunchKernelGGL((k_sweep1<3, true, true>), dim3(g.lx2, g.lx3), dim3(64, LSP),
+                         sweep1_lm_bytes(g.lx1), st, g, s, A, 1, dt);
+    } else {
+      sweep_launch<3>(g, s, dir, A, 1, LSP, dt, onthefly, st);
+    }
   } else if (na == 3 && dir != 1 && SWEEP_E7_IN_BLOCK) {
     // x2/x3, all fields: the electron wave joins the block and does energy only
     A.elec_energy = 1;
