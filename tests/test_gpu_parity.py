@@ -123,6 +123,28 @@ def test_each_entry_point(kind, shape):
     tr.configure(2, 1500.0)
 
 
+def test_long_lines_match_oracle():
+    """lx1 = 1300: x1 lines too long for the x1 sweep's LDS copy of their metric factors
+    (the register path), and 1300-row TRBDF2 columns.  Three steps against the oracle with
+    the GPU's x**2.5, every variable bit-identical.  (Against libm's pow the temperatures
+    of such long columns differ by 1.2e-12 normwise: the one-ulp differences of x**2.5
+    pass through a longer implicit solve.)"""
+    print("case cart (1300, 4, 3)", flush=True)
+    g, st = make("cart", (1300, 4, 3))
+    dt = ga.cfl_dt(g, st)
+    tr = transport_for(g, st)
+    tr.set_residency(1)
+    ref, got = st.copy(), st.copy()
+    tr.push(got)
+    with oracle.pow_mode(1):
+        for k in range(3):
+            oracle.transport_step(g, ref, dt)
+            tr.transport_step(dt)
+    tr.pull(got)
+    compare(got, ref, exact=ADV + ("Ts", "vs2", "vs3"))
+    print("  3 steps bit-identical", flush=True)
+
+
 @pytest.mark.parametrize("solver", [2, 1])
 def test_singular_band_solve_keeps_the_right_hand_side(solver):
     """dgbsv's INFO > 0 path on the GPU: zero temperature on rows 5-9 of two columns of
