@@ -35,9 +35,28 @@ __device__ __forceinline__ void cell_load(const DevGrid& g, const DevState& s, l
   }
 }
 
-// index in DevGrid::hcol of row r (1..lx1) of column gcol
+// The diffusion's column groups: 64 columns each, one lane per column.  Neighbouring
+// columns start lx1 + 4 cells apart, so with lx1 a multiple of 8 (and lx2 even, which
+// keeps the parity across x3 rows) the rows of even and of odd columns lie 4 cells (32
+// bytes) apart in their 64-byte sectors.  Then the two groups of each full block of 128
+// columns take its even and its odd columns, and all columns of a group share one sector
+// phase.  Otherwise (and in the last, partial block) group cg holds columns 64 cg .. 64 cg
+// + 63, lanes past the last column repeating it.
+__device__ __forceinline__ bool group_interleaved(const DevGrid& g, long cg) {
+  return g.lx1 % 8 == 0 && g.lx2 % 2 == 0 && ((cg >> 1) + 1) * 128 <= (long)g.lx2 * g.lx3;
+}
+__device__ __forceinline__ int group_column(const DevGrid& g, long cg, int lane) {
+  if (group_interleaved(g, cg)) return (int)((cg >> 1) * 128 + 2 * lane + (cg & 1));
+  return (int)min(cg * 64 + lane, (long)g.lx2 * g.lx3 - 1);
+}
+
+// index in DevGrid::hcol of row r (1..lx1) of column gcol: group-major, then row, then lane
 __device__ __forceinline__ long hcol_index(const DevGrid& g, int gcol, int r) {
-  return ((long)(gcol >> 6) * g.lx1 + (r - 1)) * 64 + (gcol & 63);
+  const long cg2 = (long)(gcol >> 7) * 2;
+  const bool il = group_interleaved(g, cg2);
+  const long cg = il ? cg2 + (gcol & 1) : gcol >> 6;
+  const int lane = il ? (gcol & 127) >> 1 : gcol & 63;
+  return (cg * g.lx1 + (r - 1)) * 64 + lane;
 }
 
 // conductivity (collisions.f90:162-199), times the column scale under the extension

@@ -22,7 +22,8 @@ struct DevGrid {
   const double *h1x3i, *h2x3i, *h3x3i;
   const double *dx1, *dx1i, *dx2, *dx2i, *dx3, *dx3i;
   // h1, h2, h3 of the interior rows in the energy diffusion's column-group order: factor m
-  // of row r (1..lx1) of column gcol at hcol[m hstride + ((gcol/64) lx1 + r-1) 64 + gcol%64]
+  // of row r (1..lx1) of column gcol at hcol[m hstride + (cg lx1 + r-1) 64 + lane], (cg,
+  // lane) the column's group and lane (hcol_index in diffusion_cell.h)
   const double* hcol;
   long hstride;
 };

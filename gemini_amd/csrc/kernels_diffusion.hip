@@ -187,8 +187,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
                                                 double (*sD)[LP]) {
   const long ncol = (long)g.lx2 * g.lx3;
   const int lane = threadIdx.x;
-  const long colbase0 = cg * 64;                     // first column of the group
-  const int col = (int)min(colbase0 + lane, ncol - 1);
+  const int col = group_column(g, cg, lane);
   const int N = g.lx1;
   const long grp = (long)(isp - 1) * gps + cg;
   Group q;
@@ -225,12 +224,26 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   constexpr int BATCH = ELEC ? DIFF_BATCH_E : DIFF_BATCH_I;
   static_assert(R % BATCH == 0 || R < BATCH, "chunk and batch sizes");
   constexpr int NB = R < BATCH ? R : BATCH;
+  // Row phase of the chunk windows: a column's row 1 sits 2 cells into its stride, so
+  // windows of R rows from row 1 straddle 64-byte sectors, and a sector's other part is
+  // fetched again a chunk later (long after the L2 dropped it).  The wave's windows start
+  // ph rows early instead, which puts the windows of lane 0's column, and of every column
+  // that shares its sector phase (all of an interleaved group's), on sector boundaries.
+  // The steps before row 1 are not real.  The electrons' neutral densities are stored without
+  // ghosts and are aligned at row 1, so their waves keep phase 0 in pass 1.
+  int c2, c3;
+  column_of(g, group_column(g, cg, 0), c2, c3);
+  const int pos1 = (int)(((unsigned long)(s.Ts + so + gix(g, 0, c2, c3) + 1) >> 3) & 7);
+  constexpr int RA = R < 8 ? R : 8;                  // (R a power of two)
+  const int ph1 = ELEC ? 0 : pos1 % RA;              // pass 1: windows from row 1 - ph1
+  const int ph3 = (pos1 + 1) % RA;                   // pass 3: windows from row 2 - ph3
+  const int pho = pos1 % RA;                         // output chunks from row 1 - pho
   int cbk[R];                                        // gix(0, i2, i3) of the columns this lane loads
   int kbk[R];                                        // and their interior offsets lx1*(column)
   int gck[R];                                        // and column numbers (extension only)
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    const int gcol = (int)min(colbase0 + k * CPI + cs, ncol - 1);
+    const int gcol = group_column(g, cg, k * CPI + cs);
     int a2, a3;
     column_of(g, gcol, a2, a3);
     cbk[k] = (int)gix(g, 0, a2, a3);
@@ -240,7 +253,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 
   // load and convert the rows r0..r0+R-1 of the 64 columns into LDS (rows past N repeat N)
   auto load_chunk = [&](int r0) {
-    const int r = min(r0 + lr, N);
+    const int r = clampi(r0 + lr, 1, N);
     // hcol slot of this lane's first column at row r: the group's 64 columns are contiguous
     // (slots past the last column repeat it, as these lanes do)
     const long hrow = ((long)cg * N + (r - 1)) * 64 + cs;
@@ -262,11 +275,11 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   };
 
   // walk the rows in chunks; row r's arrival completes the coefficients of row ro = r-1
-  // (2..N-1), handed to `row(ro, real, t1f, t2f, t3f, Tm, Tc, Tp)`; rows 1 and 2 and rows
+  // (2..N-1), handed to `row(ro, real, t1f, t2f, t3f, Tm, Tc, Tp)`; rows before 3 and rows
   // past N are not real and the handler must leave its state alone for them
   auto rows = [&](auto row) {
     double Tm = 0.0, Tc = 0.0, Cc = 0.0, Bc = 0.0, Dm = 0.0, Dc = 1.0;
-    for (int r0 = 1; r0 <= N; r0 += R) {
+    for (int r0 = 1 - ph1; r0 <= N; r0 += R) {
       __builtin_amdgcn_wave_barrier();
       load_chunk(r0);
       __builtin_amdgcn_wave_barrier();
@@ -302,18 +315,20 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 
   // dtbsv over stored pivot rows (slots s0 s1 s2, rhs sy), column-oriented: row i takes
   // x(i+2)*U2 off first, then x(i+1)*U1, and divides by U0; a zero x(j) contributes
-  // nothing.  Rows are taken in the aligned chunks [kR+1, kR+R] from the top; each
-  // chunk's results go to `put`, then `flush(k)` runs.
-  auto backsub = [&](int s0, int s1, int s2, int sy, bool singular, auto put, auto flush) {
+  // nothing.  Rows are taken in the chunks [kR+1-ph, kR+R-ph] from the top; each chunk's
+  // results go to `put` (rows past N and before 1 too, which must land nowhere that
+  // matters: rows past N come first and are put on row N), then `flush(k)` runs.
+  auto backsub = [&](int s0, int s1, int s2, int sy, bool singular, int ph, auto put,
+                     auto flush) {
     double pu0[R], pu1[R], pu2[R], py[R];
-    const int kt = (N - 1) / R;                      // top chunk
+    const int kt = (N - 1 + ph) / R;                 // top chunk
     auto fetch = [&](int i, int u) {
       const int ic = clampi(i, 1, N);
       pu0[u] = get2(q, ic, s0); pu1[u] = get2(q, ic, s1); pu2[u] = get2(q, ic, s2);
       py[u] = get2(q, ic, sy);
     };
 #pragma unroll
-    for (int u = 0; u < R; ++u) fetch(kt * R + 1 + u, u);     // row i lives in slot (i-1) % R
+    for (int u = 0; u < R; ++u) fetch(kt * R + 1 - ph + u, u);  // row i: slot (i-1+ph) % R
     double x1 = 0.0, x2 = 0.0;
     auto row = [&](int i, int u, bool live) {
       const double u0 = pu0[u], u1 = pu1[u], u2 = pu2[u], y = py[u];
@@ -333,26 +348,28 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     // top chunk: rows above N are skipped (their put lands on row N before row N's own)
 #pragma unroll
     for (int u = 0; u < R; ++u) {
-      const int i = kt * R + R - u;
+      const int i = kt * R + R - u - ph;
       row(i, R - 1 - u, i <= N);
     }
     flush(kt);
     for (int k = kt - 1; k >= 0; --k) {
 #pragma unroll
-      for (int u = 0; u < R; ++u) row(k * R + R - u, R - 1 - u, true);
+      for (int u = 0; u < R; ++u) row(k * R + R - u - ph, R - 1 - u, true);
       flush(k);
     }
   };
 
-  // final temperatures go out through LDS, one aligned chunk of R rows at a time
-  auto put_out = [&](int i, int u, double x) { sT[(i - 1) % R][lane] = fmax_f(x, 100.0); };
+  // final temperatures go out through LDS, one chunk of R rows (phase pho) at a time; the
+  // slots of rows past N and before 1 are never stored (those lanes store row N or 1 again)
+  auto put_out = [&](int, int u, double x) { sT[u][lane] = fmax_f(x, 100.0); };
   auto flush_out = [&](int k) {
     __builtin_amdgcn_wave_barrier();
-    const int rr = min(k * R + 1 + lr, N);
+    const int r0 = k * R + 1 - pho;
+    const int rr = clampi(r0 + lr, 1, N);
 #pragma unroll
     for (int kk = 0; kk < R; ++kk) {
       const int cc = kk * CPI + cs;
-      s.Ts[so + cbk[kk] + rr] = sT[rr - k * R - 1][cc];
+      s.Ts[so + cbk[kk] + rr] = sT[rr - r0][cc];
     }
     __builtin_amdgcn_wave_barrier();
   };
@@ -418,10 +435,10 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   // pass 2
   __builtin_amdgcn_wave_barrier();
   if (BE) {
-    backsub(SU0, SU1, SU2, SY, e.singular, put_out, flush_out);
+    backsub(SU0, SU1, SU2, SY, e.singular, pho, put_out, flush_out);
     return;
   }
-  backsub(SU0, SU1, SU2, SY, e.singular, [&](int i, int u, double x) { put2(q, i, STR, x); },
+  backsub(SU0, SU1, SU2, SY, e.singular, 0, [&](int i, int u, double x) { put2(q, i, STR, x); },
           [&](int) {});
 
   if (PH == 1) {
@@ -441,8 +458,8 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     double ptr[PF], pl[PF];
     int pb[PF];
     double tv[R];
-    auto issue_T = [&](int r0) {                     // rows r0..r0+R-1 (past N: row N)
-      const int rr = min(r0 + lr, N);
+    auto issue_T = [&](int r0) {                     // rows r0..r0+R-1 (clamped to 1..N)
+      const int rr = clampi(r0 + lr, 1, N);
 #pragma unroll
       for (int kk = 0; kk < R; ++kk) tv[kk] = s.Ts[so + cbk[kk] + rr];
     };
@@ -450,23 +467,24 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
 #pragma unroll
       for (int kk = 0; kk < R; ++kk) buf[lr][kk * CPI + cs] = tv[kk];
     };
+    const int t0 = 2 - ph3;                          // first row of chunk 0
     __builtin_amdgcn_wave_barrier();
-    issue_T(2);
+    issue_T(t0);
     store_T(sT);
     __builtin_amdgcn_wave_barrier();
-    issue_T(2 + R);
+    issue_T(t0 + R);
     auto fetch = [&](int r, int u) {                 // inputs of step j = r-1
       const int rc = clampi(r, 1, N), jc = clampi(r - 1, 1, N - 1);
       ptr[u] = get2(q, rc, STR); pl[u] = get2(q, jc, SL);
       pb[u] = q.swb[(jc >> 5) * 64];
     };
 #pragma unroll
-    for (int u = 0; u < PF; ++u) fetch(2 + u, u);
+    for (int u = 0; u < PF; ++u) fetch(t0 + u, u);
     const bool fs = f.singular;
     S2(q, 1, SY) = Tmin;
     double rb = Tmin;
-    int ch = 0;                                      // chunk of rows 2 + ch R ..
-    for (int j0 = 1; j0 <= N - 1; j0 += PF, ++ch) {
+    int ch = 0;                                      // chunk of rows t0 + ch R ..
+    for (int j0 = t0 - 1; j0 <= N - 1; j0 += PF, ++ch) {
       double (*tb)[LP] = (ch & 1) ? sC : sT;
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
@@ -474,7 +492,7 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
         const double at = -((bdf.c13 * tb[u][lane]) / bdf.dt3), trr = ptr[u], l = pl[u];
         const int bits = pb[u];
         fetch(r + PF, u);
-        const bool live = j <= N - 1;
+        const bool live = j >= 1 && j <= N - 1;
         const double d = r >= N ? Tmax : at + (bdf.c43 * trr) / bdf.dt3;
         const bool swap = (bits >> (j & 31)) & 1;
         const double pd = swap ? d : rb;
@@ -487,13 +505,13 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
       __builtin_amdgcn_wave_barrier();
       store_T((ch & 1) ? sT : sC);                   // chunk ch + 1
       __builtin_amdgcn_wave_barrier();
-      issue_T(2 + (ch + 2) * R);
+      issue_T(t0 + (ch + 2) * R);
     }
     S2(q, N, SY) = fs ? Tmax : rb;
   }
 
   // pass 4
-  backsub(SV0, SV1, SV2, SY, f.singular, put_out, flush_out);
+  backsub(SV0, SV1, SV2, SY, f.singular, pho, put_out, flush_out);
 }
 
 // PH 0: the whole solve; for a solve split in two launches, PH 1 runs passes 1-2 and
@@ -582,7 +600,7 @@ __global__ void k_fill_hcol(DevGrid g, double* hcol) {
     const int lane = (int)(t & 63);
     const long q = t >> 6;
     const int r = (int)(q % g.lx1) + 1;
-    const long gcol = min((q / g.lx1) * 64 + lane, ncol - 1);
+    const long gcol = group_column(g, q / g.lx1, lane);
     const long cb = gix(g, 0, (int)(gcol % g.lx2) + 1, (int)(gcol / g.lx2) + 1);
     hcol[t] = g.h1[cb + r];
     hcol[g.hstride + t] = g.h2[cb + r];
