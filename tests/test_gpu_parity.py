@@ -38,7 +38,11 @@ CASES = [("cart", (128, 64, 1), 0.5),        # the 2-D Cartesian configuration
          ("cart", (24, 10, 7), None),
          ("cart", (70, 6, 5), None),           # x1 blocks of 64 and 6 cells
          ("cart", (23, 10, 7), None),          # odd rows: x2/x3 loads through registers
-         ("dipole", (32, 12, 6), None)]
+         ("dipole", (32, 12, 6), None),
+         # TRBDF2 column groups: a block of 128 columns split into its even and odd
+         # columns, then a partial group of 34 (42) columns
+         ("cart", (16, 18, 9), None),
+         ("dipole", (24, 34, 5), None)]
 
 
 @pytest.mark.parametrize("kind,shape,dt", CASES)
