@@ -102,7 +102,10 @@ void launch_halo_periodic(const DevGrid& g, double* const* fields, int nfields, 
 void launch_interface_vels(const DevGrid& g, const DevState& s, int lsp, hipStream_t st);
 void launch_save_v1_boundaries(const DevGrid& g, const DevState& s, hipStream_t st);
 // dir 1,2,3; fields = F_* bits; onthefly: interface velocities from the cell velocities
-// (bit-identical to the materialised ones) instead of intvars%vs*i.
+// (bit-identical to the materialised ones) instead of intvars%vs*i.  An on-the-fly x1
+// sweep of all fields reads the x1-boundary interface velocities from s.v1b, so the
+// caller must have run launch_save_v1_boundaries on the same vs1 (the step does, before
+// its x3 sweep; no sweep writes vs1).
 void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, double dt,
                   bool onthefly, hipStream_t st, const Tuning& tune = Tuning{});
 

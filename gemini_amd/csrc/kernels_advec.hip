@@ -397,6 +397,18 @@ __device__ __forceinline__ double lane63(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// x / b from y = RN(1/b), correctly rounded: q0 = RN(x y) is within a few ulps of x/b,
+// the FMA residual x - b q0 is exact, and one correction step q0 + r y, rounded once,
+// gives RN(x/b) (Markstein's theorem for a correctly rounded reciprocal).  Where the
+// residual is zero (exact quotients, zero numerators: q0 has the sign IEEE division gives)
+// or not a number (infinite numerators) q0 is the quotient.
+__device__ __forceinline__ double div_rcp(double x, double b, double y) {
+  const double q0 = x * y;
+  const double r = __builtin_fma(-b, q0, x);
+  const double q1 = __builtin_fma(r, y, q0);
+  return __builtin_amdgcn_class(r, 0x63) ? q0 : q1;    // sNaN qNaN -0 +0
+}
+
 template <int NQ>
 struct S1Chunk {
   double f[NQ], fn[NQ];     // f(p), f(p+1)
@@ -406,7 +418,7 @@ struct S1Chunk {
   double dxp1, dxim;        // dx1(p+1), dx1i(p-2)
 };
 
-template <int NQ, bool OTF, bool LM = false>
+template <int NQ, bool OTF, int LM = 0>
 __device__ __forceinline__ void s1_load(const DevGrid& g, double* const* fp, const double* u,
                                         const double* v1i, long c0, long if0, int p,
                                         S1Chunk<NQ>& d) {
@@ -433,7 +445,7 @@ struct S1Carry {
   double u, h2, ha, dx;
 };
 
-template <int NQ, bool OTF, bool LM = false>
+template <int NQ, bool OTF, int LM = 0>
 __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, const double* u,
                                          const double* v1i, long c0, long if0, int c, int lane,
                                          double dt, double vb1, double vbN,
@@ -444,13 +456,22 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
   s1_load<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, p + 64, nxt);              // next chunk
   // h1x1i, h2x1i h3x1i (interface p), (h1 h2) h3 (cell p), dx1(p+1), dx1i(p-2)
   double i1v, h2p, hap, dx_p, dxim;
+  double ri1 = 0.0, rha1m = 0.0, rdx = 0.0, rdxs = 0.0, rdxim = 0.0;   // their reciprocals
+  const int L = lx1 + 3;
   if (LM) {
-    const int L = lx1 + 3, pi = clampi(p, 1, lx1 + 1);
+    const int pi = clampi(p, 1, lx1 + 1);
     i1v = lm[pi];
     h2p = lm[L + pi];
     hap = lm[2 * L + clampi(p, 1, lx1)];
     dx_p = lm[3 * L + clampi(p + 1, 0, lx1 + 2)];
     dxim = lm[4 * L + clampi(p - 2, 0, lx1 - 1)];
+    if (LM == 2) {
+      ri1 = lm[5 * L + pi];
+      rha1m = lm[6 * L + clampi(p - 1, 1, lx1)];
+      rdx = lm[7 * L + clampi(p + 1, 0, lx1 + 2)];
+      rdxs = lm[8 * L + clampi(p + 1, 0, lx1 + 2)];
+      rdxim = lm[9 * L + clampi(p - 2, 0, lx1 - 1)];
+    }
   } else {
     i1v = cur.i1;
     h2p = cur.i2 * cur.i3;
@@ -463,7 +484,7 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
   double v;
   if (OTF) v = p == 1 ? vb1 : (p == lx1 + 1 ? vbN : 0.5 * (um + cur.u));
   else v = cur.u;
-  const double vq = v / i1v;
+  const double vq = LM == 2 ? div_rcp(v, i1v, ri1) : v / i1v;
   const double h2m = lane_up1(h2p, k.h2);
   const double ha1m = lane_up1(hap, k.ha);                                 // ha1 of cell p-1
   const double dx_0 = lane_up1(dx_p, k.dx);                                // dx(p)
@@ -471,16 +492,18 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
   for (int q = 0; q < NQ; ++q) {
     const double f0 = cur.f[q], f1 = cur.fn[q];
     const double fm1 = lane_up1(f0, k.f[q]);
-    const double rs = (f1 - f0) / dx_p;                                    // rs(p)
+    const double rs = LM == 2 ? div_rcp(f1 - f0, dx_p, rdx) : (f1 - f0) / dx_p;  // rs(p)
     const double rsm = lane_up1(rs, k.rs[q]);                              // rs(p-1)
-    const double cs = (f1 - fm1) / (dx_0 + dx_p);
+    const double cs = LM == 2 ? div_rcp(f1 - fm1, dx_0 + dx_p, rdxs) : (f1 - fm1) / (dx_0 + dx_p);
     const double sl = minmod(cs, minmod(2.0 * rsm, 2.0 * rs));            // sl(p), p >= 0
     const double slm = lane_up1(sl, k.sl[q]);                              // sl(p-1)
     const double phn = f0 * v - ((0.5 * v) * (dx_0 + vq * dt)) * sl;
     const double phv = fm1 * v + ((0.5 * v) * (dx_0 - vq * dt)) * slm;
     const double ph = v < 0.0 ? phn : phv;                                 // phi(p), p >= 1
     const double phm = lane_up1(ph, k.ph[q]);                              // phi(p-1)
-    const double out = fm1 - ((dt * (h2p * ph - h2m * phm)) / dxim) / ha1m;
+    const double out =
+        LM == 2 ? fm1 - div_rcp(div_rcp(dt * (h2p * ph - h2m * phm), dxim, rdxim), ha1m, rha1m)
+                : fm1 - ((dt * (h2p * ph - h2m * phm)) / dxim) / ha1m;
     // cells 1..lx1 take the update; the other lanes put the ghost value they cover back
     fp[q][clampi(p - 1, -1, lx1 + 2)] = upd ? out : (p < 0 ? f0 : fm1);
     k.rs[q] = lane63(rs);
@@ -494,8 +517,9 @@ __device__ __forceinline__ void s1_chunk(const DevGrid& g, double* const* fp, co
   k.dx = lane63(dx_p);
 }
 
-// line (i2, i3) = (bx + 1, by + 1)
-template <int NQ, bool OTF, bool LM = false>
+// line (i2, i3) = (bx + 1, by + 1); under LM the block (k_sweep1) has written the LDS copy
+// of the line's factors `lm`, which the walk reads after the barrier below
+template <int NQ, bool OTF, int LM = 0>
 __device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
                                             const SweepFields& F, int isp0, double dt, int bx,
                                             int by, const double* lm = nullptr) {
@@ -512,8 +536,14 @@ __device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
   const double* u = s.vs1 + so + c0;                                        // u[p] = vs1(p)
   const double* v1i = s.vs1i + (long)(isp - 1) * nif;
   // boundary interface velocities (advec.f90:171-196), the same for the whole wave
+  // (under LM from the planes the step saved from the same vs1 before its x3 sweep: two
+  // loads, where v1i_at's branches wait for every load in flight, the factors' included)
   double vb1 = 0.0, vbN = 0.0;
-  if (OTF) {
+  if (OTF && LM) {
+    const long vb = 2L * ((i2 - 1) + (long)g.lx2 * (i3 - 1) + (long)g.lx2 * g.lx3 * (isp - 1));
+    vb1 = s.v1b[vb];
+    vbN = s.v1b[vb + 1];
+  } else if (OTF) {
     vb1 = v1i_at(g, u, 1);
     vbN = v1i_at(g, u, lx1 + 1);
   }
@@ -525,6 +555,9 @@ __device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
   const int nchunk = (lx1 + 3 + 63) / 64;                                 // p = -1 .. lx1+1
   S1Chunk<NQ> A, B;
   s1_load<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, lane - 1, A);
+  // the first chunk's loads are issued before the barrier after which every wave reads
+  // the line's factors, so their latency and the factors' overlap
+  if (LM) __syncthreads();
   int c = 0;
   for (; c + 1 < nchunk; c += 2) {
     s1_chunk<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k, lm);
@@ -532,11 +565,16 @@ __device__ __forceinline__ void sweep1_body(const DevGrid& g, const DevState& s,
   }
   if (c < nchunk) s1_chunk<NQ, OTF, LM>(g, fp, u, v1i, c0, if0, c, lane, dt, vb1, vbN, A, B, k, lm);
 }
-// LM: the line's metric factors in LDS (dynamic, sweep1_lm_bytes): the block loads h1x1i,
-// the products h2x1i h3x1i and (h1 h2) h3, dx1 and dx1i of its line once, in the order the
-// walk forms them, instead of every wave loading them for every chunk
-__host__ __device__ inline int sweep1_lm_bytes(int lx1) { return 5 * (lx1 + 3) * 8; }
-template <int NQ, bool OTF, bool LM = false>
+// LM 1: the line's metric factors in LDS (dynamic, sweep1_lm_bytes): the block loads
+// h1x1i, the products h2x1i h3x1i and (h1 h2) h3, dx1 and dx1i of its line once, in the
+// order the walk forms them, instead of every wave loading them for every chunk.  LM 2:
+// also the reciprocals of the five divisors the walk divides by (h1x1i, (h1 h2) h3, dx1,
+// dx1(p) + dx1(p+1), dx1i), each a correctly rounded 1/b, so that each of the walk's
+// thirteen divisions per cell is a multiply and two FMAs (div_rcp) with the same result.
+__host__ __device__ inline int sweep1_lm_bytes(int lx1, int lm = 1) {
+  return (lm == 2 ? 10 : 5) * (lx1 + 3) * 8;
+}
+template <int NQ, bool OTF, int LM = 0>
 __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFields F, int isp0,
                                                 double dt) {
   extern __shared__ double lm[];
@@ -546,13 +584,23 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFiel
     const long if0 = (long)(lx1 + 1) * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) - 1;
     for (int t = threadIdx.y * blockDim.x + threadIdx.x; t < L; t += blockDim.x * blockDim.y) {
       const int pi = clampi(t, 1, lx1 + 1), pc = clampi(t, 1, lx1);
-      lm[t] = g.h1x1i[if0 + pi];
+      const double i1v = g.h1x1i[if0 + pi];
+      const double ha = (g.h1[c0 + pc] * g.h2[c0 + pc]) * g.h3[c0 + pc];
+      const double dxp = g.dx1[min(t, lx1 + 2)], dxi = g.dx1i[min(t, lx1 - 1)];
+      lm[t] = i1v;
       lm[L + t] = g.h2x1i[if0 + pi] * g.h3x1i[if0 + pi];
-      lm[2 * L + t] = (g.h1[c0 + pc] * g.h2[c0 + pc]) * g.h3[c0 + pc];
-      lm[3 * L + t] = g.dx1[min(t, lx1 + 2)];
-      lm[4 * L + t] = g.dx1i[min(t, lx1 - 1)];
+      lm[2 * L + t] = ha;
+      lm[3 * L + t] = dxp;
+      lm[4 * L + t] = dxi;
+      if (LM == 2) {
+        lm[5 * L + t] = 1.0 / i1v;
+        lm[6 * L + t] = 1.0 / ha;
+        lm[7 * L + t] = 1.0 / dxp;
+        lm[8 * L + t] = 1.0 / (g.dx1[min(max(t - 1, 0), lx1 + 2)] + dxp);   // dx(p) + dx(p+1)
+        lm[9 * L + t] = 1.0 / dxi;
+      }
     }
-    __syncthreads();
+    // (the barrier is in sweep1_body, after the first chunk's loads)
   }
   if (NQ == 3 && F.elec_energy && isp0 + (int)threadIdx.y == LSP) {
     // the electron wave of an all-species block sweeps its energy only
@@ -579,6 +627,9 @@ __global__ __launch_bounds__(448) void k_sweep1(DevGrid g, DevState s, SweepFiel
 #endif
 #ifndef SWEEP1_LM_MAX
 #define SWEEP1_LM_MAX 49152   // x1 sweep: largest LDS copy of a line's metric factors (bytes)
+#endif
+#ifndef SWEEP1_RCP
+#define SWEEP1_RCP 1          // x1 sweep: divisions through the line's reciprocals where they fit
 #endif
 #ifndef SWEEP1_E7_IN_BLOCK
 #define SWEEP1_E7_IN_BLOCK 1  // x1: electrons as a seventh wave of the species block
@@ -1401,8 +1452,11 @@ void launch_sweep(const DevGrid& g, const DevState& s, int dir, int fields, doub
   } else if (na == 3 && dir == 1 && SWEEP1_E7_IN_BLOCK) {
     // x1, all fields: the electron wave joins the block and sweeps its energy only
     A.elec_energy = 1;
-    if (onthefly && sweep1_lm_bytes(g.lx1) <= SWEEP1_LM_MAX) {
-      hipLaunchKernelGGL((k_sweep1<3, true, true>), dim3(g.lx2, g.lx3), dim3(64, LSP),
+    if (onthefly && SWEEP1_RCP && sweep1_lm_bytes(g.lx1, 2) <= SWEEP1_LM_MAX) {
+      hipLaunchKernelGGL((k_sweep1<3, true, 2>), dim3(g.lx2, g.lx3), dim3(64, LSP),
+                         sweep1_lm_bytes(g.lx1, 2), st, g, s, A, 1, dt);
+    } else if (onthefly && sweep1_lm_bytes(g.lx1) <= SWEEP1_LM_MAX) {
+      hipLaunchKernelGGL((k_sweep1<3, true, 1>), dim3(g.lx2, g.lx3), dim3(64, LSP),
                          sweep1_lm_bytes(g.lx1), st, g, s, A, 1, dt);
     } else {
       sweep_launch<3>(g, s, dir, A, 1, LSP, dt, onthefly, st);
