@@ -59,7 +59,8 @@ struct Elim {
 // right-hand side is known changes nothing in either.  The pivot rows, right-hand sides
 // and solutions live in lane-interleaved HBM scratch (one 512-byte segment per row and
 // slot for the wave); outputs are staged through LDS so global stores are coalesced.
-// Blocks are numbered so that the seven species of a column group run on one XCD.
+// Blocks are numbered so that the seven species of a column group run on one XCD, the
+// electron groups (the longest waves) first.
 //
 // The steady state of every pass issues its memory operations unconditionally (clamped
 // indices; stores that must not land go where a later store of the same lane overwrites
@@ -523,10 +524,21 @@ __global__ __launch_bounds__(64) void k_diffusion(DevGrid g, DevState s, double*
   __shared__ double sT[R][LP], sC[R][LP], sB[R][LP], sD[R][LP];
   const long ncol = (long)g.lx2 * g.lx3;
   const long gps = (ncol + 63) / 64;                 // column groups per species
-  const long slot = blockIdx.x / 8;
-  const long cg = (slot / LSP) * 8 + blockIdx.x % 8;  // column group (same XCD for all species)
+  // the electron groups first (their waves are the longest, so the last wave round is
+  // shorter), then the six ion species of each run of 8 groups; block b runs on XCD b % 8,
+  // and every species of column group cg on XCD cg % 8
+  const long ne = (gps + 7) / 8 * 8;
+  long cg;
+  int isp;
+  if ((long)blockIdx.x < ne) {
+    cg = blockIdx.x;
+    isp = LSP;
+  } else {
+    const long b = blockIdx.x - ne, slot = b / 8;
+    cg = (slot / (LSP - 1)) * 8 + b % 8;
+    isp = (int)(slot % (LSP - 1)) + 1;
+  }
   if (cg >= gps) return;
-  const int isp = (int)(slot % LSP) + 1;
   if (isp == LSP)
     diffusion_group<R, BE, true, PH, EXT>(g, s, scratch, swbits, lst, k1, bdf, Teinf, isp, cg,
                                           gps, sT, sC, sB, sD);
