@@ -580,10 +580,18 @@ __global__ __launch_bounds__(64, 2) void kb_diffusion(const BatchMember* M, cons
     const long v = t - p[m];
     const long ncol = (long)g.lx2 * g.lx3;
     const long gps = (ncol + 63) / 64;
-    const long slot = v / 8;
-    const long cg = (slot / LSP) * 8 + v % 8;
+    const long ne = (gps + 7) / 8 * 8;               // (k_diffusion's numbering)
+    long cg;
+    int isp;
+    if (v < ne) {
+      cg = v;
+      isp = LSP;
+    } else {
+      const long b = v - ne, slot = b / 8;
+      cg = (slot / (LSP - 1)) * 8 + b % 8;
+      isp = (int)(slot % (LSP - 1)) + 1;
+    }
     if (cg >= gps) continue;
-    const int isp = (int)(slot % LSP) + 1;
     const double dt = M[m].dt;
     Coef tr{}, bdf{};
     tr.dt2 = dt / 2.0; tr.K = 1.0 / tr.dt2; tr.dt = dt;
