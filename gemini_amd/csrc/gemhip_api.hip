@@ -354,6 +354,11 @@ void halo_fields(std::initializer_list<Var> vars) {
 __global__ void k_eval_math(int op, const double* x, double* y, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (op == 4) {                                    // pairs (numerator, divisor)
+    const double a = x[2 * i], b = x[2 * i + 1];
+    y[i] = div_rcp(a, b, 1.0 / b);
+    return;
+  }
   const double v = x[i];
   y[i] = op == 0 ? pow25(v) : op == 1 ? sqrt(v) : op == 2 ? 1.0 / v : pow(v, 2.5);
 }
@@ -1777,12 +1782,13 @@ int gemhip_batch_release(int id) {
 int gemhip_eval_math(int op, const double* x, double* y, long n) {
   Ctx& c = ctx();
   if (!c.inited) { fail("library not initialised (gemhip_init)"); return 1; }
-  if (op < 0 || op > 3 || n < 0 || (n > 0 && (!x || !y))) { fail("bad gemhip_eval_math call"); return 2; }
+  if (op < 0 || op > 4 || n < 0 || (n > 0 && (!x || !y))) { fail("bad gemhip_eval_math call"); return 2; }
   if (n == 0) return 0;
+  const long nx = op == 4 ? 2 * n : n;               // op 4 reads n pairs
   double *dx = nullptr, *dy = nullptr;
-  bool ok = hipok(hipMalloc((void**)&dx, n * sizeof(double)), "eval alloc") &&
+  bool ok = hipok(hipMalloc((void**)&dx, nx * sizeof(double)), "eval alloc") &&
             hipok(hipMalloc((void**)&dy, n * sizeof(double)), "eval alloc") &&
-            hipok(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyDefault), "eval upload");
+            hipok(hipMemcpy(dx, x, nx * sizeof(double), hipMemcpyDefault), "eval upload");
   if (ok) {
     hipLaunchKernelGGL(k_eval_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, op,
                        dx, dy, n);

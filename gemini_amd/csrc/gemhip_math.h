@@ -52,6 +52,18 @@ GEMHIP_HD double thermal_coeff(int isp) {
 GEMHIP_HD double fmin_f(double a, double b) { return (b < a) ? b : a; }
 GEMHIP_HD double fmax_f(double a, double b) { return (b > a) ? b : a; }
 
+// x / b from y = RN(1/b), correctly rounded: q0 = RN(x y) is within a few ulps of x/b,
+// the FMA residual x - b q0 is exact, and one correction step q0 + r y, rounded once,
+// gives RN(x/b) (Markstein's theorem for a correctly rounded reciprocal).  Where the
+// residual is zero (exact quotients, zero numerators: q0 has the sign IEEE division gives)
+// or not a number (infinite numerators) q0 is the quotient.
+__device__ __forceinline__ double div_rcp(double x, double b, double y) {
+  const double q0 = x * y;
+  const double r = __builtin_fma(-b, q0, x);
+  const double q1 = __builtin_fma(r, y, q0);
+  return __builtin_amdgcn_class(r, 0x63) ? q0 : q1;    // sNaN qNaN -0 +0
+}
+
 // advec.f90:438-447
 GEMHIP_HD double minmod(double a, double b) {
   if (a * b <= 0.0) return 0.0;

@@ -397,18 +397,6 @@ __device__ __forceinline__ double lane63(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// x / b from y = RN(1/b), correctly rounded: q0 = RN(x y) is within a few ulps of x/b,
-// the FMA residual x - b q0 is exact, and one correction step q0 + r y, rounded once,
-// gives RN(x/b) (Markstein's theorem for a correctly rounded reciprocal).  Where the
-// residual is zero (exact quotients, zero numerators: q0 has the sign IEEE division gives)
-// or not a number (infinite numerators) q0 is the quotient.
-__device__ __forceinline__ double div_rcp(double x, double b, double y) {
-  const double q0 = x * y;
-  const double r = __builtin_fma(-b, q0, x);
-  const double q1 = __builtin_fma(r, y, q0);
-  return __builtin_amdgcn_class(r, 0x63) ? q0 : q1;    // sNaN qNaN -0 +0
-}
-
 template <int NQ>
 struct S1Chunk {
   double f[NQ], fn[NQ];     // f(p), f(p+1)

@@ -348,11 +348,15 @@ class Transport:
         return C.cast(f, C.c_void_p).value, C.cast(a, C.c_void_p).value, C.cast(j, C.c_void_p).value
 
     def eval_math(self, op: int, x: np.ndarray) -> np.ndarray:
-        """Device evaluation of a scalar routine (0 x**2.5, 1 sqrt, 2 1/x, 3 device pow)."""
+        """Device evaluation of a scalar routine (0 x**2.5, 1 sqrt, 2 1/x, 3 device pow;
+        4 a/b through the reciprocal 1/b, for x of shape (n, 2) holding the pairs)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
-        y = np.empty_like(x)
+        n = x.shape[0] if op == 4 else x.size
+        if op == 4 and x.shape != (n, 2):
+            raise ValueError("op 4 takes an (n, 2) array of (numerator, divisor) pairs")
+        y = np.empty(n)
         self._call("gemhip_eval_math", int(op), x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
-                   x.size)
+                   n)
         return y
 
     def kernel_timing(self, enable: bool):

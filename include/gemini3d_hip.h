@@ -281,7 +281,8 @@ int gemhip_read_initial_conditions(const char* indatsize, const char* indatfile,
 
 /* Evaluate on the device the scalar routines the kernels use, for checks against the
    host libm: op 0 x**2.5 as the conductivity uses it, 1 sqrt, 2 reciprocal 1/x,
-   3 the device library pow(x, 2.5).  x and y are host arrays of n doubles. */
+   3 the device library pow(x, 2.5).  x and y are host arrays of n doubles; for op 4,
+   the x1 sweep's division a/b through the reciprocal 1/b, x holds n pairs (a, b). */
 int gemhip_eval_math(int op, const double* x, double* y, long n);
 
 /* ---------------- part 3: contexts, tuning and ensembles (no counterpart in the reference) ---------------- */

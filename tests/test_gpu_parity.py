@@ -232,6 +232,49 @@ def test_errors_are_reported():
     tr.configure(2, 1500.0)
 
 
+def _division_pairs(rng, n):
+    """(a, b) pairs for the reciprocal division: random over 200 binades, divisors with
+    significands near 2 and near 1, numerators next to b times a rounding midpoint, exact
+    quotients, signed zeros, infinities and NaN."""
+    def mk(mant, e):
+        bits = (np.asarray(e + 1023, dtype=np.uint64) << np.uint64(52)) | (mant & np.uint64((1 << 52) - 1))
+        return bits.view(np.float64)
+    m = lambda k: rng.integers(0, 1 << 52, k, dtype=np.uint64)
+    e = lambda k: rng.integers(-100, 100, k)
+    a = mk(m(n), e(n)) * rng.choice([-1.0, 1.0], n)
+    b = mk(m(n), e(n))
+    b_hi = mk(np.uint64((1 << 52) - 1) ^ (m(n) & np.uint64(0xffff)), e(n))
+    b_lo = mk(m(n) & np.uint64(0xffff), e(n))
+    # near midpoints: a = RN(b (q + ulp(q)/2)) and its neighbours, formed in extended precision
+    q = mk(m(n), rng.integers(-50, 50, n))
+    bm = mk(m(n), rng.integers(-50, 50, n))
+    mid = (q.astype(np.longdouble) + np.spacing(q).astype(np.longdouble) / 2) * bm.astype(np.longdouble)
+    a0 = mid.astype(np.float64)
+    near = [np.nextafter(a0, np.inf), a0, np.nextafter(a0, -np.inf)]
+    k = rng.integers(1, 1000, n).astype(np.float64)
+    special_a = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 3.0])
+    pairs = [np.stack([a, b], 1), np.stack([a, b_hi], 1), np.stack([a, b_lo], 1),
+             np.stack([k * b, b], 1)]
+    pairs += [np.stack([x, bm], 1) for x in near]
+    pairs.append(np.stack([np.repeat(special_a, 16), np.tile(b[:16], special_a.size)], 1))
+    return np.concatenate(pairs)
+
+
+def test_reciprocal_division_matches_ieee_division():
+    """The x1 sweep divides by line constants through their reciprocals (q0 = a y,
+    r = fma(-b, q0, a), q = fma(r, y, q0), y = 1/b): bit-identical to a/b."""
+    g, st = make("cart", (8, 4, 3))
+    tr = transport_for(g, st)
+    x = _division_pairs(np.random.default_rng(3), 400000)
+    got = tr.eval_math(4, x)
+    with np.errstate(all="ignore"):
+        want = x[:, 0] / x[:, 1]
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.int64), want[~nan].view(np.int64))
+    print(f"{x.shape[0]} quotients bit-identical to IEEE division")
+
+
 def test_device_math_matches_host():
     """x**2.5 (double-double) and sqrt on the GPU round exactly like the host versions."""
     g, st = make("cart", (8, 4, 3))
