@@ -127,14 +127,16 @@ def test_each_entry_point(kind, shape):
     tr.configure(2, 1500.0)
 
 
-def test_long_lines_match_oracle():
-    """lx1 = 1300: x1 lines too long for the x1 sweep's LDS copy of their metric factors
-    (the register path), and 1300-row TRBDF2 columns.  Three steps against the oracle with
-    the GPU's x**2.5, every variable bit-identical.  (Against libm's pow the temperatures
-    of such long columns differ by 1.2e-12 normwise: the one-ulp differences of x**2.5
-    pass through a longer implicit solve.)"""
-    print("case cart (1300, 4, 3)", flush=True)
-    g, st = make("cart", (1300, 4, 3))
+@pytest.mark.parametrize("lx1", [700, 1300])
+def test_long_lines_match_oracle(lx1):
+    """Long x1 lines: at lx1 = 700 the x1 sweep keeps the line's metric factors in LDS but
+    not their reciprocals (true divisions), at 1300 neither (the register path); TRBDF2
+    columns of 700 and 1300 rows.  Three steps against the oracle with the GPU's x**2.5,
+    every variable bit-identical.  (Against libm's pow the temperatures of such long
+    columns differ by ~1e-12 normwise: the one-ulp differences of x**2.5 pass through a
+    longer implicit solve.)"""
+    print(f"case cart ({lx1}, 4, 3)", flush=True)
+    g, st = make("cart", (lx1, 4, 3))
     dt = ga.cfl_dt(g, st)
     tr = transport_for(g, st)
     tr.set_residency(1)
