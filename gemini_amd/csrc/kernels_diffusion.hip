@@ -144,7 +144,8 @@ __device__ __forceinline__ bool elim_rhs(const Group& q, Elim& e, int j, int sj,
 }
 
 // The same step on the matrix alone (the BDF2 factorisation): stores the pivot row, the
-// multiplier l and the interchange bit that the right-hand side will need in pass 3.
+// multiplier l and the interchange bit that the right-hand side will need in pass 3 (a
+// word of 32 steps' bits, stored once its last step, or step N-1, has run).
 struct Fact {
   double r0, r1;
   bool singular;
@@ -175,7 +176,7 @@ __device__ __forceinline__ void elim_mat(const Group& q, Fact& f, int j, int sj,
     f.bits = (w == f.word ? f.bits : 0) | ((swap ? 1 : 0) << (j & 31));
     f.word = w;
   }
-  q.swb[f.word * 64] = f.bits;
+  if ((j & 31) == 31 || j >= q.N - 1) q.swb[f.word * 64] = f.bits;   // the word is complete
 }
 
 template <int R, bool BE, bool ELEC, int PH, bool EXT>
@@ -457,7 +458,6 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
   {
     constexpr int PF = R;
     double ptr[PF], pl[PF];
-    int pb[PF];
     double tv[R];
     auto issue_T = [&](int r0) {                     // rows r0..r0+R-1 (clamped to 1..N)
       const int rr = clampi(r0 + lr, 1, N);
@@ -477,21 +477,29 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
     auto fetch = [&](int r, int u) {                 // inputs of step j = r-1
       const int rc = clampi(r, 1, N), jc = clampi(r - 1, 1, N - 1);
       ptr[u] = get2(q, rc, STR); pl[u] = get2(q, jc, SL);
-      pb[u] = q.swb[(jc >> 5) * 64];
+    };
+    // the interchange words of a chunk's steps j0 .. j0+PF-1 (at most two), a chunk ahead
+    auto words = [&](int j0, int& w0, int& w1) {
+      w0 = q.swb[(clampi(j0, 1, N - 1) >> 5) * 64];
+      w1 = q.swb[(clampi(j0 + PF - 1, 1, N - 1) >> 5) * 64];
     };
 #pragma unroll
     for (int u = 0; u < PF; ++u) fetch(t0 + u, u);
     const bool fs = f.singular;
     S2(q, 1, SY) = Tmin;
     double rb = Tmin;
+    int cw0, cw1, nw0, nw1;
+    words(t0 - 1, cw0, cw1);
     int ch = 0;                                      // chunk of rows t0 + ch R ..
     for (int j0 = t0 - 1; j0 <= N - 1; j0 += PF, ++ch) {
       double (*tb)[LP] = (ch & 1) ? sC : sT;
+      words(j0 + PF, nw0, nw1);
+      const int jw0 = clampi(j0, 1, N - 1) >> 5;
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int j = j0 + u, r = j + 1;
         const double at = -((bdf.c13 * tb[u][lane]) / bdf.dt3), trr = ptr[u], l = pl[u];
-        const int bits = pb[u];
+        const int bits = (clampi(j, 1, N - 1) >> 5) == jw0 ? cw0 : cw1;
         fetch(r + PF, u);
         const bool live = j >= 1 && j <= N - 1;
         const double d = r >= N ? Tmax : at + (bdf.c43 * trr) / bdf.dt3;
@@ -503,6 +511,8 @@ __device__ __forceinline__ void diffusion_group(const DevGrid& g, const DevState
         put2(q, live ? (fs ? r : j) : N, SY, fs ? d : pd);
         rb = live ? od : rb;
       }
+      cw0 = nw0;
+      cw1 = nw1;
       __builtin_amdgcn_wave_barrier();
       store_T((ch & 1) ? sT : sC);                   // chunk ch + 1
       __builtin_amdgcn_wave_barrier();
