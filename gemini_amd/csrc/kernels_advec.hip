@@ -79,30 +79,38 @@ __global__ void k_rhoe2T(DevGrid g, DevState s) {
 
 // rhov12v1 then rhoe2T of one padded cell: both are pointwise, so doing them cell by cell
 // gives what the two whole-array passes give, with ns read once.
+// Every input is loaded before the first store (the arrays are distinct, but the compiler
+// cannot know that, and would otherwise wait for each species' loads in turn).
 __device__ __forceinline__ void post_convert_cell(const DevGrid& g, const DevState& s, long k) {
-  {
-    double chrgflux = 0.0, sum = 0.0;
-    double n[LSP];
+  double n[LSP], rv[LSP - 1], e[LSP];
 #pragma unroll
-    for (int isp = 1; isp <= LSP - 1; ++isp) {
-      const long o = (long)(isp - 1) * g.N + k;
+  for (int isp = 1; isp <= LSP; ++isp) {
+    const long o = (long)(isp - 1) * g.N + k;
+    if (isp < LSP) {
       n[isp - 1] = s.ns[o];
-      const double v = s.rhovs1[o] / (ms(isp) * fmax_f(n[isp - 1], mindensdiv));
-      s.vs1[o] = v;
-      chrgflux = chrgflux + (n[isp - 1] * qs(isp)) * v;
+      rv[isp - 1] = s.rhovs1[o];
     }
+    e[isp - 1] = s.rhoes[o];
+  }
+  double chrgflux = 0.0, sum = 0.0;
 #pragma unroll
-    for (int isp = 1; isp <= LSP - 1; ++isp) sum = sum + n[isp - 1];
-    n[LSP - 1] = sum;
-    const long o7 = (long)(LSP - 1) * g.N + k;
-    s.ns[o7] = sum;
-    s.vs1[o7] = -(((1.0 / fmax_f(sum, mindensdiv)) / qs(LSP)) * chrgflux);
+  for (int isp = 1; isp <= LSP - 1; ++isp) {
+    const long o = (long)(isp - 1) * g.N + k;
+    const double v = rv[isp - 1] / (ms(isp) * fmax_f(n[isp - 1], mindensdiv));
+    s.vs1[o] = v;
+    chrgflux = chrgflux + (n[isp - 1] * qs(isp)) * v;
+  }
 #pragma unroll
-    for (int isp = 1; isp <= LSP; ++isp) {
-      const long o = (long)(isp - 1) * g.N + k;
-      const double t = (((gammas(isp) - 1.0) / kB) * s.rhoes[o]) / fmax_f(n[isp - 1], mindensdiv);
-      s.Ts[o] = fmax_f(t, 100.0);
-    }
+  for (int isp = 1; isp <= LSP - 1; ++isp) sum = sum + n[isp - 1];
+  n[LSP - 1] = sum;
+  const long o7 = (long)(LSP - 1) * g.N + k;
+  s.ns[o7] = sum;
+  s.vs1[o7] = -(((1.0 / fmax_f(sum, mindensdiv)) / qs(LSP)) * chrgflux);
+#pragma unroll
+  for (int isp = 1; isp <= LSP; ++isp) {
+    const long o = (long)(isp - 1) * g.N + k;
+    const double t = (((gammas(isp) - 1.0) / kB) * e[isp - 1]) / fmax_f(n[isp - 1], mindensdiv);
+    s.Ts[o] = fmax_f(t, 100.0);
   }
 }
 
@@ -158,7 +166,47 @@ __device__ __forceinline__ double pre_energy(double n, double T, int isp) {
   return ((n * kB) * T) / (gammas(isp) - 1.0);
 }
 
-// advec.f90:42-79: x1 ends, one thread per padded (x2,x3) column t and species
+// advec.f90:42-79: x1 ends, one thread per padded (x2,x3) column t and species.  The two
+// ends mirror each other: end 0 fills cells 0, -1 from cells 1, 2, 3, end 1 fills lx1+1,
+// lx1+2 from lx1, lx1-1, lx1-2 (index map m).  Each end's inputs are loaded before the
+// stores (the arrays are distinct, which the compiler cannot know), and both ends' before
+// either end's stores wherever the far end reads none of the cells the near end writes.
+struct X1End { double nm1, n0, n1, n2, n3, u0, u1, s1, r0, vb; };
+template <bool CONV>
+__device__ __forceinline__ void x1_end_load(const double* n, const double* r, const double* e,
+                                            const double* u, const double* T, int lx1, int side,
+                                            double vb, X1End& E) {
+  auto m = [&](int j) { return side ? lx1 + 1 - j : j; };
+  E.nm1 = n[m(-1)]; E.n0 = n[m(0)]; E.n1 = n[m(1)]; E.n2 = n[m(2)]; E.n3 = n[m(3)];
+  E.u0 = u[m(0)]; E.u1 = u[m(1)];
+  E.s1 = CONV ? T[m(1)] : e[m(1)];
+  E.r0 = CONV ? 0.0 : r[m(0)];
+  E.vb = vb;
+}
+template <bool CONV>
+__device__ __forceinline__ void x1_end_store(double* n, double* r, double* e, int lx1, int side,
+                                             bool inner, int isp, const X1End& E) {
+  auto m = [&](int j) { return side ? lx1 + 1 - j : j; };
+  const double msp = ms(isp);
+  double n0 = E.n0, nm1 = E.nm1, r0;
+  if (inner) {
+    const double coeff = E.n2 / E.n3;
+    n0 = fmin_f(coeff * E.n1, E.n1);
+    nm1 = fmin_f(coeff * n0, n0);
+    n[m(0)] = n0;
+    n[m(-1)] = nm1;
+    r0 = 2.0 * E.vb - E.u1;                  // E.vb: v1i at the end's boundary interface
+  } else {
+    r0 = CONV ? pre_momentum(E.n0, E.u0, isp) : E.r0;
+  }
+  const double rm = (r0 + r0) - E.u1;
+  r[m(-1)] = (rm * nm1) * msp;
+  r[m(0)] = (r0 * n0) * msp;
+  const double e1 = CONV ? pre_energy(E.n1, E.s1, isp) : E.s1;
+  const double Tb = e1 / E.n1;
+  e[m(0)] = Tb * n0;
+  e[m(-1)] = Tb * nm1;
+}
 template <bool CONV>
 __device__ __forceinline__ void bc_x1_body(const DevGrid& g, const DevState& s, int isp, long t) {
   const int i2 = (int)(t % g.L2) - 1, i3 = (int)(t / g.L2) - 1;
@@ -169,44 +217,17 @@ __device__ __forceinline__ void bc_x1_body(const DevGrid& g, const DevState& s, 
   double* e = s.rhoes + o + 1;
   const double* u = s.vs1 + o + 1;
   const double* T = s.Ts + o + 1;
-  const double msp = ms(isp);
   const int lx1 = g.lx1;
   const long vb = 2L * ((i2 - 1) + (long)g.lx2 * (i3 - 1)) + 2L * g.lx2 * g.lx3 * (isp - 1);
-  if (g.isglobal[0]) {
-    double r0;
-    if (inner) {
-      const double coeff = n[2] / n[3];
-      n[0] = fmin_f(coeff * n[1], n[1]);
-      n[-1] = fmin_f(coeff * n[0], n[0]);
-      r0 = 2.0 * s.v1b[vb] - u[1];
-    } else {
-      r0 = CONV ? pre_momentum(n[0], u[0], isp) : r[0];
-    }
-    const double rm = (r0 + r0) - u[1];
-    r[-1] = (rm * n[-1]) * msp;
-    r[0] = (r0 * n[0]) * msp;
-    const double e1 = CONV ? pre_energy(n[1], T[1], isp) : e[1];
-    const double Tb = e1 / n[1];
-    e[0] = Tb * n[0];
-    e[-1] = Tb * n[-1];
-  }
-  if (g.isglobal[1]) {
-    double r1;
-    if (inner) {
-      const double coeff = n[lx1 - 1] / n[lx1 - 2];
-      n[lx1 + 1] = fmin_f(coeff * n[lx1], n[lx1]);
-      n[lx1 + 2] = fmin_f(coeff * n[lx1 + 1], n[lx1 + 1]);
-      r1 = 2.0 * s.v1b[vb + 1] - u[lx1];
-    } else {
-      r1 = CONV ? pre_momentum(n[lx1 + 1], u[lx1 + 1], isp) : r[lx1 + 1];
-    }
-    const double r2 = (r1 + r1) - u[lx1];
-    r[lx1 + 1] = (r1 * n[lx1 + 1]) * msp;
-    r[lx1 + 2] = (r2 * n[lx1 + 2]) * msp;
-    const double eN = CONV ? pre_energy(n[lx1], T[lx1], isp) : e[lx1];
-    const double Tb = eN / n[lx1];
-    e[lx1 + 1] = Tb * n[lx1 + 1];
-    e[lx1 + 2] = Tb * n[lx1 + 2];
+  const bool lo = g.isglobal[0], hi = g.isglobal[1];
+  const bool early = lx1 >= 3;     // the far end's cells lx1-2.. are not among -1, 0
+  X1End A, B;
+  if (lo) x1_end_load<CONV>(n, r, e, u, T, lx1, 0, inner ? s.v1b[vb] : 0.0, A);
+  if (hi && early) x1_end_load<CONV>(n, r, e, u, T, lx1, 1, inner ? s.v1b[vb + 1] : 0.0, B);
+  if (lo) x1_end_store<CONV>(n, r, e, lx1, 0, inner, isp, A);
+  if (hi) {
+    if (!early) x1_end_load<CONV>(n, r, e, u, T, lx1, 1, inner ? s.v1b[vb + 1] : 0.0, B);
+    x1_end_store<CONV>(n, r, e, lx1, 1, inner, isp, B);
   }
 }
 template <bool CONV>
